@@ -1,0 +1,189 @@
+/*
+ * xdem_capi.h -- C ABI of the MI355X-native DEM time step (libxdem.so).
+ *
+ * Two layers, both `extern "C"`, plain pointers and sizes, every entry returns an int
+ * status (XDEM_OK = 0, negative on error; see xdem_types.h):
+ *
+ * 1. Launcher layer. One entry per launcher of the reference's CUDA layer
+ *    (include/xdynamics_parallel/xParallelDEM_decl.cuh:280-424, "decl" below), same
+ *    argument order and meaning, device pointers in the reference's AoS layouts, queued
+ *    on the stream set by xdem_set_stream (default stream otherwise). The C++ header
+ *    xParallelDEM_decl.hpp re-exports them under the reference's exact names and
+ *    signatures.
+ *
+ * 2. Step layer (xdem_ctx). The whole DEM step of
+ *    xIntegratorVV::OneStepSimulation (source/xdynamics_simulation/xIntegratorVV.cpp:26-64)
+ *    with all contact objects (xContactManager::deviceCollision,
+ *    source/xdynamics_manager/xContactManager.cpp:458-472) fused into a few kernels, for
+ *    one or many independent scenes advanced together. The C++ facade
+ *    xDiscreteElementMethodSimulation.hpp is built on it.
+ */
+#ifndef XDEM_CAPI_H
+#define XDEM_CAPI_H
+
+#include "xdem_host.h"
+#include "xdem_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ====================================================================== launcher layer */
+
+/* setDEMSymbolicParameter (decl:280; def:16-19). Copies the struct; later launches use it. */
+int xdem_setDEMSymbolicParameter(const xdem_dem_parameters* h_paras);
+/* Quirk switches for the launchers (default: parity profile, see xdem_quirks). */
+int xdem_set_quirks(const xdem_quirks* q);
+/* hipStream_t for all launcher-layer work (NULL = default stream). */
+int xdem_set_stream(void* hip_stream);
+/* Per-particle body-force buffers written by the wall and mesh launchers; the reference
+ * keeps them as statics of xContact (xContact::deviceBodyForceX..MomentZ,
+ * source/xdynamics_object/xContact.cpp:339-345). NULLs select library-owned buffers. */
+int xdem_set_body_force_buffers(double* fx, double* fy, double* fz, double* mx, double* my, double* mz);
+int xdem_get_body_force_buffers(double** fx, double** fy, double** fz, double** mx, double** my, double** mz);
+/* Device error word of the launcher layer (XDEM_ERR_* bits); clear != 0 resets it. */
+int xdem_get_error_word(unsigned int* out, int clear);
+
+/* vv_update_position (decl:282-283) */
+int xdem_vv_update_position(double* pos, double* ep, double* vel, double* ev, double* acc, double* ea,
+                            unsigned int np);
+/* vv_update_velocity (decl:284-285) */
+int xdem_vv_update_velocity(double* vel, double* acc, double* ep, double* ev, double* ea, double* force,
+                            double* moment, double* mass, double* iner, unsigned int np);
+/* cu_calculateHashAndIndex (decl:287) */
+int xdem_cu_calculateHashAndIndex(unsigned int* hash, unsigned int* index, double* pos, unsigned int sid,
+                                  unsigned int np);
+/* cu_calculateHashAndIndexForPolygonSphere (decl:288-290) */
+int xdem_cu_calculateHashAndIndexForPolygonSphere(unsigned int* hash, unsigned int* index, unsigned int sid,
+                                                  unsigned int nsphere, double* sphere);
+/* cu_reorderDataAndFindCellStart (decl:291): sorts (hash, index) in place by hash (stable),
+ * fills cstart/cend (0xffffffff / 0 for empty cells) and sorted_index. */
+int xdem_cu_reorderDataAndFindCellStart(unsigned int* hash, unsigned int* index, unsigned int* cstart,
+                                        unsigned int* cend, unsigned int* sorted_index, unsigned int np,
+                                        unsigned int ncell);
+/* cu_calculate_p2p (decl:293-299); cp is a device pointer, 12 history slots per particle. */
+int xdem_cu_calculate_p2p(double* pos, double* ep, double* vel, double* omega, double* force, double* moment,
+                          double* mass, double* tmax, double* rres, unsigned int* pair_count,
+                          unsigned int* pair_id, double* tsd, unsigned int* sorted_index, unsigned int* cstart,
+                          unsigned int* cend, xdem_contact_property* cp, unsigned int np);
+/* cu_plane_contact_force (decl:302-308); plan, dbi (15 doubles) and cp are device pointers,
+ * 3 wall slots per particle; writes the registered body-force buffers. */
+int xdem_cu_plane_contact_force(xdem_plane_info* plan, double* dbi, xdem_contact_property* cp, double* pos,
+                                double* ep, double* vel, double* omega, double* force, double* moment,
+                                double* mass, double* tmax, double* rres, unsigned int* pair_count,
+                                unsigned int* pair_id, double* tsd, unsigned int np);
+/* cu_particle_polygonObject_collision (decl:317-324); 5 triangle slots per particle. */
+int xdem_cu_particle_polygonObject_collision(xdem_triangle_info* dpi, double* dbi, double* pos, double* ep,
+                                             double* vel, double* omega, double* force, double* moment,
+                                             double* mass, double* tmax, double* rres, unsigned int* pair_count,
+                                             unsigned int* pair_id, double* tsd, double* dsph,
+                                             unsigned int* sidx, unsigned int* cstart, unsigned int* cend,
+                                             xdem_contact_property* cp, unsigned int np, unsigned int ntriangle);
+/* cu_decide_rolling_friction_moment (decl:336-343) */
+int xdem_cu_decide_rolling_friction_moment(double* tmax, double* rres, double* inertia, double* ep, double* ev,
+                                           double* moment, unsigned int np);
+/* reduction (decl:345): FP64 sum of np device doubles, written to *out (host). */
+int xdem_reduction(double* in, unsigned int np, double* out);
+/* cu_update_meshObjectData (decl:346-348) */
+int xdem_cu_update_meshObjectData(double* vList, double* sph, double* dlocal, xdem_triangle_info* poly,
+                                  double* dbi, unsigned int ntriangle);
+
+/* ========================================================================== step layer */
+
+typedef struct xdem_ctx xdem_ctx;
+
+#define XDEM_PRECISION_FP64  0  /* everything in FP64 */
+#define XDEM_PRECISION_MIXED 1  /* contact-force arithmetic in FP32; positions, Verlet state,
+                                   tangential history and all force sums in FP64 */
+
+typedef struct xdem_ctx_config {
+    int precision;               /* XDEM_PRECISION_* */
+    unsigned int p2p_capacity;   /* sphere-contact history slots per particle (0 = 12) */
+    xdem_quirks quirks;
+    unsigned int block_size;     /* threads per block of the contact kernel (0 = tuned default) */
+    unsigned int grid_dim_max;   /* largest bucket count per axis (power of two, 0 = 512) */
+    int device;                  /* HIP device ordinal */
+} xdem_ctx_config;
+
+/* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */
+#define XDEM_FIELD_POS     0  /* double4 x,y,z,r */
+#define XDEM_FIELD_VEL     1  /* double3 */
+#define XDEM_FIELD_ACC     2  /* double3 */
+#define XDEM_FIELD_EP      3  /* double4 */
+#define XDEM_FIELD_EV      4  /* double4 */
+#define XDEM_FIELD_EA      5  /* double4 */
+#define XDEM_FIELD_FORCE   6  /* double3, valid after xdem_ctx_collide */
+#define XDEM_FIELD_MOMENT  7  /* double3, valid after xdem_ctx_collide */
+#define XDEM_FIELD_MASS    8  /* double */
+#define XDEM_FIELD_INERTIA 9  /* double3 */
+#define XDEM_FIELD_TMAX    10 /* double3 */
+#define XDEM_FIELD_RRES    11 /* double */
+
+/* History kinds for xdem_ctx_read_history. */
+#define XDEM_HISTORY_P2P   0
+#define XDEM_HISTORY_PLANE 1
+
+typedef struct xdem_ctx_info {
+    unsigned int nscene, np, nsorted, nplane_total;
+    unsigned int nbucket;        /* buckets of all scene grids */
+    unsigned int key_bits;       /* radix-sort key width */
+    unsigned int p2p_capacity;
+    unsigned long long device_bytes;
+    unsigned long long steps_done;
+} xdem_ctx_info;
+
+int xdem_ctx_create(const xdem_ctx_config* cfg, xdem_ctx** out);
+int xdem_ctx_destroy(xdem_ctx* ctx);
+/* Adds a scene; host arrays are copied. Scenes are independent (own walls, materials,
+ * grid, time step). Must precede xdem_ctx_commit. */
+int xdem_ctx_add_scene(xdem_ctx* ctx, const xdem_scene* scene, unsigned int* scene_index);
+/* Allocates all device memory, uploads the scenes and sets the initial state of
+ * xDiscreteElementMethodSimulation::Initialize (velocity as given, acc = (0, g.y, 0),
+ * ep = (1,0,0,0), ev = ea = 0; xDiscreteElementMethodSimulation.cpp:67-80).
+ * No step call allocates afterwards. */
+int xdem_ctx_commit(xdem_ctx* ctx);
+/* Advances every scene by nsteps full steps. */
+int xdem_ctx_step(xdem_ctx* ctx, unsigned int nsteps);
+/* One step in two halves: collide = position update, grid, all contacts (force, moment,
+ * tmax, rres left in their arrays); finish = rolling resistance + velocity update. */
+int xdem_ctx_collide(xdem_ctx* ctx);
+int xdem_ctx_finish(xdem_ctx* ctx);
+int xdem_ctx_synchronize(xdem_ctx* ctx);
+/* Copies a field of one scene (its particles in input order) to/from host memory. */
+int xdem_ctx_read(xdem_ctx* ctx, int field, unsigned int scene, double* host_out);
+int xdem_ctx_write(xdem_ctx* ctx, int field, unsigned int scene, const double* host_in);
+/* Device pointer of a field over all scenes (scenes concatenated in order). */
+int xdem_ctx_device_ptr(xdem_ctx* ctx, int field, void** out);
+/* History of one scene in the reference layout: count[np], ids[np*cap], tsd[2*np*cap]
+ * (cap = p2p_capacity for XDEM_HISTORY_P2P, 3 for XDEM_HISTORY_PLANE). Sphere ids are
+ * scene-local particle indices. */
+int xdem_ctx_read_history(xdem_ctx* ctx, int kind, unsigned int scene, unsigned int* count, unsigned int* ids,
+                          double* tsd);
+/* Device error word (XDEM_ERR_* bits) accumulated since the last clear. */
+int xdem_ctx_error_word(xdem_ctx* ctx, unsigned int* out, int clear);
+/* Body wrench of every wall of a scene from the last step: out[6*k .. 6*k+5] =
+ * (Fx, Fy, Fz, Mx, My, Mz) exerted by the spheres on wall k (reference: six thrust
+ * reductions per body, xParticlePlaneContact.cpp:159-168). */
+int xdem_ctx_body_wrench(xdem_ctx* ctx, unsigned int scene, double* out);
+/* New pose of wall k of a scene, 15 doubles in the MM..ED3 layout; the wall geometry is
+ * moved with it (xParticlePlaneContact::update, xParticlePlaneContact.cpp:96-131). */
+int xdem_ctx_set_body_pose(xdem_ctx* ctx, unsigned int scene, unsigned int wall, const double dbi[15]);
+int xdem_ctx_info_get(xdem_ctx* ctx, xdem_ctx_info* out);
+/* Number of stored sphere-sphere and sphere-wall contacts of all scenes at the last step. */
+int xdem_ctx_contact_count(xdem_ctx* ctx, unsigned long long* n_p2p, unsigned long long* n_wall);
+/* hipStream_t the context works on. */
+int xdem_ctx_stream(xdem_ctx* ctx, void** out);
+/* Records nsteps full steps into a HIP graph (replaces any previous one). */
+int xdem_ctx_graph_capture(xdem_ctx* ctx, unsigned int nsteps);
+/* Launches the captured graph `nlaunch` times. */
+int xdem_ctx_graph_launch(xdem_ctx* ctx, unsigned int nlaunch);
+/* Per-kernel time of the last xdem_ctx_step call, measured with HIP events on the
+ * context's stream when timing is on: out[0] = contact kernel ms (sum over steps),
+ * out[1] = grid (position/hash/sort/cells) ms, out[2] = number of steps timed. */
+int xdem_ctx_set_timing(xdem_ctx* ctx, int on);
+int xdem_ctx_timing(xdem_ctx* ctx, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

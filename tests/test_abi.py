@@ -1,0 +1,63 @@
+"""The C-ABI libraries load and export every entry point the public headers declare
+(no device work: these run without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+import xdynamics_dll_amd as xd
+
+HEADERS = ["xdem_capi.h", "xdem_host.h"]
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|unsigned long long)\s+\**(xdem_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_headers_declare_entry_points():
+    names = declared_functions("xdem_capi.h")
+    for expected in ("xdem_cu_calculate_p2p", "xdem_cu_plane_contact_force", "xdem_ctx_step", "xdem_reduction",
+                     "xdem_setDEMSymbolicParameter", "xdem_cu_reorderDataAndFindCellStart"):
+        assert expected in names
+
+
+@pytest.mark.parametrize("header", HEADERS)
+def test_gpu_library_exports_every_declared_symbol(header):
+    lib = ctypes.CDLL(xd._lib.LIB_PATH)
+    missing = [n for n in declared_functions(header) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_host_library_exports_host_header():
+    lib = ctypes.CDLL(xd._lib.HOST_LIB_PATH)
+    missing = [n for n in declared_functions("xdem_host.h") if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_the_header():
+    declared = set(declared_functions("xdem_capi.h"))
+    bound = set(xd._lib.LAUNCHER_SIGNATURES) | set(xd._lib.CTX_SIGNATURES)
+    assert declared == bound, (declared ^ bound)
+
+
+def test_struct_layouts_match_reference_sizes():
+    # device_dem_parameters, device_contact_property, device_plane_info, device_triangle_info
+    assert ctypes.sizeof(xd._lib.DemParameters) == 144
+    assert ctypes.sizeof(xd._lib.ContactProperty) == 104
+    assert ctypes.sizeof(xd._lib.PlaneInfo) == 240
+    assert ctypes.sizeof(xd._lib.TriangleInfo) == 80
+    assert xd._lib.DemParameters.dt.offset == 64
+    assert xd._lib.DemParameters.grid_size.offset == 52
+    assert xd._lib.PlaneInfo.l1.offset == 8
+    assert xd._lib.PlaneInfo.w4.offset == 216
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    monkeypatch.setattr(xd._lib, "_lib", None)
+    monkeypatch.setattr(xd._lib, "LIB_PATH", "/nonexistent/libxdem.so")
+    with pytest.raises(ImportError):
+        xd._lib.lib()

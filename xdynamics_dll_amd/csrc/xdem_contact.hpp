@@ -1,0 +1,281 @@
+// xdem_contact.hpp -- one-contact evaluations of the DEM step, shared by the launcher
+// kernels (reference array layouts) and the fused step kernels.
+//
+// FP64 instantiations keep the reference's operation order (impl:613-733 for spheres,
+// impl:950-1080 for walls, impl:1748-1835 for triangles). FP32 instantiations (the mixed
+// path) evaluate the force law and the relative kinematics in float, while the pair
+// test, positions, tangential history and all sums stay in double.
+#pragma once
+
+#include <type_traits>
+
+#include "xdem_dev.hpp"
+
+namespace xdem {
+
+// Material constants of one contact object, computed once (host or device).
+__host__ __device__ inline ContactConst make_contact_const(const xdem_contact_property& p, double reach_radius)
+{
+    const double pi = 3.14159265358979323846;
+    ContactConst c;
+    const double iE = p.Ei, jE = p.Ej, ip = p.pri, jp = p.prj;
+    const double Eeq = (iE * jE) / (iE * (1 - jp * jp) + jE * (1 - ip * ip));
+    double Seq = (2.0 * (2.0 - ip) * (1.0 + ip) / iE) + (2.0 * (2.0 - jp) * (1.0 + jp) / jE);
+    Seq = 1.0 / Seq;
+    const double lne = log(p.rest);
+    const double beta = -lne * sqrt(1.0 / (lne * lne + pi * pi));
+    c.k43E = (4.0 / 3.0) * Eeq;
+    c.beta = beta;
+    c.ks_hm = 8.0 * Seq;
+    c.vs_hm = 2.0 * sqrt(5.0 / 6.0) * beta;
+    c.vs_dhs = beta * p.sratio;
+    c.sratio = p.sratio;
+    c.fric = p.fric;
+    c.rfric = p.rfric;
+    c.coh = p.coh;
+    const double Eeq_c = 1.0 / (((1.0 - p.pri * p.pri) / p.Ei) + ((1.0 - p.prj * p.prj) / p.Ej));
+    c.c3cp = 3.0 * p.coh * pi;
+    c.E4 = 4.0 * Eeq_c;
+    c.c8 = 8.0 * pi * p.coh * Eeq_c;
+    c.pcc = pi * pi * p.coh * p.coh;
+    c.ee = Eeq_c * Eeq_c;
+    c.jk = -(3.0 / 2.0) * pi * p.coh;
+    c.coh_reach = 0.0;
+    if (p.coh && reach_radius > 0.0) {
+        // |depth| = 0.75 (pcc R*/ee)^(1/3) grows with R*; bound it at the largest R* of
+        // the contact object and widen by a relative margin for the rounding of pow.
+        c.coh_reach = 0.75 * cbrt((c.pcc * reach_radius) / c.ee) * (1.0 + 1e-6);
+    }
+    return c;
+}
+
+// Accumulators of one sphere over all its contacts (always FP64).
+struct Accum {
+    V3<double> F, M, T;  // force, moment, rolling torque sum (tmax)
+    double R;            // rolling resistance magnitude (rres)
+};
+
+// -------------------------------------------------------------------- sphere-sphere
+// Relative geometry of a candidate pair: rp = pj - pi, dist = |rp|, cdist = ri + rj - dist.
+// The predicate cdist > 0 is evaluated in FP64 without contraction (bit-exact pair sets).
+struct PairGeom {
+    V3<double> rp;
+    double dist, cdist;
+};
+
+__device__ __forceinline__ PairGeom pair_geom(V4<double> pi_, V4<double> pj)
+{
+#pragma clang fp contract(off)
+    PairGeom g;
+    g.rp = mk3(pj.x - pi_.x, pj.y - pi_.y, pj.z - pi_.z);
+    g.dist = sqrt(g.rp.x * g.rp.x + g.rp.y * g.rp.y + g.rp.z * g.rp.z);
+    g.cdist = (pi_.w + pj.w) - g.dist;
+    return g;
+}
+
+// Contact of sphere i with sphere j (cdist > 0), impl:679-706. Updates the i-side sums
+// and the pair's history (sd0, sd1). Ft carries over between calls (quirk Q4) unless
+// the caller resets it.
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ void sphere_contact(const ContactConst& cc, double dt, V4<double> ipos, V3<T> ivel, V3<T> iom,
+                                               double im, V4<double> jpos, V3<T> jvel, V3<T> jom, double jm,
+                                               const PairGeom& g, double& sd0, double& sd1, V3<T>& Ft, Accum& acc)
+{
+    const double ir = ipos.w, jr = jpos.w;
+    const PairConst<T> c = pair_const<T, MODEL>(cc, ir, jr, im, jm);
+    V3<T> unit, dcpr, dcpr_j;
+    if constexpr (std::is_same<T, double>::value) {
+        unit = g.rp / g.dist;
+        const V3<double> ip3 = mk3(ipos.x, ipos.y, ipos.z);
+        const V3<double> cpt = ip3 + ir * unit;
+        dcpr = cpt - ip3;
+        dcpr_j = cpt - mk3(jpos.x, jpos.y, jpos.z);
+    } else {
+        const V3<float> rpf = cvt3<float>(g.rp);
+        unit = rpf / (float)g.dist;
+        dcpr = (float)ir * unit;
+        dcpr_j = dcpr - rpf;
+    }
+    const V3<T> rv = (jvel + cross(jom, dcpr_j)) - (ivel + cross(iom, dcpr));
+    V3<T> Fn;
+    force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)g.cdist, dt, sd0, sd1, rv, unit, Ft, Fn);
+    const V3<T> fnt = Fn + Ft;
+    const V3<double> fd = cvt3<double>(fnt);
+    acc.T = acc.T + cvt3<double>(cross(dcpr, fnt));
+    acc.R += (double)(c.eq_r * T(cc.rfric) * length(Fn));
+    acc.F = acc.F + fd;
+    acc.M = acc.M + cvt3<double>(cross(dcpr, fnt));
+}
+
+// Cohesion-only attraction of a separated pair (impl:708-718).
+template <class T>
+__device__ __forceinline__ void sphere_cohesion(const ContactConst& cc, double ir, double jr, const PairGeom& g, Accum& acc)
+{
+    const double Req = jr ? (ir * jr) / (ir + jr) : ir;
+    const T f = T(cc.jk) * T(Req);
+    const T cf = cohesion_force<T>(cc, T(Req), f);
+    if constexpr (std::is_same<T, double>::value) {
+        const V3<double> unit = g.rp / g.dist;
+        acc.F = acc.F - cf * unit;
+    } else {
+        const V3<float> unit = cvt3<float>(g.rp) / (float)g.dist;
+        acc.F = acc.F - cvt3<double>(cf * unit);
+    }
+}
+
+// -------------------------------------------------------------------------- walls
+// One sphere against one wall (impl:995-1066). `slots` is the sphere's wall history
+// (ids and (ds, s_dot)) in registers; the wall's id selects reset-or-append (quirk Q8).
+struct WallSlots {
+    uint32_t id[XDEM_MAX_P2PL_COUNT];
+    double s0[XDEM_MAX_P2PL_COUNT], s1[XDEM_MAX_P2PL_COUNT];
+    uint32_t count;
+};
+
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_plane_info& pl, const double* dbi, double dt,
+                                             V4<double> ipos, V3<T> ivel, V3<T> iom, double m, WallSlots& ws,
+                                             Accum& acc, double bf[6])
+{
+    const V3<double> ip3 = mk3(ipos.x, ipos.y, ipos.z);
+    const double r = ipos.w;
+    const uint32_t k = pl.id;
+    uint32_t n_new = k ? ws.count : 0u;
+    if (n_new > XDEM_MAX_P2PL_COUNT) n_new = XDEM_MAX_P2PL_COUNT;
+    const V3<double> dp = ip3 - mk3(pl.xw.x, pl.xw.y, pl.xw.z);
+    V3<double> unit = mk3(0.0, 0.0, 0.0);
+    const V3<double> wp = mk3(dot(dp, mk3(pl.u1.x, pl.u1.y, pl.u1.z)), dot(dp, mk3(pl.u2.x, pl.u2.y, pl.u2.z)),
+                              dot(dp, mk3(pl.uw.x, pl.uw.y, pl.uw.z)));
+    const double cdist = plane_detection(pl, ip3, wp, unit, r);
+    double coh_s = 0;
+    if (cc.coh) coh_s = cohesion_limit_depth(cc, r);
+    double sd0 = 0.0, sd1 = 0.0;
+    const V3<double> cpt = ip3 + r * unit;
+    const V3<double> bpos = mk3(dbi[XDEM_PX], dbi[XDEM_PY], dbi[XDEM_PZ]);
+    const V3<double> dcpr_j = cpt - bpos;
+    V3<double> sumF = mk3(0.0, 0.0, 0.0), sumM = mk3(0.0, 0.0, 0.0), tma = mk3(0.0, 0.0, 0.0);
+    double res = 0.0;
+    bool stored = false, overflow = false;
+    if (cdist > 0) {
+        for (uint32_t s = 0; s < XDEM_MAX_P2PL_COUNT; s++)
+            if (ws.id[s] == k) { sd0 = ws.s0[s]; sd1 = ws.s1[s]; break; }
+        const V3<double> dcpr = cpt - ip3;
+        V4<double> be = {dbi[XDEM_E0], dbi[XDEM_E1], dbi[XDEM_E2], dbi[XDEM_E3]};
+        V4<double> bd = {dbi[XDEM_ED0], dbi[XDEM_ED1], dbi[XDEM_ED2], dbi[XDEM_ED3]};
+        const V3<double> oj = omega_of(be, bd);
+        const V3<double> bvel = mk3(dbi[XDEM_VX], dbi[XDEM_VY], dbi[XDEM_VZ]);
+        const PairConst<T> c = pair_const<T, MODEL>(cc, r, 0.0, m, 0.0);
+        V3<T> Fn, Ft = mk3(T(0), T(0), T(0));
+        V3<T> dv;
+        if constexpr (std::is_same<T, double>::value)
+            dv = (bvel + cross(oj, dcpr_j)) - (ivel + cross(iom, dcpr));
+        else
+            dv = cvt3<float>(bvel + cross(oj, dcpr_j)) - (ivel + cross(iom, cvt3<float>(dcpr)));
+        force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)cdist, dt, sd0, sd1, dv, cvt3<T>(unit), Ft, Fn);
+        const V3<double> mf = cvt3<double>(Fn + Ft);
+        tma = tma + cross(dcpr, mf);
+        res += (double)(c.eq_r * T(cc.rfric) * length(Fn));
+        sumF = sumF + mf;
+        sumM = sumM + cross(dcpr, mf);
+        bf[0] += -mf.x; bf[1] += -mf.y; bf[2] += -mf.z;
+        const V3<double> bm = -cross(dcpr_j, mf);
+        bf[3] += bm.x; bf[4] += bm.y; bf[5] += bm.z;
+        stored = true;
+    } else if (cdist <= 0 && fabs(cdist) < fabs(coh_s)) {
+        const T f = T(cc.jk) * T(r);
+        const T cf = cohesion_force<T>(cc, T(r), f);
+        const V3<double> mf = (double)(-cf) * unit;
+        sumF = sumF + mf;
+        bf[0] += -mf.x; bf[1] += -mf.y; bf[2] += -mf.z;
+        const V3<double> bm = -cross(dcpr_j, mf);
+        bf[3] += bm.x; bf[4] += bm.y; bf[5] += bm.z;
+        stored = true;
+    }
+    if (stored) {
+        if (n_new < XDEM_MAX_P2PL_COUNT) {
+            ws.id[n_new] = k; ws.s0[n_new] = sd0; ws.s1[n_new] = sd1;
+            n_new++;
+        } else {
+            overflow = true;
+        }
+    }
+    ws.count = n_new;
+    acc.F = acc.F + sumF;
+    acc.M = acc.M + sumM;
+    acc.T = acc.T + tma;
+    acc.R += res;
+    return overflow;
+}
+
+// ---------------------------------------------------------------------- triangles
+struct TriHit {
+    uint32_t id;
+    V3<double> cpt;
+};
+
+// One sphere against one triangle contact point (impl:1748-1835). Returns true when a
+// history entry (id, sd0, sd1) is to be stored.
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ bool triangle_contact(const ContactConst& cc, const double* dbi, double dt, TriHit h,
+                                                 V3<double> ipos, double r, double m, V3<T> ivel, V3<T> iom,
+                                                 const uint32_t* old_id, const double* old_s0, const double* old_s1,
+                                                 double& sd0, double& sd1, Accum& acc, double bf[6])
+{
+    const V3<double> rp = h.cpt - ipos;
+    sd0 = 0.0; sd1 = 0.0;
+    const V3<double> po2cp = h.cpt - mk3(dbi[XDEM_PX], dbi[XDEM_PY], dbi[XDEM_PZ]);
+    const double dist = length(rp);
+    const V3<double> unit = rp / dist;
+    const double cdist = r - dist;
+    double coh_s = 0;
+    if (cc.coh) coh_s = cohesion_limit_depth(cc, r);
+    if (cdist <= 0 && fabs(cdist) < fabs(coh_s)) {
+        const T f = T(cc.jk) * T(r);
+        const T cf = cohesion_force<T>(cc, T(r), f);
+        acc.F = acc.F - (double)cf * unit;
+        return true;
+    }
+    V4<double> be = {dbi[XDEM_E0], dbi[XDEM_E1], dbi[XDEM_E2], dbi[XDEM_E3]};
+    V4<double> bd = {dbi[XDEM_ED0], dbi[XDEM_ED1], dbi[XDEM_ED2], dbi[XDEM_ED3]};
+    const V3<double> jom = omega_of(be, bd);
+    for (uint32_t s = 0; s < XDEM_MAX_P2MS_COUNT; s++)
+        if (old_id[s] == h.id) { sd0 = old_s0[s]; sd1 = old_s1[s]; break; }
+    const V3<double> rc = h.cpt - ipos;
+    const V3<double> bvel = mk3(dbi[XDEM_VX], dbi[XDEM_VY], dbi[XDEM_VZ]);
+    V3<T> dv;
+    if constexpr (std::is_same<T, double>::value)
+        dv = (bvel + cross(jom, po2cp)) - (ivel + cross(iom, rc));
+    else
+        dv = cvt3<float>(bvel + cross(jom, po2cp)) - (ivel + cross(iom, cvt3<float>(rc)));
+    const PairConst<T> c = pair_const<T, MODEL>(cc, r, 0.0, m, dbi[XDEM_MM]);
+    V3<T> Fn, Ft = mk3(T(0), T(0), T(0));
+    force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)cdist, dt, sd0, sd1, dv, cvt3<T>(unit), Ft, Fn);
+    const V3<double> mf = cvt3<double>(Fn + Ft);
+    acc.T = acc.T + cross(rc, mf);
+    acc.R += (double)(c.eq_r * T(cc.rfric) * length(Fn));
+    acc.F = acc.F + mf;
+    acc.M = acc.M + cross(rc, mf);
+    bf[0] += -mf.x; bf[1] += -mf.y; bf[2] += -mf.z;
+    const V3<double> mm = -cross(po2cp, mf);
+    bf[3] += mm.x; bf[4] += mm.y; bf[5] += mm.z;
+    return true;
+}
+
+// ---------------------------------------------------------------- rolling resistance
+// impl:2420-2454 on the accumulated sums; returns true when tmax/rres are consumed
+// (reset), false when rres == 0 and they are left as they are.
+__device__ __forceinline__ bool rolling_moment(double dt, double J, V3<double> omega, V3<double> Tm, double Tr, V3<double>& M)
+{
+    if (!Tr) return false;
+    const double Jdt = J * dt;
+    const V3<double> T = mk3(Jdt * omega.x - Tm.x, Jdt * omega.y - Tm.y, Jdt * omega.z - Tm.z);
+    const double lT = length(T);
+    if (lT && Tr) {
+        V3<double> Trv = Tr * (T / lT);
+        if (length(Trv) >= lT) Trv = T;
+        M = M + Trv;
+    }
+    return true;
+}
+
+}  // namespace xdem

@@ -1,0 +1,752 @@
+// xdem_ctx.hip -- the step layer of the C ABI: device state of one or many scenes and
+// the fused step (integrate+hash, radix sort, bucket ranges + sorted copies, contacts +
+// Verlet), optionally recorded in a HIP graph.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "xdem_capi.h"
+#include "xdem_sort.hpp"
+#include "xdem_step.hpp"
+
+using namespace xdem;
+
+namespace {
+
+struct SceneHost {
+    xdem_scene s;               // deep copy of the caller's arrays
+    std::vector<double> pos, vel, mass, inertia, dbi;
+    std::vector<xdem_plane_info> planes;
+    std::vector<xdem_contact_property> plane_cp;
+    uint32_t p_begin = 0;
+    uint32_t wall_begin = 0;
+    SceneDev dev;
+};
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_zero_counters(unsigned long long* c)
+{
+    if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
+}
+
+// Sum of wrench[slot][c][p_begin .. p_end) for every (slot, c) = one block each, fixed order.
+__global__ __launch_bounds__(256) void k_wrench_reduce(const double* wrench, uint32_t np, const uint32_t* range,
+                                                      double* out)
+{
+    __shared__ double s[4];
+    const uint32_t sc = blockIdx.x;       // slot * 6 + c
+    const uint32_t slot = sc / 6;
+    const double* src = wrench + (size_t)sc * np;
+    const uint32_t b = range[2 * slot], e = range[2 * slot + 1];
+    double v = 0.0;
+    for (uint32_t i = b + threadIdx.x; i < e; i += 256) v += src[i];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[sc] = ((s[0] + s[1]) + s[2]) + s[3];
+}
+
+uint32_t next_pow2(uint32_t v)
+{
+    uint32_t p = 1;
+    while (p < v && p < 0x80000000u) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+struct xdem_ctx {
+    xdem_ctx_config cfg;
+    std::vector<SceneHost> scenes;
+    std::vector<WallDev> walls;
+    std::vector<uint32_t> wslot_range;  // [2*slot] sphere range of each dynamic wall's scene
+    std::vector<int> wall_slot;         // global wall -> slot or -1
+    bool committed = false;
+    bool multi = false;
+    int model = -1;
+    uint32_t np = 0, nbucket = 0, key_bits = 1, cap = XDEM_MAX_P2P_COUNT, nwslot = 0;
+    int tmax_persist = 0;
+    hipStream_t stream = nullptr;
+    StepLaunch L = {nullptr, nullptr};
+
+    double *pos = nullptr, *vel = nullptr, *acc = nullptr, *ep = nullptr, *ev = nullptr, *ea = nullptr;
+    double *force = nullptr, *moment = nullptr, *mass = nullptr, *inertia = nullptr, *tmax = nullptr, *rres = nullptr;
+    uint32_t* scene_of = nullptr;
+    SceneDev* d_scenes = nullptr;
+    WallDev* d_walls = nullptr;
+    uint32_t *keys = nullptr, *vals = nullptr, *skeys = nullptr, *svals = nullptr, *ktmp = nullptr, *vtmp = nullptr;
+    void* sort_scratch = nullptr;
+    uint2* cell_range = nullptr;
+    double4* spos = nullptr;
+    void* svm = nullptr;
+    void* som = nullptr;
+    uint32_t* pp_count[2] = {nullptr, nullptr};
+    uint32_t* pp_id[2] = {nullptr, nullptr};
+    double2* pp_tsd[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint32_t* pl_count = nullptr;
+    uint32_t* pl_id = nullptr;
+    double2* pl_tsd = nullptr;
+    double* wrench_pp = nullptr;
+    double* wrench_out = nullptr;
+    uint32_t* d_wslot_range = nullptr;
+    uint32_t* err = nullptr;
+    unsigned long long* counters = nullptr;
+    std::vector<void*> allocs;
+    size_t bytes = 0;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    unsigned graph_steps = 0;
+
+    bool timing = false;
+    std::vector<hipEvent_t> events;  // 3 per timed step
+    double t_contact = 0.0, t_grid = 0.0;
+    unsigned t_steps = 0;
+    unsigned long long steps_done = 0;
+
+    template <class P> hipError_t alloc(P** p, size_t bytes_)
+    {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes_ ? bytes_ : 16);
+        if (e != hipSuccess) return e;
+        allocs.push_back(q);
+        bytes += bytes_;
+        *p = static_cast<P*>(q);
+        return hipSuccess;
+    }
+
+    StepArgs args() const
+    {
+        StepArgs A;
+        std::memset(&A, 0, sizeof(A));
+        A.pos = pos; A.vel = vel; A.acc = acc; A.ep = ep; A.ev = ev; A.ea = ea;
+        A.force = force; A.moment = moment; A.mass = mass; A.inertia = inertia; A.tmax = tmax; A.rres = rres;
+        A.scene_of = scene_of; A.scenes = d_scenes; A.walls = d_walls;
+        A.keys = keys; A.vals = vals; A.skeys = skeys; A.svals = svals; A.cell_range = cell_range;
+        A.spos = spos; A.svm = svm; A.som = som;
+        A.pp_count_in = pp_count[cur]; A.pp_id_in = pp_id[cur]; A.pp_tsd_in = pp_tsd[cur];
+        A.pp_count_out = pp_count[cur ^ 1]; A.pp_id_out = pp_id[cur ^ 1]; A.pp_tsd_out = pp_tsd[cur ^ 1];
+        A.pl_count = pl_count; A.pl_id = pl_id; A.pl_tsd = pl_tsd;
+        A.wrench = wrench_pp; A.err = err; A.counters = counters;
+        A.np = np; A.nsorted = np; A.cap = cap; A.nwslot = nwslot;
+        A.stale_ft = cfg.quirks.stale_ft; A.flat_inertia = cfg.quirks.flat_inertia;
+        A.keep_velocity = cfg.quirks.velocity_not_updated; A.tmax_persist = tmax_persist;
+        return A;
+    }
+
+    // First half of a step: integrate + key, sort, cells + gather, contacts.
+    hipError_t collide_half(bool fuse, hipEvent_t* ev3)
+    {
+        hipError_t e;
+        const StepArgs A = args();
+        if (ev3 && (e = hipEventRecord(ev3[0], stream)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, counters);
+        if ((e = launch_integrate_hash(A, multi, stream)) != hipSuccess) return e;
+        if ((e = radix_sort_pairs(keys, vals, skeys, svals, ktmp, vtmp, np, (int)key_bits, sort_scratch, stream)) !=
+            hipSuccess)
+            return e;
+        if ((e = L.gather(A, stream)) != hipSuccess) return e;
+        if (ev3 && (e = hipEventRecord(ev3[1], stream)) != hipSuccess) return e;
+        if ((e = L.collide(A, multi, fuse, model, cfg.quirks.gpu_force_law != 0, stream)) != hipSuccess) return e;
+        if (ev3 && (e = hipEventRecord(ev3[2], stream)) != hipSuccess) return e;
+        if (nwslot) {
+            hipLaunchKernelGGL(k_wrench_reduce, dim3(nwslot * 6), dim3(256), 0, stream, wrench_pp, np, d_wslot_range,
+                               wrench_out);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        cur ^= 1;
+        return hipSuccess;
+    }
+};
+
+static int hip_status(hipError_t e) { return e == hipSuccess ? XDEM_OK : XDEM_E_HIP; }
+
+#define CTX_CHECK(expr)                                  \
+    do {                                                 \
+        hipError_t e_ = (expr);                          \
+        if (e_ != hipSuccess) return XDEM_E_HIP;         \
+    } while (0)
+
+extern "C" {
+
+int xdem_ctx_create(const xdem_ctx_config* cfg, xdem_ctx** out)
+{
+    if (!out) return XDEM_E_INVALID;
+    *out = nullptr;
+    xdem_ctx* c = new (std::nothrow) xdem_ctx();
+    if (!c) return XDEM_E_INVALID;
+    if (cfg) c->cfg = *cfg;
+    else {
+        std::memset(&c->cfg, 0, sizeof(c->cfg));
+        c->cfg.quirks.stale_ft = 1;
+        c->cfg.quirks.flat_inertia = 1;
+    }
+    if (c->cfg.precision != XDEM_PRECISION_FP64 && c->cfg.precision != XDEM_PRECISION_MIXED) { delete c; return XDEM_E_INVALID; }
+    c->cap = c->cfg.p2p_capacity ? c->cfg.p2p_capacity : XDEM_MAX_P2P_COUNT;
+    if (c->cap > 64) { delete c; return XDEM_E_INVALID; }
+    if (!c->cfg.grid_dim_max) c->cfg.grid_dim_max = 512;
+    if (hipSetDevice(c->cfg.device) != hipSuccess) { delete c; return XDEM_E_HIP; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return XDEM_E_HIP; }
+    *out = c;
+    return XDEM_OK;
+}
+
+int xdem_ctx_destroy(xdem_ctx* c)
+{
+    if (!c) return XDEM_OK;
+    hipSetDevice(c->cfg.device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    for (hipEvent_t e : c->events) hipEventDestroy(e);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return XDEM_OK;
+}
+
+int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
+{
+    if (!c || !s) return XDEM_E_INVALID;
+    if (c->committed) return XDEM_E_NOT_READY;
+    if (s->np == 0 || !s->pos || !s->mass || !s->inertia) return XDEM_E_INVALID;
+    if (s->nplane > (unsigned)kMaxWallsPerScene) return XDEM_E_INVALID;
+    if (s->nplane && (!s->planes || !s->plane_dbi || !s->plane_cp)) return XDEM_E_INVALID;
+    if (!(s->cell_size > 0.0) || !(s->dt > 0.0)) return XDEM_E_INVALID;
+    if (c->model >= 0 && c->model != s->contact_model) return XDEM_E_INVALID;  // one force model per run
+    if ((unsigned long long)c->np + s->np >= 0xfffffff0ull) return XDEM_E_INVALID;
+    for (unsigned k = 0; k < s->nplane; k++)
+        if (s->planes[k].id != k) return XDEM_E_INVALID;  // wall ids follow the wall order
+    c->model = s->contact_model;
+    SceneHost h;
+    h.s = *s;
+    h.pos.assign(s->pos, s->pos + 4 * (size_t)s->np);
+    h.vel.assign(3 * (size_t)s->np, 0.0);
+    if (s->vel) h.vel.assign(s->vel, s->vel + 3 * (size_t)s->np);
+    h.mass.assign(s->mass, s->mass + s->np);
+    h.inertia.assign(s->inertia, s->inertia + 3 * (size_t)s->np);
+    h.planes.assign(s->planes, s->planes + s->nplane);
+    h.plane_cp.assign(s->plane_cp, s->plane_cp + s->nplane);
+    h.dbi.assign(s->plane_dbi, s->plane_dbi + 15 * (size_t)s->nplane);
+    h.p_begin = c->np;
+    h.wall_begin = (uint32_t)c->walls.size();
+    c->np += s->np;
+    double rmax = 0.0;
+    for (unsigned i = 0; i < s->np; i++) rmax = std::max(rmax, h.pos[4 * i + 3]);
+    // bucket grid: the reference's origin and cell size, power-of-two dims covering the
+    // scene (wrapping beyond them, as the reference's 128^3 grid does)
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (unsigned i = 0; i < s->np; i++)
+        for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], h.pos[4 * i + a]); hi[a] = std::max(hi[a], h.pos[4 * i + a]); }
+    for (unsigned k = 0; k < s->nplane; k++) {
+        const xdem_double3 cs[4] = {s->planes[k].xw, s->planes[k].w2, s->planes[k].w3, s->planes[k].w4};
+        for (const auto& q : cs) {
+            const double v[3] = {q.x, q.y, q.z};
+            for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], v[a]); hi[a] = std::max(hi[a], v[a]); }
+        }
+    }
+    const double wo[3] = {s->world_origin[0], s->world_origin[1], s->world_origin[2]};
+    uint32_t dims[3];
+    for (int a = 0; a < 3; a++) {
+        const double span = std::floor((hi[a] - wo[a]) / s->cell_size) - std::floor((lo[a] - wo[a]) / s->cell_size) + 3.0;
+        uint32_t d = next_pow2((uint32_t)std::min(span, 1048576.0));
+        d = std::max(4u, std::min(d, c->cfg.grid_dim_max));
+        dims[a] = d;
+    }
+    SceneDev& D = h.dev;
+    std::memset(&D, 0, sizeof(D));
+    D.dt = s->dt;
+    D.half2dt = 0.5 * s->dt * s->dt;
+    D.gx = s->gravity[0]; D.gy = s->gravity[1]; D.gz = s->gravity[2];
+    D.grid.wo_x = wo[0]; D.grid.wo_y = wo[1]; D.grid.wo_z = wo[2];
+    D.grid.cs = s->cell_size;
+    D.grid.mask_x = dims[0] - 1; D.grid.mask_y = dims[1] - 1; D.grid.mask_z = dims[2] - 1;
+    D.grid.dim_x = dims[0];
+    D.grid.dim_xy = dims[0] * dims[1];
+    D.grid.base = c->nbucket;
+    const unsigned long long nb = (unsigned long long)dims[0] * dims[1] * dims[2];
+    if ((unsigned long long)c->nbucket + nb >= 0xffffffffull) return XDEM_E_INVALID;
+    c->nbucket += (uint32_t)nb;
+    D.cc = make_contact_const(s->cp_pp, 0.5 * rmax);
+    D.p_begin = h.p_begin;
+    D.p_end = h.p_begin + s->np;
+    D.wall_begin = h.wall_begin;
+    D.nwall = s->nplane;
+    D.has_pp = 1;
+    if (s->cp_pp.rfric == 0.0) c->tmax_persist = 1;
+    for (unsigned k = 0; k < s->nplane; k++) {
+        WallDev W;
+        std::memset(&W, 0, sizeof(W));
+        W.info = s->planes[k];
+        std::memcpy(W.dbi, s->plane_dbi + 15 * k, sizeof(double) * 15);
+        W.cc = make_contact_const(s->plane_cp[k], rmax);
+        if (s->plane_cp[k].rfric == 0.0) c->tmax_persist = 1;
+        W.wslot = -1;
+        if (W.dbi[XDEM_MM] > 0.0) {  // a wall with mass is a dynamic body: reduce its wrench
+            W.wslot = (int32_t)c->nwslot++;
+            c->wslot_range.push_back(D.p_begin);
+            c->wslot_range.push_back(D.p_end);
+        }
+        c->wall_slot.push_back(W.wslot);
+        c->walls.push_back(W);
+    }
+    if (idx) *idx = (unsigned)c->scenes.size();
+    c->scenes.push_back(std::move(h));
+    return XDEM_OK;
+}
+
+int xdem_ctx_commit(xdem_ctx* c)
+{
+    if (!c) return XDEM_E_INVALID;
+    if (c->committed) return XDEM_E_NOT_READY;
+    if (c->scenes.empty()) return XDEM_E_INVALID;
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    c->multi = c->scenes.size() > 1;
+    c->L = c->cfg.precision == XDEM_PRECISION_MIXED ? step_launch_mixed() : step_launch_f64();
+    const size_t n = c->np;
+    c->key_bits = 1;
+    while (c->key_bits < 32 && (1ull << c->key_bits) < (unsigned long long)c->nbucket) c->key_bits++;
+    const size_t vsz = c->cfg.precision == XDEM_PRECISION_MIXED ? sizeof(float4) : sizeof(double4);
+    CTX_CHECK(c->alloc(&c->pos, sizeof(double) * 4 * n));
+    CTX_CHECK(c->alloc(&c->vel, sizeof(double) * 3 * n));
+    CTX_CHECK(c->alloc(&c->acc, sizeof(double) * 3 * n));
+    CTX_CHECK(c->alloc(&c->ep, sizeof(double) * 4 * n));
+    CTX_CHECK(c->alloc(&c->ev, sizeof(double) * 4 * n));
+    CTX_CHECK(c->alloc(&c->ea, sizeof(double) * 4 * n));
+    CTX_CHECK(c->alloc(&c->force, sizeof(double) * 3 * n));
+    CTX_CHECK(c->alloc(&c->moment, sizeof(double) * 3 * n));
+    CTX_CHECK(c->alloc(&c->mass, sizeof(double) * n));
+    CTX_CHECK(c->alloc(&c->inertia, sizeof(double) * 3 * n));
+    CTX_CHECK(c->alloc(&c->tmax, sizeof(double) * 3 * n));
+    CTX_CHECK(c->alloc(&c->rres, sizeof(double) * n));
+    if (c->multi) CTX_CHECK(c->alloc(&c->scene_of, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->d_scenes, sizeof(SceneDev) * c->scenes.size()));
+    CTX_CHECK(c->alloc(&c->d_walls, sizeof(WallDev) * std::max<size_t>(1, c->walls.size())));
+    CTX_CHECK(c->alloc(&c->keys, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->vals, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->skeys, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->svals, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->ktmp, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->vtmp, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->sort_scratch, radix_sort_scratch_bytes((uint32_t)n)));
+    CTX_CHECK(c->alloc(&c->cell_range, sizeof(uint2) * (size_t)c->nbucket));
+    CTX_CHECK(c->alloc(&c->spos, sizeof(double4) * n));
+    CTX_CHECK(c->alloc(&c->svm, vsz * n));
+    CTX_CHECK(c->alloc(&c->som, vsz * n));
+    for (int b = 0; b < 2; b++) {
+        CTX_CHECK(c->alloc(&c->pp_count[b], sizeof(uint32_t) * n));
+        CTX_CHECK(c->alloc(&c->pp_id[b], sizeof(uint32_t) * n * c->cap));
+        CTX_CHECK(c->alloc(&c->pp_tsd[b], sizeof(double2) * n * c->cap));
+    }
+    CTX_CHECK(c->alloc(&c->pl_count, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->pl_id, sizeof(uint32_t) * n * XDEM_MAX_P2PL_COUNT));
+    CTX_CHECK(c->alloc(&c->pl_tsd, sizeof(double2) * n * XDEM_MAX_P2PL_COUNT));
+    if (c->nwslot) {
+        CTX_CHECK(c->alloc(&c->wrench_pp, sizeof(double) * 6 * n * c->nwslot));
+        CTX_CHECK(c->alloc(&c->wrench_out, sizeof(double) * 6 * c->nwslot));
+        CTX_CHECK(c->alloc(&c->d_wslot_range, sizeof(uint32_t) * 2 * c->nwslot));
+    }
+    CTX_CHECK(c->alloc(&c->err, sizeof(uint32_t)));
+    CTX_CHECK(c->alloc(&c->counters, sizeof(unsigned long long) * 2));
+
+    // host staging of the initial state (xDiscreteElementMethodSimulation.cpp:60-83)
+    std::vector<double> pos(4 * n), vel(3 * n), acc(3 * n, 0.0), ep(4 * n, 0.0), mass(n), inertia(3 * n);
+    std::vector<uint32_t> scene_of(c->multi ? n : 0);
+    std::vector<SceneDev> sd;
+    for (size_t si = 0; si < c->scenes.size(); si++) {
+        const SceneHost& h = c->scenes[si];
+        const size_t b = h.p_begin, m = h.s.np;
+        std::copy(h.pos.begin(), h.pos.end(), pos.begin() + 4 * b);
+        std::copy(h.vel.begin(), h.vel.end(), vel.begin() + 3 * b);
+        std::copy(h.mass.begin(), h.mass.end(), mass.begin() + b);
+        std::copy(h.inertia.begin(), h.inertia.end(), inertia.begin() + 3 * b);
+        for (size_t i = 0; i < m; i++) {
+            acc[3 * (b + i) + 1] = h.s.gravity[1];  // acc = (0, g.y, 0) as the reference initialises it
+            ep[4 * (b + i)] = 1.0;
+            if (c->multi) scene_of[b + i] = (uint32_t)si;
+        }
+        sd.push_back(h.dev);
+    }
+    hipStream_t s = c->stream;
+    CTX_CHECK(hipMemcpyAsync(c->pos, pos.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemcpyAsync(c->vel, vel.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemcpyAsync(c->acc, acc.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemcpyAsync(c->ep, ep.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemcpyAsync(c->mass, mass.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemcpyAsync(c->inertia, inertia.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    if (c->multi) CTX_CHECK(hipMemcpyAsync(c->scene_of, scene_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemcpyAsync(c->d_scenes, sd.data(), sizeof(SceneDev) * sd.size(), hipMemcpyHostToDevice, s));
+    if (!c->walls.empty())
+        CTX_CHECK(hipMemcpyAsync(c->d_walls, c->walls.data(), sizeof(WallDev) * c->walls.size(), hipMemcpyHostToDevice, s));
+    if (c->nwslot)
+        CTX_CHECK(hipMemcpyAsync(c->d_wslot_range, c->wslot_range.data(), sizeof(uint32_t) * 2 * c->nwslot,
+                                 hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemsetAsync(c->ev, 0, sizeof(double) * 4 * n, s));
+    CTX_CHECK(hipMemsetAsync(c->ea, 0, sizeof(double) * 4 * n, s));
+    CTX_CHECK(hipMemsetAsync(c->force, 0, sizeof(double) * 3 * n, s));
+    CTX_CHECK(hipMemsetAsync(c->moment, 0, sizeof(double) * 3 * n, s));
+    CTX_CHECK(hipMemsetAsync(c->tmax, 0, sizeof(double) * 3 * n, s));
+    CTX_CHECK(hipMemsetAsync(c->rres, 0, sizeof(double) * n, s));
+    for (int b = 0; b < 2; b++) {
+        CTX_CHECK(hipMemsetAsync(c->pp_count[b], 0, sizeof(uint32_t) * n, s));
+        CTX_CHECK(hipMemsetAsync(c->pp_id[b], 0, sizeof(uint32_t) * n * c->cap, s));
+        CTX_CHECK(hipMemsetAsync(c->pp_tsd[b], 0, sizeof(double2) * n * c->cap, s));
+    }
+    CTX_CHECK(hipMemsetAsync(c->pl_count, 0, sizeof(uint32_t) * n, s));
+    CTX_CHECK(hipMemsetAsync(c->pl_id, 0, sizeof(uint32_t) * n * XDEM_MAX_P2PL_COUNT, s));
+    CTX_CHECK(hipMemsetAsync(c->pl_tsd, 0, sizeof(double2) * n * XDEM_MAX_P2PL_COUNT, s));
+    CTX_CHECK(hipMemsetAsync(c->err, 0, sizeof(uint32_t), s));
+    CTX_CHECK(hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 2, s));
+    if (c->nwslot) CTX_CHECK(hipMemsetAsync(c->wrench_out, 0, sizeof(double) * 6 * c->nwslot, s));
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->keys, kInvalid, n);
+    const size_t nb2 = 2 * (size_t)c->nbucket;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nb2 + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t*>(c->cell_range), kInvalid, nb2);
+    CTX_CHECK(hipGetLastError());
+    CTX_CHECK(hipStreamSynchronize(s));
+    c->committed = true;
+    return XDEM_OK;
+}
+
+static int run_steps(xdem_ctx* c, unsigned nsteps)
+{
+    const bool time_it = c->timing;
+    if (time_it) {
+        const size_t need = 3 * (size_t)std::min(nsteps, 2048u);
+        while (c->events.size() < need) {
+            hipEvent_t e;
+            CTX_CHECK(hipEventCreate(&e));
+            c->events.push_back(e);
+        }
+    }
+    for (unsigned k = 0; k < nsteps; k++) {
+        hipEvent_t* ev3 = (time_it && k < 2048u) ? &c->events[3 * k] : nullptr;
+        CTX_CHECK(c->collide_half(true, ev3));
+        c->steps_done++;
+    }
+    if (time_it) {
+        CTX_CHECK(hipStreamSynchronize(c->stream));
+        c->t_contact = 0.0;
+        c->t_grid = 0.0;
+        c->t_steps = std::min(nsteps, 2048u);
+        for (unsigned k = 0; k < c->t_steps; k++) {
+            float a = 0.f, b = 0.f;
+            CTX_CHECK(hipEventElapsedTime(&a, c->events[3 * k], c->events[3 * k + 1]));
+            CTX_CHECK(hipEventElapsedTime(&b, c->events[3 * k + 1], c->events[3 * k + 2]));
+            c->t_grid += a;
+            c->t_contact += b;
+        }
+    }
+    return XDEM_OK;
+}
+
+int xdem_ctx_step(xdem_ctx* c, unsigned int nsteps)
+{
+    if (!c) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    return run_steps(c, nsteps);
+}
+
+int xdem_ctx_collide(xdem_ctx* c)
+{
+    if (!c) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    CTX_CHECK(c->collide_half(false, nullptr));
+    return XDEM_OK;
+}
+
+int xdem_ctx_finish(xdem_ctx* c)
+{
+    if (!c) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    CTX_CHECK(launch_finish(c->args(), c->multi, c->stream));
+    c->steps_done++;
+    return XDEM_OK;
+}
+
+int xdem_ctx_synchronize(xdem_ctx* c)
+{
+    if (!c) return XDEM_E_INVALID;
+    return hip_status(hipStreamSynchronize(c->stream));
+}
+
+static int field_info(xdem_ctx* c, int field, double** p, int* width)
+{
+    switch (field) {
+    case XDEM_FIELD_POS: *p = c->pos; *width = 4; break;
+    case XDEM_FIELD_VEL: *p = c->vel; *width = 3; break;
+    case XDEM_FIELD_ACC: *p = c->acc; *width = 3; break;
+    case XDEM_FIELD_EP: *p = c->ep; *width = 4; break;
+    case XDEM_FIELD_EV: *p = c->ev; *width = 4; break;
+    case XDEM_FIELD_EA: *p = c->ea; *width = 4; break;
+    case XDEM_FIELD_FORCE: *p = c->force; *width = 3; break;
+    case XDEM_FIELD_MOMENT: *p = c->moment; *width = 3; break;
+    case XDEM_FIELD_MASS: *p = c->mass; *width = 1; break;
+    case XDEM_FIELD_INERTIA: *p = c->inertia; *width = 3; break;
+    case XDEM_FIELD_TMAX: *p = c->tmax; *width = 3; break;
+    case XDEM_FIELD_RRES: *p = c->rres; *width = 1; break;
+    default: return XDEM_E_INVALID;
+    }
+    return XDEM_OK;
+}
+
+int xdem_ctx_read(xdem_ctx* c, int field, unsigned int scene, double* out)
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    double* p;
+    int w;
+    if (field_info(c, field, &p, &w)) return XDEM_E_INVALID;
+    const SceneHost& h = c->scenes[scene];
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    CTX_CHECK(hipMemcpyAsync(out, p + (size_t)w * h.p_begin, sizeof(double) * w * h.s.np, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    return XDEM_OK;
+}
+
+int xdem_ctx_write(xdem_ctx* c, int field, unsigned int scene, const double* in)
+{
+    if (!c || !in) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    double* p;
+    int w;
+    if (field_info(c, field, &p, &w)) return XDEM_E_INVALID;
+    const SceneHost& h = c->scenes[scene];
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    CTX_CHECK(hipMemcpyAsync(p + (size_t)w * h.p_begin, in, sizeof(double) * w * h.s.np, hipMemcpyHostToDevice, c->stream));
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    return XDEM_OK;
+}
+
+int xdem_ctx_device_ptr(xdem_ctx* c, int field, void** out)
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    double* p;
+    int w;
+    if (field_info(c, field, &p, &w)) return XDEM_E_INVALID;
+    *out = p;
+    return XDEM_OK;
+}
+
+int xdem_ctx_read_history(xdem_ctx* c, int kind, unsigned int scene, unsigned int* count, unsigned int* ids, double* tsd)
+{
+    if (!c || !count || !ids || !tsd) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    const SceneHost& h = c->scenes[scene];
+    const size_t b = h.p_begin, m = h.s.np;
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    uint32_t cap;
+    const uint32_t *dc, *di;
+    const double2* dt;
+    if (kind == XDEM_HISTORY_P2P) {
+        cap = c->cap; dc = c->pp_count[c->cur]; di = c->pp_id[c->cur]; dt = c->pp_tsd[c->cur];
+    } else if (kind == XDEM_HISTORY_PLANE) {
+        cap = XDEM_MAX_P2PL_COUNT; dc = c->pl_count; di = c->pl_id; dt = c->pl_tsd;
+    } else {
+        return XDEM_E_INVALID;
+    }
+    CTX_CHECK(hipMemcpyAsync(count, dc + b, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipMemcpyAsync(ids, di + b * cap, sizeof(uint32_t) * m * cap, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipMemcpyAsync(tsd, dt + b * cap, sizeof(double2) * m * cap, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    if (kind == XDEM_HISTORY_P2P)
+        for (size_t i = 0; i < m; i++)
+            for (uint32_t s = 0; s < count[i] && s < cap; s++) ids[i * cap + s] -= (uint32_t)b;
+    return XDEM_OK;
+}
+
+int xdem_ctx_error_word(xdem_ctx* c, unsigned int* out, int clear)
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    if (!c->committed) { *out = 0; return XDEM_OK; }
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    CTX_CHECK(hipMemcpyAsync(out, c->err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    if (clear) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->err, 0u, (size_t)1);
+        CTX_CHECK(hipGetLastError());
+        CTX_CHECK(hipStreamSynchronize(c->stream));
+    }
+    return XDEM_OK;
+}
+
+int xdem_ctx_body_wrench(xdem_ctx* c, unsigned int scene, double* out)
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    const SceneHost& h = c->scenes[scene];
+    std::vector<double> all(6 * (size_t)c->nwslot);
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    if (c->nwslot) {
+        CTX_CHECK(hipMemcpyAsync(all.data(), c->wrench_out, sizeof(double) * 6 * c->nwslot, hipMemcpyDeviceToHost, c->stream));
+        CTX_CHECK(hipStreamSynchronize(c->stream));
+    }
+    for (unsigned k = 0; k < h.s.nplane; k++) {
+        const int slot = c->wall_slot[h.wall_begin + k];
+        for (int q = 0; q < 6; q++) out[6 * k + q] = slot >= 0 ? all[6 * (size_t)slot + q] : 0.0;
+    }
+    return XDEM_OK;
+}
+
+int xdem_ctx_set_body_pose(xdem_ctx* c, unsigned int scene, unsigned int wall, const double dbi[15])
+{
+    if (!c || !dbi) return XDEM_E_INVALID;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    SceneHost& h = c->scenes[scene];
+    if (wall >= h.s.nplane) return XDEM_E_INVALID;
+    WallDev& W = c->walls[h.wall_begin + wall];
+    // xParticlePlaneContact::update: corners = pose position + A(e) * local corner, where the
+    // local corners were fixed at definition relative to the initial pose.
+    const double* old = W.dbi;
+    auto rot = [](const double* e, const double v[3], bool transpose, double o[3]) {
+        const double a00 = 2 * (e[0] * e[0] + e[1] * e[1] - 0.5), a01 = 2 * (e[1] * e[2] - e[0] * e[3]), a02 = 2 * (e[1] * e[3] + e[0] * e[2]);
+        const double a10 = 2 * (e[1] * e[2] + e[0] * e[3]), a11 = 2 * (e[0] * e[0] + e[2] * e[2] - 0.5), a12 = 2 * (e[2] * e[3] - e[0] * e[1]);
+        const double a20 = 2 * (e[1] * e[3] - e[0] * e[2]), a21 = 2 * (e[2] * e[3] + e[0] * e[1]), a22 = 2 * (e[0] * e[0] + e[3] * e[3] - 0.5);
+        if (!transpose) {
+            o[0] = a00 * v[0] + a01 * v[1] + a02 * v[2]; o[1] = a10 * v[0] + a11 * v[1] + a12 * v[2]; o[2] = a20 * v[0] + a21 * v[1] + a22 * v[2];
+        } else {
+            o[0] = a00 * v[0] + a10 * v[1] + a20 * v[2]; o[1] = a01 * v[0] + a11 * v[1] + a21 * v[2]; o[2] = a02 * v[0] + a12 * v[1] + a22 * v[2];
+        }
+    };
+    const xdem_double3 corners[4] = {W.info.xw, W.info.w2, W.info.w3, W.info.w4};
+    double g[4][3];
+    for (int q = 0; q < 4; q++) {
+        const double rel[3] = {corners[q].x - old[XDEM_PX], corners[q].y - old[XDEM_PY], corners[q].z - old[XDEM_PZ]};
+        double loc[3], gl[3];
+        rot(old + XDEM_E0, rel, true, loc);
+        rot(dbi + XDEM_E0, loc, false, gl);
+        for (int a = 0; a < 3; a++) g[q][a] = dbi[XDEM_PX + a] + gl[a];
+    }
+    xdem_plane_info pi = W.info;
+    pi.xw = {g[0][0], g[0][1], g[0][2]};
+    pi.w2 = {g[1][0], g[1][1], g[1][2]};
+    pi.w3 = {g[2][0], g[2][1], g[2][2]};
+    pi.w4 = {g[3][0], g[3][1], g[3][2]};
+    const double pa[3] = {g[1][0] - g[0][0], g[1][1] - g[0][1], g[1][2] - g[0][2]};
+    const double pb[3] = {g[3][0] - g[0][0], g[3][1] - g[0][1], g[3][2] - g[0][2]};
+    pi.pa = {pa[0], pa[1], pa[2]};
+    pi.pb = {pb[0], pb[1], pb[2]};
+    pi.l1 = std::sqrt(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]);
+    pi.l2 = std::sqrt(pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2]);
+    const double u1[3] = {pa[0] / pi.l1, pa[1] / pi.l1, pa[2] / pi.l1};
+    const double u2[3] = {pb[0] / pi.l2, pb[1] / pi.l2, pb[2] / pi.l2};
+    pi.u1 = {u1[0], u1[1], u1[2]};
+    pi.u2 = {u2[0], u2[1], u2[2]};
+    pi.uw = {u1[1] * u2[2] - u1[2] * u2[1], u1[2] * u2[0] - u1[0] * u2[2], u1[0] * u2[1] - u1[1] * u2[0]};
+    W.info = pi;
+    std::memcpy(W.dbi, dbi, sizeof(double) * 15);
+    if (c->committed) {
+        CTX_CHECK(hipSetDevice(c->cfg.device));
+        CTX_CHECK(hipMemcpyAsync(c->d_walls + h.wall_begin + wall, &W, sizeof(WallDev), hipMemcpyHostToDevice, c->stream));
+        CTX_CHECK(hipStreamSynchronize(c->stream));
+    }
+    return XDEM_OK;
+}
+
+int xdem_ctx_info_get(xdem_ctx* c, xdem_ctx_info* out)
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    out->nscene = (unsigned)c->scenes.size();
+    out->np = c->np;
+    out->nsorted = c->np;
+    out->nplane_total = (unsigned)c->walls.size();
+    out->nbucket = c->nbucket;
+    out->key_bits = c->key_bits;
+    out->p2p_capacity = c->cap;
+    out->device_bytes = c->bytes;
+    out->steps_done = c->steps_done;
+    return XDEM_OK;
+}
+
+int xdem_ctx_contact_count(xdem_ctx* c, unsigned long long* n_p2p, unsigned long long* n_wall)
+{
+    if (!c || !n_p2p || !n_wall) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    unsigned long long h[2];
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    CTX_CHECK(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    *n_p2p = h[0];
+    *n_wall = h[1];
+    return XDEM_OK;
+}
+
+int xdem_ctx_stream(xdem_ctx* c, void** out)
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    *out = c->stream;
+    return XDEM_OK;
+}
+
+int xdem_ctx_graph_capture(xdem_ctx* c, unsigned int nsteps)
+{
+    if (!c) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    if (nsteps == 0 || (nsteps & 1u)) return XDEM_E_INVALID;  // even: history buffers return to the same parity
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    const int parity = c->cur;
+    CTX_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (unsigned k = 0; k < nsteps && e == hipSuccess; k++) e = c->collide_half(true, nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    c->cur = parity;
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return XDEM_E_HIP;
+    }
+    c->graph = g;
+    CTX_CHECK(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    c->graph_steps = nsteps;
+    return XDEM_OK;
+}
+
+int xdem_ctx_graph_launch(xdem_ctx* c, unsigned int nlaunch)
+{
+    if (!c) return XDEM_E_INVALID;
+    if (!c->gexec) return XDEM_E_NOT_READY;
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    for (unsigned k = 0; k < nlaunch; k++) {
+        CTX_CHECK(hipGraphLaunch(c->gexec, c->stream));
+        c->steps_done += c->graph_steps;
+    }
+    return XDEM_OK;
+}
+
+int xdem_ctx_set_timing(xdem_ctx* c, int on)
+{
+    if (!c) return XDEM_E_INVALID;
+    c->timing = on != 0;
+    return XDEM_OK;
+}
+
+int xdem_ctx_timing(xdem_ctx* c, double out[3])
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    out[0] = c->t_contact;
+    out[1] = c->t_grid;
+    out[2] = (double)c->t_steps;
+    return XDEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// xdem_step_f64.hip -- FP64 instantiation of the fused step kernels.
+// Built with -ffp-contract=off so every multiply and add rounds separately, in the
+// reference's operation order.
+#include "xdem_step.hpp"
+
+namespace xdem {
+namespace {
+
+constexpr unsigned kBlock = 256;
+inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
+
+hipError_t gather_f64(const StepArgs& A, hipStream_t s)
+{
+    if (A.nsorted == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cells_gather<double>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+template <bool MULTI, bool FUSE>
+void collide_models(const StepArgs& A, int model, bool gpu_law, hipStream_t s)
+{
+    const dim3 g = grid_for(A.nsorted), b(kBlock);
+    if (model == 1) {
+        if (gpu_law) hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, 1, true>), g, b, 0, s, A);
+        else hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, 1, false>), g, b, 0, s, A);
+    } else {
+        if (gpu_law) hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, 0, true>), g, b, 0, s, A);
+        else hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, 0, false>), g, b, 0, s, A);
+    }
+}
+
+hipError_t collide_f64(const StepArgs& A, bool multi, bool fuse, int model, bool gpu_law, hipStream_t s)
+{
+    if (A.nsorted == 0) return hipSuccess;
+    if (multi) {
+        if (fuse) collide_models<true, true>(A, model, gpu_law, s);
+        else collide_models<true, false>(A, model, gpu_law, s);
+    } else {
+        if (fuse) collide_models<false, true>(A, model, gpu_law, s);
+        else collide_models<false, false>(A, model, gpu_law, s);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+StepLaunch step_launch_f64() { return {gather_f64, collide_f64}; }
+
+hipError_t launch_integrate_hash(const StepArgs& A, bool multi, hipStream_t s)
+{
+    if (A.np == 0) return hipSuccess;
+    if (multi) hipLaunchKernelGGL(k_integrate_hash<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    else hipLaunchKernelGGL(k_integrate_hash<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(const StepArgs& A, bool multi, hipStream_t s)
+{
+    if (A.np == 0) return hipSuccess;
+    if (multi) hipLaunchKernelGGL(k_finish<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    else hipLaunchKernelGGL(k_finish<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+}  // namespace xdem

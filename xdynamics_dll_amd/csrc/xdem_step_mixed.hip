@@ -1,0 +1,49 @@
+// xdem_step_mixed.hip -- mixed-precision instantiation of the fused step kernels:
+// contact-force arithmetic and the neighbours' velocity copies in FP32; the pair test,
+// positions, tangential history, force sums and the Verlet update in FP64.
+#include "xdem_step.hpp"
+
+namespace xdem {
+namespace {
+
+constexpr unsigned kBlock = 256;
+inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
+
+hipError_t gather_mixed(const StepArgs& A, hipStream_t s)
+{
+    if (A.nsorted == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cells_gather<float>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+template <bool MULTI, bool FUSE>
+void collide_models(const StepArgs& A, int model, bool gpu_law, hipStream_t s)
+{
+    const dim3 g = grid_for(A.nsorted), b(kBlock);
+    if (model == 1) {
+        if (gpu_law) hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, 1, true>), g, b, 0, s, A);
+        else hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, 1, false>), g, b, 0, s, A);
+    } else {
+        if (gpu_law) hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, 0, true>), g, b, 0, s, A);
+        else hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, 0, false>), g, b, 0, s, A);
+    }
+}
+
+hipError_t collide_mixed(const StepArgs& A, bool multi, bool fuse, int model, bool gpu_law, hipStream_t s)
+{
+    if (A.nsorted == 0) return hipSuccess;
+    if (multi) {
+        if (fuse) collide_models<true, true>(A, model, gpu_law, s);
+        else collide_models<true, false>(A, model, gpu_law, s);
+    } else {
+        if (fuse) collide_models<false, true>(A, model, gpu_law, s);
+        else collide_models<false, false>(A, model, gpu_law, s);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+StepLaunch step_launch_mixed() { return {gather_mixed, collide_mixed}; }
+
+}  // namespace xdem

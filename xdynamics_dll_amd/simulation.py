@@ -1,0 +1,181 @@
+"""Python view of the step layer (xdem_ctx) and of the reference's DEM entry points.
+
+`DemContext` owns one device context with one or many scenes. `xIntegratorVV` mirrors the
+method set of the reference's xDiscreteElementMethodSimulation / xIntegratorVV
+(include/xdynamics_simulation/xDiscreteElementMethodSimulation.h:20-37,
+xIngegratorVV.h:13-14) on top of it. All arithmetic runs in the gfx950 library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .scene import PARITY_QUIRKS, SceneData
+
+
+class DemContext:
+    def __init__(self, scenes, precision: str = "fp64", p2p_capacity: int = 12, quirks: dict | None = None,
+                 grid_dim_max: int = 512, device: int = 0):
+        if isinstance(scenes, SceneData):
+            scenes = [scenes]
+        self.scenes = list(scenes)
+        self.L = _lib.lib()
+        cfg = _lib.CtxConfig()
+        cfg.precision = {"fp64": _lib.PRECISION_FP64, "mixed": _lib.PRECISION_MIXED}[precision]
+        cfg.p2p_capacity = p2p_capacity
+        q = dict(PARITY_QUIRKS)
+        q.update(quirks or {})
+        cfg.quirks = _lib.Quirks(q["gpu_force_law"], q["velocity_not_updated"], q["stale_ft"], q["flat_inertia"])
+        cfg.grid_dim_max = grid_dim_max
+        cfg.device = device
+        self.precision = precision
+        self.capacity = p2p_capacity
+        self._h = C.c_void_p()
+        _lib.check(self.L.xdem_ctx_create(C.byref(cfg), C.byref(self._h)), "xdem_ctx_create")
+        for s in self.scenes:
+            cs, keep = s.to_c()
+            _lib.check(self.L.xdem_ctx_add_scene(self._h, C.byref(cs), None), "xdem_ctx_add_scene")
+            del keep
+        _lib.check(self.L.xdem_ctx_commit(self._h), "xdem_ctx_commit")
+
+    # ------------------------------------------------------------------ stepping
+    def step(self, n: int = 1) -> None:
+        _lib.check(self.L.xdem_ctx_step(self._h, n), "xdem_ctx_step")
+
+    def collide(self) -> None:
+        _lib.check(self.L.xdem_ctx_collide(self._h), "xdem_ctx_collide")
+
+    def finish(self) -> None:
+        _lib.check(self.L.xdem_ctx_finish(self._h), "xdem_ctx_finish")
+
+    def synchronize(self) -> None:
+        _lib.check(self.L.xdem_ctx_synchronize(self._h), "xdem_ctx_synchronize")
+
+    def graph_capture(self, nsteps: int) -> None:
+        _lib.check(self.L.xdem_ctx_graph_capture(self._h, nsteps), "xdem_ctx_graph_capture")
+
+    def graph_launch(self, n: int) -> None:
+        _lib.check(self.L.xdem_ctx_graph_launch(self._h, n), "xdem_ctx_graph_launch")
+
+    # ------------------------------------------------------------------ state
+    def read(self, name: str, scene: int = 0) -> np.ndarray:
+        f, w = _lib.FIELD[name]
+        n = self.scenes[scene].np
+        out = np.empty(n * w, dtype=np.float64)
+        _lib.check(self.L.xdem_ctx_read(self._h, f, scene, out.ctypes.data), "xdem_ctx_read")
+        return out.reshape(n, w) if w > 1 else out
+
+    def write(self, name: str, values: np.ndarray, scene: int = 0) -> None:
+        f, w = _lib.FIELD[name]
+        a = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        if a.size != self.scenes[scene].np * w:
+            raise ValueError(f"{name}: expected {self.scenes[scene].np * w} values, got {a.size}")
+        _lib.check(self.L.xdem_ctx_write(self._h, f, scene, a.ctypes.data), "xdem_ctx_write")
+
+    def device_ptr(self, name: str) -> int:
+        p = C.c_void_p()
+        _lib.check(self.L.xdem_ctx_device_ptr(self._h, _lib.FIELD[name][0], C.byref(p)), "xdem_ctx_device_ptr")
+        return p.value
+
+    def history(self, kind: str = "p2p", scene: int = 0):
+        n = self.scenes[scene].np
+        k = _lib.HISTORY_P2P if kind == "p2p" else _lib.HISTORY_PLANE
+        cap = self.capacity if kind == "p2p" else 3
+        count = np.empty(n, dtype=np.uint32)
+        ids = np.empty(n * cap, dtype=np.uint32)
+        tsd = np.empty(n * cap * 2, dtype=np.float64)
+        _lib.check(self.L.xdem_ctx_read_history(self._h, k, scene, count.ctypes.data, ids.ctypes.data,
+                                                tsd.ctypes.data), "xdem_ctx_read_history")
+        return count, ids.reshape(n, cap), tsd.reshape(n, cap, 2)
+
+    def error_word(self, clear: bool = True) -> int:
+        w = C.c_uint()
+        _lib.check(self.L.xdem_ctx_error_word(self._h, C.byref(w), 1 if clear else 0), "xdem_ctx_error_word")
+        return w.value
+
+    def wrench(self, scene: int = 0) -> np.ndarray:
+        nw = len(self.scenes[scene].planes)
+        out = np.zeros(6 * max(nw, 1))
+        _lib.check(self.L.xdem_ctx_body_wrench(self._h, scene, out.ctypes.data), "xdem_ctx_body_wrench")
+        return out[: 6 * nw].reshape(nw, 6)
+
+    def set_body_pose(self, scene: int, wall: int, dbi) -> None:
+        a = np.ascontiguousarray(dbi, dtype=np.float64)
+        _lib.check(self.L.xdem_ctx_set_body_pose(self._h, scene, wall, a.ctypes.data), "xdem_ctx_set_body_pose")
+
+    def contact_count(self):
+        a, b = C.c_ulonglong(), C.c_ulonglong()
+        _lib.check(self.L.xdem_ctx_contact_count(self._h, C.byref(a), C.byref(b)), "xdem_ctx_contact_count")
+        return a.value, b.value
+
+    def info(self) -> dict:
+        i = _lib.CtxInfo()
+        _lib.check(self.L.xdem_ctx_info_get(self._h, C.byref(i)), "xdem_ctx_info_get")
+        return {n: getattr(i, n) for n, _ in i._fields_}
+
+    def set_timing(self, on: bool) -> None:
+        _lib.check(self.L.xdem_ctx_set_timing(self._h, 1 if on else 0), "xdem_ctx_set_timing")
+
+    def timing(self):
+        out = (C.c_double * 3)()
+        _lib.check(self.L.xdem_ctx_timing(self._h, out), "xdem_ctx_timing")
+        return {"contact_ms": out[0], "grid_ms": out[1], "steps": int(out[2])}
+
+    def close(self) -> None:
+        if self._h:
+            self.L.xdem_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class xIntegratorVV:
+    """Method set of xDiscreteElementMethodSimulation + xIntegratorVV over DemContext."""
+
+    def __init__(self):
+        self.ctx: DemContext | None = None
+        self._scenes: list[SceneData] = []
+
+    def Initialize(self, scenes, precision: str = "fp64", **kw) -> int:
+        self.ctx = DemContext(scenes, precision=precision, **kw)
+        self._scenes = self.ctx.scenes
+        return 0
+
+    def Initialized(self) -> bool:
+        return self.ctx is not None
+
+    def OneStepSimulation(self, ct: float, cstep: int) -> int:
+        self.ctx.step(1)
+        return 0
+
+    def SaveStepResult(self, pt: int) -> dict:
+        return {n: self.ctx.read(n) for n in ("pos", "vel", "acc", "ep", "ev", "ea")}
+
+    def CriticalTimeStep(self, min_rad: float) -> float:
+        s = self._scenes[0]
+        return _lib.host().xdem_critical_time_step(min_rad, s.density, s.cp_pp.Ei, s.cp_pp.pri)
+
+    def updateObjectFromMBD(self, poses: dict | None = None) -> None:
+        for (scene, wall), dbi in (poses or {}).items():
+            self.ctx.set_body_pose(scene, wall, dbi)
+
+    def Position(self) -> int:
+        return self.ctx.device_ptr("pos")
+
+    def Velocity(self) -> int:
+        return self.ctx.device_ptr("vel")
+
+    def num_particles(self) -> int:
+        return sum(s.np for s in self._scenes)
+
+    def set_dem_data(self, cpos, pos, vel, acc, ep, ev, ea) -> int:
+        for name, a in (("pos", pos), ("vel", vel), ("acc", acc), ("ep", ep), ("ev", ev), ("ea", ea)):
+            if a is not None:
+                self.ctx.write(name, a)
+        return 0
