@@ -14,6 +14,8 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the gfx950 library)")
+    import xdem_ref
+    xdem_ref.set_threads(min(8, os.cpu_count() or 1))
 
 
 def load_fixture(name: str):

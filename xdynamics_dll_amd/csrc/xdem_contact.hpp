@@ -98,7 +98,8 @@ __device__ __forceinline__ void sphere_contact(const ContactConst& cc, double dt
     }
     const V3<T> rv = (jvel + cross(jom, dcpr_j)) - (ivel + cross(iom, dcpr));
     V3<T> Fn;
-    force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)g.cdist, dt, sd0, sd1, rv, unit, Ft, Fn);
+    force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)g.cdist, dt, sd0, sd1, rv, unit, Ft, Fn, g.cdist, c.eq_r_d,
+                                 c.eq_m_d);
     const V3<T> fnt = Fn + Ft;
     const V3<double> fd = cvt3<double>(fnt);
     acc.T = acc.T + cvt3<double>(cross(dcpr, fnt));
@@ -171,7 +172,8 @@ __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_
             dv = (bvel + cross(oj, dcpr_j)) - (ivel + cross(iom, dcpr));
         else
             dv = cvt3<float>(bvel + cross(oj, dcpr_j)) - (ivel + cross(iom, cvt3<float>(dcpr)));
-        force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)cdist, dt, sd0, sd1, dv, cvt3<T>(unit), Ft, Fn);
+        force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)cdist, dt, sd0, sd1, dv, cvt3<T>(unit), Ft, Fn, cdist, c.eq_r_d,
+                                     c.eq_m_d);
         const V3<double> mf = cvt3<double>(Fn + Ft);
         tma = tma + cross(dcpr, mf);
         res += (double)(c.eq_r * T(cc.rfric) * length(Fn));
@@ -249,7 +251,8 @@ __device__ __forceinline__ bool triangle_contact(const ContactConst& cc, const d
         dv = cvt3<float>(bvel + cross(jom, po2cp)) - (ivel + cross(iom, cvt3<float>(rc)));
     const PairConst<T> c = pair_const<T, MODEL>(cc, r, 0.0, m, dbi[XDEM_MM]);
     V3<T> Fn, Ft = mk3(T(0), T(0), T(0));
-    force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)cdist, dt, sd0, sd1, dv, cvt3<T>(unit), Ft, Fn);
+    force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)cdist, dt, sd0, sd1, dv, cvt3<T>(unit), Ft, Fn, cdist, c.eq_r_d,
+                                     c.eq_m_d);
     const V3<double> mf = cvt3<double>(Fn + Ft);
     acc.T = acc.T + cross(rc, mf);
     acc.R += (double)(c.eq_r * T(cc.rfric) * length(Fn));

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "xdem_types.h"
 
 namespace xdem {
@@ -152,6 +154,7 @@ struct ContactConst {
 // Per-pair constants at precision T.
 template <class T> struct PairConst {
     T kn, vn, ks, vs, mu, eq_r, eq_m;
+    double eq_r_d, eq_m_d;  // effective radius and mass in FP64
 };
 
 // Cohesion force (impl:388-413). Req is the pair's effective radius, Fn the normal force.
@@ -181,20 +184,39 @@ __device__ __forceinline__ double cohesion_limit_depth(const ContactConst& cc, d
 //   GPU_LAW selects the kernel's damping terms (impl) instead of xContact.cpp's.
 // ds/dots is the tangential history, kept in FP64 for every T. Ft is updated only
 // when the tangential slip direction is defined (|e| != 0), as in the reference.
+// In the mixed path (T = float) the normal-force magnitude -- Hertz, damping and JKR terms
+// and their sum -- is still evaluated in FP64 from FP64 overlap and effective radius/mass
+// (cdist_d, Req_d, Meq_d): the elastic and cohesive terms nearly cancel near contact onset,
+// and an FP32 sum would lose the net force there. Directions, relative velocity, the
+// tangential law and the moments stay in T.
 template <class T, int MODEL, bool GPU_LAW>
 __device__ __forceinline__ void force_law(const ContactConst& cc, const PairConst<T>& c, T Req, T cdist,
                                           double dt, double& ds, double& dots, V3<T> dv, V3<T> unit,
-                                          V3<T>& Ft, V3<T>& Fn)
+                                          V3<T>& Ft, V3<T>& Fn, double cdist_d = 0.0, double Req_d = 0.0,
+                                          double Meq_d = 0.0)
 {
-    const T fsn = -c.kn * (cdist * sqrt(cdist));
     const T dvu = dot(dv, unit);
-    T fdn;
-    if (MODEL == 1 && !GPU_LAW)
-        fdn = c.vn * (T(2.0) * sqrt(c.eq_m * c.kn)) * dvu;
-    else
-        fdn = c.vn * dvu;
-    const T fca = -cohesion_force<T>(cc, Req, fsn + fdn);
-    const T fnm = !GPU_LAW ? (fsn + fdn + fca) : (fsn + fca + fdn);
+    T fnm;
+    if constexpr (std::is_same<T, double>::value) {
+        const T fsn = -c.kn * (cdist * sqrt(cdist));
+        T fdn;
+        if (MODEL == 1 && !GPU_LAW)
+            fdn = c.vn * (T(2.0) * sqrt(c.eq_m * c.kn)) * dvu;
+        else
+            fdn = c.vn * dvu;
+        const T fca = -cohesion_force<T>(cc, Req, fsn + fdn);
+        fnm = !GPU_LAW ? (fsn + fdn + fca) : (fsn + fca + fdn);
+    } else {
+        const double kn = cc.k43E * sqrt(Req_d);
+        const double fsn = -kn * (cdist_d * sqrt(cdist_d));
+        double fdn;
+        if (MODEL == 1 && !GPU_LAW)
+            fdn = cc.beta * (2.0 * sqrt(Meq_d * kn)) * (double)dvu;
+        else
+            fdn = cc.beta * (double)dvu;
+        const double fca = -cohesion_force<double>(cc, Req_d, fsn + fdn);
+        fnm = (T)(!GPU_LAW ? (fsn + fdn + fca) : (fsn + fca + fdn));
+    }
     Fn = fnm * unit;
     const V3<T> e = dv - dvu * unit;
     const T mag_e = length(e);
@@ -228,6 +250,8 @@ __device__ __forceinline__ PairConst<T> pair_const(const ContactConst& cc, doubl
     const T Req_t = T(Req);
     c.eq_m = T(Meq);
     c.eq_r = Req_t;
+    c.eq_m_d = Meq;
+    c.eq_r_d = Req;
     c.kn = T(cc.k43E) * sqrt(Req_t);
     c.vn = T(cc.beta);
     c.ks = MODEL == 1 ? T(cc.ks_hm) : c.kn * T(cc.sratio);

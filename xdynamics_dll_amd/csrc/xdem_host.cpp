@@ -374,6 +374,7 @@ int xdem_scene_build(int workload, unsigned long long seed, unsigned int np_over
         }
         // box volume = 3 x packed volume (solid fraction 0.6), cube
         double side = std::cbrt(3.0 * vol / 0.6);
+        std::sort(radii.begin(), radii.end(), [](double a, double b) { return a > b; });
         V3 lo = {0, 0, 0}, hi = {side, side, side};
         Rsa rsa(lo, hi, 2.0e-3);
         for (unsigned i = 0; i < n; i++) {
@@ -405,6 +406,7 @@ int xdem_scene_build(int workload, unsigned long long seed, unsigned int np_over
         }
         const double R = 0.030;
         double H = vol / (0.30 * pi * R * R);
+        std::sort(radii.begin(), radii.end(), [](double a, double b) { return a > b; });
         const double cx = 0.2, cy = 0.2;  // cylinder axis over the middle of a 0.4 m floor
         V3 lo = {cx - R, cy - R, 0.0}, hi = {cx + R, cy + R, H};
         Rsa rsa(lo, hi, 1.5e-3);

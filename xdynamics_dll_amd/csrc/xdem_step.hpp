@@ -256,7 +256,8 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
                 // ---- rolling resistance (impl:2420-2454)
                 const V4<double> e = ld4(A.ep + 4 * i);
                 V4<double> av = ld4(A.ev + 4 * i);
-                const double J = A.flat_inertia ? A.inertia[i] : A.inertia[3 * i];
+                // quirk Q5: the scene's inertia array read as flat doubles at the scene-local id
+                const double J = A.flat_inertia ? A.inertia[i + 2 * S.p_begin] : A.inertia[3 * i];
                 const bool consumed = rolling_moment(S.dt, J, omega_of(e, av), tot.T, tot.R, tot.M);
                 if (A.tmax_persist) st3(A.tmax + 3 * i, consumed ? mk3(0.0, 0.0, 0.0) : tot.T);
                 // ---- velocity and orientation-rate update (xIntegratorVV.cpp:131-152; impl:192-222)
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
     const V4<double> e = ld4(A.ep + 4 * i);
     V4<double> av = ld4(A.ev + 4 * i);
     V3<double> M = ld3(A.moment + 3 * i);
-    const double J = A.flat_inertia ? A.inertia[i] : A.inertia[3 * i];
+    const double J = A.flat_inertia ? A.inertia[i + 2 * S.p_begin] : A.inertia[3 * i];
     if (rolling_moment(S.dt, J, omega_of(e, av), ld3(A.tmax + 3 * i), A.rres[i], M)) {
         st3(A.tmax + 3 * i, mk3(0.0, 0.0, 0.0));
         A.rres[i] = 0.0;
