@@ -34,19 +34,21 @@ def step_bytes(cbar: float) -> float:
     return 436.0 + 40.0 * cbar
 
 
-def contact_kernel_bytes(cbar: float) -> float:
-    """Algorithmic bytes per sphere of the fused contact kernel (contacts + rolling + Verlet
-    velocity half): own state read pos 32, vel 24, acc 24, ep 32, ev 32, ea 32, mass 8,
-    inertia 24 (= 208); state write vel 24, acc 24, ev 32, ea 32 (= 112); sphere and wall
-    history: counts 2*8, entries 2*20*cbar."""
-    return 336.0 + 40.0 * cbar
+def contact_kernel_bytes(cbar: float, fused: bool) -> float:
+    """Algorithmic bytes per sphere of the contact kernel k_collide.
+    Fused (contacts + rolling + velocity half): own state read pos 32, vel 24, acc 24,
+    ep 32, ev 32, ea 32, mass 8, inertia 24 (= 208); state write vel 24, acc 24, ev 32,
+    ea 32 (= 112); sphere and wall history: counts 2*8, entries 2*20*cbar.
+    Split (contacts only): read pos 32, vel 24, ep 32, ev 32, mass 8 (= 128); write force,
+    moment, rolling torque sum and magnitude 24+24+24+8 (= 80); history 16 + 40*cbar."""
+    return (336.0 if fused else 224.0) + 40.0 * cbar
 
 
-def run_gpu(scene, precision, steps, warmup, capacity):
+def run_gpu(scene, precision, steps, warmup, capacity, fuse=False):
     import torch
     from xdynamics_dll_amd import DemContext
 
-    ctx = DemContext(scene, precision=precision, p2p_capacity=capacity)
+    ctx = DemContext(scene, precision=precision, p2p_capacity=capacity, fuse_velocity=fuse)
     ctx.step(warmup)
     ctx.synchronize()
     torch.cuda.synchronize()
@@ -65,35 +67,51 @@ def run_gpu(scene, precision, steps, warmup, capacity):
     return {"seconds": t1 - t0, "timing": tm, "n_pp": n_pp, "n_wall": n_wall, "err": err, "info": info}
 
 
-def force_error(scene, capacity, nsteps=20):
-    """Mixed vs FP64 GPU forces at the same state: advance an FP64 context nsteps, copy its
-    state into a mixed context, run one contact pass on both, compare per sphere:
-    |F_mixed - F_fp64| / (|F_fp64| + max |F_fp64| * 1e-6)."""
+def force_error(scene, capacity, nsteps=40):
+    """Mixed and FP64 GPU contact forces against the FP64 oracle at one state: an FP64
+    context is advanced nsteps, its state is copied into a fresh mixed context, a fresh FP64
+    context and the oracle, and each runs one contact pass. Per sphere, the mixed error is
+    |F_mixed - F_oracle| / (sum of that sphere's |F_pair| + |F_oracle|); the FP64 GPU error
+    is max |F_fp64 - F_oracle| / max |F_oracle|. Pair sets are compared exactly."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import xdem_ref
     from xdynamics_dll_amd import DemContext
 
+    fields = ("pos", "vel", "acc", "ep", "ev", "ea")
     a = DemContext(scene, precision="fp64", p2p_capacity=capacity)
     a.step(nsteps)
+    state = {f: a.read(f) for f in fields}
+    a.close()
     b = DemContext(scene, precision="mixed", p2p_capacity=capacity)
-    for f in ("pos", "vel", "acc", "ep", "ev", "ea"):
-        b.write(f, a.read(f))
-    # the histories are part of the state; copy through a short replay instead: both
-    # contexts run the same collide from the same state (no history yet in b, so reset a's)
-    a2 = DemContext(scene, precision="fp64", p2p_capacity=capacity)
-    for f in ("pos", "vel", "acc", "ep", "ev", "ea"):
-        a2.write(f, a.read(f))
-    a2.collide()
+    c = DemContext(scene, precision="fp64", p2p_capacity=capacity)
+    r = xdem_ref.RefSim(scene, capacity=capacity)
+    for f in fields:
+        b.write(f, state[f])
+        c.write(f, state[f])
+        r.write(f, state[f])
     b.collide()
-    fa, fb = a2.read("force"), b.read("force")
-    na = np.linalg.norm(fa, axis=1)
-    scale = na.max()
-    e = np.linalg.norm(fb - fa, axis=1) / (na + 1e-6 * scale)
-    ca, ia, _ = a2.history("p2p")
-    cb, ib, _ = b.history("p2p")
-    pairs_equal = bool(np.array_equal(ca, cb) and all(
-        np.array_equal(np.sort(ia[i, : ca[i]]), np.sort(ib[i, : cb[i]])) for i in np.nonzero(ca)[0]))
-    for c in (a, a2, b):
-        c.close()
-    return float(e.max()), pairs_equal
+    c.collide()
+    mag = np.zeros(scene.np)
+    r.collide(pair_magnitude=mag)
+    fr = r.force
+    fb, fc = b.read("force"), c.read("force")
+    norm = mag + np.linalg.norm(fr, axis=1)
+    has = norm > 0
+    mixed_err = float((np.linalg.norm(fb - fr, axis=1)[has] / norm[has]).max()) if has.any() else 0.0
+    fp64_err = float(np.abs(fc - fr).max() / max(np.abs(fr).max(), 1e-300))
+
+    def same_pairs(ctx):
+        cnt, ids, _ = ctx.history("p2p")
+        if not np.array_equal(cnt, r.pp_count):
+            return False
+        return all(np.array_equal(np.sort(ids[i, : cnt[i]]), np.sort(r.pp_id[i, : r.pp_count[i]]))
+                   for i in np.nonzero(cnt)[0])
+
+    out = {"mixed_force_err": mixed_err, "fp64_force_err": fp64_err, "pairs_exact_mixed": same_pairs(b),
+           "pairs_exact_fp64": same_pairs(c), "contacts_checked": int(r.pp_count.sum()), "state_step": nsteps}
+    b.close()
+    c.close()
+    return out
 
 
 def cpu_baseline(scene, capacity, budget_s=12.0):
@@ -124,6 +142,7 @@ def main():
     ap.add_argument("--capacity", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fuse", action="store_true", help="rolling + velocity update inside the contact kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described "
@@ -139,12 +158,12 @@ def main():
     t_build = time.perf_counter() - t_build
     n = scene.np
 
-    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity)
+    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, args.fuse)
     value = n * args.steps / mixed["seconds"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
     k_ms = tm["contact_ms"] / max(tm["steps"], 1)
-    k_bytes = contact_kernel_bytes(cbar) * n
+    k_bytes = contact_kernel_bytes(cbar, args.fuse) * n
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     step_gbps = step_bytes(cbar) * n * args.steps / mixed["seconds"] / 1e9
 
@@ -166,12 +185,13 @@ def main():
                    "np": n, "seed": SEED, "p2p_capacity": args.capacity, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "k_collide (fused contacts + rolling + velocity update)",
-                     "kernel_ms": k_ms, "bytes_per_sphere": contact_kernel_bytes(cbar)},
+                     "kernel": "k_collide (" + ("contacts + rolling + velocity update" if args.fuse else "contacts") + ")",
+                     "kernel_ms": k_ms, "bytes_per_sphere": contact_kernel_bytes(cbar, args.fuse)},
         "step_gbps": step_gbps,
         "step_roofline_frac": step_gbps / HBM_PEAK_GBPS,
         "mean_contacts_per_sphere": cbar,
         "grid_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
+        "finish_ms_per_step": tm["finish_ms"] / max(tm["steps"], 1),
         "error_word": mixed["err"],
         "scene_build_s": t_build,
     }
@@ -180,9 +200,7 @@ def main():
         f64_value = n * max(20, args.steps // 2) / f64["seconds"]
         out["fp64_value"] = f64_value
         out["mixed_over_fp64"] = value / f64_value
-        ferr, pairs_equal = force_error(scene, args.capacity)
-        out["max_force_err_mixed_vs_fp64"] = ferr
-        out["pairs_bit_exact_mixed_vs_fp64"] = pairs_equal
+        out["force_check"] = force_error(scene, args.capacity)
     if not args.no_cpu:
         cb = cpu_baseline(scene, args.capacity)
         out["cpu_baseline"] = cb

@@ -102,6 +102,8 @@ typedef struct xdem_ctx_config {
     xdem_quirks quirks;
     unsigned int block_size;     /* threads per block of the contact kernel (0 = tuned default) */
     unsigned int grid_dim_max;   /* largest bucket count per axis (power of two, 0 = 512) */
+    int fuse_velocity;           /* 1: rolling + velocity update inside the contact kernel;
+                                    0: separate kernel (fewer registers in the contact kernel) */
     int device;                  /* HIP device ordinal */
 } xdem_ctx_config;
 
@@ -179,9 +181,10 @@ int xdem_ctx_graph_capture(xdem_ctx* ctx, unsigned int nsteps);
 int xdem_ctx_graph_launch(xdem_ctx* ctx, unsigned int nlaunch);
 /* Per-kernel time of the last xdem_ctx_step call, measured with HIP events on the
  * context's stream when timing is on: out[0] = contact kernel ms (sum over steps),
- * out[1] = grid (position/hash/sort/cells) ms, out[2] = number of steps timed. */
+ * out[1] = grid (position/hash/sort/cells) ms, out[2] = number of steps timed,
+ * out[3] = rolling + velocity kernel ms (0 when fused into the contact kernel). */
 int xdem_ctx_set_timing(xdem_ctx* ctx, int on);
-int xdem_ctx_timing(xdem_ctx* ctx, double out[3]);
+int xdem_ctx_timing(xdem_ctx* ctx, double out[4]);
 
 #ifdef __cplusplus
 }

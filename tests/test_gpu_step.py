@@ -70,10 +70,11 @@ def test_fp64_collide_matches_oracle(name):
     gpu.close()
 
 
+@pytest.mark.parametrize("fuse", [False, True])
 @pytest.mark.parametrize("name", ["lattice", "block"])
-def test_fp64_fused_steps_match_oracle(name):
+def test_fp64_fused_steps_match_oracle(name, fuse):
     scene = SCENES[name]()
-    gpu = DemContext(scene, precision="fp64")
+    gpu = DemContext(scene, precision="fp64", fuse_velocity=fuse)
     ref = xdem_ref.RefSim(scene)
     for n in (1, 9, 40):
         gpu.step(n)
@@ -116,10 +117,11 @@ def test_multi_scene_equals_single_scenes():
         assert np.array_equal(both.read(f, 1), sb.read(f)), f
 
 
-def test_graph_replay_equals_eager():
+@pytest.mark.parametrize("fuse", [False, True])
+def test_graph_replay_equals_eager(fuse):
     scene = compressed_lattice()
-    eager = DemContext(scene, precision="fp64")
-    graph = DemContext(scene, precision="fp64")
+    eager = DemContext(scene, precision="fp64", fuse_velocity=fuse)
+    graph = DemContext(scene, precision="fp64", fuse_velocity=fuse)
     graph.graph_capture(2)
     for k in range(5):
         graph.graph_launch(1)

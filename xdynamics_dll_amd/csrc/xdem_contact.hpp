@@ -3,8 +3,9 @@
 //
 // FP64 instantiations keep the reference's operation order (impl:613-733 for spheres,
 // impl:950-1080 for walls, impl:1748-1835 for triangles). FP32 instantiations (the mixed
-// path) evaluate the force law and the relative kinematics in float, while the pair
-// test, positions, tangential history and all sums stay in double.
+// path) evaluate the tangential spring-damper magnitude and the moment arithmetic in
+// float; the pair test, positions, relative velocity, normal force, tangential history
+// and all sums stay in double (see force_law).
 #pragma once
 
 #include <type_traits>
@@ -77,51 +78,37 @@ __device__ __forceinline__ PairGeom pair_geom(V4<double> pi_, V4<double> pj)
 // and the pair's history (sd0, sd1). Ft carries over between calls (quirk Q4) unless
 // the caller resets it.
 template <class T, int MODEL, bool GPU_LAW>
-__device__ __forceinline__ void sphere_contact(const ContactConst& cc, double dt, V4<double> ipos, V3<T> ivel, V3<T> iom,
-                                               double im, V4<double> jpos, V3<T> jvel, V3<T> jom, double jm,
-                                               const PairGeom& g, double& sd0, double& sd1, V3<T>& Ft, Accum& acc)
+__device__ __forceinline__ void sphere_contact(const ContactConst& cc, double dt, V4<double> ipos, V3<double> ivel,
+                                               V3<double> iom, double im, V4<double> jpos, V3<double> jvel,
+                                               V3<double> jom, double jm, const PairGeom& g, double& sd0, double& sd1,
+                                               V3<T>& Ft, Accum& acc)
 {
     const double ir = ipos.w, jr = jpos.w;
     const PairConst<T> c = pair_const<T, MODEL>(cc, ir, jr, im, jm);
-    V3<T> unit, dcpr, dcpr_j;
-    if constexpr (std::is_same<T, double>::value) {
-        unit = g.rp / g.dist;
-        const V3<double> ip3 = mk3(ipos.x, ipos.y, ipos.z);
-        const V3<double> cpt = ip3 + ir * unit;
-        dcpr = cpt - ip3;
-        dcpr_j = cpt - mk3(jpos.x, jpos.y, jpos.z);
-    } else {
-        const V3<float> rpf = cvt3<float>(g.rp);
-        unit = rpf / (float)g.dist;
-        dcpr = (float)ir * unit;
-        dcpr_j = dcpr - rpf;
-    }
-    const V3<T> rv = (jvel + cross(jom, dcpr_j)) - (ivel + cross(iom, dcpr));
+    const V3<double> unit = g.rp / g.dist;
+    const V3<double> ip3 = mk3(ipos.x, ipos.y, ipos.z);
+    const V3<double> cpt = ip3 + ir * unit;
+    const V3<double> dcpr = cpt - ip3;
+    const V3<double> dcpr_j = cpt - mk3(jpos.x, jpos.y, jpos.z);
+    const V3<double> rv = (jvel + cross(jom, dcpr_j)) - (ivel + cross(iom, dcpr));
     V3<T> Fn;
-    force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)g.cdist, dt, sd0, sd1, rv, unit, Ft, Fn, g.cdist, c.eq_r_d,
-                                 c.eq_m_d);
+    force_law<T, MODEL, GPU_LAW>(cc, c, g.cdist, dt, sd0, sd1, rv, unit, Ft, Fn);
     const V3<T> fnt = Fn + Ft;
-    const V3<double> fd = cvt3<double>(fnt);
-    acc.T = acc.T + cvt3<double>(cross(dcpr, fnt));
+    const V3<T> dc = cvt3<T>(dcpr);
+    acc.T = acc.T + cvt3<double>(cross(dc, fnt));
     acc.R += (double)(c.eq_r * T(cc.rfric) * length(Fn));
-    acc.F = acc.F + fd;
-    acc.M = acc.M + cvt3<double>(cross(dcpr, fnt));
+    acc.F = acc.F + cvt3<double>(fnt);
+    acc.M = acc.M + cvt3<double>(cross(dc, fnt));
 }
 
-// Cohesion-only attraction of a separated pair (impl:708-718).
-template <class T>
+// Cohesion-only attraction of a separated pair (impl:708-718), FP64 for every precision.
 __device__ __forceinline__ void sphere_cohesion(const ContactConst& cc, double ir, double jr, const PairGeom& g, Accum& acc)
 {
     const double Req = jr ? (ir * jr) / (ir + jr) : ir;
-    const T f = T(cc.jk) * T(Req);
-    const T cf = cohesion_force<T>(cc, T(Req), f);
-    if constexpr (std::is_same<T, double>::value) {
-        const V3<double> unit = g.rp / g.dist;
-        acc.F = acc.F - cf * unit;
-    } else {
-        const V3<float> unit = cvt3<float>(g.rp) / (float)g.dist;
-        acc.F = acc.F - cvt3<double>(cf * unit);
-    }
+    const double f = cc.jk * Req;
+    const double cf = cohesion_force<double>(cc, Req, f);
+    const V3<double> unit = g.rp / g.dist;
+    acc.F = acc.F - cf * unit;
 }
 
 // -------------------------------------------------------------------------- walls
@@ -135,8 +122,8 @@ struct WallSlots {
 
 template <class T, int MODEL, bool GPU_LAW>
 __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_plane_info& pl, const double* dbi, double dt,
-                                             V4<double> ipos, V3<T> ivel, V3<T> iom, double m, WallSlots& ws,
-                                             Accum& acc, double bf[6])
+                                             V4<double> ipos, V3<double> ivel, V3<double> iom, double m,
+                                             WallSlots& ws, Accum& acc, double bf[6])
 {
     const V3<double> ip3 = mk3(ipos.x, ipos.y, ipos.z);
     const double r = ipos.w;
@@ -167,13 +154,8 @@ __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_
         const V3<double> bvel = mk3(dbi[XDEM_VX], dbi[XDEM_VY], dbi[XDEM_VZ]);
         const PairConst<T> c = pair_const<T, MODEL>(cc, r, 0.0, m, 0.0);
         V3<T> Fn, Ft = mk3(T(0), T(0), T(0));
-        V3<T> dv;
-        if constexpr (std::is_same<T, double>::value)
-            dv = (bvel + cross(oj, dcpr_j)) - (ivel + cross(iom, dcpr));
-        else
-            dv = cvt3<float>(bvel + cross(oj, dcpr_j)) - (ivel + cross(iom, cvt3<float>(dcpr)));
-        force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)cdist, dt, sd0, sd1, dv, cvt3<T>(unit), Ft, Fn, cdist, c.eq_r_d,
-                                     c.eq_m_d);
+        const V3<double> dv = (bvel + cross(oj, dcpr_j)) - (ivel + cross(iom, dcpr));
+        force_law<T, MODEL, GPU_LAW>(cc, c, cdist, dt, sd0, sd1, dv, unit, Ft, Fn);
         const V3<double> mf = cvt3<double>(Fn + Ft);
         tma = tma + cross(dcpr, mf);
         res += (double)(c.eq_r * T(cc.rfric) * length(Fn));
@@ -184,9 +166,9 @@ __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_
         bf[3] += bm.x; bf[4] += bm.y; bf[5] += bm.z;
         stored = true;
     } else if (cdist <= 0 && fabs(cdist) < fabs(coh_s)) {
-        const T f = T(cc.jk) * T(r);
-        const T cf = cohesion_force<T>(cc, T(r), f);
-        const V3<double> mf = (double)(-cf) * unit;
+        const double f = cc.jk * r;
+        const double cf = cohesion_force<double>(cc, r, f);
+        const V3<double> mf = (-cf) * unit;
         sumF = sumF + mf;
         bf[0] += -mf.x; bf[1] += -mf.y; bf[2] += -mf.z;
         const V3<double> bm = -cross(dcpr_j, mf);
@@ -219,7 +201,7 @@ struct TriHit {
 // history entry (id, sd0, sd1) is to be stored.
 template <class T, int MODEL, bool GPU_LAW>
 __device__ __forceinline__ bool triangle_contact(const ContactConst& cc, const double* dbi, double dt, TriHit h,
-                                                 V3<double> ipos, double r, double m, V3<T> ivel, V3<T> iom,
+                                                 V3<double> ipos, double r, double m, V3<double> ivel, V3<double> iom,
                                                  const uint32_t* old_id, const double* old_s0, const double* old_s1,
                                                  double& sd0, double& sd1, Accum& acc, double bf[6])
 {
@@ -232,9 +214,9 @@ __device__ __forceinline__ bool triangle_contact(const ContactConst& cc, const d
     double coh_s = 0;
     if (cc.coh) coh_s = cohesion_limit_depth(cc, r);
     if (cdist <= 0 && fabs(cdist) < fabs(coh_s)) {
-        const T f = T(cc.jk) * T(r);
-        const T cf = cohesion_force<T>(cc, T(r), f);
-        acc.F = acc.F - (double)cf * unit;
+        const double f = cc.jk * r;
+        const double cf = cohesion_force<double>(cc, r, f);
+        acc.F = acc.F - cf * unit;
         return true;
     }
     V4<double> be = {dbi[XDEM_E0], dbi[XDEM_E1], dbi[XDEM_E2], dbi[XDEM_E3]};
@@ -244,15 +226,10 @@ __device__ __forceinline__ bool triangle_contact(const ContactConst& cc, const d
         if (old_id[s] == h.id) { sd0 = old_s0[s]; sd1 = old_s1[s]; break; }
     const V3<double> rc = h.cpt - ipos;
     const V3<double> bvel = mk3(dbi[XDEM_VX], dbi[XDEM_VY], dbi[XDEM_VZ]);
-    V3<T> dv;
-    if constexpr (std::is_same<T, double>::value)
-        dv = (bvel + cross(jom, po2cp)) - (ivel + cross(iom, rc));
-    else
-        dv = cvt3<float>(bvel + cross(jom, po2cp)) - (ivel + cross(iom, cvt3<float>(rc)));
+    const V3<double> dv = (bvel + cross(jom, po2cp)) - (ivel + cross(iom, rc));
     const PairConst<T> c = pair_const<T, MODEL>(cc, r, 0.0, m, dbi[XDEM_MM]);
     V3<T> Fn, Ft = mk3(T(0), T(0), T(0));
-    force_law<T, MODEL, GPU_LAW>(cc, c, c.eq_r, (T)cdist, dt, sd0, sd1, dv, cvt3<T>(unit), Ft, Fn, cdist, c.eq_r_d,
-                                     c.eq_m_d);
+    force_law<T, MODEL, GPU_LAW>(cc, c, cdist, dt, sd0, sd1, dv, unit, Ft, Fn);
     const V3<double> mf = cvt3<double>(Fn + Ft);
     acc.T = acc.T + cross(rc, mf);
     acc.R += (double)(c.eq_r * T(cc.rfric) * length(Fn));

@@ -31,11 +31,6 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
     if (i < n) p[i] = v;
 }
 
-__global__ void k_zero_counters(unsigned long long* c)
-{
-    if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
-}
-
 // Sum of wrench[slot][c][p_begin .. p_end) for every (slot, c) = one block each, fixed order.
 __global__ __launch_bounds__(256) void k_wrench_reduce(const double* wrench, uint32_t np, const uint32_t* range,
                                                       double* out)
@@ -74,7 +69,7 @@ struct xdem_ctx {
     uint32_t np = 0, nbucket = 0, key_bits = 1, cap = XDEM_MAX_P2P_COUNT, nwslot = 0;
     int tmax_persist = 0;
     hipStream_t stream = nullptr;
-    StepLaunch L = {nullptr, nullptr};
+    CollideLaunch collide_fn = nullptr;
 
     double *pos = nullptr, *vel = nullptr, *acc = nullptr, *ep = nullptr, *ev = nullptr, *ea = nullptr;
     double *force = nullptr, *moment = nullptr, *mass = nullptr, *inertia = nullptr, *tmax = nullptr, *rres = nullptr;
@@ -85,8 +80,9 @@ struct xdem_ctx {
     void* sort_scratch = nullptr;
     uint2* cell_range = nullptr;
     double4* spos = nullptr;
-    void* svm = nullptr;
-    void* som = nullptr;
+    float4* sposf = nullptr;
+    double4* svm = nullptr;
+    double4* som = nullptr;
     uint32_t* pp_count[2] = {nullptr, nullptr};
     uint32_t* pp_id[2] = {nullptr, nullptr};
     double2* pp_tsd[2] = {nullptr, nullptr};
@@ -108,7 +104,7 @@ struct xdem_ctx {
 
     bool timing = false;
     std::vector<hipEvent_t> events;  // 3 per timed step
-    double t_contact = 0.0, t_grid = 0.0;
+    double t_contact = 0.0, t_grid = 0.0, t_finish = 0.0;
     unsigned t_steps = 0;
     unsigned long long steps_done = 0;
 
@@ -131,7 +127,7 @@ struct xdem_ctx {
         A.force = force; A.moment = moment; A.mass = mass; A.inertia = inertia; A.tmax = tmax; A.rres = rres;
         A.scene_of = scene_of; A.scenes = d_scenes; A.walls = d_walls;
         A.keys = keys; A.vals = vals; A.skeys = skeys; A.svals = svals; A.cell_range = cell_range;
-        A.spos = spos; A.svm = svm; A.som = som;
+        A.spos = spos; A.sposf = sposf; A.svm = svm; A.som = som;
         A.pp_count_in = pp_count[cur]; A.pp_id_in = pp_id[cur]; A.pp_tsd_in = pp_tsd[cur];
         A.pp_count_out = pp_count[cur ^ 1]; A.pp_id_out = pp_id[cur ^ 1]; A.pp_tsd_out = pp_tsd[cur ^ 1];
         A.pl_count = pl_count; A.pl_id = pl_id; A.pl_tsd = pl_tsd;
@@ -139,6 +135,7 @@ struct xdem_ctx {
         A.np = np; A.nsorted = np; A.cap = cap; A.nwslot = nwslot;
         A.stale_ft = cfg.quirks.stale_ft; A.flat_inertia = cfg.quirks.flat_inertia;
         A.keep_velocity = cfg.quirks.velocity_not_updated; A.tmax_persist = tmax_persist;
+        A.zero_force = 1;
         return A;
     }
 
@@ -148,14 +145,13 @@ struct xdem_ctx {
         hipError_t e;
         const StepArgs A = args();
         if (ev3 && (e = hipEventRecord(ev3[0], stream)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, counters);
         if ((e = launch_integrate_hash(A, multi, stream)) != hipSuccess) return e;
         if ((e = radix_sort_pairs(keys, vals, skeys, svals, ktmp, vtmp, np, (int)key_bits, sort_scratch, stream)) !=
             hipSuccess)
             return e;
-        if ((e = L.gather(A, stream)) != hipSuccess) return e;
+        if ((e = launch_gather(A, stream)) != hipSuccess) return e;
         if (ev3 && (e = hipEventRecord(ev3[1], stream)) != hipSuccess) return e;
-        if ((e = L.collide(A, multi, fuse, model, cfg.quirks.gpu_force_law != 0, stream)) != hipSuccess) return e;
+        if ((e = collide_fn(A, multi, fuse, model, cfg.quirks.gpu_force_law != 0, stream)) != hipSuccess) return e;
         if (ev3 && (e = hipEventRecord(ev3[2], stream)) != hipSuccess) return e;
         if (nwslot) {
             hipLaunchKernelGGL(k_wrench_reduce, dim3(nwslot * 6), dim3(256), 0, stream, wrench_pp, np, d_wslot_range,
@@ -163,6 +159,23 @@ struct xdem_ctx {
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         cur ^= 1;
+        return hipSuccess;
+    }
+
+    // One full step; ev4 (optional) brackets grid, contacts and the velocity kernel.
+    hipError_t full_step(hipEvent_t* ev4)
+    {
+        hipError_t e;
+        if (cfg.fuse_velocity) {
+            if ((e = collide_half(true, ev4)) != hipSuccess) return e;
+            if (ev4) return hipEventRecord(ev4[3], stream);
+            return hipSuccess;
+        }
+        if ((e = collide_half(false, ev4)) != hipSuccess) return e;
+        StepArgs A = args();
+        A.zero_force = 0;  // the next contact pass overwrites force and moment
+        if ((e = launch_finish(A, multi, stream)) != hipSuccess) return e;
+        if (ev4) return hipEventRecord(ev4[3], stream);
         return hipSuccess;
     }
 };
@@ -310,11 +323,11 @@ int xdem_ctx_commit(xdem_ctx* c)
     if (c->scenes.empty()) return XDEM_E_INVALID;
     CTX_CHECK(hipSetDevice(c->cfg.device));
     c->multi = c->scenes.size() > 1;
-    c->L = c->cfg.precision == XDEM_PRECISION_MIXED ? step_launch_mixed() : step_launch_f64();
+    c->collide_fn = c->cfg.precision == XDEM_PRECISION_MIXED ? collide_launch_mixed() : collide_launch_f64();
     const size_t n = c->np;
     c->key_bits = 1;
     while (c->key_bits < 32 && (1ull << c->key_bits) < (unsigned long long)c->nbucket) c->key_bits++;
-    const size_t vsz = c->cfg.precision == XDEM_PRECISION_MIXED ? sizeof(float4) : sizeof(double4);
+    const size_t vsz = sizeof(double4);
     CTX_CHECK(c->alloc(&c->pos, sizeof(double) * 4 * n));
     CTX_CHECK(c->alloc(&c->vel, sizeof(double) * 3 * n));
     CTX_CHECK(c->alloc(&c->acc, sizeof(double) * 3 * n));
@@ -339,6 +352,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->sort_scratch, radix_sort_scratch_bytes((uint32_t)n)));
     CTX_CHECK(c->alloc(&c->cell_range, sizeof(uint2) * (size_t)c->nbucket));
     CTX_CHECK(c->alloc(&c->spos, sizeof(double4) * n));
+    CTX_CHECK(c->alloc(&c->sposf, sizeof(float4) * n));
     CTX_CHECK(c->alloc(&c->svm, vsz * n));
     CTX_CHECK(c->alloc(&c->som, vsz * n));
     for (int b = 0; b < 2; b++) {
@@ -419,8 +433,9 @@ int xdem_ctx_commit(xdem_ctx* c)
 static int run_steps(xdem_ctx* c, unsigned nsteps)
 {
     const bool time_it = c->timing;
+    const unsigned kmax = 2048u;
     if (time_it) {
-        const size_t need = 3 * (size_t)std::min(nsteps, 2048u);
+        const size_t need = 4 * (size_t)std::min(nsteps, kmax);
         while (c->events.size() < need) {
             hipEvent_t e;
             CTX_CHECK(hipEventCreate(&e));
@@ -428,21 +443,24 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
         }
     }
     for (unsigned k = 0; k < nsteps; k++) {
-        hipEvent_t* ev3 = (time_it && k < 2048u) ? &c->events[3 * k] : nullptr;
-        CTX_CHECK(c->collide_half(true, ev3));
+        hipEvent_t* ev4 = (time_it && k < kmax) ? &c->events[4 * k] : nullptr;
+        CTX_CHECK(c->full_step(ev4));
         c->steps_done++;
     }
     if (time_it) {
         CTX_CHECK(hipStreamSynchronize(c->stream));
         c->t_contact = 0.0;
         c->t_grid = 0.0;
-        c->t_steps = std::min(nsteps, 2048u);
+        c->t_finish = 0.0;
+        c->t_steps = std::min(nsteps, kmax);
         for (unsigned k = 0; k < c->t_steps; k++) {
-            float a = 0.f, b = 0.f;
-            CTX_CHECK(hipEventElapsedTime(&a, c->events[3 * k], c->events[3 * k + 1]));
-            CTX_CHECK(hipEventElapsedTime(&b, c->events[3 * k + 1], c->events[3 * k + 2]));
+            float a = 0.f, b = 0.f, f = 0.f;
+            CTX_CHECK(hipEventElapsedTime(&a, c->events[4 * k], c->events[4 * k + 1]));
+            CTX_CHECK(hipEventElapsedTime(&b, c->events[4 * k + 1], c->events[4 * k + 2]));
+            CTX_CHECK(hipEventElapsedTime(&f, c->events[4 * k + 2], c->events[4 * k + 3]));
             c->t_grid += a;
             c->t_contact += b;
+            c->t_finish += f;
         }
     }
     return XDEM_OK;
@@ -707,7 +725,7 @@ int xdem_ctx_graph_capture(xdem_ctx* c, unsigned int nsteps)
     const int parity = c->cur;
     CTX_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    for (unsigned k = 0; k < nsteps && e == hipSuccess; k++) e = c->collide_half(true, nullptr);
+    for (unsigned k = 0; k < nsteps && e == hipSuccess; k++) e = c->full_step(nullptr);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     c->cur = parity;
@@ -740,12 +758,13 @@ int xdem_ctx_set_timing(xdem_ctx* c, int on)
     return XDEM_OK;
 }
 
-int xdem_ctx_timing(xdem_ctx* c, double out[3])
+int xdem_ctx_timing(xdem_ctx* c, double out[4])
 {
     if (!c || !out) return XDEM_E_INVALID;
     out[0] = c->t_contact;
     out[1] = c->t_grid;
     out[2] = (double)c->t_steps;
+    out[3] = c->t_finish;
     return XDEM_OK;
 }
 

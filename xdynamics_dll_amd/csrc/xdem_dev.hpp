@@ -184,60 +184,51 @@ __device__ __forceinline__ double cohesion_limit_depth(const ContactConst& cc, d
 //   GPU_LAW selects the kernel's damping terms (impl) instead of xContact.cpp's.
 // ds/dots is the tangential history, kept in FP64 for every T. Ft is updated only
 // when the tangential slip direction is defined (|e| != 0), as in the reference.
-// In the mixed path (T = float) the normal-force magnitude -- Hertz, damping and JKR terms
-// and their sum -- is still evaluated in FP64 from FP64 overlap and effective radius/mass
-// (cdist_d, Req_d, Meq_d): the elastic and cohesive terms nearly cancel near contact onset,
-// and an FP32 sum would lose the net force there. Directions, relative velocity, the
-// tangential law and the moments stay in T.
+//
+// The relative velocity dv, the contact normal, the normal force (Hertz + damping + JKR)
+// and the split of dv into normal and tangential parts are FP64 for every T: near contact
+// onset the damping and cohesive terms cancel to ~1e-3 of either, and when dv is nearly
+// normal the tangential slip is a difference of nearly equal velocities, so FP32 there
+// would miss the 1e-4 force bound by orders of magnitude. With T = float (the mixed path)
+// the tangential spring-damper magnitude, its Coulomb cap and the output vectors are FP32.
 template <class T, int MODEL, bool GPU_LAW>
-__device__ __forceinline__ void force_law(const ContactConst& cc, const PairConst<T>& c, T Req, T cdist,
-                                          double dt, double& ds, double& dots, V3<T> dv, V3<T> unit,
-                                          V3<T>& Ft, V3<T>& Fn, double cdist_d = 0.0, double Req_d = 0.0,
-                                          double Meq_d = 0.0)
+__device__ __forceinline__ void force_law(const ContactConst& cc, const PairConst<T>& c, double cdist, double dt,
+                                          double& ds, double& dots, V3<double> dv, V3<double> unit, V3<T>& Ft,
+                                          V3<T>& Fn)
 {
-    const T dvu = dot(dv, unit);
-    T fnm;
-    if constexpr (std::is_same<T, double>::value) {
-        const T fsn = -c.kn * (cdist * sqrt(cdist));
-        T fdn;
-        if (MODEL == 1 && !GPU_LAW)
-            fdn = c.vn * (T(2.0) * sqrt(c.eq_m * c.kn)) * dvu;
-        else
-            fdn = c.vn * dvu;
-        const T fca = -cohesion_force<T>(cc, Req, fsn + fdn);
-        fnm = !GPU_LAW ? (fsn + fdn + fca) : (fsn + fca + fdn);
-    } else {
-        const double kn = cc.k43E * sqrt(Req_d);
-        const double fsn = -kn * (cdist_d * sqrt(cdist_d));
-        double fdn;
-        if (MODEL == 1 && !GPU_LAW)
-            fdn = cc.beta * (2.0 * sqrt(Meq_d * kn)) * (double)dvu;
-        else
-            fdn = cc.beta * (double)dvu;
-        const double fca = -cohesion_force<double>(cc, Req_d, fsn + fdn);
-        fnm = (T)(!GPU_LAW ? (fsn + fdn + fca) : (fsn + fca + fdn));
-    }
-    Fn = fnm * unit;
-    const V3<T> e = dv - dvu * unit;
-    const T mag_e = length(e);
+    const double Req = c.eq_r_d, Meq = c.eq_m_d;
+    const double kn = std::is_same<T, double>::value ? (double)c.kn : cc.k43E * sqrt(Req);
+    const double fsn = -kn * (cdist * sqrt(cdist));
+    const double dvu = dot(dv, unit);
+    double fdn;
+    if (MODEL == 1 && !GPU_LAW)
+        fdn = cc.beta * (2.0 * sqrt(Meq * kn)) * dvu;
+    else
+        fdn = cc.beta * dvu;
+    const double fca = -cohesion_force<double>(cc, Req, fsn + fdn);
+    const double fnm = !GPU_LAW ? (fsn + fdn + fca) : (fsn + fca + fdn);
+    const V3<double> Fn_d = fnm * unit;
+    Fn = cvt3<T>(Fn_d);
+    const V3<double> e = dv - dvu * unit;
+    const double mag_e = length(e);
     if (mag_e) {
-        const V3<T> sh = e / mag_e;
-        const T s_dot = dot(dv, sh);
-        const double nds = ds + dt * ((double)s_dot + dots);
+        const V3<double> sh = e / mag_e;
+        const double s_dot = dot(dv, sh);
+        const double nds = ds + dt * (s_dot + dots);
         ds = nds;
-        dots = (double)s_dot;
+        dots = s_dot;
         T ft1;
         if (MODEL == 1) {
-            const T St = c.ks * sqrt(c.eq_r * cdist);
+            const T St = c.ks * sqrt(c.eq_r * T(cdist));
             if (GPU_LAW)
                 ft1 = St * T(nds) + c.vs * sqrt(St * c.eq_m);
             else
-                ft1 = St * T(nds) + c.vs * sqrt(St * c.eq_m) * s_dot;
+                ft1 = St * T(nds) + c.vs * sqrt(St * c.eq_m) * T(s_dot);
         } else {
-            ft1 = c.ks * T(nds) + c.vs * s_dot;
+            ft1 = c.ks * T(nds) + c.vs * T(s_dot);
         }
         const T ft2 = c.mu * length(Fn);
-        Ft = fmin(ft1, ft2) * sh;
+        Ft = fmin(ft1, ft2) * cvt3<T>(sh);
     }
 }
 

@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kThreads) void k_p2p(LaunchPrm L, const double* __r
                                                                sd0, sd1, Ft, acc);
                         store = true;
                     } else if (g.cdist <= 0 && fabs(g.cdist) < fabs(coh_s)) {
-                        sphere_cohesion<double>(cc, ir, jr, g, acc);
+                        sphere_cohesion(cc, ir, jr, g, acc);
                         store = true;
                     }
                     if (store) {

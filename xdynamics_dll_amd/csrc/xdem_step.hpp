@@ -56,8 +56,9 @@ struct StepArgs {
     uint2* cell_range;                       // [start, end) per bucket, start = kInvalid if empty
     // sorted-order neighbour data
     double4* spos;
-    void* svm;    // V4<T>: velocity xyz + mass
-    void* som;    // V4<T>: angular velocity xyz
+    float4* sposf;  // position relative to the scene's grid origin and radius, FP32 (prefilter)
+    double4* svm;  // velocity xyz + mass
+    double4* som;  // angular velocity xyz
     // sphere history (double-buffered) and wall history (in place)
     const uint32_t* pp_count_in; const uint32_t* pp_id_in; const double2* pp_tsd_in;
     uint32_t* pp_count_out; uint32_t* pp_id_out; double2* pp_tsd_out;
@@ -68,11 +69,8 @@ struct StepArgs {
     uint32_t np, nsorted, cap;
     uint32_t nwslot;
     int stale_ft, flat_inertia, keep_velocity, tmax_persist;
+    int zero_force;  // k_finish leaves force/moment zero (the reference's state after a step)
 };
-
-template <class T> struct Vec4Of;
-template <> struct Vec4Of<double> { using type = double4; };
-template <> struct Vec4Of<float> { using type = float4; };
 
 __device__ __forceinline__ uint32_t scene_index(const StepArgs& A, uint32_t i, bool multi)
 {
@@ -84,6 +82,10 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) void k_integrate_hash(StepArgs A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {  // stored-contact counters of this step
+        A.counters[0] = 0ull;
+        A.counters[1] = 0ull;
+    }
     if (i >= A.np) return;
     const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
     const uint32_t old = A.keys[i];
@@ -110,10 +112,10 @@ __global__ __launch_bounds__(256) void k_integrate_hash(StepArgs A)
 }
 
 // --------------------------------------------------------- bucket ranges + sorted copies
-template <class T>
-__global__ __launch_bounds__(256) void k_cells_gather(StepArgs A)
+// Internal linkage: each translation unit that includes this header gets its own copy;
+// only the FP64 unit launches it.
+static __global__ __launch_bounds__(256) void k_cells_gather(StepArgs A)
 {
-    using V4T = typename Vec4Of<T>::type;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= A.nsorted) return;
     const uint32_t key = A.skeys[j];
@@ -123,13 +125,16 @@ __global__ __launch_bounds__(256) void k_cells_gather(StepArgs A)
     if (k < A.np) {
         const double4 p = reinterpret_cast<const double4*>(A.pos)[k];
         A.spos[j] = p;
+        const Grid& gr = A.scenes[A.scene_of ? A.scene_of[k] : 0u].grid;
+        A.sposf[j] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), (float)p.w);
         const V3<double> v = ld3(A.vel + 3 * k);
         const V3<double> w = omega_of(ld4(A.ep + 4 * k), ld4(A.ev + 4 * k));
-        V4T vm, om;
-        vm.x = (T)v.x; vm.y = (T)v.y; vm.z = (T)v.z; vm.w = (T)A.mass[k];
-        om.x = (T)w.x; om.y = (T)w.y; om.z = (T)w.z; om.w = (T)0;
-        static_cast<V4T*>(A.svm)[j] = vm;
-        static_cast<V4T*>(A.som)[j] = om;
+        A.svm[j] = make_double4(v.x, v.y, v.z, A.mass[k]);
+        A.som[j] = make_double4(w.x, w.y, w.z, 0.0);
+    } else {
+        // mesh-proxy slot: infinitely far, never a sphere-sphere candidate
+        A.spos[j] = make_double4(__builtin_huge_val(), __builtin_huge_val(), __builtin_huge_val(), 0.0);
+        A.sposf[j] = make_float4(__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf(), 0.0f);
     }
 }
 
@@ -137,7 +142,6 @@ __global__ __launch_bounds__(256) void k_cells_gather(StepArgs A)
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide(StepArgs A)
 {
-    using V4T = typename Vec4Of<T>::type;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n_pp = 0, n_wall = 0;
     if (j < A.nsorted) {
@@ -146,10 +150,10 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
             const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
             const double4 ip4 = A.spos[j];
             const V4<double> ipos = {ip4.x, ip4.y, ip4.z, ip4.w};
-            const V4T ivm = static_cast<const V4T*>(A.svm)[j];
-            const V4T iow = static_cast<const V4T*>(A.som)[j];
-            const V3<T> ivel = mk3(ivm.x, ivm.y, ivm.z);
-            const V3<T> iom = mk3(iow.x, iow.y, iow.z);
+            const double4 ivm = A.svm[j];
+            const double4 iow = A.som[j];
+            const V3<double> ivel = mk3(ivm.x, ivm.y, ivm.z);
+            const V3<double> iom = mk3(iow.x, iow.y, iow.z);
             const double im = A.mass[i];
             uint32_t err = 0;
 
@@ -165,52 +169,82 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
                 V3<T> Ft = mk3(T(0), T(0), T(0));
                 uint32_t n_new = 0, n_seen = 0;
                 const double reach = S.cc.coh_reach;
-                for (int z = -1; z <= 1; z++)
-                    for (int y = -1; y <= 1; y++)
-                        for (int x = -1; x <= 1; x++) {
-                            const uint2 rg = A.cell_range[grid_bucket(S.grid, gx + x, gy + y, gz + z)];
-                            if (rg.x == kInvalid) continue;
-                            for (uint32_t m = rg.x; m < rg.y; m++) {
-                                if (m == j) continue;
-                                const uint32_t k = A.svals[m];
-                                if (k >= A.np) continue;
-                                const double4 q = A.spos[m];
-                                const V4<double> jpos = {q.x, q.y, q.z, q.w};
-                                const PairGeom g = pair_geom(ipos, jpos);
-                                double sd0 = 0.0, sd1 = 0.0;
-                                bool store = false;
-                                if (g.cdist > 0) {
-                                    for (uint32_t s = 0; s < old_count; s++)
-                                        if (A.pp_id_in[hb + s] == k) {
-                                            const double2 h = A.pp_tsd_in[hb + s];
-                                            sd0 = h.x; sd1 = h.y;
-                                            break;
-                                        }
-                                    const V4T jvm = static_cast<const V4T*>(A.svm)[m];
-                                    const V4T jow = static_cast<const V4T*>(A.som)[m];
-                                    if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
-                                    sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, ipos, ivel, iom, im, jpos,
-                                                                      mk3(jvm.x, jvm.y, jvm.z), mk3(jow.x, jow.y, jow.z),
-                                                                      (double)jvm.w, g, sd0, sd1, Ft, pp);
-                                    store = true;
-                                } else if (reach > 0.0 && -g.cdist < reach) {
-                                    const double jr = jpos.w, ir = ipos.w;
-                                    const double coh_s = cohesion_limit_depth(S.cc, jr ? (ir * jr) / (ir + jr) : ir);
-                                    if (fabs(g.cdist) < fabs(coh_s)) {
-                                        sphere_cohesion<T>(S.cc, ir, jr, g, pp);
-                                        store = true;
-                                    }
+                // FP32 prefilter: a candidate is skipped only when its FP32 distance exceeds
+                // r_i + r_j + reach by more than the rounding of the FP32 coordinates
+                // (|x| 2^-24 per coordinate and operand, bounded by mg) and of the FP32
+                // radius sum (factor 1 + 1e-6), so every pair the exact FP64 test below
+                // could accept reaches it. Mesh-proxy slots sit at infinity.
+                const float4 fi = A.sposf[j];
+                const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
+                const float reachf = (float)reach;
+                // candidates of one run of sorted slots [m0, m1)
+                auto visit = [&](uint32_t m0, uint32_t m1) {
+                    for (uint32_t m = m0; m < m1; m++) {
+                        if (m == j) continue;
+                        {
+                            const float4 f = A.sposf[m];
+                            const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
+                            const float lim = (fi.w + f.w + reachf) * 1.000001f + mg;
+                            if (dx * dx + dy * dy + dz * dz > lim * lim) continue;
+                        }
+                        const double4 q = A.spos[m];
+                        const uint32_t k = A.svals[m];
+                        const V4<double> jpos = {q.x, q.y, q.z, q.w};
+                        const PairGeom g = pair_geom(ipos, jpos);
+                        double sd0 = 0.0, sd1 = 0.0;
+                        bool store = false;
+                        if (g.cdist > 0) {
+                            for (uint32_t s = 0; s < old_count; s++)
+                                if (A.pp_id_in[hb + s] == k) {
+                                    const double2 h = A.pp_tsd_in[hb + s];
+                                    sd0 = h.x; sd1 = h.y;
+                                    break;
                                 }
-                                if (store) {
-                                    n_seen++;
-                                    if (n_new < cap) {
-                                        A.pp_id_out[hb + n_new] = k;
-                                        A.pp_tsd_out[hb + n_new] = make_double2(sd0, sd1);
-                                        n_new++;
-                                    }
-                                }
+                            const double4 jvm = A.svm[m];
+                            const double4 jow = A.som[m];
+                            if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
+                            sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, ipos, ivel, iom, im, jpos,
+                                                              mk3(jvm.x, jvm.y, jvm.z), mk3(jow.x, jow.y, jow.z),
+                                                              jvm.w, g, sd0, sd1, Ft, pp);
+                            store = true;
+                        } else if (reach > 0.0 && -g.cdist < reach) {
+                            const double jr = jpos.w, ir = ipos.w;
+                            const double coh_s = cohesion_limit_depth(S.cc, jr ? (ir * jr) / (ir + jr) : ir);
+                            if (fabs(g.cdist) < fabs(coh_s)) {
+                                sphere_cohesion(S.cc, ir, jr, g, pp);
+                                store = true;
                             }
                         }
+                        if (store) {
+                            n_seen++;
+                            if (n_new < cap) {
+                                A.pp_id_out[hb + n_new] = k;
+                                A.pp_tsd_out[hb + n_new] = make_double2(sd0, sd1);
+                                n_new++;
+                            }
+                        }
+                    }
+                };
+                // 27 neighbour buckets in the reference's (z, y, x) order. The three buckets
+                // of an x-row have consecutive keys unless the row wraps, so their spheres
+                // form one run of sorted slots, visited in the same order.
+                const bool row_contiguous = (((uint32_t)(gx - 1) & S.grid.mask_x) + 2u) == ((uint32_t)(gx + 1) & S.grid.mask_x);
+                for (int z = -1; z <= 1; z++)
+                    for (int y = -1; y <= 1; y++) {
+                        const uint32_t b0 = grid_bucket(S.grid, gx - 1, gy + y, gz + z);
+                        if (row_contiguous) {
+                            const uint2 r0 = A.cell_range[b0], r1 = A.cell_range[b0 + 1], r2 = A.cell_range[b0 + 2];
+                            const uint32_t s = r0.x != kInvalid ? r0.x : (r1.x != kInvalid ? r1.x : r2.x);
+                            if (s == kInvalid) continue;
+                            const uint32_t e = r2.x != kInvalid ? r2.y : (r1.x != kInvalid ? r1.y : r0.y);
+                            visit(s, e);
+                        } else {
+                            for (int x = -1; x <= 1; x++) {
+                                const uint2 rg = A.cell_range[grid_bucket(S.grid, gx + x, gy + y, gz + z)];
+                                if (rg.x != kInvalid) visit(rg.x, rg.y);
+                            }
+                        }
+                    }
                 A.pp_count_out[i] = n_new;
                 if (n_seen > cap) err |= XDEM_ERR_P2P_OVERFLOW;
                 n_pp = n_new;
@@ -347,18 +381,20 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
     st4(A.ev + 4 * i, av);
     st3(A.acc + 3 * i, a);
     st4(A.ea + 4 * i, mea);
-    st3(A.force + 3 * i, mk3(0.0, 0.0, 0.0));
-    st3(A.moment + 3 * i, mk3(0.0, 0.0, 0.0));
+    if (A.zero_force) {
+        st3(A.force + 3 * i, mk3(0.0, 0.0, 0.0));
+        st3(A.moment + 3 * i, mk3(0.0, 0.0, 0.0));
+    }
 }
 
 // Kernel entry points of one precision (defined in xdem_step_f64.hip / xdem_step_mixed.hip).
-struct StepLaunch {
-    hipError_t (*gather)(const StepArgs&, hipStream_t);
-    hipError_t (*collide)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, hipStream_t);
-};
-StepLaunch step_launch_f64();
-StepLaunch step_launch_mixed();
+// Contact kernel of one precision (xdem_step_f64.hip / xdem_step_mixed.hip); the other
+// kernels are FP64 only and live in xdem_step_f64.hip.
+using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, hipStream_t);
+CollideLaunch collide_launch_f64();
+CollideLaunch collide_launch_mixed();
 hipError_t launch_integrate_hash(const StepArgs&, bool multi, hipStream_t);
+hipError_t launch_gather(const StepArgs&, hipStream_t);
 hipError_t launch_finish(const StepArgs&, bool multi, hipStream_t);
 
 }  // namespace xdem

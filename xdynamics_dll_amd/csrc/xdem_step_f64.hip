@@ -9,13 +9,6 @@ namespace {
 constexpr unsigned kBlock = 256;
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
-hipError_t gather_f64(const StepArgs& A, hipStream_t s)
-{
-    if (A.nsorted == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cells_gather<double>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-    return hipGetLastError();
-}
-
 template <bool MULTI, bool FUSE>
 void collide_models(const StepArgs& A, int model, bool gpu_law, hipStream_t s)
 {
@@ -44,13 +37,20 @@ hipError_t collide_f64(const StepArgs& A, bool multi, bool fuse, int model, bool
 
 }  // namespace
 
-StepLaunch step_launch_f64() { return {gather_f64, collide_f64}; }
+CollideLaunch collide_launch_f64() { return collide_f64; }
 
 hipError_t launch_integrate_hash(const StepArgs& A, bool multi, hipStream_t s)
 {
     if (A.np == 0) return hipSuccess;
     if (multi) hipLaunchKernelGGL(k_integrate_hash<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
     else hipLaunchKernelGGL(k_integrate_hash<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const StepArgs& A, hipStream_t s)
+{
+    if (A.nsorted == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cells_gather, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 

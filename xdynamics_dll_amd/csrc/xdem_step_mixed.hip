@@ -9,13 +9,6 @@ namespace {
 constexpr unsigned kBlock = 256;
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
-hipError_t gather_mixed(const StepArgs& A, hipStream_t s)
-{
-    if (A.nsorted == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cells_gather<float>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-    return hipGetLastError();
-}
-
 template <bool MULTI, bool FUSE>
 void collide_models(const StepArgs& A, int model, bool gpu_law, hipStream_t s)
 {
@@ -44,6 +37,6 @@ hipError_t collide_mixed(const StepArgs& A, bool multi, bool fuse, int model, bo
 
 }  // namespace
 
-StepLaunch step_launch_mixed() { return {gather_mixed, collide_mixed}; }
+CollideLaunch collide_launch_mixed() { return collide_mixed; }
 
 }  // namespace xdem

@@ -17,7 +17,7 @@ from .scene import PARITY_QUIRKS, SceneData
 
 class DemContext:
     def __init__(self, scenes, precision: str = "fp64", p2p_capacity: int = 12, quirks: dict | None = None,
-                 grid_dim_max: int = 512, device: int = 0):
+                 grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -29,6 +29,7 @@ class DemContext:
         q.update(quirks or {})
         cfg.quirks = _lib.Quirks(q["gpu_force_law"], q["velocity_not_updated"], q["stale_ft"], q["flat_inertia"])
         cfg.grid_dim_max = grid_dim_max
+        cfg.fuse_velocity = 1 if fuse_velocity else 0
         cfg.device = device
         self.precision = precision
         self.capacity = p2p_capacity
@@ -119,9 +120,9 @@ class DemContext:
         _lib.check(self.L.xdem_ctx_set_timing(self._h, 1 if on else 0), "xdem_ctx_set_timing")
 
     def timing(self):
-        out = (C.c_double * 3)()
+        out = (C.c_double * 4)()
         _lib.check(self.L.xdem_ctx_timing(self._h, out), "xdem_ctx_timing")
-        return {"contact_ms": out[0], "grid_ms": out[1], "steps": int(out[2])}
+        return {"contact_ms": out[0], "grid_ms": out[1], "steps": int(out[2]), "finish_ms": out[3]}
 
     def close(self) -> None:
         if self._h:
