@@ -133,16 +133,54 @@ def cpu_baseline(scene, capacity, budget_s=12.0):
                       f"FP64 C restatement, OpenMP over spheres"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_collide_dam_break_1m.json")
+
+
+def measured_traffic(np_, fused):
+    """HBM bytes per k_collide launch from the committed rocprofv3 --pmc summary (FETCH_SIZE
+    and WRITE_SIZE per dispatch, each scaled by the ratio the same counter shows on a plain
+    device copy of known size), or None when no summary matches this workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("fused" if fused else "split")
+    if not k or t.get("np") != np_:
+        return None
+    return (k["fetch_kib"] * t["copy_scale_fetch"] + k["write_kib"] * t["copy_scale_write"]) * 1024.0
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of mib MiB, repeated 5 times: the calibration kernel
+    for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4,
+                      "ms": dt * 1e3, "gbps": 2 * n * 4 / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--np", type=int, default=0, help="sphere count override (default 1,000,000)")
     ap.add_argument("--capacity", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fuse", action="store_true", help="rolling + velocity update inside the contact kernel")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described "
@@ -153,6 +191,9 @@ def main():
     from xdynamics_dll_amd import SceneData
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     t_build = time.perf_counter()
     scene = SceneData.build(2, SEED, np_override=args.np)
     t_build = time.perf_counter() - t_build
@@ -184,7 +225,7 @@ def main():
                                "5 walls, mixed FP32 contact arithmetic (FP64 positions/history/sums)",
                    "np": n, "seed": SEED, "p2p_capacity": args.capacity, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(n, args.fuse),
                      "kernel": "k_collide (" + ("contacts + rolling + velocity update" if args.fuse else "contacts") + ")",
                      "kernel_ms": k_ms, "bytes_per_sphere": contact_kernel_bytes(cbar, args.fuse)},
         "step_gbps": step_gbps,

@@ -61,3 +61,42 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(xd._lib, "LIB_PATH", "/nonexistent/libxdem.so")
     with pytest.raises(ImportError):
         xd._lib.lib()
+
+
+SHIM_CALLER = r"""
+#include "xParallelDEM_decl.hpp"
+#include <cstdio>
+// Takes the address of every reference-named launcher, as a caller built against
+// xParallelDEM_decl.cuh would reference them; nothing is called, so no GPU is needed.
+int main()
+{
+    void* fns[] = {(void*)&setDEMSymbolicParameter, (void*)&vv_update_position, (void*)&vv_update_velocity,
+                   (void*)&cu_calculateHashAndIndex, (void*)&cu_calculateHashAndIndexForPolygonSphere,
+                   (void*)&cu_reorderDataAndFindCellStart, (void*)&cu_calculate_p2p,
+                   (void*)&cu_plane_contact_force, (void*)&cu_particle_polygonObject_collision,
+                   (void*)&cu_decide_rolling_friction_moment, (void*)&reduction, (void*)&cu_update_meshObjectData};
+    static_assert(sizeof(device_dem_parameters) == 144, "device_dem_parameters");
+    static_assert(sizeof(device_contact_property) == 104, "device_contact_property");
+    static_assert(sizeof(device_plane_info) == 240, "device_plane_info");
+    static_assert(sizeof(device_triangle_info) == 80, "device_triangle_info");
+    std::printf("%d\n", (int)(sizeof(fns) / sizeof(fns[0])));
+    return xdem_shim_status();
+}
+"""
+
+
+def test_reference_named_shim_compiles_and_links(tmp_path):
+    """include/xParallelDEM_decl.hpp: the reference's launcher names, forwarded to the C ABI."""
+    import shutil
+    import subprocess
+
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "caller.cpp"
+    src.write_text(SHIM_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(xd._lib.LIB_PATH)
+    cmd = ["g++", "-std=c++17", "-O1", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe),
+           f"-L{libdir}", "-lxdem", f"-Wl,-rpath,{libdir}", "-Wl,--allow-shlib-undefined"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

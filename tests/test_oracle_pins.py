@@ -1,0 +1,482 @@
+"""Pins of the oracle (the FP64 C restatement under oracle/) by checks that do not restate its
+formulas: brute force, the reference's own particle files, physical identities the force
+laws must satisfy, independent geometry, and conservation laws. Each check is chosen so that
+a plausible slip in the restatement (a dropped term, a swapped material index, a wrong sign
+or factor) fails it. CPU only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy.optimize import minimize_scalar
+from scipy.spatial import cKDTree
+
+import xdem_ref
+from scenes import block_scene, compressed_lattice, sphere_drop_scene
+from xdynamics_dll_amd import SceneData, _lib, contact_property, make_plane
+
+L = xdem_ref.lib()
+
+
+def f64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def ptr(a):
+    return a.ctypes.data
+
+
+def effective_modulus(Ei, Ej, nui, nuj):
+    return 1.0 / ((1.0 - nui ** 2) / Ei + (1.0 - nuj ** 2) / Ej)
+
+
+def effective_shear_modulus(Ei, Ej, nui, nuj):
+    """Mindlin: 1/G* = (2 - nu_i)/G_i + (2 - nu_j)/G_j with G = E / (2 (1 + nu))."""
+    Gi, Gj = Ei / (2 * (1 + nui)), Ej / (2 * (1 + nuj))
+    return 1.0 / ((2 - nui) / Gi + (2 - nuj) / Gj)
+
+
+def params(model=1, dt=1e-5):
+    s = compressed_lattice(n_side=2, walls=False)
+    p = s.dem_parameters()
+    p.contact_model = model
+    p.dt = dt
+    return p
+
+
+def quirks(gpu_law=0):
+    return xdem_ref.quirks_struct(dict(gpu_force_law=gpu_law))
+
+
+def pair_force(p, q, cp, ri, rj, mi, mj, cdist, dv, unit, sd=(0.0, 0.0), ft=(0.0, 0.0, 0.0)):
+    dv, unit = f64(dv), f64(unit)
+    sd, ft, fn = f64(sd).copy(), f64(ft).copy(), np.zeros(3)
+    L.ref_pair_force(C.byref(p), C.byref(q), C.byref(cp), ri, rj, mi, mj, cdist, ptr(dv), ptr(unit), ptr(sd),
+                     ptr(ft), ptr(fn))
+    return fn, ft, sd
+
+
+# A material pair with different moduli and Poisson ratios on the two sides, so that a
+# swapped index in the effective moduli changes the result.
+EI, EJ, NUI, NUJ, REST, FRIC = 3.0e7, 8.0e7, 0.21, 0.34, 0.35, 0.45
+RI, RJ, MI, MJ = 1.3e-3, 0.8e-3, 2.1e-5, 0.6e-5
+RSTAR, MSTAR = RI * RJ / (RI + RJ), MI * MJ / (MI + MJ)
+ESTAR = effective_modulus(EI, EJ, NUI, NUJ)
+GSTAR = effective_shear_modulus(EI, EJ, NUI, NUJ)
+UNIT = np.array([0.6, 0.0, 0.8])
+TANG = np.array([0.0, 1.0, 0.0])
+
+
+def cp_pair(coh=0.0, sratio=0.0, fric=FRIC):
+    return contact_property(EI, EJ, NUI, NUJ, REST, fric, 0.0, coh, sratio)
+
+
+# ------------------------------------------------------------------ grid, sort, pair sets
+def lattice_compressed():
+    """The 10,000-sphere jittered lattice (2.2 mm pitch, r = 1 mm, +-0.05 mm jitter) pressed
+    to 0.91 of its pitch, so that most but not all lattice neighbours overlap."""
+    s = SceneData.build(1, 1001)
+    pos = s.pos.copy()
+    c = pos[:, :3].min(0)
+    pos[:, :3] = c + (pos[:, :3] - c) * 0.91
+    return SceneData.from_positions(pos, 2500.0, s.cp_pp, walls=(), dt=s.dt)
+
+
+SCENES = {"sphere_drop": sphere_drop_scene, "block": block_scene, "lattice_cfg1": lattice_compressed}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_hash_sort_and_cell_bounds(name):
+    """impl:54-72, 290-360: 128^3 hash, stable sort by (hash, id), first/last+1 per cell."""
+    s = SCENES[name]()
+    ref = xdem_ref.RefSim(s)
+    ref.grid()
+    n = s.np
+    g = np.floor((s.pos[:, :3] - np.asarray(s.world_origin)) / s.cell_size).astype(np.int64)
+    h = (((g[:, 2] & 127) * 128 + (g[:, 1] & 127)) * 128 + (g[:, 0] & 127)).astype(np.uint32)
+    order = np.argsort(h, kind="stable")
+    assert np.array_equal(ref.hash, h[order])
+    assert np.array_equal(ref.index, order.astype(np.uint32))
+    assert np.array_equal(ref.sorted_index, order.astype(np.uint32))
+    counts = np.bincount(h, minlength=ref.ncell)
+    occupied = np.nonzero(counts)[0]
+    first = np.searchsorted(h[order], occupied, side="left")
+    assert np.array_equal(ref.cstart[occupied], first)
+    assert np.array_equal(ref.cend[occupied], first + counts[occupied])
+    assert np.all(ref.cstart[counts == 0] == 0xFFFFFFFF)
+    assert n == counts.sum()
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_pair_sets_equal_brute_force(name):
+    """impl:613-733 stores every pair with r_i + r_j - |p_j - p_i| > 0 (no cohesion here);
+    the brute force uses a k-d tree for candidates and the exact predicate."""
+    s = SCENES[name]()
+    ref = xdem_ref.RefSim(s, capacity=16)
+    ref.collide()
+    assert ref.overflow["p2p"] == 0
+    pos = ref.pos
+    tree = cKDTree(pos[:, :3])
+    cand = tree.query_pairs(2.0 * pos[:, 3].max() * (1 + 1e-9), output_type="ndarray")
+    i, j = cand[:, 0], cand[:, 1]
+    d = pos[j, :3] - pos[i, :3]
+    dist = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+    cdist = (pos[i, 3] + pos[j, 3]) - dist
+    sure = cdist > 1e-15
+    unsure = np.abs(cdist) <= 1e-15
+    expect = [set() for _ in range(s.np)]
+    for a, b in zip(i[sure], j[sure]):
+        expect[a].add(int(b))
+        expect[b].add(int(a))
+    skip = set(map(tuple, cand[unsure]))
+    got = [set(map(int, ref.pp_id[k, : ref.pp_count[k]])) for k in range(s.np)]
+    for k in range(s.np):
+        extra = {m for m in got[k] ^ expect[k] if (min(k, m), max(k, m)) not in skip}
+        assert not extra, (k, extra)
+    assert sum(len(e) for e in expect) > s.np // 2   # the fixture really is a contact network
+
+
+# ------------------------------------------------------------------------ force laws
+@pytest.mark.parametrize("model", [1, 0])
+def test_hertz_normal_force(model):
+    """F_n = -(4/3) E* sqrt(R*) delta^1.5 along the unit vector, E* from both materials."""
+    p, q = params(model), quirks()
+    for delta in (1e-7, 3e-6, 2e-5):
+        fn, _, _ = pair_force(p, q, cp_pair(sratio=0.3), RI, RJ, MI, MJ, delta, [0, 0, 0], UNIT)
+        expect = -(4.0 / 3.0) * ESTAR * math.sqrt(RSTAR) * delta ** 1.5 * UNIT
+        assert np.allclose(fn, expect, rtol=1e-13, atol=0), (delta, fn, expect)
+
+
+@pytest.mark.parametrize("gpu_law", [0, 1])
+def test_normal_damping_is_the_restitution_damping_ratio(gpu_law):
+    """The damping coefficient beta must be the damping ratio whose linear spring-dashpot
+    restitution is e: e = exp(-pi beta / sqrt(1 - beta^2)). The CPU law scales it by the
+    critical damping 2 sqrt(m* k_n) (xContact.cpp:244), the kernels' law does not (impl:438)."""
+    p, q = params(1), quirks(gpu_law)
+    delta, v = 4e-6, 0.37
+    kn = (4.0 / 3.0) * ESTAR * math.sqrt(RSTAR)
+    f0, _, _ = pair_force(p, q, cp_pair(), RI, RJ, MI, MJ, delta, [0, 0, 0], UNIT)
+    f1, _, _ = pair_force(p, q, cp_pair(), RI, RJ, MI, MJ, delta, v * UNIT, UNIT)
+    fdn = float(np.dot(f1 - f0, UNIT))
+    beta = fdn / v if gpu_law else fdn / (2.0 * math.sqrt(MSTAR * kn) * v)
+    assert 0 < beta < 1
+    assert math.isclose(math.exp(-math.pi * beta / math.sqrt(1 - beta * beta)), REST, rel_tol=1e-9)
+
+
+def test_tangential_history_under_constant_slip_hertz_mindlin():
+    """With the slip rate s' constant from rest, ds_n = dt (2n - 1) s' and s'_prev = s'
+    (trapezoidal history, impl:443-447), and the CPU law's tangential force is
+    St ds + vs sqrt(St m*) s' with St = 8 G* sqrt(R* delta), vs = 2 sqrt(5/6) beta."""
+    dt, sdot, delta = 1e-5, 0.02, 5e-6
+    p, q = params(1, dt), quirks()
+    cp = cp_pair(fric=1e6)                    # no Coulomb cap
+    lne = math.log(REST)
+    beta = -lne / math.sqrt(lne * lne + math.pi * math.pi)
+    St = 8.0 * GSTAR * math.sqrt(RSTAR * delta)
+    vs = 2.0 * math.sqrt(5.0 / 6.0) * beta
+    sd = (0.0, 0.0)
+    for n in range(1, 8):
+        _, ft, sd = pair_force(p, q, cp, RI, RJ, MI, MJ, delta, sdot * TANG, UNIT, sd=sd)
+        assert math.isclose(sd[0], dt * (2 * n - 1) * sdot, rel_tol=1e-12)
+        assert sd[1] == sdot
+        expect = St * sd[0] + vs * math.sqrt(St * MSTAR) * sdot
+        assert np.allclose(ft, expect * TANG, rtol=1e-12, atol=0)
+
+
+def test_tangential_force_dhs_model():
+    """DHS (model 0): k_s = k_n sratio, c_s = beta sratio; first step ds = dt s'."""
+    dt, sdot, delta, ratio = 1e-5, 0.03, 5e-6, 0.7
+    p, q = params(0, dt), quirks()
+    kn = (4.0 / 3.0) * ESTAR * math.sqrt(RSTAR)
+    lne = math.log(REST)
+    beta = -lne / math.sqrt(lne * lne + math.pi * math.pi)
+    _, ft, sd = pair_force(p, q, cp_pair(sratio=ratio, fric=1e6), RI, RJ, MI, MJ, delta, sdot * TANG, UNIT)
+    expect = kn * ratio * (dt * sdot) + beta * ratio * sdot
+    assert np.allclose(ft, expect * TANG, rtol=1e-12, atol=0)
+
+
+def test_coulomb_cap():
+    """Once the spring force exceeds mu |F_n|, |F_t| = mu |F_n| along the slip."""
+    p, q = params(1), quirks()
+    fn, ft, _ = pair_force(p, q, cp_pair(), RI, RJ, MI, MJ, 5e-6, 0.1 * TANG, UNIT, sd=(1e-3, 0.1))
+    assert math.isclose(np.linalg.norm(ft), FRIC * np.linalg.norm(fn), rel_tol=1e-14)
+    assert np.allclose(ft / np.linalg.norm(ft), TANG, rtol=0, atol=1e-15)
+
+
+def test_stale_tangential_force_when_slip_direction_undefined():
+    """impl:443-451: with |e| = 0 the tangential force keeps the value carried in."""
+    p, q = params(1), quirks()
+    carried = (1e-4, -2e-4, 3e-4)
+    _, ft, sd = pair_force(p, q, cp_pair(), RI, RJ, MI, MJ, 5e-6, 0.2 * UNIT, UNIT, sd=(1e-6, 0.5), ft=carried)
+    assert np.array_equal(ft, carried)
+    assert np.array_equal(sd, [1e-6, 0.5])
+
+
+# ---------------------------------------------------------------------------- cohesion
+def test_jkr_force_satisfies_the_jkr_load_relation():
+    """JKR: an external load F and the contact radius a satisfy
+    F = 4 E* a^3 / (3 R*) - sqrt(8 pi w E* a^3). The oracle returns the adhesive term
+    cf = -sqrt(8 pi w E* a^3) for a given F; recover a^3 from it and check the relation."""
+    w = 0.05
+    cp = cp_pair(coh=w)
+    c0 = 3.0 * math.pi * w * RSTAR
+    for F in (-0.5 * c0, -0.3 * c0, 0.0, 0.7 * c0, 5 * c0, 40 * c0):
+        cf = L.ref_cohesion_force(RI, RJ, C.byref(cp), F)
+        a3 = cf * cf / (8.0 * math.pi * w * ESTAR)
+        assert cf < 0
+        assert math.isclose(4.0 * ESTAR * a3 / (3.0 * RSTAR) + cf, F, rel_tol=1e-9, abs_tol=1e-12 * c0), F
+
+
+def test_jkr_pull_off_force():
+    """Below the pull-off load -1.5 pi w R* (textbook JKR) the contact radius stays at its
+    pull-off value, so the load relation returns exactly the pull-off force."""
+    w = 0.05
+    cp = cp_pair(coh=w)
+    F_po = -1.5 * math.pi * w * RSTAR
+    for F in (F_po, 2 * F_po, 10 * F_po):
+        cf = L.ref_cohesion_force(RI, RJ, C.byref(cp), F)
+        a3 = cf * cf / (8.0 * math.pi * w * ESTAR)
+        assert math.isclose(4.0 * ESTAR * a3 / (3.0 * RSTAR) + cf, F_po, rel_tol=1e-9)
+
+
+def test_cohesion_separation_depth_is_the_fixed_grip_instability():
+    """The JKR approach delta(a) = a^2/R* - sqrt(2 pi w a / E*) has its minimum (separation
+    under fixed grips) at -3/4 (pi^2 w^2 R* / E*^2)^(1/3); found here by minimisation."""
+    for w in (0.01, 0.05, 0.3):
+        cp = cp_pair(coh=w)
+        got = L.ref_cohesion_limit_depth(RI, RJ, C.byref(cp))
+        f = lambda a: a * a / RSTAR - math.sqrt(2 * math.pi * w * a / ESTAR)
+        res = minimize_scalar(f, bounds=(1e-12, 1e-3), method="bounded", options={"xatol": 1e-16})
+        assert math.isclose(got, res.fun, rel_tol=1e-7), (w, got, res.fun)
+
+
+# ------------------------------------------------------------------------ wall geometry
+def _plane(Lx=0.1, Wy=0.05):
+    info, dbi = make_plane((0, 0, 0), (Lx, 0, 0), (Lx, Wy, 0), (0, Wy, 0), 0)
+    return info, Lx, Wy
+
+
+def _detect(info, x, r):
+    u = np.zeros(3)
+    c = L.ref_plane_detection(C.byref(info), ptr(f64(x)), r, ptr(u))
+    return c, u
+
+
+def test_plane_face_edge_and_corner_distances():
+    """impl:735-815 against analytic distances: face (one-sided normal -uw, Q7), edge (unit
+    from the sphere towards the edge line), corner (unit from the corner to the sphere, as
+    written in the reference), and no contact outside the reach."""
+    info, Lx, Wy = _plane()
+    r = 1e-3
+    # face: 0.4 mm above the face
+    c, u = _detect(info, (0.03, 0.02, 4e-4), r)
+    assert math.isclose(c, r - 4e-4, rel_tol=1e-12) and np.allclose(u, [0, 0, -1], atol=1e-15)
+    # edge y = 0: the centre 0.3 mm beyond the edge and 0.5 mm above the plane
+    x = np.array([0.05, -3e-4, 5e-4])
+    c, u = _detect(info, x, r)
+    d = math.hypot(3e-4, 5e-4)
+    assert math.isclose(c, r - d, rel_tol=1e-12)
+    assert np.allclose(u, np.array([0, 3e-4, -5e-4]) / d, atol=1e-13)
+    # edge y = Wy
+    c, u = _detect(info, (0.05, Wy + 2e-4, 6e-4), r)
+    assert math.isclose(c, r - math.hypot(2e-4, 6e-4), rel_tol=1e-12)
+    # edge x = Lx
+    c, u = _detect(info, (Lx + 1e-4, 0.02, 7e-4), r)
+    assert math.isclose(c, r - math.hypot(1e-4, 7e-4), rel_tol=1e-12)
+    # corner (Lx, Wy, 0)
+    x = np.array([Lx + 2e-4, Wy + 3e-4, 4e-4])
+    c, u = _detect(info, x, r)
+    X = x - np.array([Lx, Wy, 0])
+    assert math.isclose(c, r - np.linalg.norm(X), rel_tol=1e-12)
+    assert np.allclose(u, X / np.linalg.norm(X), atol=1e-13)
+    # out of reach beyond the corner, and beyond the edge x = 0 (the unreachable branch)
+    assert _detect(info, (Lx + 9e-4, Wy + 9e-4, 9e-4), r)[0] == 0.0
+    assert _detect(info, (-2e-4, 0.02, 3e-4), r)[0] == 0.0
+
+
+# ------------------------------------------------------------------- triangle geometry
+def closest_point_independent(a, b, c, p):
+    """Projection onto the triangle's plane if it lies inside, else the nearest of the
+    three clamped edge projections (not Ericson's region tests)."""
+    n = np.cross(b - a, c - a)
+    n = n / np.linalg.norm(n)
+    q = p - np.dot(p - a, n) * n
+    M = np.stack([b - a, c - a], axis=1)
+    lam, *_ = np.linalg.lstsq(M, q - a, rcond=None)
+    if lam[0] >= 0 and lam[1] >= 0 and lam.sum() <= 1:
+        return q
+    best, bd = None, np.inf
+    for s, e in ((a, b), (b, c), (c, a)):
+        t = np.clip(np.dot(p - s, e - s) / np.dot(e - s, e - s), 0.0, 1.0)
+        x = s + t * (e - s)
+        d = np.linalg.norm(p - x)
+        if d < bd:
+            best, bd = x, d
+    return best
+
+
+def test_closest_point_on_triangle():
+    rng = np.random.default_rng(11)
+    kinds = set()
+    for _ in range(3000):
+        a, b, c = rng.normal(size=(3, 3))
+        p = rng.normal(size=3) * 2.0
+        t = _lib.TriangleInfo()
+        for name, v in zip("PQR", (a, b, c)):
+            setattr(t, name, _lib.double3(*v))
+        out = np.zeros(3)
+        ct = L.ref_closest_pt_point_triangle(C.byref(t), ptr(f64(p)), ptr(out))
+        kinds.add(ct)
+        exp = closest_point_independent(a, b, c, p)
+        assert np.allclose(out, exp, rtol=0, atol=1e-9), (a, b, c, p, out, exp)
+    assert kinds == {0, 1, 2}
+
+
+def test_triangle_proxy_sphere_passes_through_the_vertices():
+    """xUtilityFunction.cpp:379-415: the proxy sphere's centre lies on the normal through
+    the circumcentre, so all three vertices are on the sphere, and its offset from the
+    circumcentre is ft times the radius (to the iteration's tolerance 1e-4)."""
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        P, Q, R = rng.normal(size=(3, 3))
+        out = np.zeros(4)
+        L.ref_fit_sphere_to_triangle(ptr(f64(P)), ptr(f64(Q)), ptr(f64(R)), 0.8, ptr(out))
+        c, r = out[:3], out[3]
+        for v in (P, Q, R):
+            assert math.isclose(np.linalg.norm(v - c), r, rel_tol=1e-8)
+        A = np.stack([Q - P, R - P])
+        rhs = 0.5 * np.array([np.dot(Q, Q) - np.dot(P, P), np.dot(R, R) - np.dot(P, P)])
+        n = np.cross(Q - P, R - P)
+        cc = np.linalg.solve(np.vstack([A, n]), np.append(rhs, np.dot(n, P)))   # circumcentre
+        off = c - cc
+        assert np.linalg.norm(np.cross(off, n)) <= 1e-7 * np.linalg.norm(off) * np.linalg.norm(n)
+        assert abs(np.linalg.norm(off) / r - 0.8) < 2e-4
+
+
+# ------------------------------------------------------------------- rigid-body rotation
+def G(e):
+    return np.array([[-e[1], e[0], e[3], -e[2]], [-e[2], -e[3], e[0], e[1]], [-e[3], e[2], -e[1], e[0]]])
+
+
+def test_uceom_satisfies_euler_equations():
+    """impl:74-133: the 4x4 solve must give Euler's equations in the body frame,
+    J w' + w x (J w) = n with w = 2 G(e) e', w' = 2 G(e) e'', and the unit-norm constraint
+    e . e'' = -e' . e'."""
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        e = rng.normal(size=4)
+        e /= np.linalg.norm(e)
+        d = rng.normal(size=4)
+        d -= np.dot(d, e) * e
+        J = rng.uniform(0.5, 2.0, size=3) * 1e-9
+        n = rng.normal(size=3) * 1e-9
+        out = np.zeros(4)
+        L.ref_calculate_uceom(ptr(f64(J)), ptr(f64(e)), ptr(f64(d)), ptr(f64(n)), ptr(out))
+        w = 2 * G(e) @ d
+        wdot = 2 * G(e) @ out
+        lhs = J * wdot + np.cross(w, J * w)
+        assert np.allclose(lhs, n, rtol=1e-9, atol=1e-9 * np.abs(n).max())
+        assert math.isclose(np.dot(e, out), -np.dot(d, d), rel_tol=1e-9, abs_tol=1e-12)
+
+
+def test_angular_velocity_is_the_global_rate():
+    """impl:34-42: w = 2 E(e) e' is the global angular velocity: A(e) (2 G(e) e') with
+    A = E G^T."""
+    rng = np.random.default_rng(4)
+    for _ in range(100):
+        e = rng.normal(size=4)
+        e /= np.linalg.norm(e)
+        d = rng.normal(size=4)
+        d -= np.dot(d, e) * e
+        E = np.array([[-e[1], e[0], -e[3], e[2]], [-e[2], e[3], e[0], -e[1]], [-e[3], -e[2], e[1], e[0]]])
+        A = E @ G(e).T
+        out = np.zeros(3)
+        L.ref_to_angular_velocity(ptr(f64(e)), ptr(f64(d)), ptr(out))
+        assert np.allclose(out, A @ (2 * G(e) @ d), rtol=1e-12, atol=1e-14)
+        assert np.allclose(A @ A.T, np.eye(3), atol=1e-14)
+
+
+def test_rolling_resistance_clamp():
+    """impl:2420-2454 with the reference's flat inertia off: the added moment is along
+    T = J dt w - Tmax, of magnitude min(Mr, |T|); sums are cleared after use."""
+    rng = np.random.default_rng(8)
+    n = 64
+    p = params(1, 1e-5)
+    q = xdem_ref.quirks_struct(dict(flat_inertia=0))
+    ep = rng.normal(size=(n, 4))
+    ep /= np.linalg.norm(ep, axis=1)[:, None]
+    ev = rng.normal(size=(n, 4)) * 10
+    ev -= (ev * ep).sum(1)[:, None] * ep
+    inertia = np.repeat(rng.uniform(1e-10, 1e-9, size=(n, 1)), 3, axis=1)
+    tmax = rng.normal(size=(n, 3)) * 1e-14
+    rres = rng.uniform(0, 3e-13, size=n)
+    moment = np.zeros((n, 3))
+    w = np.array([2 * (np.array([[-e[1], e[0], -e[3], e[2]], [-e[2], e[3], e[0], -e[1]],
+                                 [-e[3], -e[2], e[1], e[0]]]) @ d) for e, d in zip(ep, ev)])
+    T = inertia[:, :1] * p.dt * w - tmax
+    r0 = rres.copy()
+    L.ref_decide_rolling_friction_moment(C.byref(p), C.byref(q), ptr(tmax), ptr(rres), ptr(f64(inertia)),
+                                         ptr(ep), ptr(ev), ptr(moment), n)
+    mag = np.linalg.norm(moment, axis=1)
+    assert np.allclose(mag, np.minimum(r0, np.linalg.norm(T, axis=1)), rtol=1e-12)
+    assert np.allclose(np.cross(moment, T), 0.0, atol=1e-12 * np.abs(T).max() * np.abs(moment).max())
+    assert (np.einsum("ij,ij->i", moment, T) > 0).all()
+    assert np.all(tmax == 0) and np.all(rres == 0)
+    assert (r0 < np.linalg.norm(T, axis=1)).any() and (r0 > np.linalg.norm(T, axis=1)).any()
+
+
+# ------------------------------------------------------------------ Verlet, conservation
+@pytest.mark.parametrize("gdir", ["y", "z"])
+def test_free_fall(gdir):
+    """One sphere, no walls. With gravity along y the initial acceleration (0, g.y, 0)
+    (xDiscreteElementMethodSimulation.cpp:74-80) is g, so x_n = x0 + g (n dt)^2 / 2; along z
+    it starts at 0, so x_n = x0 + g dt^2 n (n - 1) / 2."""
+    g = (0.0, -9.81, 0.0) if gdir == "y" else (0.0, 0.0, -9.81)
+    pos = np.array([[0.1, 0.2, 0.3, 1e-3]])
+    cp = contact_property(1e7, 1e7, 0.3, 0.3, 0.5, 0.3, 0.0)
+    dt = 1e-4
+    s = SceneData.from_positions(pos, 2500.0, cp, walls=(), dt=dt, gravity=g)
+    ref = xdem_ref.RefSim(s)
+    n = 50
+    ref.step(n)
+    k = n * n if gdir == "y" else n * (n - 1)
+    expect = pos[0, :3] + 0.5 * np.asarray(g) * dt * dt * k
+    assert np.allclose(ref.pos[0, :3], expect, rtol=1e-13, atol=1e-15)
+    assert ref.pos[0, 3] == pos[0, 3]
+
+
+@pytest.mark.parametrize("coh", [0.0, 0.05])
+def test_newton_third_law(coh):
+    """Without rotation, every pair force enters its two spheres with opposite signs, so
+    the contact forces of an isolated cluster sum to zero (to rounding)."""
+    s = compressed_lattice(n_side=6, walls=False, coh=coh)
+    ref = xdem_ref.RefSim(s)
+    ref.collide()
+    assert ref.pp_count.sum() > 0
+    total = ref.force.sum(0)
+    assert np.abs(total).max() <= 1e-13 * np.abs(ref.force).sum()
+    assert np.abs(ref.force).max() > 0
+
+
+def test_wall_forces_balance_the_body_wrench():
+    """The body force of each wall is minus the sum of the wall's forces on the spheres."""
+    s = compressed_lattice()
+    ref = xdem_ref.RefSim(s)
+    bare = xdem_ref.RefSim(s)
+    bare.planes, bare.plane_dbi, bare.plane_cp = [], [], []
+    ref.collide()
+    bare.collide()
+    wall_force = (ref.force - bare.force).sum(0)
+    body = ref.wrench[:, :3].sum(0)
+    assert np.abs(wall_force).max() > 0
+    assert np.allclose(body, -wall_force, rtol=1e-12, atol=1e-12 * np.abs(wall_force).max())
+
+
+def test_reduction_is_an_fp64_sum():
+    rng = np.random.default_rng(9)
+    x = rng.normal(size=100003) * 10.0 ** rng.uniform(-5, 5, size=100003)
+    got = L.ref_reduction(ptr(x), x.size)
+    assert abs(got - math.fsum(x)) <= x.size * np.finfo(float).eps * np.abs(x).sum()
