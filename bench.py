@@ -44,11 +44,12 @@ def contact_kernel_bytes(cbar: float, fused: bool) -> float:
     return (336.0 if fused else 224.0) + 40.0 * cbar
 
 
-def run_gpu(scene, precision, steps, warmup, capacity, fuse=False):
+def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0):
     import torch
     from xdynamics_dll_amd import DemContext
 
-    ctx = DemContext(scene, precision=precision, p2p_capacity=capacity, fuse_velocity=fuse)
+    ctx = DemContext(scene, precision=precision, p2p_capacity=capacity, fuse_velocity=fuse,
+                     buckets_per_sphere=buckets_per_sphere)
     ctx.step(warmup)
     ctx.synchronize()
     torch.cuda.synchronize()
@@ -179,6 +180,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fuse", action="store_true", help="rolling + velocity update inside the contact kernel")
+    ap.add_argument("--buckets-per-sphere", type=int, default=0,
+                    help="bucket budget of the grid per sphere (0 = library default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -199,7 +202,7 @@ def main():
     t_build = time.perf_counter() - t_build
     n = scene.np
 
-    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, args.fuse)
+    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, args.fuse, args.buckets_per_sphere)
     value = n * args.steps / mixed["seconds"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
@@ -223,7 +226,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "dam-break column, 1,000,000 polydisperse spheres r~U[1,2] mm, dt=5e-6 s, "
                                "5 walls, mixed FP32 contact arithmetic (FP64 positions/history/sums)",
-                   "np": n, "seed": SEED, "p2p_capacity": args.capacity, "parallelism": "single GPU"},
+                   "np": n, "seed": SEED, "p2p_capacity": args.capacity, "parallelism": "single GPU",
+                   "nbucket": mixed["info"]["nbucket"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(n, args.fuse),
                      "kernel": "k_collide (" + ("contacts + rolling + velocity update" if args.fuse else "contacts") + ")",
@@ -237,7 +241,8 @@ def main():
         "scene_build_s": t_build,
     }
     if not args.no_fp64:
-        f64 = run_gpu(scene, "fp64", max(20, args.steps // 2), max(5, args.warmup // 2), args.capacity)
+        f64 = run_gpu(scene, "fp64", max(20, args.steps // 2), max(5, args.warmup // 2), args.capacity,
+                      args.fuse, args.buckets_per_sphere)
         f64_value = n * max(20, args.steps // 2) / f64["seconds"]
         out["fp64_value"] = f64_value
         out["mixed_over_fp64"] = value / f64_value

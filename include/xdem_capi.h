@@ -105,6 +105,9 @@ typedef struct xdem_ctx_config {
     int fuse_velocity;           /* 1: rolling + velocity update inside the contact kernel;
                                     0: separate kernel (fewer registers in the contact kernel) */
     int device;                  /* HIP device ordinal */
+    unsigned int buckets_per_sphere; /* bucket budget of a scene: the power of two at or above
+                                        this times its sphere count (0 = 4); the largest axes
+                                        are halved until the grid fits */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */
@@ -128,7 +131,7 @@ typedef struct xdem_ctx_config {
 typedef struct xdem_ctx_info {
     unsigned int nscene, np, nsorted, nplane_total;
     unsigned int nbucket;        /* buckets of all scene grids */
-    unsigned int key_bits;       /* radix-sort key width */
+    unsigned int key_bits;       /* bits of the largest bucket key */
     unsigned int p2p_capacity;
     unsigned long long device_bytes;
     unsigned long long steps_done;

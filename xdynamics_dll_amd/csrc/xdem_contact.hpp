@@ -118,6 +118,26 @@ struct WallSlots {
     uint32_t id[XDEM_MAX_P2PL_COUNT];
     double s0[XDEM_MAX_P2PL_COUNT], s1[XDEM_MAX_P2PL_COUNT];
     uint32_t count;
+    uint32_t dirty;              // a slot was written (the caller writes the slots back only then)
+    uint32_t loaded;             // slots read from memory (only done once a wall needs them)
+    const uint32_t* src_id;      // the sphere's slots in memory
+    const double* src_tsd;       // (ds, s_dot_prev) pairs
+
+    __device__ __forceinline__ void init(uint32_t cnt, const uint32_t* ids, const double* tsd)
+    {
+        count = cnt; dirty = 0; loaded = 0; src_id = ids; src_tsd = tsd;
+    }
+    __device__ __forceinline__ void load()
+    {
+        if (loaded) return;
+#pragma unroll
+        for (int s = 0; s < XDEM_MAX_P2PL_COUNT; s++) {
+            id[s] = src_id[s];
+            s0[s] = src_tsd[2 * s];
+            s1[s] = src_tsd[2 * s + 1];
+        }
+        loaded = 1;
+    }
 };
 
 template <class T, int MODEL, bool GPU_LAW>
@@ -145,6 +165,7 @@ __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_
     double res = 0.0;
     bool stored = false, overflow = false;
     if (cdist > 0) {
+        ws.load();
         for (uint32_t s = 0; s < XDEM_MAX_P2PL_COUNT; s++)
             if (ws.id[s] == k) { sd0 = ws.s0[s]; sd1 = ws.s1[s]; break; }
         const V3<double> dcpr = cpt - ip3;
@@ -177,7 +198,9 @@ __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_
     }
     if (stored) {
         if (n_new < XDEM_MAX_P2PL_COUNT) {
+            ws.load();
             ws.id[n_new] = k; ws.s0[n_new] = sd0; ws.s1[n_new] = sd1;
+            ws.dirty = 1;
             n_new++;
         } else {
             overflow = true;

@@ -1,6 +1,6 @@
 // xdem_ctx.hip -- the step layer of the C ABI: device state of one or many scenes and
-// the fused step (integrate+hash, radix sort, bucket ranges + sorted copies, contacts +
-// Verlet), optionally recorded in a HIP graph.
+// the fused step (position update + bucket key, bucket counting sort + sorted copies,
+// contacts + Verlet), optionally recorded in a HIP graph.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -76,9 +76,8 @@ struct xdem_ctx {
     uint32_t* scene_of = nullptr;
     SceneDev* d_scenes = nullptr;
     WallDev* d_walls = nullptr;
-    uint32_t *keys = nullptr, *vals = nullptr, *skeys = nullptr, *svals = nullptr, *ktmp = nullptr, *vtmp = nullptr;
-    void* sort_scratch = nullptr;
-    uint2* cell_range = nullptr;
+    uint32_t *keys = nullptr, *rank = nullptr, *svals = nullptr, *slot_tmp = nullptr;
+    uint32_t *bucket_count = nullptr, *bucket_start = nullptr, *scan_part = nullptr;
     double4* spos = nullptr;
     float4* sposf = nullptr;
     double4* svm = nullptr;
@@ -126,13 +125,14 @@ struct xdem_ctx {
         A.pos = pos; A.vel = vel; A.acc = acc; A.ep = ep; A.ev = ev; A.ea = ea;
         A.force = force; A.moment = moment; A.mass = mass; A.inertia = inertia; A.tmax = tmax; A.rres = rres;
         A.scene_of = scene_of; A.scenes = d_scenes; A.walls = d_walls;
-        A.keys = keys; A.vals = vals; A.skeys = skeys; A.svals = svals; A.cell_range = cell_range;
+        A.keys = keys; A.rank = rank; A.svals = svals; A.slot_tmp = slot_tmp;
+        A.bucket_count = bucket_count; A.bucket_start = bucket_start; A.scan_part = scan_part;
         A.spos = spos; A.sposf = sposf; A.svm = svm; A.som = som;
         A.pp_count_in = pp_count[cur]; A.pp_id_in = pp_id[cur]; A.pp_tsd_in = pp_tsd[cur];
         A.pp_count_out = pp_count[cur ^ 1]; A.pp_id_out = pp_id[cur ^ 1]; A.pp_tsd_out = pp_tsd[cur ^ 1];
         A.pl_count = pl_count; A.pl_id = pl_id; A.pl_tsd = pl_tsd;
         A.wrench = wrench_pp; A.err = err; A.counters = counters;
-        A.np = np; A.nsorted = np; A.cap = cap; A.nwslot = nwslot;
+        A.np = np; A.nsorted = np; A.cap = cap; A.nwslot = nwslot; A.nbucket = nbucket;
         A.stale_ft = cfg.quirks.stale_ft; A.flat_inertia = cfg.quirks.flat_inertia;
         A.keep_velocity = cfg.quirks.velocity_not_updated; A.tmax_persist = tmax_persist;
         A.zero_force = 1;
@@ -146,10 +146,7 @@ struct xdem_ctx {
         const StepArgs A = args();
         if (ev3 && (e = hipEventRecord(ev3[0], stream)) != hipSuccess) return e;
         if ((e = launch_integrate_hash(A, multi, stream)) != hipSuccess) return e;
-        if ((e = radix_sort_pairs(keys, vals, skeys, svals, ktmp, vtmp, np, (int)key_bits, sort_scratch, stream)) !=
-            hipSuccess)
-            return e;
-        if ((e = launch_gather(A, stream)) != hipSuccess) return e;
+        if ((e = launch_grid(A, stream)) != hipSuccess) return e;
         if (ev3 && (e = hipEventRecord(ev3[1], stream)) != hipSuccess) return e;
         if ((e = collide_fn(A, multi, fuse, model, cfg.quirks.gpu_force_law != 0, stream)) != hipSuccess) return e;
         if (ev3 && (e = hipEventRecord(ev3[2], stream)) != hipSuccess) return e;
@@ -274,6 +271,22 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
         d = std::max(4u, std::min(d, c->cfg.grid_dim_max));
         dims[a] = d;
     }
+    // Bucket budget: halve the largest axis (wrapping it, as the reference's 128^3 grid
+    // wraps) until the scene's grid has at most next_pow2(buckets_per_sphere * np) buckets.
+    // Every axis keeps at least 4 buckets, so the 27 neighbours of a cell stay 27 distinct
+    // buckets and the visiting order of real contacts does not change.
+    {
+        const unsigned long long per = c->cfg.buckets_per_sphere ? c->cfg.buckets_per_sphere : 4u;
+        unsigned long long budget = 64;
+        while (budget < per * (unsigned long long)std::max(1u, s->np)) budget <<= 1;
+        while ((unsigned long long)dims[0] * dims[1] * dims[2] > budget) {
+            int a = 0;
+            if (dims[1] > dims[a]) a = 1;
+            if (dims[2] > dims[a]) a = 2;
+            if (dims[a] <= 4) break;
+            dims[a] >>= 1;
+        }
+    }
     SceneDev& D = h.dev;
     std::memset(&D, 0, sizeof(D));
     D.dt = s->dt;
@@ -344,13 +357,13 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->d_scenes, sizeof(SceneDev) * c->scenes.size()));
     CTX_CHECK(c->alloc(&c->d_walls, sizeof(WallDev) * std::max<size_t>(1, c->walls.size())));
     CTX_CHECK(c->alloc(&c->keys, sizeof(uint32_t) * n));
-    CTX_CHECK(c->alloc(&c->vals, sizeof(uint32_t) * n));
-    CTX_CHECK(c->alloc(&c->skeys, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->rank, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->svals, sizeof(uint32_t) * n));
-    CTX_CHECK(c->alloc(&c->ktmp, sizeof(uint32_t) * n));
-    CTX_CHECK(c->alloc(&c->vtmp, sizeof(uint32_t) * n));
-    CTX_CHECK(c->alloc(&c->sort_scratch, radix_sort_scratch_bytes((uint32_t)n)));
-    CTX_CHECK(c->alloc(&c->cell_range, sizeof(uint2) * (size_t)c->nbucket));
+    CTX_CHECK(c->alloc(&c->slot_tmp, sizeof(uint32_t) * n));
+    const size_t nbt = bucket_table_size(c->nbucket);
+    CTX_CHECK(c->alloc(&c->bucket_count, sizeof(uint32_t) * nbt));
+    CTX_CHECK(c->alloc(&c->bucket_start, sizeof(uint32_t) * (nbt + 1)));
+    CTX_CHECK(c->alloc(&c->scan_part, sizeof(uint32_t) * (nbt / 4096 + 1)));
     CTX_CHECK(c->alloc(&c->spos, sizeof(double4) * n));
     CTX_CHECK(c->alloc(&c->sposf, sizeof(float4) * n));
     CTX_CHECK(c->alloc(&c->svm, vsz * n));
@@ -420,10 +433,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(hipMemsetAsync(c->err, 0, sizeof(uint32_t), s));
     CTX_CHECK(hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 2, s));
     if (c->nwslot) CTX_CHECK(hipMemsetAsync(c->wrench_out, 0, sizeof(double) * 6 * c->nwslot, s));
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->keys, kInvalid, n);
-    const size_t nb2 = 2 * (size_t)c->nbucket;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nb2 + 255) / 256)), dim3(256), 0, s,
-                       reinterpret_cast<uint32_t*>(c->cell_range), kInvalid, nb2);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nbt + 255) / 256)), dim3(256), 0, s, c->bucket_count, 0u, nbt);
     CTX_CHECK(hipGetLastError());
     CTX_CHECK(hipStreamSynchronize(s));
     c->committed = true;

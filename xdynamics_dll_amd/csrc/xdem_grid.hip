@@ -1,0 +1,224 @@
+// xdem_grid.hip -- position update and the bucket grid of the fused step (xdem_ctx path).
+//
+// The reference sorts (hash, index) pairs with a radix sort every step and then finds the
+// first and last sorted position of every cell (def:94-124; impl:317-360). Here the grid is
+// a counting sort over a dense bucket table:
+//   k_integrate_hash  Velocity-Verlet position/orientation update (impl:135-147), bucket key,
+//                     and an atomic count per bucket (the returned value is a provisional
+//                     rank inside the bucket)
+//   k_scan_reduce     \
+//   k_scan_parts       > exclusive prefix of the counts: start[b] = first sorted slot of
+//   k_scan_apply      /  bucket b, start[nbucket] = N; the counts are cleared on the way
+//   k_scatter         provisional slot start[key] + rank for every sphere
+//   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
+//                     smaller id, i.e. exactly the reference's (hash, index) order; writes
+//                     the sorted id and the sorted copies the contact kernel reads
+// The bucket of slot range [start[b], start[b+1]) needs no clearing between steps, and a
+// row of three consecutive buckets is the single range [start[b], start[b+3]).
+// Built with -ffp-contract=off: the position update is bit-identical to the reference's.
+#include "xdem_step.hpp"
+
+namespace xdem {
+namespace {
+
+constexpr unsigned kBlock = 256;
+constexpr unsigned kScanItems = 16;                   // per thread
+constexpr unsigned kScanTile = kBlock * kScanItems;   // 4096 buckets per block
+constexpr unsigned kPartsBlock = 1024;
+
+inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_integrate_hash(StepArgs A)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {  // stored-contact counters of this step
+        A.counters[0] = 0ull;
+        A.counters[1] = 0ull;
+    }
+    if (i >= A.np) return;
+    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    const V4<double> e = ld4(A.ep + 4 * i), d = ld4(A.ev + 4 * i), a = ld4(A.ea + 4 * i);
+    V4<double> ne;
+    ne.x = e.x + S.dt * d.x + S.half2dt * a.x;
+    ne.y = e.y + S.dt * d.y + S.half2dt * a.y;
+    ne.z = e.z + S.dt * d.z + S.half2dt * a.z;
+    ne.w = e.w + S.dt * d.w + S.half2dt * a.w;
+    const double l = sqrt(ne.x * ne.x + ne.y * ne.y + ne.z * ne.z + ne.w * ne.w);
+    ne.x = ne.x / l; ne.y = ne.y / l; ne.z = ne.z / l; ne.w = ne.w / l;
+    st4(A.ep + 4 * i, ne);
+    const V3<double> v = ld3(A.vel + 3 * i), ac = ld3(A.acc + 3 * i);
+    V4<double> p = ld4(A.pos + 4 * i);
+    p.x += S.dt * v.x + S.half2dt * ac.x;
+    p.y += S.dt * v.y + S.half2dt * ac.y;
+    p.z += S.dt * v.z + S.half2dt * ac.z;
+    st4(A.pos + 4 * i, p);
+    int gx, gy, gz;
+    grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
+    const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
+    A.keys[i] = key;
+    A.rank[i] = atomicAdd(&A.bucket_count[key], 1u);
+}
+
+// Sum of 16 consecutive buckets per thread, reduced per block of 4096 buckets.
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ count,
+                                                        uint32_t* __restrict__ part)
+{
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint4* c = reinterpret_cast<const uint4*>(count + (size_t)blockIdx.x * kScanTile) + threadIdx.x * 4;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint4 v = c[k];
+        s += v.x + v.y + v.z + v.w;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (unsigned w = 0; w < kBlock / 64; w++) t += wsum[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+// Inclusive scan of one value per lane across a wave64.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of the block sums by one block; writes the grand total to *total.
+__global__ __launch_bounds__(kPartsBlock) void k_scan_parts(uint32_t* __restrict__ part, uint32_t nparts,
+                                                            uint32_t* __restrict__ total)
+{
+    __shared__ uint32_t wsum[kPartsBlock / 64];
+    const uint32_t per = (nparts + kPartsBlock - 1) / kPartsBlock;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < per; k++)
+        if (b0 + k < nparts) s += part[b0 + k];
+    const uint32_t inc = wave_inclusive_scan(s);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) wsum[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (unsigned k = 0; k < kPartsBlock / 64; k++) {
+            const uint32_t t = wsum[k];
+            wsum[k] = run;
+            run += t;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    uint32_t run = wsum[w] + inc - s;
+    for (uint32_t k = 0; k < per; k++)
+        if (b0 + k < nparts) {
+            const uint32_t t = part[b0 + k];
+            part[b0 + k] = run;
+            run += t;
+        }
+}
+
+// start[b] = part[tile] + exclusive prefix inside the tile; the counts are zeroed.
+__global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ count,
+                                                       const uint32_t* __restrict__ part,
+                                                       uint32_t* __restrict__ start)
+{
+    __shared__ uint32_t wsum[kBlock / 64];
+    const size_t base = (size_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint4* c = reinterpret_cast<uint4*>(count + base);
+    uint32_t v[kScanItems];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint4 q = c[k];
+        v[4 * k + 0] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        c[k] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < (int)kScanItems; k++) {
+        const uint32_t t = v[k];
+        v[k] = s;
+        s += t;
+    }
+    const uint32_t inc = wave_inclusive_scan(s);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t off = part[blockIdx.x] + inc - s;
+    for (uint32_t k = 0; k < w; k++) off += wsum[k];
+    uint4* o = reinterpret_cast<uint4*>(start + base);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        o[k] = make_uint4(off + v[4 * k], off + v[4 * k + 1], off + v[4 * k + 2], off + v[4 * k + 3]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(StepArgs A)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.np) return;
+    A.slot_tmp[A.bucket_start[A.keys[i]] + A.rank[i]] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.nsorted) return;
+    const uint32_t k = A.slot_tmp[j];
+    const uint32_t key = A.keys[k];
+    const uint32_t s = A.bucket_start[key], e = A.bucket_start[key + 1];
+    uint32_t r = 0;
+    for (uint32_t m = s; m < e; m++) r += A.slot_tmp[m] < k ? 1u : 0u;
+    const uint32_t f = s + r;
+    A.svals[f] = k;
+    const double4 p = reinterpret_cast<const double4*>(A.pos)[k];
+    A.spos[f] = p;
+    const Grid& gr = A.scenes[A.scene_of ? A.scene_of[k] : 0u].grid;
+    A.sposf[f] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), (float)p.w);
+    const V3<double> v = ld3(A.vel + 3 * k);
+    const V3<double> w = omega_of(ld4(A.ep + 4 * k), ld4(A.ev + 4 * k));
+    A.svm[f] = make_double4(v.x, v.y, v.z, A.mass[k]);
+    A.som[f] = make_double4(w.x, w.y, w.z, 0.0);
+}
+
+}  // namespace
+
+uint32_t bucket_table_size(uint32_t nbucket)
+{
+    return (nbucket + kScanTile - 1) / kScanTile * kScanTile;
+}
+
+hipError_t launch_integrate_hash(const StepArgs& A, bool multi, hipStream_t s)
+{
+    if (A.np == 0) return hipSuccess;
+    if (multi) hipLaunchKernelGGL(k_integrate_hash<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    else hipLaunchKernelGGL(k_integrate_hash<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid(const StepArgs& A, hipStream_t s)
+{
+    const uint32_t nb = bucket_table_size(A.nbucket);
+    const uint32_t ntile = nb / kScanTile;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kPartsBlock), 0, s, A.scan_part, ntile, A.bucket_start + nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part,
+                       A.bucket_start);
+    if (A.np) {
+        hipLaunchKernelGGL(k_scatter, grid_for(A.np), dim3(kBlock), 0, s, A);
+        hipLaunchKernelGGL(k_order_gather, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace xdem

@@ -271,20 +271,13 @@ __global__ __launch_bounds__(kThreads) void k_plane(LaunchPrm L, const xdem_plan
     for (int q = 0; q < 15; q++) b[q] = dbi[q];
     const unsigned sid = id * CAP;
     WallSlots ws;
-#pragma unroll
-    for (unsigned s = 0; s < CAP; s++) {
-        ws.id[s] = pair_id[sid + s];
-        ws.s0[s] = tsd[2 * (sid + s)];
-        ws.s1[s] = tsd[2 * (sid + s) + 1];
-    }
-    ws.count = pair_count[id];
+    ws.init(pair_count[id], pair_id + sid, tsd + 2 * sid);
+    ws.load();
     const V4<double> ipos = ld4(pos + 4 * id);
     const V3<double> ivel = ld3(vel + 3 * id);
     const V3<double> iom = omega_of(ld4(ep + 4 * id), ld4(ev + 4 * id));
     Accum acc = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
     double bf[6] = {0, 0, 0, 0, 0, 0};
-    const unsigned before = ws.count;
-    (void)before;
     const bool ovf = wall_contact<double, MODEL, GPU_LAW>(cc, pl, b, L.dt, ipos, ivel, iom, mass[id], ws, acc, bf);
     if (ovf) atomicOr(err, XDEM_ERR_PLANE_OVERFLOW);
 #pragma unroll

@@ -1,17 +1,12 @@
 // xdem_step.hpp -- kernels of the fused DEM step (the xdem_ctx path).
 //
-// One step is four kernels plus the sort:
-//   k_integrate_hash  Velocity-Verlet position/orientation update (impl:135-147) and the
-//                     bucket key of every sphere; clears the buckets the sphere occupied
-//                     in the previous step (so no per-step clear of the whole table).
-//   radix sort        (key, sphere id) pairs, stable: buckets in key order, spheres of a
-//                     bucket in ascending id order, exactly the reference's sorted order.
-//   k_cells_gather    bucket ranges [start, end) and a sorted-order copy of what the
-//                     neighbours of a sphere read: position+radius, velocity+mass, omega.
+// One step is the grid build of xdem_grid.hip (position update, bucket counting sort,
+// sorted copies) followed by
 //   k_collide         one thread per sorted slot: sphere-sphere contacts over the 27
 //                     neighbour buckets in the reference's (z, y, x) order, every wall of
 //                     the scene in id order, then (fused mode) rolling resistance and the
-//                     velocity/orientation-rate update (xIntegratorVV.cpp:131-152).
+//                     velocity/orientation-rate update (xIntegratorVV.cpp:131-152);
+//   k_finish          (split mode) rolling resistance and the velocity update.
 // Several independent scenes share the launches: each scene owns a range of buckets, so
 // spheres of different scenes never meet in a bucket.
 #pragma once
@@ -50,10 +45,13 @@ struct StepArgs {
     const uint32_t* scene_of;  // sphere -> scene (multi-scene only)
     const SceneDev* scenes;
     const WallDev* walls;
-    // grid
-    uint32_t* keys; uint32_t* vals;          // unsorted (key, id) of this step
-    const uint32_t* skeys; const uint32_t* svals;  // sorted
-    uint2* cell_range;                       // [start, end) per bucket, start = kInvalid if empty
+    // grid: bucket key and provisional rank per sphere; dense bucket table
+    uint32_t* keys; uint32_t* rank;
+    uint32_t* bucket_count;                  // zero between steps
+    uint32_t* bucket_start;                  // first sorted slot of each bucket, [nbucket] = N
+    uint32_t* scan_part;                     // per 4096-bucket tile sums / offsets
+    uint32_t* slot_tmp;                      // sphere id per provisional slot
+    uint32_t* svals;                         // sphere id per sorted slot
     // sorted-order neighbour data
     double4* spos;
     float4* sposf;  // position relative to the scene's grid origin and radius, FP32 (prefilter)
@@ -67,7 +65,7 @@ struct StepArgs {
     uint32_t* err;
     unsigned long long* counters;  // [0] stored sphere contacts, [1] stored wall contacts
     uint32_t np, nsorted, cap;
-    uint32_t nwslot;
+    uint32_t nwslot, nbucket;
     int stale_ft, flat_inertia, keep_velocity, tmax_persist;
     int zero_force;  // k_finish leaves force/moment zero (the reference's state after a step)
 };
@@ -77,68 +75,6 @@ __device__ __forceinline__ uint32_t scene_index(const StepArgs& A, uint32_t i, b
     return multi ? A.scene_of[i] : 0u;
 }
 
-// ------------------------------------------------------------------- position + key
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_integrate_hash(StepArgs A)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {  // stored-contact counters of this step
-        A.counters[0] = 0ull;
-        A.counters[1] = 0ull;
-    }
-    if (i >= A.np) return;
-    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-    const uint32_t old = A.keys[i];
-    if (old != kInvalid) A.cell_range[old].x = kInvalid;
-    const V4<double> e = ld4(A.ep + 4 * i), d = ld4(A.ev + 4 * i), a = ld4(A.ea + 4 * i);
-    V4<double> ne;
-    ne.x = e.x + S.dt * d.x + S.half2dt * a.x;
-    ne.y = e.y + S.dt * d.y + S.half2dt * a.y;
-    ne.z = e.z + S.dt * d.z + S.half2dt * a.z;
-    ne.w = e.w + S.dt * d.w + S.half2dt * a.w;
-    const double l = sqrt(ne.x * ne.x + ne.y * ne.y + ne.z * ne.z + ne.w * ne.w);
-    ne.x = ne.x / l; ne.y = ne.y / l; ne.z = ne.z / l; ne.w = ne.w / l;
-    st4(A.ep + 4 * i, ne);
-    const V3<double> v = ld3(A.vel + 3 * i), ac = ld3(A.acc + 3 * i);
-    V4<double> p = ld4(A.pos + 4 * i);
-    p.x += S.dt * v.x + S.half2dt * ac.x;
-    p.y += S.dt * v.y + S.half2dt * ac.y;
-    p.z += S.dt * v.z + S.half2dt * ac.z;
-    st4(A.pos + 4 * i, p);
-    int gx, gy, gz;
-    grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
-    A.keys[i] = grid_bucket(S.grid, gx, gy, gz);
-    A.vals[i] = i;
-}
-
-// --------------------------------------------------------- bucket ranges + sorted copies
-// Internal linkage: each translation unit that includes this header gets its own copy;
-// only the FP64 unit launches it.
-static __global__ __launch_bounds__(256) void k_cells_gather(StepArgs A)
-{
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nsorted) return;
-    const uint32_t key = A.skeys[j];
-    if (j == 0 || A.skeys[j - 1] != key) A.cell_range[key].x = j;
-    if (j == A.nsorted - 1 || A.skeys[j + 1] != key) A.cell_range[key].y = j + 1;
-    const uint32_t k = A.svals[j];
-    if (k < A.np) {
-        const double4 p = reinterpret_cast<const double4*>(A.pos)[k];
-        A.spos[j] = p;
-        const Grid& gr = A.scenes[A.scene_of ? A.scene_of[k] : 0u].grid;
-        A.sposf[j] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), (float)p.w);
-        const V3<double> v = ld3(A.vel + 3 * k);
-        const V3<double> w = omega_of(ld4(A.ep + 4 * k), ld4(A.ev + 4 * k));
-        A.svm[j] = make_double4(v.x, v.y, v.z, A.mass[k]);
-        A.som[j] = make_double4(w.x, w.y, w.z, 0.0);
-    } else {
-        // mesh-proxy slot: infinitely far, never a sphere-sphere candidate
-        A.spos[j] = make_double4(__builtin_huge_val(), __builtin_huge_val(), __builtin_huge_val(), 0.0);
-        A.sposf[j] = make_float4(__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf(), 0.0f);
-    }
-}
-
-// ------------------------------------------------------------------------ contacts
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide(StepArgs A)
 {
@@ -177,71 +113,64 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
                 const float4 fi = A.sposf[j];
                 const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
                 const float reachf = (float)reach;
-                // candidates of one run of sorted slots [m0, m1)
-                auto visit = [&](uint32_t m0, uint32_t m1) {
-                    for (uint32_t m = m0; m < m1; m++) {
-                        if (m == j) continue;
-                        {
-                            const float4 f = A.sposf[m];
-                            const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
-                            const float lim = (fi.w + f.w + reachf) * 1.000001f + mg;
-                            if (dx * dx + dy * dy + dz * dz > lim * lim) continue;
-                        }
-                        const double4 q = A.spos[m];
-                        const uint32_t k = A.svals[m];
-                        const V4<double> jpos = {q.x, q.y, q.z, q.w};
-                        const PairGeom g = pair_geom(ipos, jpos);
-                        double sd0 = 0.0, sd1 = 0.0;
-                        bool store = false;
-                        if (g.cdist > 0) {
-                            for (uint32_t s = 0; s < old_count; s++)
-                                if (A.pp_id_in[hb + s] == k) {
-                                    const double2 h = A.pp_tsd_in[hb + s];
-                                    sd0 = h.x; sd1 = h.y;
-                                    break;
-                                }
-                            const double4 jvm = A.svm[m];
-                            const double4 jow = A.som[m];
-                            if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
-                            sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, ipos, ivel, iom, im, jpos,
-                                                              mk3(jvm.x, jvm.y, jvm.z), mk3(jow.x, jow.y, jow.z),
-                                                              jvm.w, g, sd0, sd1, Ft, pp);
-                            store = true;
-                        } else if (reach > 0.0 && -g.cdist < reach) {
-                            const double jr = jpos.w, ir = ipos.w;
-                            const double coh_s = cohesion_limit_depth(S.cc, jr ? (ir * jr) / (ir + jr) : ir);
-                            if (fabs(g.cdist) < fabs(coh_s)) {
-                                sphere_cohesion(S.cc, ir, jr, g, pp);
-                                store = true;
-                            }
-                        }
-                        if (store) {
-                            n_seen++;
-                            if (n_new < cap) {
-                                A.pp_id_out[hb + n_new] = k;
-                                A.pp_tsd_out[hb + n_new] = make_double2(sd0, sd1);
-                                n_new++;
-                            }
-                        }
-                    }
-                };
                 // 27 neighbour buckets in the reference's (z, y, x) order. The three buckets
-                // of an x-row have consecutive keys unless the row wraps, so their spheres
-                // form one run of sorted slots, visited in the same order.
-                const bool row_contiguous = (((uint32_t)(gx - 1) & S.grid.mask_x) + 2u) == ((uint32_t)(gx + 1) & S.grid.mask_x);
+                // of an x-row are consecutive keys, or two runs when the row wraps (the
+                // buckets up to the end of the row, then those from its start), so a row is
+                // at most two runs of sorted slots, walked by one loop in the same order as
+                // bucket by bucket. One loop for both cases keeps the lanes of a wave on the
+                // same code path.
+                const uint32_t bx0 = (uint32_t)(gx - 1) & S.grid.mask_x;
+                const uint32_t n1 = min(3u, S.grid.dim_x - bx0);   // buckets before the wrap
                 for (int z = -1; z <= 1; z++)
                     for (int y = -1; y <= 1; y++) {
-                        const uint32_t b0 = grid_bucket(S.grid, gx - 1, gy + y, gz + z);
-                        if (row_contiguous) {
-                            const uint2 r0 = A.cell_range[b0], r1 = A.cell_range[b0 + 1], r2 = A.cell_range[b0 + 2];
-                            const uint32_t s = r0.x != kInvalid ? r0.x : (r1.x != kInvalid ? r1.x : r2.x);
-                            if (s == kInvalid) continue;
-                            const uint32_t e = r2.x != kInvalid ? r2.y : (r1.x != kInvalid ? r1.y : r0.y);
-                            visit(s, e);
-                        } else {
-                            for (int x = -1; x <= 1; x++) {
-                                const uint2 rg = A.cell_range[grid_bucket(S.grid, gx + x, gy + y, gz + z)];
-                                if (rg.x != kInvalid) visit(rg.x, rg.y);
+                        const uint32_t rb = grid_bucket(S.grid, 0, gy + y, gz + z);
+                        const uint32_t s1 = A.bucket_start[rb + bx0], e1 = A.bucket_start[rb + bx0 + n1];
+                        const uint32_t s2 = A.bucket_start[rb], e2 = A.bucket_start[rb + 3 - n1];
+                        const uint32_t len1 = e1 - s1, total = len1 + (e2 - s2);
+                        for (uint32_t t = 0; t < total; t++) {
+                            const uint32_t m = t < len1 ? s1 + t : s2 + (t - len1);
+                            if (m == j) continue;
+                            {
+                                const float4 f = A.sposf[m];
+                                const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
+                                const float lim = (fi.w + f.w + reachf) * 1.000001f + mg;
+                                if (dx * dx + dy * dy + dz * dz > lim * lim) continue;
+                            }
+                            const double4 q = A.spos[m];
+                            const uint32_t k = A.svals[m];
+                            const V4<double> jpos = {q.x, q.y, q.z, q.w};
+                            const PairGeom g = pair_geom(ipos, jpos);
+                            double sd0 = 0.0, sd1 = 0.0;
+                            bool store = false;
+                            if (g.cdist > 0) {
+                                for (uint32_t s = 0; s < old_count; s++)
+                                    if (A.pp_id_in[hb + s] == k) {
+                                        const double2 h = A.pp_tsd_in[hb + s];
+                                        sd0 = h.x; sd1 = h.y;
+                                        break;
+                                    }
+                                const double4 jvm = A.svm[m];
+                                const double4 jow = A.som[m];
+                                if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
+                                sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, ipos, ivel, iom, im, jpos,
+                                                                  mk3(jvm.x, jvm.y, jvm.z), mk3(jow.x, jow.y, jow.z),
+                                                                  jvm.w, g, sd0, sd1, Ft, pp);
+                                store = true;
+                            } else if (reach > 0.0 && -g.cdist < reach) {
+                                const double jr = jpos.w, ir = ipos.w;
+                                const double coh_s = cohesion_limit_depth(S.cc, jr ? (ir * jr) / (ir + jr) : ir);
+                                if (fabs(g.cdist) < fabs(coh_s)) {
+                                    sphere_cohesion(S.cc, ir, jr, g, pp);
+                                    store = true;
+                                }
+                            }
+                            if (store) {
+                                n_seen++;
+                                if (n_new < cap) {
+                                    A.pp_id_out[hb + n_new] = k;
+                                    A.pp_tsd_out[hb + n_new] = make_double2(sd0, sd1);
+                                    n_new++;
+                                }
                             }
                         }
                     }
@@ -258,14 +187,12 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
             tot.T = t0 + pp.T;
             tot.R = 0.0 + pp.R;
             if (S.nwall) {
+                // wall slots are read only when a wall needs them and written back only
+                // when one changed (the values in memory are the same either way)
                 WallSlots ws;
-#pragma unroll
-                for (int s = 0; s < XDEM_MAX_P2PL_COUNT; s++) {
-                    ws.id[s] = A.pl_id[XDEM_MAX_P2PL_COUNT * i + s];
-                    const double2 h = A.pl_tsd[XDEM_MAX_P2PL_COUNT * i + s];
-                    ws.s0[s] = h.x; ws.s1[s] = h.y;
-                }
-                ws.count = A.pl_count[i];
+                const uint32_t count0 = A.pl_count[i];
+                ws.init(count0, A.pl_id + XDEM_MAX_P2PL_COUNT * i,
+                        reinterpret_cast<const double*>(A.pl_tsd + XDEM_MAX_P2PL_COUNT * i));
                 for (uint32_t w = 0; w < S.nwall; w++) {
                     const WallDev& W = A.walls[S.wall_begin + w];
                     double bf[6] = {0, 0, 0, 0, 0, 0};
@@ -277,12 +204,14 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
                         for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = bf[c];
                     }
                 }
+                if (ws.dirty) {
 #pragma unroll
-                for (int s = 0; s < XDEM_MAX_P2PL_COUNT; s++) {
-                    A.pl_id[XDEM_MAX_P2PL_COUNT * i + s] = ws.id[s];
-                    A.pl_tsd[XDEM_MAX_P2PL_COUNT * i + s] = make_double2(ws.s0[s], ws.s1[s]);
+                    for (int s = 0; s < XDEM_MAX_P2PL_COUNT; s++) {
+                        A.pl_id[XDEM_MAX_P2PL_COUNT * i + s] = ws.id[s];
+                        A.pl_tsd[XDEM_MAX_P2PL_COUNT * i + s] = make_double2(ws.s0[s], ws.s1[s]);
+                    }
                 }
-                A.pl_count[i] = ws.count;
+                if (ws.count != count0) A.pl_count[i] = ws.count;
                 n_wall = ws.count;
             }
 
@@ -393,8 +322,10 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
 using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, hipStream_t);
 CollideLaunch collide_launch_f64();
 CollideLaunch collide_launch_mixed();
+// xdem_grid.hip
+uint32_t bucket_table_size(uint32_t nbucket);   // bucket_count / bucket_start entries (+1 for start)
 hipError_t launch_integrate_hash(const StepArgs&, bool multi, hipStream_t);
-hipError_t launch_gather(const StepArgs&, hipStream_t);
+hipError_t launch_grid(const StepArgs&, hipStream_t);
 hipError_t launch_finish(const StepArgs&, bool multi, hipStream_t);
 
 }  // namespace xdem

@@ -1,4 +1,4 @@
-// xdem_step_f64.hip -- FP64 instantiation of the fused step kernels.
+// xdem_step_f64.hip -- FP64 instantiation of the contact and velocity kernels.
 // Built with -ffp-contract=off so every multiply and add rounds separately, in the
 // reference's operation order.
 #include "xdem_step.hpp"
@@ -38,21 +38,6 @@ hipError_t collide_f64(const StepArgs& A, bool multi, bool fuse, int model, bool
 }  // namespace
 
 CollideLaunch collide_launch_f64() { return collide_f64; }
-
-hipError_t launch_integrate_hash(const StepArgs& A, bool multi, hipStream_t s)
-{
-    if (A.np == 0) return hipSuccess;
-    if (multi) hipLaunchKernelGGL(k_integrate_hash<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
-    else hipLaunchKernelGGL(k_integrate_hash<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
-    return hipGetLastError();
-}
-
-hipError_t launch_gather(const StepArgs& A, hipStream_t s)
-{
-    if (A.nsorted == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cells_gather, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-    return hipGetLastError();
-}
 
 hipError_t launch_finish(const StepArgs& A, bool multi, hipStream_t s)
 {

@@ -17,7 +17,8 @@ from .scene import PARITY_QUIRKS, SceneData
 
 class DemContext:
     def __init__(self, scenes, precision: str = "fp64", p2p_capacity: int = 12, quirks: dict | None = None,
-                 grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False):
+                 grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False,
+                 buckets_per_sphere: int = 0):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -31,6 +32,7 @@ class DemContext:
         cfg.grid_dim_max = grid_dim_max
         cfg.fuse_velocity = 1 if fuse_velocity else 0
         cfg.device = device
+        cfg.buckets_per_sphere = buckets_per_sphere
         self.precision = precision
         self.capacity = p2p_capacity
         self._h = C.c_void_p()
