@@ -237,6 +237,7 @@ def main():
         "mean_contacts_per_sphere": cbar,
         "grid_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
         "finish_ms_per_step": tm["finish_ms"] / max(tm["steps"], 1),
+        "pair_search_ms_per_step": tm["pairs_ms"] / max(tm["steps"], 1),
         "error_word": mixed["err"],
         "scene_build_s": t_build,
     }

@@ -157,7 +157,9 @@ int xdem_ctx_synchronize(xdem_ctx* ctx);
 /* Copies a field of one scene (its particles in input order) to/from host memory. */
 int xdem_ctx_read(xdem_ctx* ctx, int field, unsigned int scene, double* host_out);
 int xdem_ctx_write(xdem_ctx* ctx, int field, unsigned int scene, const double* host_in);
-/* Device pointer of a field over all scenes (scenes concatenated in order). */
+/* Device pointer of a field over all scenes (scenes concatenated in order). With
+ * fuse_velocity = 1, XDEM_FIELD_VEL and XDEM_FIELD_EV alternate between two buffers from
+ * step to step: fetch the pointer again after every step. */
 int xdem_ctx_device_ptr(xdem_ctx* ctx, int field, void** out);
 /* History of one scene in the reference layout: count[np], ids[np*cap], tsd[2*np*cap]
  * (cap = p2p_capacity for XDEM_HISTORY_P2P, 3 for XDEM_HISTORY_PLANE). Sphere ids are
@@ -183,11 +185,12 @@ int xdem_ctx_graph_capture(xdem_ctx* ctx, unsigned int nsteps);
 /* Launches the captured graph `nlaunch` times. */
 int xdem_ctx_graph_launch(xdem_ctx* ctx, unsigned int nlaunch);
 /* Per-kernel time of the last xdem_ctx_step call, measured with HIP events on the
- * context's stream when timing is on: out[0] = contact kernel ms (sum over steps),
- * out[1] = grid (position/hash/sort/cells) ms, out[2] = number of steps timed,
- * out[3] = rolling + velocity kernel ms (0 when fused into the contact kernel). */
+ * context's stream when timing is on (sums over the steps timed, at most 2048):
+ * out[0] = contact kernel (k_collide) ms, out[1] = grid (position update, bucket sort,
+ * sorted copies) ms, out[2] = number of steps timed, out[3] = rolling + velocity kernel
+ * ms (0 when fused into the contact kernel), out[4] = pair search (k_pairs) ms. */
 int xdem_ctx_set_timing(xdem_ctx* ctx, int on);
-int xdem_ctx_timing(xdem_ctx* ctx, double out[4]);
+int xdem_ctx_timing(xdem_ctx* ctx, double out[5]);
 
 #ifdef __cplusplus
 }

@@ -110,6 +110,8 @@ typedef struct xdem_quirks {
 #define XDEM_ERR_MESH_OVERFLOW  0x4u  /* more triangle contacts than triangle slots */
 #define XDEM_ERR_NONFINITE      0x8u  /* NaN/Inf in a force or a state update */
 #define XDEM_ERR_MESH_LIST      0x10u /* more than 20 triangle candidates of one kind */
+#define XDEM_ERR_NBR_OVERFLOW   0x20u /* more near neighbours than the pair-search list holds
+                                         (step layer; the contacts beyond it are missed) */
 
 /* Status codes returned by every C entry point. */
 #define XDEM_OK              0

@@ -23,6 +23,7 @@ ERR_PLANE_OVERFLOW = 0x2
 ERR_MESH_OVERFLOW = 0x4
 ERR_NONFINITE = 0x8
 ERR_MESH_LIST = 0x10
+ERR_NBR_OVERFLOW = 0x20
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1

@@ -154,6 +154,20 @@ __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_
     V3<double> unit = mk3(0.0, 0.0, 0.0);
     const V3<double> wp = mk3(dot(dp, mk3(pl.u1.x, pl.u1.y, pl.u1.z)), dot(dp, mk3(pl.u2.x, pl.u2.y, pl.u2.z)),
                               dot(dp, mk3(pl.uw.x, pl.uw.y, pl.uw.z)));
+    // Exact early exit for a wall without cohesion. Every contact region of
+    // plane_detection needs (dp . uw)^2 < r^2 max(1, |uw|^2): corners and edges test
+    // (dp . uw)^2 plus non-negative terms against r^2, the face needs |dp . uw| / |uw| < r.
+    // Beyond that (with a 1e-6 margin for rounding) the sphere cannot touch the wall and
+    // the outcome is the no-contact one: zero force, slot count as below.
+    const double uw2 = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
+    if (cc.coh == 0.0 && wp.z * wp.z >= (r * r) * fmax(1.0, uw2) * (1.0 + 1e-6)) {
+        ws.count = n_new;
+        acc.F = acc.F + mk3(0.0, 0.0, 0.0);
+        acc.M = acc.M + mk3(0.0, 0.0, 0.0);
+        acc.T = acc.T + mk3(0.0, 0.0, 0.0);
+        acc.R += 0.0;
+        return false;
+    }
     const double cdist = plane_detection(pl, ip3, wp, unit, r);
     double coh_s = 0;
     if (cc.coh) coh_s = cohesion_limit_depth(cc, r);

@@ -77,11 +77,16 @@ struct xdem_ctx {
     SceneDev* d_scenes = nullptr;
     WallDev* d_walls = nullptr;
     uint32_t *keys = nullptr, *rank = nullptr, *svals = nullptr, *slot_tmp = nullptr;
+    uint32_t *nbr_count = nullptr, *nbr = nullptr;
+    uint32_t list_cap = 48;
     uint32_t *bucket_count = nullptr, *bucket_start = nullptr, *scan_part = nullptr;
-    double4* spos = nullptr;
     float4* sposf = nullptr;
-    double4* svm = nullptr;
-    double4* som = nullptr;
+    // vel and ev: two buffers each in fused mode (the contact kernel writes the new values
+    // into the other one while neighbours still read the old ones); vel/ev point at the
+    // current one
+    double* vel_buf[2] = {nullptr, nullptr};
+    double* ev_buf[2] = {nullptr, nullptr};
+    int sv = 0;
     uint32_t* pp_count[2] = {nullptr, nullptr};
     uint32_t* pp_id[2] = {nullptr, nullptr};
     double2* pp_tsd[2] = {nullptr, nullptr};
@@ -100,10 +105,11 @@ struct xdem_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     unsigned graph_steps = 0;
+    int graph_parity = 0, graph_sparity = 0;
 
     bool timing = false;
-    std::vector<hipEvent_t> events;  // 3 per timed step
-    double t_contact = 0.0, t_grid = 0.0, t_finish = 0.0;
+    std::vector<hipEvent_t> events;  // kEventsPerStep per timed step
+    double t_contact = 0.0, t_grid = 0.0, t_finish = 0.0, t_pairs = 0.0;
     unsigned t_steps = 0;
     unsigned long long steps_done = 0;
 
@@ -127,7 +133,9 @@ struct xdem_ctx {
         A.scene_of = scene_of; A.scenes = d_scenes; A.walls = d_walls;
         A.keys = keys; A.rank = rank; A.svals = svals; A.slot_tmp = slot_tmp;
         A.bucket_count = bucket_count; A.bucket_start = bucket_start; A.scan_part = scan_part;
-        A.spos = spos; A.sposf = sposf; A.svm = svm; A.som = som;
+        A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap;
+        A.sposf = sposf;
+        A.vel_out = vel; A.ev_out = ev;
         A.pp_count_in = pp_count[cur]; A.pp_id_in = pp_id[cur]; A.pp_tsd_in = pp_tsd[cur];
         A.pp_count_out = pp_count[cur ^ 1]; A.pp_id_out = pp_id[cur ^ 1]; A.pp_tsd_out = pp_tsd[cur ^ 1];
         A.pl_count = pl_count; A.pl_id = pl_id; A.pl_tsd = pl_tsd;
@@ -139,43 +147,61 @@ struct xdem_ctx {
         return A;
     }
 
-    // First half of a step: integrate + key, sort, cells + gather, contacts.
-    hipError_t collide_half(bool fuse, hipEvent_t* ev3)
+    // First half of a step: position update + key, bucket sort + sorted copies, pair
+    // search, contacts. ev (optional) gets: start, grid done, pairs done, contacts done.
+    hipError_t collide_half(bool fuse, hipEvent_t* ev)
     {
         hipError_t e;
-        const StepArgs A = args();
-        if (ev3 && (e = hipEventRecord(ev3[0], stream)) != hipSuccess) return e;
+        StepArgs A = args();
+        if (fuse) {
+            A.vel_out = vel_buf[sv ^ 1];
+            A.ev_out = ev_buf[sv ^ 1];
+        }
+        if (ev && (e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
         if ((e = launch_integrate_hash(A, multi, stream)) != hipSuccess) return e;
         if ((e = launch_grid(A, stream)) != hipSuccess) return e;
-        if (ev3 && (e = hipEventRecord(ev3[1], stream)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
+        if ((e = launch_pairs(A, multi, stream)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[2], stream)) != hipSuccess) return e;
         if ((e = collide_fn(A, multi, fuse, model, cfg.quirks.gpu_force_law != 0, stream)) != hipSuccess) return e;
-        if (ev3 && (e = hipEventRecord(ev3[2], stream)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[3], stream)) != hipSuccess) return e;
         if (nwslot) {
             hipLaunchKernelGGL(k_wrench_reduce, dim3(nwslot * 6), dim3(256), 0, stream, wrench_pp, np, d_wslot_range,
                                wrench_out);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
         cur ^= 1;
+        if (fuse) set_state_parity(sv ^ 1);
         return hipSuccess;
     }
 
-    // One full step; ev4 (optional) brackets grid, contacts and the velocity kernel.
-    hipError_t full_step(hipEvent_t* ev4)
+    void set_state_parity(int p)
+    {
+        sv = p;
+        vel = vel_buf[p];
+        ev = ev_buf[p];
+    }
+
+    // One full step; ev (optional, kEventsPerStep) brackets grid, pair search, contacts and
+    // the velocity kernel.
+    hipError_t full_step(hipEvent_t* ev)
     {
         hipError_t e;
         if (cfg.fuse_velocity) {
-            if ((e = collide_half(true, ev4)) != hipSuccess) return e;
-            if (ev4) return hipEventRecord(ev4[3], stream);
+            if ((e = collide_half(true, ev)) != hipSuccess) return e;
+            if (ev) return hipEventRecord(ev[4], stream);
             return hipSuccess;
         }
-        if ((e = collide_half(false, ev4)) != hipSuccess) return e;
+        if ((e = collide_half(false, ev)) != hipSuccess) return e;
         StepArgs A = args();
         A.zero_force = 0;  // the next contact pass overwrites force and moment
         if ((e = launch_finish(A, multi, stream)) != hipSuccess) return e;
-        if (ev4) return hipEventRecord(ev4[3], stream);
+        if (ev) return hipEventRecord(ev[4], stream);
         return hipSuccess;
     }
 };
+
+constexpr unsigned kEventsPerStep = 5;
 
 static int hip_status(hipError_t e) { return e == hipSuccess ? XDEM_OK : XDEM_E_HIP; }
 
@@ -340,12 +366,15 @@ int xdem_ctx_commit(xdem_ctx* c)
     const size_t n = c->np;
     c->key_bits = 1;
     while (c->key_bits < 32 && (1ull << c->key_bits) < (unsigned long long)c->nbucket) c->key_bits++;
-    const size_t vsz = sizeof(double4);
     CTX_CHECK(c->alloc(&c->pos, sizeof(double) * 4 * n));
-    CTX_CHECK(c->alloc(&c->vel, sizeof(double) * 3 * n));
+    CTX_CHECK(c->alloc(&c->vel_buf[0], sizeof(double) * 3 * n));
+    if (c->cfg.fuse_velocity) CTX_CHECK(c->alloc(&c->vel_buf[1], sizeof(double) * 3 * n));
+    c->vel = c->vel_buf[0];
     CTX_CHECK(c->alloc(&c->acc, sizeof(double) * 3 * n));
     CTX_CHECK(c->alloc(&c->ep, sizeof(double) * 4 * n));
-    CTX_CHECK(c->alloc(&c->ev, sizeof(double) * 4 * n));
+    CTX_CHECK(c->alloc(&c->ev_buf[0], sizeof(double) * 4 * n));
+    if (c->cfg.fuse_velocity) CTX_CHECK(c->alloc(&c->ev_buf[1], sizeof(double) * 4 * n));
+    c->ev = c->ev_buf[0];
     CTX_CHECK(c->alloc(&c->ea, sizeof(double) * 4 * n));
     CTX_CHECK(c->alloc(&c->force, sizeof(double) * 3 * n));
     CTX_CHECK(c->alloc(&c->moment, sizeof(double) * 3 * n));
@@ -360,14 +389,14 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->rank, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->svals, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->slot_tmp, sizeof(uint32_t) * n));
+    c->list_cap = std::max(48u, 2u * c->cap);
+    CTX_CHECK(c->alloc(&c->nbr_count, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->nbr, sizeof(uint32_t) * n * c->list_cap));
     const size_t nbt = bucket_table_size(c->nbucket);
     CTX_CHECK(c->alloc(&c->bucket_count, sizeof(uint32_t) * nbt));
     CTX_CHECK(c->alloc(&c->bucket_start, sizeof(uint32_t) * (nbt + 1)));
     CTX_CHECK(c->alloc(&c->scan_part, sizeof(uint32_t) * (nbt / 4096 + 1)));
-    CTX_CHECK(c->alloc(&c->spos, sizeof(double4) * n));
     CTX_CHECK(c->alloc(&c->sposf, sizeof(float4) * n));
-    CTX_CHECK(c->alloc(&c->svm, vsz * n));
-    CTX_CHECK(c->alloc(&c->som, vsz * n));
     for (int b = 0; b < 2; b++) {
         CTX_CHECK(c->alloc(&c->pp_count[b], sizeof(uint32_t) * n));
         CTX_CHECK(c->alloc(&c->pp_id[b], sizeof(uint32_t) * n * c->cap));
@@ -445,7 +474,7 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
     const bool time_it = c->timing;
     const unsigned kmax = 2048u;
     if (time_it) {
-        const size_t need = 4 * (size_t)std::min(nsteps, kmax);
+        const size_t need = kEventsPerStep * (size_t)std::min(nsteps, kmax);
         while (c->events.size() < need) {
             hipEvent_t e;
             CTX_CHECK(hipEventCreate(&e));
@@ -453,8 +482,8 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
         }
     }
     for (unsigned k = 0; k < nsteps; k++) {
-        hipEvent_t* ev4 = (time_it && k < kmax) ? &c->events[4 * k] : nullptr;
-        CTX_CHECK(c->full_step(ev4));
+        hipEvent_t* ev = (time_it && k < kmax) ? &c->events[kEventsPerStep * k] : nullptr;
+        CTX_CHECK(c->full_step(ev));
         c->steps_done++;
     }
     if (time_it) {
@@ -462,13 +491,17 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
         c->t_contact = 0.0;
         c->t_grid = 0.0;
         c->t_finish = 0.0;
+        c->t_pairs = 0.0;
         c->t_steps = std::min(nsteps, kmax);
         for (unsigned k = 0; k < c->t_steps; k++) {
-            float a = 0.f, b = 0.f, f = 0.f;
-            CTX_CHECK(hipEventElapsedTime(&a, c->events[4 * k], c->events[4 * k + 1]));
-            CTX_CHECK(hipEventElapsedTime(&b, c->events[4 * k + 1], c->events[4 * k + 2]));
-            CTX_CHECK(hipEventElapsedTime(&f, c->events[4 * k + 2], c->events[4 * k + 3]));
+            const hipEvent_t* ev = &c->events[kEventsPerStep * k];
+            float a = 0.f, p = 0.f, b = 0.f, f = 0.f;
+            CTX_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
+            CTX_CHECK(hipEventElapsedTime(&p, ev[1], ev[2]));
+            CTX_CHECK(hipEventElapsedTime(&b, ev[2], ev[3]));
+            CTX_CHECK(hipEventElapsedTime(&f, ev[3], ev[4]));
             c->t_grid += a;
+            c->t_pairs += p;
             c->t_contact += b;
             c->t_finish += f;
         }
@@ -732,13 +765,16 @@ int xdem_ctx_graph_capture(xdem_ctx* c, unsigned int nsteps)
     if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
     CTX_CHECK(hipStreamSynchronize(c->stream));
-    const int parity = c->cur;
+    const int parity = c->cur, sparity = c->sv;
     CTX_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (unsigned k = 0; k < nsteps && e == hipSuccess; k++) e = c->full_step(nullptr);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     c->cur = parity;
+    c->set_state_parity(sparity);
+    c->graph_parity = parity;
+    c->graph_sparity = sparity;
     if (e != hipSuccess || e2 != hipSuccess) {
         if (g) hipGraphDestroy(g);
         return XDEM_E_HIP;
@@ -753,6 +789,9 @@ int xdem_ctx_graph_launch(xdem_ctx* c, unsigned int nlaunch)
 {
     if (!c) return XDEM_E_INVALID;
     if (!c->gexec) return XDEM_E_NOT_READY;
+    // the graph's buffers are those of the parity it was recorded at (an odd number of
+    // eager steps since then swaps them)
+    if (c->cur != c->graph_parity || c->sv != c->graph_sparity) return XDEM_E_NOT_READY;
     CTX_CHECK(hipSetDevice(c->cfg.device));
     for (unsigned k = 0; k < nlaunch; k++) {
         CTX_CHECK(hipGraphLaunch(c->gexec, c->stream));
@@ -768,13 +807,14 @@ int xdem_ctx_set_timing(xdem_ctx* c, int on)
     return XDEM_OK;
 }
 
-int xdem_ctx_timing(xdem_ctx* c, double out[4])
+int xdem_ctx_timing(xdem_ctx* c, double out[5])
 {
     if (!c || !out) return XDEM_E_INVALID;
     out[0] = c->t_contact;
     out[1] = c->t_grid;
     out[2] = (double)c->t_steps;
     out[3] = c->t_finish;
+    out[4] = c->t_pairs;
     return XDEM_OK;
 }
 

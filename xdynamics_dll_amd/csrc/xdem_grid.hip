@@ -12,7 +12,9 @@
 //   k_scatter         provisional slot start[key] + rank for every sphere
 //   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
 //                     smaller id, i.e. exactly the reference's (hash, index) order; writes
-//                     the sorted id and the sorted copies the contact kernel reads
+//                     the sorted id and the FP32 position copy the pair search reads
+// The pair search k_pairs (xdem_step.hpp) is launched from here too: its FP64 pair
+// predicate must round exactly like the reference's.
 // The bucket of slot range [start[b], start[b+1]) needs no clearing between steps, and a
 // row of three consecutive buckets is the single range [start[b], start[b+3]).
 // Built with -ffp-contract=off: the position update is bit-identical to the reference's.
@@ -182,13 +184,8 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     const uint32_t f = s + r;
     A.svals[f] = k;
     const double4 p = reinterpret_cast<const double4*>(A.pos)[k];
-    A.spos[f] = p;
     const Grid& gr = A.scenes[A.scene_of ? A.scene_of[k] : 0u].grid;
     A.sposf[f] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), (float)p.w);
-    const V3<double> v = ld3(A.vel + 3 * k);
-    const V3<double> w = omega_of(ld4(A.ep + 4 * k), ld4(A.ev + 4 * k));
-    A.svm[f] = make_double4(v.x, v.y, v.z, A.mass[k]);
-    A.som[f] = make_double4(w.x, w.y, w.z, 0.0);
 }
 
 }  // namespace
@@ -203,6 +200,14 @@ hipError_t launch_integrate_hash(const StepArgs& A, bool multi, hipStream_t s)
     if (A.np == 0) return hipSuccess;
     if (multi) hipLaunchKernelGGL(k_integrate_hash<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
     else hipLaunchKernelGGL(k_integrate_hash<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs(const StepArgs& A, bool multi, hipStream_t s)
+{
+    if (A.nsorted == 0) return hipSuccess;
+    if (multi) hipLaunchKernelGGL(k_pairs<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+    else hipLaunchKernelGGL(k_pairs<false>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 

@@ -42,6 +42,8 @@ struct StepArgs {
     double* pos;  double* vel;  double* acc;  double* ep;  double* ev;  double* ea;
     double* force; double* moment; const double* mass; const double* inertia;
     double* tmax; double* rres;
+    double* vel_out; double* ev_out;  // where the fused contact kernel writes vel and ev (the
+                                      // other buffer of a pair: neighbours read the old ones)
     const uint32_t* scene_of;  // sphere -> scene (multi-scene only)
     const SceneDev* scenes;
     const WallDev* walls;
@@ -52,11 +54,10 @@ struct StepArgs {
     uint32_t* scan_part;                     // per 4096-bucket tile sums / offsets
     uint32_t* slot_tmp;                      // sphere id per provisional slot
     uint32_t* svals;                         // sphere id per sorted slot
-    // sorted-order neighbour data
-    double4* spos;
-    float4* sposf;  // position relative to the scene's grid origin and radius, FP32 (prefilter)
-    double4* svm;  // velocity xyz + mass
-    double4* som;  // angular velocity xyz
+    uint32_t* nbr_count;                     // near neighbours per sphere (k_pairs)
+    uint32_t* nbr;                           // their sphere ids, list_cap per sphere
+    float4* sposf;  // per sorted slot: position relative to the scene's grid origin and
+                    // radius, FP32 (the pair search reads only this)
     // sphere history (double-buffered) and wall history (in place)
     const uint32_t* pp_count_in; const uint32_t* pp_id_in; const double2* pp_tsd_in;
     uint32_t* pp_count_out; uint32_t* pp_id_out; double2* pp_tsd_out;
@@ -66,6 +67,7 @@ struct StepArgs {
     unsigned long long* counters;  // [0] stored sphere contacts, [1] stored wall contacts
     uint32_t np, nsorted, cap;
     uint32_t nwslot, nbucket;
+    uint32_t list_cap;                       // neighbour-list entries per sorted slot
     int stale_ft, flat_inertia, keep_velocity, tmax_persist;
     int zero_force;  // k_finish leaves force/moment zero (the reference's state after a step)
 };
@@ -75,21 +77,90 @@ __device__ __forceinline__ uint32_t scene_index(const StepArgs& A, uint32_t i, b
     return multi ? A.scene_of[i] : 0u;
 }
 
+// Pair search: for the sphere in sorted slot j, the ids of every neighbour that can pass
+// the reference's pair predicate (cdist > 0, or inside the JKR separation distance of a
+// cohesive pair), in the reference's visiting order: 27 buckets in (z, y, x) order, spheres
+// of a bucket in id order. The three buckets of an x-row are consecutive keys, or two runs
+// when the row wraps (the buckets up to the end of the row, then those from its start).
+// One flat loop walks all nine rows, so each lane advances through its own candidates
+// without waiting for the other lanes row by row. Only an FP32 test runs here (few
+// registers, many waves per SIMD to hide the latency of the candidate loads); k_collide
+// applies the exact FP64 predicate to the list, in the same order.
+//
+// FP32 test: a candidate is dropped only when its FP32 distance exceeds r_i + r_j + reach
+// by more than the rounding of the FP32 coordinates (|x| 2^-24 per coordinate and operand,
+// bounded by mg) and of the FP32 radius sum (factor 1 + 1e-6), so every pair the exact test
+// accepts is in the list. reach bounds the JKR separation distance of the scene's pairs.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_pairs(StepArgs A)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.nsorted) return;
+    const uint32_t i = A.svals[j];
+    uint32_t n = 0;
+    if (i < A.np) {
+        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+        if (S.has_pp) {
+            const double4 ip4 = reinterpret_cast<const double4*>(A.pos)[i];
+            int gx, gy, gz;
+            grid_cell(S.grid, ip4.x, ip4.y, ip4.z, gx, gy, gz);
+            const float4 fi = A.sposf[j];
+            const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
+            const float reachf = (float)S.cc.coh_reach;
+            const uint32_t lc = A.list_cap;
+            uint32_t* out = A.nbr + (size_t)i * lc;
+            const uint32_t bx0 = (uint32_t)(gx - 1) & S.grid.mask_x;
+            const uint32_t n1 = min(3u, S.grid.dim_x - bx0);   // buckets before the wrap
+            uint32_t row = 0, t = 0, s1 = 0, len1 = 0, s2 = 0, total = 0;
+            auto load_row = [&](uint32_t r) {
+                const uint32_t rb = grid_bucket(S.grid, 0, gy + (int)(r % 3u) - 1, gz + (int)(r / 3u) - 1);
+                s1 = A.bucket_start[rb + bx0];
+                const uint32_t e1 = A.bucket_start[rb + bx0 + n1];
+                s2 = A.bucket_start[rb];
+                const uint32_t e2 = A.bucket_start[rb + 3 - n1];
+                len1 = e1 - s1;
+                total = len1 + (e2 - s2);
+                t = 0;
+            };
+            load_row(0);
+            while (true) {
+                if (t < total) {
+                    const uint32_t m = t < len1 ? s1 + t : s2 + (t - len1);
+                    t++;
+                    if (m == j) continue;
+                    const float4 f = A.sposf[m];
+                    const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
+                    const float lim = (fi.w + f.w + reachf) * 1.000001f + mg;
+                    if (dx * dx + dy * dy + dz * dz > lim * lim) continue;
+                    if (n < lc) out[n] = A.svals[m];
+                    n++;
+                } else {
+                    if (++row == 9u) break;
+                    load_row(row);
+                }
+            }
+        }
+    }
+    if (i < A.np) A.nbr_count[i] = n;
+}
+
+// Contacts of sphere i: the near neighbours k_pairs listed, in their order, under the exact
+// FP64 predicate; every wall of the scene in id order; then (fused mode) rolling resistance
+// and the velocity update. One thread per sphere id, so the sphere's own state and outputs
+// are contiguous; neighbours are read by id only for listed pairs.
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide(StepArgs A)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n_pp = 0, n_wall = 0;
-    if (j < A.nsorted) {
-        const uint32_t i = A.svals[j];
+    {
         if (i < A.np) {
             const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-            const double4 ip4 = A.spos[j];
-            const V4<double> ipos = {ip4.x, ip4.y, ip4.z, ip4.w};
-            const double4 ivm = A.svm[j];
-            const double4 iow = A.som[j];
-            const V3<double> ivel = mk3(ivm.x, ivm.y, ivm.z);
-            const V3<double> iom = mk3(iow.x, iow.y, iow.z);
+            const V4<double> ipos = ld4(A.pos + 4 * i);
+            const V3<double> ivel = ld3(A.vel + 3 * i);
+            const V4<double> iep = ld4(A.ep + 4 * i);
+            V4<double> iev = ld4(A.ev + 4 * i);
+            const V3<double> iom = omega_of(iep, iev);
             const double im = A.mass[i];
             uint32_t err = 0;
 
@@ -100,80 +171,45 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
                 const uint32_t hb = i * cap;
                 uint32_t old_count = A.pp_count_in[i];
                 if (old_count > cap) old_count = cap;
-                int gx, gy, gz;
-                grid_cell(S.grid, ipos.x, ipos.y, ipos.z, gx, gy, gz);
                 V3<T> Ft = mk3(T(0), T(0), T(0));
                 uint32_t n_new = 0, n_seen = 0;
+                const uint32_t nlist_all = A.nbr_count[i];
+                const uint32_t lc = A.list_cap;
+                if (nlist_all > lc) err |= XDEM_ERR_NBR_OVERFLOW;
+                const uint32_t nlist = min(nlist_all, lc);
+                const uint32_t* lst = A.nbr + (size_t)i * lc;
                 const double reach = S.cc.coh_reach;
-                // FP32 prefilter: a candidate is skipped only when its FP32 distance exceeds
-                // r_i + r_j + reach by more than the rounding of the FP32 coordinates
-                // (|x| 2^-24 per coordinate and operand, bounded by mg) and of the FP32
-                // radius sum (factor 1 + 1e-6), so every pair the exact FP64 test below
-                // could accept reaches it. Mesh-proxy slots sit at infinity.
-                const float4 fi = A.sposf[j];
-                const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
-                const float reachf = (float)reach;
-                // 27 neighbour buckets in the reference's (z, y, x) order. The three buckets
-                // of an x-row are consecutive keys, or two runs when the row wraps (the
-                // buckets up to the end of the row, then those from its start), so a row is
-                // at most two runs of sorted slots, walked by one loop in the same order as
-                // bucket by bucket. One loop for both cases keeps the lanes of a wave on the
-                // same code path.
-                const uint32_t bx0 = (uint32_t)(gx - 1) & S.grid.mask_x;
-                const uint32_t n1 = min(3u, S.grid.dim_x - bx0);   // buckets before the wrap
-                for (int z = -1; z <= 1; z++)
-                    for (int y = -1; y <= 1; y++) {
-                        const uint32_t rb = grid_bucket(S.grid, 0, gy + y, gz + z);
-                        const uint32_t s1 = A.bucket_start[rb + bx0], e1 = A.bucket_start[rb + bx0 + n1];
-                        const uint32_t s2 = A.bucket_start[rb], e2 = A.bucket_start[rb + 3 - n1];
-                        const uint32_t len1 = e1 - s1, total = len1 + (e2 - s2);
-                        for (uint32_t t = 0; t < total; t++) {
-                            const uint32_t m = t < len1 ? s1 + t : s2 + (t - len1);
-                            if (m == j) continue;
-                            {
-                                const float4 f = A.sposf[m];
-                                const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
-                                const float lim = (fi.w + f.w + reachf) * 1.000001f + mg;
-                                if (dx * dx + dy * dy + dz * dz > lim * lim) continue;
+                for (uint32_t t = 0; t < nlist; t++) {
+                    const uint32_t k = lst[t];
+                    const V4<double> jpos = ld4(A.pos + 4 * (size_t)k);
+                    const PairGeom g = pair_geom(ipos, jpos);
+                    double sd0 = 0.0, sd1 = 0.0;
+                    if (g.cdist > 0) {
+                        for (uint32_t s = 0; s < old_count; s++)
+                            if (A.pp_id_in[hb + s] == k) {
+                                const double2 h = A.pp_tsd_in[hb + s];
+                                sd0 = h.x; sd1 = h.y;
+                                break;
                             }
-                            const double4 q = A.spos[m];
-                            const uint32_t k = A.svals[m];
-                            const V4<double> jpos = {q.x, q.y, q.z, q.w};
-                            const PairGeom g = pair_geom(ipos, jpos);
-                            double sd0 = 0.0, sd1 = 0.0;
-                            bool store = false;
-                            if (g.cdist > 0) {
-                                for (uint32_t s = 0; s < old_count; s++)
-                                    if (A.pp_id_in[hb + s] == k) {
-                                        const double2 h = A.pp_tsd_in[hb + s];
-                                        sd0 = h.x; sd1 = h.y;
-                                        break;
-                                    }
-                                const double4 jvm = A.svm[m];
-                                const double4 jow = A.som[m];
-                                if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
-                                sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, ipos, ivel, iom, im, jpos,
-                                                                  mk3(jvm.x, jvm.y, jvm.z), mk3(jow.x, jow.y, jow.z),
-                                                                  jvm.w, g, sd0, sd1, Ft, pp);
-                                store = true;
-                            } else if (reach > 0.0 && -g.cdist < reach) {
-                                const double jr = jpos.w, ir = ipos.w;
-                                const double coh_s = cohesion_limit_depth(S.cc, jr ? (ir * jr) / (ir + jr) : ir);
-                                if (fabs(g.cdist) < fabs(coh_s)) {
-                                    sphere_cohesion(S.cc, ir, jr, g, pp);
-                                    store = true;
-                                }
-                            }
-                            if (store) {
-                                n_seen++;
-                                if (n_new < cap) {
-                                    A.pp_id_out[hb + n_new] = k;
-                                    A.pp_tsd_out[hb + n_new] = make_double2(sd0, sd1);
-                                    n_new++;
-                                }
-                            }
-                        }
+                        const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
+                        const V3<double> jom = omega_of(ld4(A.ep + 4 * (size_t)k), ld4(A.ev + 4 * (size_t)k));
+                        if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
+                        sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, ipos, ivel, iom, im, jpos, jvel, jom,
+                                                          A.mass[k], g, sd0, sd1, Ft, pp);
+                    } else {
+                        if (!(reach > 0.0 && -g.cdist < reach)) continue;
+                        const double jr = jpos.w, ir = ipos.w;
+                        const double coh_s = cohesion_limit_depth(S.cc, jr ? (ir * jr) / (ir + jr) : ir);
+                        if (!(fabs(g.cdist) < fabs(coh_s))) continue;
+                        sphere_cohesion(S.cc, ir, jr, g, pp);
                     }
+                    n_seen++;
+                    if (n_new < cap) {
+                        A.pp_id_out[hb + n_new] = k;
+                        A.pp_tsd_out[hb + n_new] = make_double2(sd0, sd1);
+                        n_new++;
+                    }
+                }
                 A.pp_count_out[i] = n_new;
                 if (n_seen > cap) err |= XDEM_ERR_P2P_OVERFLOW;
                 n_pp = n_new;
@@ -217,8 +253,8 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
 
             if (FUSE) {
                 // ---- rolling resistance (impl:2420-2454)
-                const V4<double> e = ld4(A.ep + 4 * i);
-                V4<double> av = ld4(A.ev + 4 * i);
+                const V4<double> e = iep;
+                V4<double> av = iev;
                 // quirk Q5: the scene's inertia array read as flat doubles at the scene-local id
                 const double J = A.flat_inertia ? A.inertia[i + 2 * S.p_begin] : A.inertia[3 * i];
                 const bool consumed = rolling_moment(S.dt, J, omega_of(e, av), tot.T, tot.R, tot.M);
@@ -243,9 +279,10 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
                     v = mk3(v.x + hdt * a.x, v.y + hdt * a.y, v.z + hdt * a.z);
                     av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y;
                     av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
-                    st3(A.vel + 3 * i, v);
+                    st3(A.vel_out + 3 * i, v);
                 }
-                st4(A.ev + 4 * i, av);
+                if (A.keep_velocity && A.vel_out != A.vel) st3(A.vel_out + 3 * i, ivel);
+                st4(A.ev_out + 4 * i, av);
                 st3(A.acc + 3 * i, a);
                 st4(A.ea + 4 * i, mea);
                 if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) err |= XDEM_ERR_NONFINITE;
@@ -321,6 +358,7 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
 // kernels are FP64 only and live in xdem_step_f64.hip.
 using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, hipStream_t);
 CollideLaunch collide_launch_f64();
+hipError_t launch_pairs(const StepArgs&, bool multi, hipStream_t);
 CollideLaunch collide_launch_mixed();
 // xdem_grid.hip
 uint32_t bucket_table_size(uint32_t nbucket);   // bucket_count / bucket_start entries (+1 for start)

@@ -12,7 +12,7 @@ inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 template <bool MULTI, bool FUSE>
 void collide_models(const StepArgs& A, int model, bool gpu_law, hipStream_t s)
 {
-    const dim3 g = grid_for(A.nsorted), b(kBlock);
+    const dim3 g = grid_for(A.np), b(kBlock);
     if (model == 1) {
         if (gpu_law) hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, 1, true>), g, b, 0, s, A);
         else hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, 1, false>), g, b, 0, s, A);
@@ -24,7 +24,7 @@ void collide_models(const StepArgs& A, int model, bool gpu_law, hipStream_t s)
 
 hipError_t collide_mixed(const StepArgs& A, bool multi, bool fuse, int model, bool gpu_law, hipStream_t s)
 {
-    if (A.nsorted == 0) return hipSuccess;
+    if (A.np == 0) return hipSuccess;
     if (multi) {
         if (fuse) collide_models<true, true>(A, model, gpu_law, s);
         else collide_models<true, false>(A, model, gpu_law, s);
