@@ -76,7 +76,7 @@ struct xdem_ctx {
     uint32_t* scene_of = nullptr;
     SceneDev* d_scenes = nullptr;
     WallDev* d_walls = nullptr;
-    uint32_t *keys = nullptr, *rank = nullptr, *svals = nullptr, *slot_tmp = nullptr;
+    uint32_t *keys = nullptr, *svals = nullptr, *slot_tmp = nullptr;
     uint32_t *nbr_count = nullptr, *nbr = nullptr;
     uint32_t list_cap = 48;
     uint32_t *bucket_count = nullptr, *bucket_start = nullptr, *scan_part = nullptr;
@@ -131,7 +131,7 @@ struct xdem_ctx {
         A.pos = pos; A.vel = vel; A.acc = acc; A.ep = ep; A.ev = ev; A.ea = ea;
         A.force = force; A.moment = moment; A.mass = mass; A.inertia = inertia; A.tmax = tmax; A.rres = rres;
         A.scene_of = scene_of; A.scenes = d_scenes; A.walls = d_walls;
-        A.keys = keys; A.rank = rank; A.svals = svals; A.slot_tmp = slot_tmp;
+        A.keys = keys; A.svals = svals; A.slot_tmp = slot_tmp;
         A.bucket_count = bucket_count; A.bucket_start = bucket_start; A.scan_part = scan_part;
         A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap;
         A.sposf = sposf;
@@ -386,7 +386,6 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->d_scenes, sizeof(SceneDev) * c->scenes.size()));
     CTX_CHECK(c->alloc(&c->d_walls, sizeof(WallDev) * std::max<size_t>(1, c->walls.size())));
     CTX_CHECK(c->alloc(&c->keys, sizeof(uint32_t) * n));
-    CTX_CHECK(c->alloc(&c->rank, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->svals, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->slot_tmp, sizeof(uint32_t) * n));
     c->list_cap = std::max(48u, 2u * c->cap);

@@ -4,15 +4,15 @@
 // first and last sorted position of every cell (def:94-124; impl:317-360). Here the grid is
 // a counting sort over a dense bucket table:
 //   k_integrate_hash  Velocity-Verlet position/orientation update (impl:135-147), bucket key,
-//                     and an atomic count per bucket (the returned value is a provisional
-//                     rank inside the bucket)
+//                     and a count per bucket (an atomic add whose result is not waited for)
 //   k_scan_reduce     \
 //   k_scan_parts       > exclusive prefix of the counts: start[b] = first sorted slot of
 //   k_scan_apply      /  bucket b, start[nbucket] = N; the counts are cleared on the way
-//   k_scatter         provisional slot start[key] + rank for every sphere
+//   k_scatter         provisional slot start[key] + (atomic rank inside the bucket)
 //   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
 //                     smaller id, i.e. exactly the reference's (hash, index) order; writes
-//                     the sorted id and the FP32 position copy the pair search reads
+//                     the sorted id and the FP32 position copy the pair search reads, and
+//                     clears the bucket's count for the next step
 // The pair search k_pairs (xdem_step.hpp) is launched from here too: its FP64 pair
 // predicate must round exactly like the reference's.
 // The bucket of slot range [start[b], start[b+1]) needs no clearing between steps, and a
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_hash(StepArgs A)
     grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
     const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
     A.keys[i] = key;
-    A.rank[i] = atomicAdd(&A.bucket_count[key], 1u);
+    atomicAdd(&A.bucket_count[key], 1u);
 }
 
 // Sum of 16 consecutive buckets per thread, reduced per block of 4096 buckets.
@@ -169,7 +169,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(StepArgs A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.np) return;
-    A.slot_tmp[A.bucket_start[A.keys[i]] + A.rank[i]] = i;
+    const uint32_t key = A.keys[i];
+    A.slot_tmp[A.bucket_start[key] + atomicAdd(&A.bucket_count[key], 1u)] = i;
 }
 
 __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
@@ -183,6 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     for (uint32_t m = s; m < e; m++) r += A.slot_tmp[m] < k ? 1u : 0u;
     const uint32_t f = s + r;
     A.svals[f] = k;
+    A.bucket_count[key] = 0u;
     const double4 p = reinterpret_cast<const double4*>(A.pos)[k];
     const Grid& gr = A.scenes[A.scene_of ? A.scene_of[k] : 0u].grid;
     A.sposf[f] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), (float)p.w);

@@ -47,8 +47,8 @@ struct StepArgs {
     const uint32_t* scene_of;  // sphere -> scene (multi-scene only)
     const SceneDev* scenes;
     const WallDev* walls;
-    // grid: bucket key and provisional rank per sphere; dense bucket table
-    uint32_t* keys; uint32_t* rank;
+    // grid: bucket key per sphere; dense bucket table
+    uint32_t* keys;
     uint32_t* bucket_count;                  // zero between steps
     uint32_t* bucket_start;                  // first sorted slot of each bucket, [nbucket] = N
     uint32_t* scan_part;                     // per 4096-bucket tile sums / offsets
@@ -91,6 +91,8 @@ __device__ __forceinline__ uint32_t scene_index(const StepArgs& A, uint32_t i, b
 // by more than the rounding of the FP32 coordinates (|x| 2^-24 per coordinate and operand,
 // bounded by mg) and of the FP32 radius sum (factor 1 + 1e-6), so every pair the exact test
 // accepts is in the list. reach bounds the JKR separation distance of the scene's pairs.
+// The traversal works on bucket coordinates, i.e. the cell coordinates masked to the
+// grid, exactly as grid_bucket(grid_cell(position)) of the reference's traversal.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 {
@@ -101,19 +103,25 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
     if (i < A.np) {
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
         if (S.has_pp) {
-            const double4 ip4 = reinterpret_cast<const double4*>(A.pos)[i];
-            int gx, gy, gz;
-            grid_cell(S.grid, ip4.x, ip4.y, ip4.z, gx, gy, gz);
+            // the sphere's bucket coordinates from its key (dims are powers of two)
+            const Grid& G = S.grid;
+            const uint32_t kb = A.keys[i] - G.base;
+            const uint32_t bx = kb & G.mask_x;
+            const uint32_t by = (kb >> __builtin_ctz(G.dim_x)) & G.mask_y;
+            const uint32_t bz = kb >> __builtin_ctz(G.dim_xy);
             const float4 fi = A.sposf[j];
             const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
-            const float reachf = (float)S.cc.coh_reach;
+            // lim = r_j (1 + 1e-6) + [(r_i + reach)(1 + 1e-6) + mg]: the same bound with the
+            // sphere's own part hoisted (each FP32 rounding is far below the 1e-6 margin)
+            const float limi = ((fi.w + (float)S.cc.coh_reach) * 1.000001f) + mg;
             const uint32_t lc = A.list_cap;
             uint32_t* out = A.nbr + (size_t)i * lc;
-            const uint32_t bx0 = (uint32_t)(gx - 1) & S.grid.mask_x;
-            const uint32_t n1 = min(3u, S.grid.dim_x - bx0);   // buckets before the wrap
+            const uint32_t bx0 = (bx - 1u) & G.mask_x;
+            const uint32_t n1 = min(3u, G.dim_x - bx0);   // buckets before the wrap
             uint32_t row = 0, t = 0, s1 = 0, len1 = 0, s2 = 0, total = 0;
             auto load_row = [&](uint32_t r) {
-                const uint32_t rb = grid_bucket(S.grid, 0, gy + (int)(r % 3u) - 1, gz + (int)(r / 3u) - 1);
+                const uint32_t rb = G.base + ((bz + r / 3u - 1u) & G.mask_z) * G.dim_xy +
+                                    ((by + r % 3u - 1u) & G.mask_y) * G.dim_x;
                 s1 = A.bucket_start[rb + bx0];
                 const uint32_t e1 = A.bucket_start[rb + bx0 + n1];
                 s2 = A.bucket_start[rb];
@@ -130,7 +138,7 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
                     if (m == j) continue;
                     const float4 f = A.sposf[m];
                     const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
-                    const float lim = (fi.w + f.w + reachf) * 1.000001f + mg;
+                    const float lim = f.w * 1.000001f + limi;
                     if (dx * dx + dy * dy + dz * dz > lim * lim) continue;
                     if (n < lc) out[n] = A.svals[m];
                     n++;
