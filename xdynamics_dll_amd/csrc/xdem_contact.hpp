@@ -140,6 +140,39 @@ struct WallSlots {
     }
 };
 
+// True when the sphere cannot touch the wall, for a wall without cohesion. Every contact
+// region of plane_detection needs (dp . uw)^2 < r^2 max(1, |uw|^2): corners and edges test
+// (dp . uw)^2 plus non-negative terms against r^2, the face needs |dp . uw| / |uw| < r.
+// Beyond that, with a 1e-6 margin for rounding, the outcome is the no-contact one.
+__device__ __forceinline__ bool wall_out_of_reach_raw(double coh, V3<double> xw, V3<double> uw, double uw2,
+                                                      V4<double> ipos)
+{
+    if (coh != 0.0) return false;
+    const V3<double> dp = mk3(ipos.x, ipos.y, ipos.z) - xw;
+    const double wz = dot(dp, uw);
+    const double r = ipos.w;
+    return wz * wz >= (r * r) * fmax(1.0, uw2) * (1.0 + 1e-6);
+}
+
+__device__ __forceinline__ bool wall_out_of_reach(const ContactConst& cc, const xdem_plane_info& pl, V4<double> ipos)
+{
+    const double uw2 = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
+    return wall_out_of_reach_raw(cc.coh, mk3(pl.xw.x, pl.xw.y, pl.xw.z), mk3(pl.uw.x, pl.uw.y, pl.uw.z), uw2, ipos);
+}
+
+// The state update of wall_contact for a sphere out of the wall's reach: no force, the
+// slot count reset by the first wall (id 0), as impl:989 does.
+__device__ __forceinline__ void wall_no_contact(const xdem_plane_info& pl, WallSlots& ws, Accum& acc)
+{
+    uint32_t n_new = pl.id ? ws.count : 0u;
+    if (n_new > XDEM_MAX_P2PL_COUNT) n_new = XDEM_MAX_P2PL_COUNT;
+    ws.count = n_new;
+    acc.F = acc.F + mk3(0.0, 0.0, 0.0);
+    acc.M = acc.M + mk3(0.0, 0.0, 0.0);
+    acc.T = acc.T + mk3(0.0, 0.0, 0.0);
+    acc.R += 0.0;
+}
+
 template <class T, int MODEL, bool GPU_LAW>
 __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_plane_info& pl, const double* dbi, double dt,
                                              V4<double> ipos, V3<double> ivel, V3<double> iom, double m,
@@ -154,18 +187,8 @@ __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_
     V3<double> unit = mk3(0.0, 0.0, 0.0);
     const V3<double> wp = mk3(dot(dp, mk3(pl.u1.x, pl.u1.y, pl.u1.z)), dot(dp, mk3(pl.u2.x, pl.u2.y, pl.u2.z)),
                               dot(dp, mk3(pl.uw.x, pl.uw.y, pl.uw.z)));
-    // Exact early exit for a wall without cohesion. Every contact region of
-    // plane_detection needs (dp . uw)^2 < r^2 max(1, |uw|^2): corners and edges test
-    // (dp . uw)^2 plus non-negative terms against r^2, the face needs |dp . uw| / |uw| < r.
-    // Beyond that (with a 1e-6 margin for rounding) the sphere cannot touch the wall and
-    // the outcome is the no-contact one: zero force, slot count as below.
-    const double uw2 = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
-    if (cc.coh == 0.0 && wp.z * wp.z >= (r * r) * fmax(1.0, uw2) * (1.0 + 1e-6)) {
-        ws.count = n_new;
-        acc.F = acc.F + mk3(0.0, 0.0, 0.0);
-        acc.M = acc.M + mk3(0.0, 0.0, 0.0);
-        acc.T = acc.T + mk3(0.0, 0.0, 0.0);
-        acc.R += 0.0;
+    if (wall_out_of_reach(cc, pl, ipos)) {
+        wall_no_contact(pl, ws, acc);
         return false;
     }
     const double cdist = plane_detection(pl, ip3, wp, unit, r);

@@ -77,7 +77,7 @@ struct xdem_ctx {
     SceneDev* d_scenes = nullptr;
     WallDev* d_walls = nullptr;
     uint32_t *keys = nullptr, *svals = nullptr, *slot_tmp = nullptr;
-    uint32_t *nbr_count = nullptr, *nbr = nullptr;
+    uint32_t *nbr_count = nullptr, *nbr = nullptr, *work = nullptr;
     uint32_t list_cap = 48;
     uint32_t *bucket_count = nullptr, *bucket_start = nullptr, *scan_part = nullptr;
     float4* sposf = nullptr;
@@ -133,7 +133,7 @@ struct xdem_ctx {
         A.scene_of = scene_of; A.scenes = d_scenes; A.walls = d_walls;
         A.keys = keys; A.svals = svals; A.slot_tmp = slot_tmp;
         A.bucket_count = bucket_count; A.bucket_start = bucket_start; A.scan_part = scan_part;
-        A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap;
+        A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap; A.work = work;
         A.sposf = sposf;
         A.vel_out = vel; A.ev_out = ev;
         A.pp_count_in = pp_count[cur]; A.pp_id_in = pp_id[cur]; A.pp_tsd_in = pp_tsd[cur];
@@ -347,6 +347,8 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
             c->wslot_range.push_back(D.p_begin);
             c->wslot_range.push_back(D.p_end);
         }
+        if (W.info.id == 0) D.wall_id0 = 1;
+        if (W.wslot >= 0) D.wall_dynamic = 1;
         c->wall_slot.push_back(W.wslot);
         c->walls.push_back(W);
     }
@@ -390,6 +392,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->slot_tmp, sizeof(uint32_t) * n));
     c->list_cap = std::max(48u, 2u * c->cap);
     CTX_CHECK(c->alloc(&c->nbr_count, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->work, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->nbr, sizeof(uint32_t) * n * c->list_cap));
     const size_t nbt = bucket_table_size(c->nbucket);
     CTX_CHECK(c->alloc(&c->bucket_count, sizeof(uint32_t) * nbt));
@@ -410,7 +413,7 @@ int xdem_ctx_commit(xdem_ctx* c)
         CTX_CHECK(c->alloc(&c->d_wslot_range, sizeof(uint32_t) * 2 * c->nwslot));
     }
     CTX_CHECK(c->alloc(&c->err, sizeof(uint32_t)));
-    CTX_CHECK(c->alloc(&c->counters, sizeof(unsigned long long) * 2));
+    CTX_CHECK(c->alloc(&c->counters, sizeof(unsigned long long) * 4));
 
     // host staging of the initial state (xDiscreteElementMethodSimulation.cpp:60-83)
     std::vector<double> pos(4 * n), vel(3 * n), acc(3 * n, 0.0), ep(4 * n, 0.0), mass(n), inertia(3 * n);
@@ -459,7 +462,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(hipMemsetAsync(c->pl_id, 0, sizeof(uint32_t) * n * XDEM_MAX_P2PL_COUNT, s));
     CTX_CHECK(hipMemsetAsync(c->pl_tsd, 0, sizeof(double2) * n * XDEM_MAX_P2PL_COUNT, s));
     CTX_CHECK(hipMemsetAsync(c->err, 0, sizeof(uint32_t), s));
-    CTX_CHECK(hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 2, s));
+    CTX_CHECK(hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 4, s));
     if (c->nwslot) CTX_CHECK(hipMemsetAsync(c->wrench_out, 0, sizeof(double) * 6 * c->nwslot, s));
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nbt + 255) / 256)), dim3(256), 0, s, c->bucket_count, 0u, nbt);
     CTX_CHECK(hipGetLastError());

@@ -34,9 +34,10 @@ template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_integrate_hash(StepArgs A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {  // stored-contact counters of this step
+    if (i == 0) {  // stored-contact counters and full-pass queue length of this step
         A.counters[0] = 0ull;
         A.counters[1] = 0ull;
+        A.counters[2] = 0ull;
     }
     if (i >= A.np) return;
     const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];

@@ -26,6 +26,8 @@ struct SceneDev {
     uint32_t p_begin, p_end;  // sphere range of the scene
     uint32_t wall_begin, nwall;
     uint32_t has_pp;
+    uint32_t wall_id0;        // a wall of the scene has plane id 0 (it resets the wall slot count)
+    uint32_t wall_dynamic;    // a wall of the scene is a dynamic body (per-sphere wrench output)
     uint32_t pad_;
 };
 
@@ -54,6 +56,7 @@ struct StepArgs {
     uint32_t* scan_part;                     // per 4096-bucket tile sums / offsets
     uint32_t* slot_tmp;                      // sphere id per provisional slot
     uint32_t* svals;                         // sphere id per sorted slot
+    uint32_t* work;                          // spheres for the full contact pass
     uint32_t* nbr_count;                     // near neighbours per sphere (k_pairs)
     uint32_t* nbr;                           // their sphere ids, list_cap per sphere
     float4* sposf;  // per sorted slot: position relative to the scene's grid origin and
@@ -64,7 +67,8 @@ struct StepArgs {
     uint32_t* pl_count; uint32_t* pl_id; double2* pl_tsd;
     double* wrench;          // [nwslot][6][np] per-sphere contributions to dynamic walls
     uint32_t* err;
-    unsigned long long* counters;  // [0] stored sphere contacts, [1] stored wall contacts
+    unsigned long long* counters;  // [0] stored sphere contacts, [1] stored wall contacts,
+                                   // [2] spheres queued for the full contact pass
     uint32_t np, nsorted, cap;
     uint32_t nwslot, nbucket;
     uint32_t list_cap;                       // neighbour-list entries per sorted slot
@@ -75,6 +79,17 @@ struct StepArgs {
 __device__ __forceinline__ uint32_t scene_index(const StepArgs& A, uint32_t i, bool multi)
 {
     return multi ? A.scene_of[i] : 0u;
+}
+
+// Work-group ids are dealt round-robin to the 8 XCDs (each with its own L2). This maps the
+// id so that each XCD gets one contiguous eighth of the range: neighbouring slots, whose
+// candidates overlap, then share an L2. The map is a permutation of [0, n).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
+{
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = n / kXcd, extra = n % kXcd;   // the first `extra` XCDs get per + 1
+    const uint32_t x = b % kXcd, r = b / kXcd;
+    return x * per + min(x, extra) + r;
 }
 
 // Pair search: for the sphere in sorted slot j, the ids of every neighbour that can pass
@@ -96,7 +111,7 @@ __device__ __forceinline__ uint32_t scene_index(const StepArgs& A, uint32_t i, b
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (j >= A.nsorted) return;
     const uint32_t i = A.svals[j];
     uint32_t n = 0;
@@ -152,159 +167,194 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
     if (i < A.np) A.nbr_count[i] = n;
 }
 
-// Contacts of sphere i: the near neighbours k_pairs listed, in their order, under the exact
-// FP64 predicate; every wall of the scene in id order; then (fused mode) rolling resistance
-// and the velocity update. One thread per sphere id, so the sphere's own state and outputs
-// are contiguous; neighbours are read by id only for listed pairs.
-template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
-__global__ __launch_bounds__(256) void k_collide(StepArgs A)
+// A sphere's own state, loaded together before any branch (independent loads in flight at
+// once). Fused mode also needs what the velocity update reads.
+struct OwnState {
+    V4<double> pos, ep, ev;
+    V3<double> vel, t0;          // t0: carried rolling torque sum (zero unless it persists)
+    double mass;
+    V4<double> ea;               // fused mode only
+    V3<double> acc, inertia;     // fused mode only
+    double J;                    // fused mode only: rolling inertia (quirk Q5 aware)
+};
+
+// The fields of the velocity update (fused mode).
+__device__ __forceinline__ void load_own_fused(const StepArgs& A, const SceneDev& S, uint32_t i, OwnState& o)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n_pp = 0, n_wall = 0;
-    {
-        if (i < A.np) {
-            const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-            const V4<double> ipos = ld4(A.pos + 4 * i);
-            const V3<double> ivel = ld3(A.vel + 3 * i);
-            const V4<double> iep = ld4(A.ep + 4 * i);
-            V4<double> iev = ld4(A.ev + 4 * i);
-            const V3<double> iom = omega_of(iep, iev);
-            const double im = A.mass[i];
-            uint32_t err = 0;
+    o.ea = ld4(A.ea + 4 * i);
+    o.acc = ld3(A.acc + 3 * i);
+    o.inertia = ld3(A.inertia + 3 * i);
+    // quirk Q5: the scene's inertia array read as flat doubles at the scene-local id
+    o.J = A.flat_inertia ? A.inertia[i + 2 * S.p_begin] : o.inertia.x;
+}
 
-            // ---- sphere-sphere (impl:613-733)
-            Accum pp = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
-            if (S.has_pp) {
-                const uint32_t cap = A.cap;
-                const uint32_t hb = i * cap;
-                uint32_t old_count = A.pp_count_in[i];
-                if (old_count > cap) old_count = cap;
-                V3<T> Ft = mk3(T(0), T(0), T(0));
-                uint32_t n_new = 0, n_seen = 0;
-                const uint32_t nlist_all = A.nbr_count[i];
-                const uint32_t lc = A.list_cap;
-                if (nlist_all > lc) err |= XDEM_ERR_NBR_OVERFLOW;
-                const uint32_t nlist = min(nlist_all, lc);
-                const uint32_t* lst = A.nbr + (size_t)i * lc;
-                const double reach = S.cc.coh_reach;
-                for (uint32_t t = 0; t < nlist; t++) {
-                    const uint32_t k = lst[t];
-                    const V4<double> jpos = ld4(A.pos + 4 * (size_t)k);
-                    const PairGeom g = pair_geom(ipos, jpos);
-                    double sd0 = 0.0, sd1 = 0.0;
-                    if (g.cdist > 0) {
-                        for (uint32_t s = 0; s < old_count; s++)
-                            if (A.pp_id_in[hb + s] == k) {
-                                const double2 h = A.pp_tsd_in[hb + s];
-                                sd0 = h.x; sd1 = h.y;
-                                break;
-                            }
-                        const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
-                        const V3<double> jom = omega_of(ld4(A.ep + 4 * (size_t)k), ld4(A.ev + 4 * (size_t)k));
-                        if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
-                        sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, ipos, ivel, iom, im, jpos, jvel, jom,
-                                                          A.mass[k], g, sd0, sd1, Ft, pp);
-                    } else {
-                        if (!(reach > 0.0 && -g.cdist < reach)) continue;
-                        const double jr = jpos.w, ir = ipos.w;
-                        const double coh_s = cohesion_limit_depth(S.cc, jr ? (ir * jr) / (ir + jr) : ir);
-                        if (!(fabs(g.cdist) < fabs(coh_s))) continue;
-                        sphere_cohesion(S.cc, ir, jr, g, pp);
-                    }
-                    n_seen++;
-                    if (n_new < cap) {
-                        A.pp_id_out[hb + n_new] = k;
-                        A.pp_tsd_out[hb + n_new] = make_double2(sd0, sd1);
-                        n_new++;
-                    }
-                }
-                A.pp_count_out[i] = n_new;
-                if (n_seen > cap) err |= XDEM_ERR_P2P_OVERFLOW;
-                n_pp = n_new;
-            }
+// FUSED_TOO: also load the velocity-update fields now (the lite pass, which keeps few
+// values live; the full pass loads them just before its tail instead).
+template <bool FUSED_TOO>
+__device__ __forceinline__ OwnState load_own(const StepArgs& A, const SceneDev& S, uint32_t i)
+{
+    OwnState o;
+    o.pos = ld4(A.pos + 4 * i);
+    o.vel = ld3(A.vel + 3 * i);
+    o.ep = ld4(A.ep + 4 * i);
+    o.ev = ld4(A.ev + 4 * i);
+    o.mass = A.mass[i];
+    o.t0 = A.tmax_persist ? ld3(A.tmax + 3 * i) : mk3(0.0, 0.0, 0.0);
+    if (FUSED_TOO) load_own_fused(A, S, i, o);
+    return o;
+}
 
-            // ---- walls, in id order (impl:950-1080 once per wall)
-            Accum tot;
-            tot.F = mk3(0.0, 0.0, 0.0) + pp.F;
-            tot.M = mk3(0.0, 0.0, 0.0) + pp.M;
-            const V3<double> t0 = A.tmax_persist ? ld3(A.tmax + 3 * i) : mk3(0.0, 0.0, 0.0);
-            tot.T = t0 + pp.T;
-            tot.R = 0.0 + pp.R;
-            if (S.nwall) {
-                // wall slots are read only when a wall needs them and written back only
-                // when one changed (the values in memory are the same either way)
-                WallSlots ws;
-                const uint32_t count0 = A.pl_count[i];
-                ws.init(count0, A.pl_id + XDEM_MAX_P2PL_COUNT * i,
-                        reinterpret_cast<const double*>(A.pl_tsd + XDEM_MAX_P2PL_COUNT * i));
-                for (uint32_t w = 0; w < S.nwall; w++) {
-                    const WallDev& W = A.walls[S.wall_begin + w];
-                    double bf[6] = {0, 0, 0, 0, 0, 0};
-                    if (wall_contact<T, MODEL, GPU_LAW>(W.cc, W.info, W.dbi, S.dt, ipos, ivel, iom, im, ws, tot, bf))
-                        err |= XDEM_ERR_PLANE_OVERFLOW;
-                    if (W.wslot >= 0) {
-                        double* wr = A.wrench + (size_t)W.wslot * 6 * A.np;
-#pragma unroll
-                        for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = bf[c];
-                    }
-                }
-                if (ws.dirty) {
-#pragma unroll
-                    for (int s = 0; s < XDEM_MAX_P2PL_COUNT; s++) {
-                        A.pl_id[XDEM_MAX_P2PL_COUNT * i + s] = ws.id[s];
-                        A.pl_tsd[XDEM_MAX_P2PL_COUNT * i + s] = make_double2(ws.s0[s], ws.s1[s]);
-                    }
-                }
-                if (ws.count != count0) A.pl_count[i] = ws.count;
-                n_wall = ws.count;
-            }
-
-            if (FUSE) {
-                // ---- rolling resistance (impl:2420-2454)
-                const V4<double> e = iep;
-                V4<double> av = iev;
-                // quirk Q5: the scene's inertia array read as flat doubles at the scene-local id
-                const double J = A.flat_inertia ? A.inertia[i + 2 * S.p_begin] : A.inertia[3 * i];
-                const bool consumed = rolling_moment(S.dt, J, omega_of(e, av), tot.T, tot.R, tot.M);
-                if (A.tmax_persist) st3(A.tmax + 3 * i, consumed ? mk3(0.0, 0.0, 0.0) : tot.T);
-                // ---- velocity and orientation-rate update (xIntegratorVV.cpp:131-152; impl:192-222)
-                const double m = im;
-                const V4<double> aa = ld4(A.ea + 4 * i);
-                const V3<double> np_ = rot_local(tot.M, e);
-                const V4<double> mea = uceom(ld3(A.inertia + 3 * i), e, av, np_);
-                const double inv_m = 1.0 / m;
-                const V3<double> a = mk3(inv_m * (tot.F.x + m * S.gx), inv_m * (tot.F.y + m * S.gy),
-                                         inv_m * (tot.F.z + m * S.gz));
-                const double hdt = 0.5 * S.dt;
-                if (A.keep_velocity) {
-                    av.x = av.x + hdt * (aa.x + mea.x); av.y = av.y + hdt * (aa.y + mea.y);
-                    av.z = av.z + hdt * (aa.z + mea.z); av.w = av.w + hdt * (aa.w + mea.w);
-                } else {
-                    V3<double> v = ld3(A.vel + 3 * i);
-                    const V3<double> ao = ld3(A.acc + 3 * i);
-                    v = mk3(v.x + hdt * ao.x, v.y + hdt * ao.y, v.z + hdt * ao.z);
-                    av.x = av.x + hdt * aa.x; av.y = av.y + hdt * aa.y; av.z = av.z + hdt * aa.z; av.w = av.w + hdt * aa.w;
-                    v = mk3(v.x + hdt * a.x, v.y + hdt * a.y, v.z + hdt * a.z);
-                    av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y;
-                    av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
-                    st3(A.vel_out + 3 * i, v);
-                }
-                if (A.keep_velocity && A.vel_out != A.vel) st3(A.vel_out + 3 * i, ivel);
-                st4(A.ev_out + 4 * i, av);
-                st3(A.acc + 3 * i, a);
-                st4(A.ea + 4 * i, mea);
-                if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) err |= XDEM_ERR_NONFINITE;
-            } else {
-                st3(A.force + 3 * i, tot.F);
-                st3(A.moment + 3 * i, tot.M);
-                st3(A.tmax + 3 * i, tot.T);
-                A.rres[i] = tot.R;
-            }
-            if (err) atomicOr(A.err, err);
+// End of a sphere's contact pass. Fused mode: rolling resistance (impl:2420-2454) and the
+// velocity and orientation-rate update (xIntegratorVV.cpp:131-152; impl:192-222). Split
+// mode: the sums are stored for k_finish.
+template <bool FUSE>
+__device__ __forceinline__ void sphere_tail(const StepArgs& A, const SceneDev& S, uint32_t i, const OwnState& o,
+                                            Accum tot, uint32_t& err)
+{
+    if (FUSE) {
+        const V4<double> e = o.ep;
+        V4<double> av = o.ev;
+        const bool consumed = rolling_moment(S.dt, o.J, omega_of(e, av), tot.T, tot.R, tot.M);
+        if (A.tmax_persist) st3(A.tmax + 3 * i, consumed ? mk3(0.0, 0.0, 0.0) : tot.T);
+        const double m = o.mass;
+        const V4<double> aa = o.ea;
+        const V3<double> np_ = rot_local(tot.M, e);
+        const V4<double> mea = uceom(o.inertia, e, av, np_);
+        const double inv_m = 1.0 / m;
+        const V3<double> a = mk3(inv_m * (tot.F.x + m * S.gx), inv_m * (tot.F.y + m * S.gy),
+                                 inv_m * (tot.F.z + m * S.gz));
+        const double hdt = 0.5 * S.dt;
+        if (A.keep_velocity) {
+            av.x = av.x + hdt * (aa.x + mea.x); av.y = av.y + hdt * (aa.y + mea.y);
+            av.z = av.z + hdt * (aa.z + mea.z); av.w = av.w + hdt * (aa.w + mea.w);
+            if (A.vel_out != A.vel) st3(A.vel_out + 3 * i, o.vel);
+        } else {
+            V3<double> v = o.vel;
+            const V3<double> ao = o.acc;
+            v = mk3(v.x + hdt * ao.x, v.y + hdt * ao.y, v.z + hdt * ao.z);
+            av.x = av.x + hdt * aa.x; av.y = av.y + hdt * aa.y; av.z = av.z + hdt * aa.z; av.w = av.w + hdt * aa.w;
+            v = mk3(v.x + hdt * a.x, v.y + hdt * a.y, v.z + hdt * a.z);
+            av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y;
+            av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
+            st3(A.vel_out + 3 * i, v);
         }
+        st4(A.ev_out + 4 * i, av);
+        st3(A.acc + 3 * i, a);
+        st4(A.ea + 4 * i, mea);
+        if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) err |= XDEM_ERR_NONFINITE;
+    } else {
+        st3(A.force + 3 * i, tot.F);
+        st3(A.moment + 3 * i, tot.M);
+        st3(A.tmax + 3 * i, tot.T);
+        A.rres[i] = tot.R;
     }
-    // stored-contact counters: one atomic per wave
-    unsigned long long c0 = n_pp, c1 = n_wall;
+}
+
+// Full contact pass of sphere i: the near neighbours k_pairs listed, in their order, under
+// the exact FP64 predicate; every wall of the scene in id order; then sphere_tail. The
+// sphere's own state and outputs are read and written by id.
+template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, uint32_t& n_pp, uint32_t& n_wall)
+{
+    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    OwnState o = load_own<false>(A, S, i);
+    const V3<double> iom = omega_of(o.ep, o.ev);
+    uint32_t err = 0;
+
+    // ---- sphere-sphere (impl:613-733)
+    Accum pp = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
+    if (S.has_pp) {
+        const uint32_t cap = A.cap;
+        const uint32_t hb = i * cap;
+        uint32_t old_count = A.pp_count_in[i];
+        if (old_count > cap) old_count = cap;
+        V3<T> Ft = mk3(T(0), T(0), T(0));
+        uint32_t n_new = 0, n_seen = 0;
+        const uint32_t nlist_all = A.nbr_count[i];
+        const uint32_t lc = A.list_cap;
+        if (nlist_all > lc) err |= XDEM_ERR_NBR_OVERFLOW;
+        const uint32_t nlist = min(nlist_all, lc);
+        const uint32_t* lst = A.nbr + (size_t)i * lc;
+        const double reach = S.cc.coh_reach;
+        for (uint32_t t = 0; t < nlist; t++) {
+            const uint32_t k = lst[t];
+            const V4<double> jpos = ld4(A.pos + 4 * (size_t)k);
+            const PairGeom g = pair_geom(o.pos, jpos);
+            double sd0 = 0.0, sd1 = 0.0;
+            if (g.cdist > 0) {
+                for (uint32_t s = 0; s < old_count; s++)
+                    if (A.pp_id_in[hb + s] == k) {
+                        const double2 h = A.pp_tsd_in[hb + s];
+                        sd0 = h.x; sd1 = h.y;
+                        break;
+                    }
+                const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
+                const V3<double> jom = omega_of(ld4(A.ep + 4 * (size_t)k), ld4(A.ev + 4 * (size_t)k));
+                if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
+                sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, o.pos, o.vel, iom, o.mass, jpos, jvel, jom,
+                                                  A.mass[k], g, sd0, sd1, Ft, pp);
+            } else {
+                if (!(reach > 0.0 && -g.cdist < reach)) continue;
+                const double jr = jpos.w, ir = o.pos.w;
+                const double coh_s = cohesion_limit_depth(S.cc, jr ? (ir * jr) / (ir + jr) : ir);
+                if (!(fabs(g.cdist) < fabs(coh_s))) continue;
+                sphere_cohesion(S.cc, ir, jr, g, pp);
+            }
+            n_seen++;
+            if (n_new < cap) {
+                A.pp_id_out[hb + n_new] = k;
+                A.pp_tsd_out[hb + n_new] = make_double2(sd0, sd1);
+                n_new++;
+            }
+        }
+        A.pp_count_out[i] = n_new;
+        if (n_seen > cap) err |= XDEM_ERR_P2P_OVERFLOW;
+        n_pp = n_new;
+    }
+
+    // ---- walls, in id order (impl:950-1080 once per wall)
+    Accum tot;
+    tot.F = mk3(0.0, 0.0, 0.0) + pp.F;
+    tot.M = mk3(0.0, 0.0, 0.0) + pp.M;
+    tot.T = o.t0 + pp.T;
+    tot.R = 0.0 + pp.R;
+    if (S.nwall) {
+        // wall slots are read only when a wall needs them and written back only
+        // when one changed (the values in memory are the same either way)
+        WallSlots ws;
+        const uint32_t count0 = A.pl_count[i];
+        ws.init(count0, A.pl_id + XDEM_MAX_P2PL_COUNT * i,
+                reinterpret_cast<const double*>(A.pl_tsd + XDEM_MAX_P2PL_COUNT * i));
+        for (uint32_t w = 0; w < S.nwall; w++) {
+            const WallDev& W = A.walls[S.wall_begin + w];
+            double bf[6] = {0, 0, 0, 0, 0, 0};
+            if (wall_contact<T, MODEL, GPU_LAW>(W.cc, W.info, W.dbi, S.dt, o.pos, o.vel, iom, o.mass, ws, tot, bf))
+                err |= XDEM_ERR_PLANE_OVERFLOW;
+            if (W.wslot >= 0) {
+                double* wr = A.wrench + (size_t)W.wslot * 6 * A.np;
+#pragma unroll
+                for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = bf[c];
+            }
+        }
+        if (ws.dirty) {
+#pragma unroll
+            for (int s = 0; s < XDEM_MAX_P2PL_COUNT; s++) {
+                A.pl_id[XDEM_MAX_P2PL_COUNT * i + s] = ws.id[s];
+                A.pl_tsd[XDEM_MAX_P2PL_COUNT * i + s] = make_double2(ws.s0[s], ws.s1[s]);
+            }
+        }
+        if (ws.count != count0) A.pl_count[i] = ws.count;
+        n_wall = ws.count;
+    }
+
+    if (FUSE) load_own_fused(A, S, i, o);
+    sphere_tail<FUSE>(A, S, i, o, tot, err);
+    if (err) atomicOr(A.err, err);
+}
+
+// Wave-wide sums of the stored-contact counts, one atomic per wave.
+__device__ __forceinline__ void add_contact_counts(const StepArgs& A, unsigned long long c0, unsigned long long c1)
+{
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         c0 += __shfl_xor(c0, off, 64);
@@ -314,6 +364,104 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
         atomicAdd(&A.counters[0], c0);
         atomicAdd(&A.counters[1], c1);
     }
+}
+
+// Lite pass over all spheres. It completes every sphere with no listed neighbour and no
+// wall within reach (wall_out_of_reach): the full pass would add only zero sums, so the
+// lite pass applies the same tail to zero sums, and the walls' slot-count rule of
+// wall_no_contact through the scene's summary (the count drops to 0 if the scene has a wall
+// with id 0, impl:989). All loads are issued before the first branch, and the walls of a
+// single-scene launch are read from LDS. Other spheres are appended to A.work (one atomic
+// per wave; their order in the list does not affect any result).
+template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
+__global__ __launch_bounds__(256) void k_collide_lite(StepArgs A)
+{
+    __shared__ double wall_sh[kMaxWallsPerScene][8];   // xw, uw, |uw|^2, cohesion flag
+    if (!MULTI) {
+        const SceneDev& S0 = A.scenes[0];
+        const uint32_t nw = min(S0.nwall, (uint32_t)kMaxWallsPerScene);
+        if (threadIdx.x < nw) {
+            const WallDev& W = A.walls[S0.wall_begin + threadIdx.x];
+            const xdem_plane_info& pl = W.info;
+            wall_sh[threadIdx.x][0] = pl.xw.x; wall_sh[threadIdx.x][1] = pl.xw.y; wall_sh[threadIdx.x][2] = pl.xw.z;
+            wall_sh[threadIdx.x][3] = pl.uw.x; wall_sh[threadIdx.x][4] = pl.uw.y; wall_sh[threadIdx.x][5] = pl.uw.z;
+            wall_sh[threadIdx.x][6] = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
+            wall_sh[threadIdx.x][7] = W.cc.coh;
+        }
+        __syncthreads();
+    }
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n_pp = 0, n_wall = 0;
+    bool full = false;
+    if (i < A.np) {
+        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+        const OwnState o = load_own<FUSE>(A, S, i);
+        const uint32_t nb = S.has_pp ? A.nbr_count[i] : 0u;
+        const uint32_t count0 = S.nwall ? A.pl_count[i] : 0u;
+        bool far = nb == 0u;
+        if (!MULTI && S.nwall <= kMaxWallsPerScene) {
+            for (uint32_t w = 0; w < S.nwall; w++) {
+                const double* q = wall_sh[w];
+                far = far & wall_out_of_reach_raw(q[7], mk3(q[0], q[1], q[2]), mk3(q[3], q[4], q[5]), q[6], o.pos);
+            }
+        } else {
+            for (uint32_t w = 0; w < S.nwall; w++) {
+                const WallDev& W = A.walls[S.wall_begin + w];
+                far = far & wall_out_of_reach(W.cc, W.info, o.pos);
+            }
+        }
+        if (far) {
+            uint32_t err = 0;
+            if (S.has_pp) A.pp_count_out[i] = 0u;
+            Accum tot;
+            tot.F = mk3(0.0, 0.0, 0.0);
+            tot.M = mk3(0.0, 0.0, 0.0);
+            tot.T = o.t0 + mk3(0.0, 0.0, 0.0);
+            tot.R = 0.0;
+            if (S.nwall) {
+                if (S.wall_dynamic)
+                    for (uint32_t w = 0; w < S.nwall; w++) {
+                        const int32_t slot = A.walls[S.wall_begin + w].wslot;
+                        if (slot >= 0) {
+                            double* wr = A.wrench + (size_t)slot * 6 * A.np;
+#pragma unroll
+                            for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
+                        }
+                    }
+                const uint32_t count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
+                if (count != count0) A.pl_count[i] = count;
+                n_wall = count;
+            }
+            sphere_tail<FUSE>(A, S, i, o, tot, err);
+            if (err) atomicOr(A.err, err);
+        } else {
+            full = true;
+        }
+    }
+    const unsigned long long mask = __ballot(full);
+    if (mask) {
+        const uint32_t lane = threadIdx.x & 63u;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(&A.counters[2], (unsigned long long)__popcll(mask));
+        base = __shfl(base, 0, 64);
+        if (full) A.work[base + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+    }
+    add_contact_counts(A, n_pp, n_wall);
+}
+
+// Full pass over the spheres the lite pass queued (grid-stride).
+template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
+__global__ __launch_bounds__(256) void k_collide(StepArgs A)
+{
+    const uint32_t nwork = (uint32_t)A.counters[2];
+    unsigned long long c0 = 0, c1 = 0;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwork; w += gridDim.x * blockDim.x) {
+        uint32_t n_pp = 0, n_wall = 0;
+        collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, A.work[w], n_pp, n_wall);
+        c0 += n_pp;
+        c1 += n_wall;
+    }
+    add_contact_counts(A, c0, c1);
 }
 
 // Second half of a split step: rolling resistance and the velocity update from the

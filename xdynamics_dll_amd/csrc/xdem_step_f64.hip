@@ -1,6 +1,8 @@
 // xdem_step_f64.hip -- FP64 instantiation of the contact and velocity kernels.
 // Built with -ffp-contract=off so every multiply and add rounds separately, in the
 // reference's operation order.
+#include <algorithm>
+
 #include "xdem_step.hpp"
 
 namespace xdem {
@@ -9,16 +11,28 @@ namespace {
 constexpr unsigned kBlock = 256;
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
+// Lite pass over every sphere, then the full pass over the spheres it queued (grid-stride
+// over at most kFullBlocks blocks; the queue length is read on the device).
+constexpr unsigned kFullBlocks = 1024;
+
+template <bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
+void collide_passes(const StepArgs& A, hipStream_t s)
+{
+    const dim3 g = grid_for(A.np), b(kBlock);
+    const dim3 gf(std::min(g.x, kFullBlocks));
+    hipLaunchKernelGGL((k_collide_lite<double, MULTI, FUSE, MODEL, GPU_LAW>), g, b, 0, s, A);
+    hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW>), gf, b, 0, s, A);
+}
+
 template <bool MULTI, bool FUSE>
 void collide_models(const StepArgs& A, int model, bool gpu_law, hipStream_t s)
 {
-    const dim3 g = grid_for(A.np), b(kBlock);
     if (model == 1) {
-        if (gpu_law) hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, 1, true>), g, b, 0, s, A);
-        else hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, 1, false>), g, b, 0, s, A);
+        if (gpu_law) collide_passes<MULTI, FUSE, 1, true>(A, s);
+        else collide_passes<MULTI, FUSE, 1, false>(A, s);
     } else {
-        if (gpu_law) hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, 0, true>), g, b, 0, s, A);
-        else hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, 0, false>), g, b, 0, s, A);
+        if (gpu_law) collide_passes<MULTI, FUSE, 0, true>(A, s);
+        else collide_passes<MULTI, FUSE, 0, false>(A, s);
     }
 }
 
