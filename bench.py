@@ -35,7 +35,7 @@ def step_bytes(cbar: float) -> float:
 
 
 def contact_kernel_bytes(cbar: float, fused: bool) -> float:
-    """Algorithmic bytes per sphere of the contact kernel k_collide.
+    """Algorithmic bytes per sphere of the contact pass (k_collide_lite + k_collide).
     Fused (contacts + rolling + velocity half): own state read pos 32, vel 24, acc 24,
     ep 32, ev 32, ea 32, mass 8, inertia 24 (= 208); state write vel 24, acc 24, ev 32,
     ea 32 (= 112); sphere and wall history: counts 2*8, entries 2*20*cbar.
@@ -138,9 +138,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_collide_dam_break_1m.js
 
 
 def measured_traffic(np_, fused):
-    """HBM bytes per k_collide launch from the committed rocprofv3 --pmc summary (FETCH_SIZE
-    and WRITE_SIZE per dispatch, each scaled by the ratio the same counter shows on a plain
-    device copy of known size), or None when no summary matches this workload."""
+    """HBM bytes per step of the contact pass (k_collide_lite + k_collide) from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the ratio
+    the same counter shows on a plain device copy of known size), or None when no summary
+    matches this workload."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -179,7 +180,8 @@ def main():
     ap.add_argument("--capacity", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--fuse", action="store_true", help="rolling + velocity update inside the contact kernel")
+    ap.add_argument("--split", action="store_true",
+                    help="rolling + velocity update in a kernel of their own (default: inside the contact pass)")
     ap.add_argument("--buckets-per-sphere", type=int, default=0,
                     help="bucket budget of the grid per sphere (0 = library default)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -202,12 +204,13 @@ def main():
     t_build = time.perf_counter() - t_build
     n = scene.np
 
-    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, args.fuse, args.buckets_per_sphere)
+    fuse = not args.split
+    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere)
     value = n * args.steps / mixed["seconds"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
     k_ms = tm["contact_ms"] / max(tm["steps"], 1)
-    k_bytes = contact_kernel_bytes(cbar, args.fuse) * n
+    k_bytes = contact_kernel_bytes(cbar, fuse) * n
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     step_gbps = step_bytes(cbar) * n * args.steps / mixed["seconds"] / 1e9
 
@@ -229,9 +232,10 @@ def main():
                    "np": n, "seed": SEED, "p2p_capacity": args.capacity, "parallelism": "single GPU",
                    "nbucket": mixed["info"]["nbucket"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(n, args.fuse),
-                     "kernel": "k_collide (" + ("contacts + rolling + velocity update" if args.fuse else "contacts") + ")",
-                     "kernel_ms": k_ms, "bytes_per_sphere": contact_kernel_bytes(cbar, args.fuse)},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(n, fuse),
+                     "kernel": "contact pass k_collide_lite + k_collide (" +
+                               ("contacts, rolling, velocity update" if fuse else "contacts") + ")",
+                     "kernel_ms": k_ms, "bytes_per_sphere": contact_kernel_bytes(cbar, fuse)},
         "step_gbps": step_gbps,
         "step_roofline_frac": step_gbps / HBM_PEAK_GBPS,
         "mean_contacts_per_sphere": cbar,
@@ -243,7 +247,7 @@ def main():
     }
     if not args.no_fp64:
         f64 = run_gpu(scene, "fp64", max(20, args.steps // 2), max(5, args.warmup // 2), args.capacity,
-                      args.fuse, args.buckets_per_sphere)
+                      fuse, args.buckets_per_sphere)
         f64_value = n * max(20, args.steps // 2) / f64["seconds"]
         out["fp64_value"] = f64_value
         out["mixed_over_fp64"] = value / f64_value
