@@ -480,3 +480,44 @@ def test_reduction_is_an_fp64_sum():
     x = rng.normal(size=100003) * 10.0 ** rng.uniform(-5, 5, size=100003)
     got = L.ref_reduction(ptr(x), x.size)
     assert abs(got - math.fsum(x)) <= x.size * np.finfo(float).eps * np.abs(x).sum()
+
+
+def test_mesh_face_contact_is_hertz_and_balances_the_body():
+    """A sphere pressed on the face of one large triangle of a body at rest: the force is the
+    Hertz normal force with R* = r (the wall side has no radius), along the face normal, and
+    the body receives the opposite force (impl:1748-1835)."""
+    from mesh_scenes import triangle_infos
+
+    r, delta = 1e-3, 4e-6
+    pos = np.array([[0.01, 0.01, r - delta, r]])
+    cp = contact_property(EI, EJ, NUI, NUJ, REST, FRIC, 0.0)
+    s = SceneData.from_positions(pos, 2500.0, cp, walls=(), dt=1e-5)
+    prm = s.dem_parameters()
+    q = xdem_ref.quirks_struct({})
+    tri = triangle_infos(1)
+    for name, v in zip("PQR", ((-0.05, -0.05, 0.0), (0.1, -0.05, 0.0), (0.01, 0.1, 0.0))):
+        setattr(tri[0], name, _lib.double3(*v))
+    dbi = np.zeros(15); dbi[0] = 5.0; dbi[7] = 1.0
+    ncell = 128 ** 3
+    sph = np.array([[0.0, 0.0, 0.0, 0.02]])
+    h = np.zeros(2, np.uint32); idx = np.zeros(2, np.uint32)
+    L.ref_calculate_hash_and_index(C.byref(prm), ptr(h), ptr(idx), ptr(pos), 0, 1)
+    # the proxy sphere binned in the sphere's own cell, so the scan finds the triangle
+    sph[0, :3] = pos[0, :3]
+    L.ref_calculate_hash_and_index(C.byref(prm), ptr(h), ptr(idx), ptr(sph), 1, 1)
+    cs = np.zeros(ncell, np.uint32); ce = np.zeros(ncell, np.uint32); srt = np.zeros(2, np.uint32)
+    L.ref_reorder_data_and_find_cell_start(ptr(h), ptr(idx), ptr(cs), ptr(ce), ptr(srt), 2, ncell)
+    F = np.zeros((1, 3)); M = np.zeros((1, 3)); T = np.zeros((1, 3)); R = np.zeros(1)
+    cnt = np.zeros(1, np.uint32); ids = np.zeros((1, 5), np.uint32); tsd = np.zeros((1, 5, 2))
+    bf = [np.zeros(1) for _ in range(6)]
+    z4, z3 = np.zeros((1, 4)), np.zeros((1, 3))
+    ep = np.array([[1.0, 0.0, 0.0, 0.0]])
+    L.ref_particle_polygon_collision(C.byref(prm), C.byref(q), C.addressof(tri), ptr(dbi), *(ptr(b) for b in bf),
+                                     ptr(pos), ptr(ep), ptr(z3), ptr(z4), ptr(F), ptr(M), ptr(s.mass), ptr(T), ptr(R),
+                                     ptr(cnt), ptr(ids), ptr(tsd), ptr(srt), ptr(cs), ptr(ce), C.byref(cp), 1, 1, 5)
+    kn = (4.0 / 3.0) * ESTAR * math.sqrt(r)
+    assert cnt[0] == 1 and ids[0, 0] == 0
+    fz = kn * delta ** 1.5
+    # x and y only carry the rounding of the closest point (barycentric), ~1e-14 relative
+    assert np.allclose(F[0], [0.0, 0.0, fz], rtol=1e-12, atol=1e-12 * fz)
+    assert np.allclose([bf[0][0], bf[1][0], bf[2][0]], -F[0], rtol=1e-12, atol=1e-12 * fz)
