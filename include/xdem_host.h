@@ -46,7 +46,8 @@ typedef struct xdem_scene {
 /* Workload ids for xdem_scene_build. */
 #define XDEM_SCENE_BOX_LATTICE   1  /* 10,000 spheres r=1 mm on a 2.2 mm jittered lattice, 0.1x0.1x0.3 m box */
 #define XDEM_SCENE_DAM_BREAK     2  /* 1,000,000 spheres r~U[1,2] mm, 0.3x0.3x0.6 m column, 1.2x0.3x0.7 m box */
-#define XDEM_SCENE_BATCH_MEMBER  4  /* one box-settling scene of the 256-scene sweep (index = variant) */
+#define XDEM_SCENE_BATCH_MEMBER  4  /* one box-settling scene of the 256-scene sweep (index = variant);
+                                     dt = min(1e-5 s, 0.2 x Rayleigh time of its smallest sphere) */
 #define XDEM_SCENE_PILE          5  /* 20,000 spheres r~U[1,1.5] mm in a 30 mm cylinder region over a floor */
 
 /* Build a scene. np_override > 0 replaces the particle count (same generator, same

@@ -105,10 +105,12 @@ def test_mixed_force_error_and_pairs(name):
     assert err.max() <= 1e-4
 
 
-def test_multi_scene_equals_single_scenes():
+@pytest.mark.parametrize("precision,fuse", [("fp64", False), ("fp64", True), ("mixed", True)])
+def test_multi_scene_equals_single_scenes(precision, fuse):
     a, b = compressed_lattice(), block_scene()
-    both = DemContext([a, b], precision="fp64")
-    sa, sb = DemContext(a, precision="fp64"), DemContext(b, precision="fp64")
+    both = DemContext([a, b], precision=precision, fuse_velocity=fuse)
+    sa = DemContext(a, precision=precision, fuse_velocity=fuse)
+    sb = DemContext(b, precision=precision, fuse_velocity=fuse)
     both.step(5)
     sa.step(5)
     sb.step(5)

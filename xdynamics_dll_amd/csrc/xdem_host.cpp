@@ -392,7 +392,12 @@ int xdem_scene_build(int workload, unsigned long long seed, unsigned int np_over
         xdem_contact_property_make(E, E, 0.3, 0.3, rest, mu, 0.02, 0.0, 0.0, &s->cp_pp);
         box_walls(side, side, side, pl, dbi);
         set_walls(s, pl, dbi, s->cp_pp);
-        s->dt = 1e-5;
+        // dt = 1e-5 s, capped at 0.2 of the Rayleigh time of the smallest sphere
+        // (T_R = pi r sqrt(rho / G) / (0.1631 nu + 0.8766), G = E / (2 (1 + nu))): the
+        // stiffest scenes (E near 1e8 Pa, r = 0.5 mm) diverge at 1e-5 s.
+        const double rmin = radii.back(), nu = 0.3, G = E / (2.0 * (1.0 + nu));
+        const double t_rayleigh = pi * rmin * std::sqrt(2500.0 / G) / (0.1631 * nu + 0.8766);
+        s->dt = std::min(1e-5, 0.2 * t_rayleigh);
         s->nsteps = 1000;
         return XDEM_OK;
     }
