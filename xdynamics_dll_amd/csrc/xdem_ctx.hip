@@ -87,6 +87,12 @@ struct xdem_ctx {
     double* vel_buf[2] = {nullptr, nullptr};
     double* ev_buf[2] = {nullptr, nullptr};
     int sv = 0;
+    // pos and ep: two buffers each; a step's velocity update writes the next step's drift
+    // into the other one (drift_valid), which the next step makes current
+    double* pos_buf[2] = {nullptr, nullptr};
+    double* ep_buf[2] = {nullptr, nullptr};
+    int spp = 0;
+    bool drift_valid = false;
     uint32_t* pp_count[2] = {nullptr, nullptr};
     uint32_t* pp_id[2] = {nullptr, nullptr};
     double2* pp_tsd[2] = {nullptr, nullptr};
@@ -105,7 +111,7 @@ struct xdem_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     unsigned graph_steps = 0;
-    int graph_parity = 0, graph_sparity = 0;
+    int graph_parity = 0, graph_sparity = 0, graph_pparity = 0;
 
     bool timing = false;
     std::vector<hipEvent_t> events;  // kEventsPerStep per timed step
@@ -136,6 +142,8 @@ struct xdem_ctx {
         A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap; A.work = work;
         A.sposf = sposf;
         A.vel_out = vel; A.ev_out = ev;
+        A.pos_next = pos_buf[spp ^ 1]; A.ep_next = ep_buf[spp ^ 1];
+        A.drift = 1;
         A.pp_count_in = pp_count[cur]; A.pp_id_in = pp_id[cur]; A.pp_tsd_in = pp_tsd[cur];
         A.pp_count_out = pp_count[cur ^ 1]; A.pp_id_out = pp_id[cur ^ 1]; A.pp_tsd_out = pp_tsd[cur ^ 1];
         A.pl_count = pl_count; A.pl_id = pl_id; A.pl_tsd = pl_tsd;
@@ -152,13 +160,18 @@ struct xdem_ctx {
     hipError_t collide_half(bool fuse, hipEvent_t* ev)
     {
         hipError_t e;
+        if (ev && (e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
+        if (!drift_valid) {
+            if ((e = launch_drift(args(), multi, stream)) != hipSuccess) return e;
+            drift_valid = true;
+        }
+        set_pos_parity(spp ^ 1);   // the drifted positions become current
         StepArgs A = args();
+        A.drift = fuse ? 1 : 0;    // split mode: k_finish computes the next drift
         if (fuse) {
             A.vel_out = vel_buf[sv ^ 1];
             A.ev_out = ev_buf[sv ^ 1];
         }
-        if (ev && (e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
-        if ((e = launch_integrate_hash(A, multi, stream)) != hipSuccess) return e;
         if ((e = launch_grid(A, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
         if ((e = launch_pairs(A, multi, stream)) != hipSuccess) return e;
@@ -172,6 +185,7 @@ struct xdem_ctx {
         }
         cur ^= 1;
         if (fuse) set_state_parity(sv ^ 1);
+        drift_valid = fuse;
         return hipSuccess;
     }
 
@@ -180,6 +194,23 @@ struct xdem_ctx {
         sv = p;
         vel = vel_buf[p];
         ev = ev_buf[p];
+    }
+
+    void set_pos_parity(int p)
+    {
+        spp = p;
+        pos = pos_buf[p];
+        ep = ep_buf[p];
+    }
+
+    // Rolling resistance + velocity update (+ the next drift) of a split step.
+    hipError_t finish_half(bool zero_force)
+    {
+        StepArgs A = args();
+        A.zero_force = zero_force ? 1 : 0;
+        const hipError_t e = launch_finish(A, multi, stream);
+        if (e == hipSuccess) drift_valid = true;
+        return e;
     }
 
     // One full step; ev (optional, kEventsPerStep) brackets grid, pair search, contacts and
@@ -193,9 +224,8 @@ struct xdem_ctx {
             return hipSuccess;
         }
         if ((e = collide_half(false, ev)) != hipSuccess) return e;
-        StepArgs A = args();
-        A.zero_force = 0;  // the next contact pass overwrites force and moment
-        if ((e = launch_finish(A, multi, stream)) != hipSuccess) return e;
+        // the next contact pass overwrites force and moment: no need to clear them
+        if ((e = finish_half(false)) != hipSuccess) return e;
         if (ev) return hipEventRecord(ev[4], stream);
         return hipSuccess;
     }
@@ -368,12 +398,16 @@ int xdem_ctx_commit(xdem_ctx* c)
     const size_t n = c->np;
     c->key_bits = 1;
     while (c->key_bits < 32 && (1ull << c->key_bits) < (unsigned long long)c->nbucket) c->key_bits++;
-    CTX_CHECK(c->alloc(&c->pos, sizeof(double) * 4 * n));
+    CTX_CHECK(c->alloc(&c->pos_buf[0], sizeof(double) * 4 * n));
+    CTX_CHECK(c->alloc(&c->pos_buf[1], sizeof(double) * 4 * n));
     CTX_CHECK(c->alloc(&c->vel_buf[0], sizeof(double) * 3 * n));
     if (c->cfg.fuse_velocity) CTX_CHECK(c->alloc(&c->vel_buf[1], sizeof(double) * 3 * n));
     c->vel = c->vel_buf[0];
     CTX_CHECK(c->alloc(&c->acc, sizeof(double) * 3 * n));
-    CTX_CHECK(c->alloc(&c->ep, sizeof(double) * 4 * n));
+    CTX_CHECK(c->alloc(&c->ep_buf[0], sizeof(double) * 4 * n));
+    CTX_CHECK(c->alloc(&c->ep_buf[1], sizeof(double) * 4 * n));
+    c->set_pos_parity(0);
+    c->drift_valid = false;
     CTX_CHECK(c->alloc(&c->ev_buf[0], sizeof(double) * 4 * n));
     if (c->cfg.fuse_velocity) CTX_CHECK(c->alloc(&c->ev_buf[1], sizeof(double) * 4 * n));
     c->ev = c->ev_buf[0];
@@ -533,7 +567,7 @@ int xdem_ctx_finish(xdem_ctx* c)
     if (!c) return XDEM_E_INVALID;
     if (!c->committed) return XDEM_E_NOT_READY;
     CTX_CHECK(hipSetDevice(c->cfg.device));
-    CTX_CHECK(launch_finish(c->args(), c->multi, c->stream));
+    CTX_CHECK(c->finish_half(true));
     c->steps_done++;
     return XDEM_OK;
 }
@@ -591,6 +625,7 @@ int xdem_ctx_write(xdem_ctx* c, int field, unsigned int scene, const double* in)
     CTX_CHECK(hipSetDevice(c->cfg.device));
     CTX_CHECK(hipMemcpyAsync(p + (size_t)w * h.p_begin, in, sizeof(double) * w * h.s.np, hipMemcpyHostToDevice, c->stream));
     CTX_CHECK(hipStreamSynchronize(c->stream));
+    c->drift_valid = false;   // the precomputed drift used the old state
     return XDEM_OK;
 }
 
@@ -602,6 +637,7 @@ int xdem_ctx_device_ptr(xdem_ctx* c, int field, void** out)
     int w;
     if (field_info(c, field, &p, &w)) return XDEM_E_INVALID;
     *out = p;
+    c->drift_valid = false;   // the caller may write the state through the pointer
     return XDEM_OK;
 }
 
@@ -766,8 +802,12 @@ int xdem_ctx_graph_capture(xdem_ctx* c, unsigned int nsteps)
     CTX_CHECK(hipSetDevice(c->cfg.device));
     if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    if (!c->drift_valid) {   // the graph's steps start from a precomputed drift
+        CTX_CHECK(launch_drift(c->args(), c->multi, c->stream));
+        c->drift_valid = true;
+    }
     CTX_CHECK(hipStreamSynchronize(c->stream));
-    const int parity = c->cur, sparity = c->sv;
+    const int parity = c->cur, sparity = c->sv, pparity = c->spp;
     CTX_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (unsigned k = 0; k < nsteps && e == hipSuccess; k++) e = c->full_step(nullptr);
@@ -775,8 +815,11 @@ int xdem_ctx_graph_capture(xdem_ctx* c, unsigned int nsteps)
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     c->cur = parity;
     c->set_state_parity(sparity);
+    c->set_pos_parity(pparity);
+    c->drift_valid = true;
     c->graph_parity = parity;
     c->graph_sparity = sparity;
+    c->graph_pparity = pparity;
     if (e != hipSuccess || e2 != hipSuccess) {
         if (g) hipGraphDestroy(g);
         return XDEM_E_HIP;
@@ -793,7 +836,8 @@ int xdem_ctx_graph_launch(xdem_ctx* c, unsigned int nlaunch)
     if (!c->gexec) return XDEM_E_NOT_READY;
     // the graph's buffers are those of the parity it was recorded at (an odd number of
     // eager steps since then swaps them)
-    if (c->cur != c->graph_parity || c->sv != c->graph_sparity) return XDEM_E_NOT_READY;
+    if (c->cur != c->graph_parity || c->sv != c->graph_sparity || c->spp != c->graph_pparity || !c->drift_valid)
+        return XDEM_E_NOT_READY;
     CTX_CHECK(hipSetDevice(c->cfg.device));
     for (unsigned k = 0; k < nlaunch; k++) {
         CTX_CHECK(hipGraphLaunch(c->gexec, c->stream));

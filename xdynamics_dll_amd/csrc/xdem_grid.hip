@@ -1,23 +1,25 @@
-// xdem_grid.hip -- position update and the bucket grid of the fused step (xdem_ctx path).
+// xdem_grid.hip -- the bucket grid of the fused step (xdem_ctx path).
 //
 // The reference sorts (hash, index) pairs with a radix sort every step and then finds the
 // first and last sorted position of every cell (def:94-124; impl:317-360). Here the grid is
-// a counting sort over a dense bucket table:
-//   k_integrate_hash  Velocity-Verlet position/orientation update (impl:135-147), bucket key,
-//                     and a count per bucket (an atomic add whose result is not waited for)
+// a counting sort over a dense bucket table. The drift of a step (position and orientation
+// update, impl:135-147) is computed at the end of the previous step's velocity update
+// (drift_sphere), which also writes each sphere's bucket key and counts it (posted atomic
+// add); k_drift does the same when no previous step left it. Then:
 //   k_scan_reduce     \
 //   k_scan_parts       > exclusive prefix of the counts: start[b] = first sorted slot of
 //   k_scan_apply      /  bucket b, start[nbucket] = N; the counts are cleared on the way
-//   k_scatter         provisional slot start[key] + (atomic rank inside the bucket)
+//   k_scatter         provisional slot start[key] + (atomic rank inside the bucket); the
+//                     counts are back to the occupancy afterwards
 //   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
 //                     smaller id, i.e. exactly the reference's (hash, index) order; writes
 //                     the sorted id and the FP32 position copy the pair search reads, and
-//                     clears the bucket's count for the next step
+//                     clears the bucket's count for the next step's drift
 // The pair search k_pairs (xdem_step.hpp) is launched from here too: its FP64 pair
 // predicate must round exactly like the reference's.
 // The bucket of slot range [start[b], start[b+1]) needs no clearing between steps, and a
 // row of three consecutive buckets is the single range [start[b], start[b+3]).
-// Built with -ffp-contract=off: the position update is bit-identical to the reference's.
+// Built with -ffp-contract=off: the drift is bit-identical to the reference's.
 #include "xdem_step.hpp"
 
 namespace xdem {
@@ -30,37 +32,17 @@ constexpr unsigned kPartsBlock = 1024;
 
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
+// Drift of every sphere from the current state into the next position/orientation buffer
+// (impl:135-147), with bucket keys and counts: used when the previous step did not leave its
+// drift (the first step, or after the state was written from the host).
 template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_integrate_hash(StepArgs A)
+__global__ __launch_bounds__(kBlock) void k_drift(StepArgs A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {  // stored-contact counters and full-pass queue length of this step
-        A.counters[0] = 0ull;
-        A.counters[1] = 0ull;
-        A.counters[2] = 0ull;
-    }
     if (i >= A.np) return;
     const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-    const V4<double> e = ld4(A.ep + 4 * i), d = ld4(A.ev + 4 * i), a = ld4(A.ea + 4 * i);
-    V4<double> ne;
-    ne.x = e.x + S.dt * d.x + S.half2dt * a.x;
-    ne.y = e.y + S.dt * d.y + S.half2dt * a.y;
-    ne.z = e.z + S.dt * d.z + S.half2dt * a.z;
-    ne.w = e.w + S.dt * d.w + S.half2dt * a.w;
-    const double l = sqrt(ne.x * ne.x + ne.y * ne.y + ne.z * ne.z + ne.w * ne.w);
-    ne.x = ne.x / l; ne.y = ne.y / l; ne.z = ne.z / l; ne.w = ne.w / l;
-    st4(A.ep + 4 * i, ne);
-    const V3<double> v = ld3(A.vel + 3 * i), ac = ld3(A.acc + 3 * i);
-    V4<double> p = ld4(A.pos + 4 * i);
-    p.x += S.dt * v.x + S.half2dt * ac.x;
-    p.y += S.dt * v.y + S.half2dt * ac.y;
-    p.z += S.dt * v.z + S.half2dt * ac.z;
-    st4(A.pos + 4 * i, p);
-    int gx, gy, gz;
-    grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
-    const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
-    A.keys[i] = key;
-    atomicAdd(&A.bucket_count[key], 1u);
+    drift_sphere(A, S, i, ld4(A.pos + 4 * i), ld4(A.ep + 4 * i), ld3(A.vel + 3 * i), ld3(A.acc + 3 * i),
+                 ld4(A.ev + 4 * i), ld4(A.ea + 4 * i));
 }
 
 // Sum of 16 consecutive buckets per thread, reduced per block of 4096 buckets.
@@ -99,10 +81,13 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x)
     return x;
 }
 
-// Exclusive scan of the block sums by one block; writes the grand total to *total.
+// Exclusive scan of the block sums by one block; writes the grand total to *total. Also
+// clears the step's contact counters and the full-pass queue length.
 __global__ __launch_bounds__(kPartsBlock) void k_scan_parts(uint32_t* __restrict__ part, uint32_t nparts,
-                                                            uint32_t* __restrict__ total)
+                                                            uint32_t* __restrict__ total,
+                                                            unsigned long long* __restrict__ counters)
 {
+    if (threadIdx.x < 3) counters[threadIdx.x] = 0ull;
     __shared__ uint32_t wsum[kPartsBlock / 64];
     const uint32_t per = (nparts + kPartsBlock - 1) / kPartsBlock;
     const uint32_t b0 = threadIdx.x * per;
@@ -185,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     for (uint32_t m = s; m < e; m++) r += A.slot_tmp[m] < k ? 1u : 0u;
     const uint32_t f = s + r;
     A.svals[f] = k;
-    A.bucket_count[key] = 0u;
+    A.bucket_count[key] = 0u;   // for the next step's counts
     const double4 p = reinterpret_cast<const double4*>(A.pos)[k];
     const Grid& gr = A.scenes[A.scene_of ? A.scene_of[k] : 0u].grid;
     A.sposf[f] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), (float)p.w);
@@ -198,11 +183,11 @@ uint32_t bucket_table_size(uint32_t nbucket)
     return (nbucket + kScanTile - 1) / kScanTile * kScanTile;
 }
 
-hipError_t launch_integrate_hash(const StepArgs& A, bool multi, hipStream_t s)
+hipError_t launch_drift(const StepArgs& A, bool multi, hipStream_t s)
 {
     if (A.np == 0) return hipSuccess;
-    if (multi) hipLaunchKernelGGL(k_integrate_hash<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
-    else hipLaunchKernelGGL(k_integrate_hash<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    if (multi) hipLaunchKernelGGL(k_drift<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    else hipLaunchKernelGGL(k_drift<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 
@@ -219,7 +204,8 @@ hipError_t launch_grid(const StepArgs& A, hipStream_t s)
     const uint32_t nb = bucket_table_size(A.nbucket);
     const uint32_t ntile = nb / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part);
-    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kPartsBlock), 0, s, A.scan_part, ntile, A.bucket_start + nb);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kPartsBlock), 0, s, A.scan_part, ntile, A.bucket_start + nb,
+                       A.counters);
     hipLaunchKernelGGL(k_scan_apply, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part,
                        A.bucket_start);
     if (A.np) {

@@ -46,6 +46,8 @@ struct StepArgs {
     double* tmax; double* rres;
     double* vel_out; double* ev_out;  // where the fused contact kernel writes vel and ev (the
                                       // other buffer of a pair: neighbours read the old ones)
+    double* pos_next; double* ep_next; // the next step's drifted position and orientation
+    int drift;                        // the velocity update also computes the next drift
     const uint32_t* scene_of;  // sphere -> scene (multi-scene only)
     const SceneDev* scenes;
     const WallDev* walls;
@@ -167,6 +169,33 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
     if (i < A.np) A.nbr_count[i] = n;
 }
 
+// Drift of the next step (impl:135-147; xIntegratorVV.cpp:66-100): Euler parameters and
+// position from the state the velocity update just produced, the bucket key of the new
+// position and its count (a posted atomic add). Written to the second position/orientation
+// buffer, which the next step makes current. Same expressions, in the same order, as a
+// separate position-update pass would evaluate.
+__device__ __forceinline__ void drift_sphere(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
+                                             V4<double> e, V3<double> v, V3<double> ac, V4<double> d, V4<double> a)
+{
+    V4<double> ne;
+    ne.x = e.x + S.dt * d.x + S.half2dt * a.x;
+    ne.y = e.y + S.dt * d.y + S.half2dt * a.y;
+    ne.z = e.z + S.dt * d.z + S.half2dt * a.z;
+    ne.w = e.w + S.dt * d.w + S.half2dt * a.w;
+    const double l = sqrt(ne.x * ne.x + ne.y * ne.y + ne.z * ne.z + ne.w * ne.w);
+    ne.x = ne.x / l; ne.y = ne.y / l; ne.z = ne.z / l; ne.w = ne.w / l;
+    st4(A.ep_next + 4 * i, ne);
+    p.x += S.dt * v.x + S.half2dt * ac.x;
+    p.y += S.dt * v.y + S.half2dt * ac.y;
+    p.z += S.dt * v.z + S.half2dt * ac.z;
+    st4(A.pos_next + 4 * i, p);
+    int gx, gy, gz;
+    grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
+    const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
+    A.keys[i] = key;
+    atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: k_scatter takes the rank
+}
+
 // A sphere's own state, loaded together before any branch (independent loads in flight at
 // once). Fused mode also needs what the velocity update reads.
 struct OwnState {
@@ -224,12 +253,12 @@ __device__ __forceinline__ void sphere_tail(const StepArgs& A, const SceneDev& S
         const V3<double> a = mk3(inv_m * (tot.F.x + m * S.gx), inv_m * (tot.F.y + m * S.gy),
                                  inv_m * (tot.F.z + m * S.gz));
         const double hdt = 0.5 * S.dt;
+        V3<double> v = o.vel;
         if (A.keep_velocity) {
             av.x = av.x + hdt * (aa.x + mea.x); av.y = av.y + hdt * (aa.y + mea.y);
             av.z = av.z + hdt * (aa.z + mea.z); av.w = av.w + hdt * (aa.w + mea.w);
-            if (A.vel_out != A.vel) st3(A.vel_out + 3 * i, o.vel);
+            if (A.vel_out != A.vel) st3(A.vel_out + 3 * i, v);
         } else {
-            V3<double> v = o.vel;
             const V3<double> ao = o.acc;
             v = mk3(v.x + hdt * ao.x, v.y + hdt * ao.y, v.z + hdt * ao.z);
             av.x = av.x + hdt * aa.x; av.y = av.y + hdt * aa.y; av.z = av.z + hdt * aa.z; av.w = av.w + hdt * aa.w;
@@ -242,6 +271,7 @@ __device__ __forceinline__ void sphere_tail(const StepArgs& A, const SceneDev& S
         st3(A.acc + 3 * i, a);
         st4(A.ea + 4 * i, mea);
         if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) err |= XDEM_ERR_NONFINITE;
+        if (A.drift) drift_sphere(A, S, i, o.pos, e, v, a, av, mea);
     } else {
         st3(A.force + 3 * i, tot.F);
         st3(A.moment + 3 * i, tot.M);
@@ -507,6 +537,8 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
         st3(A.force + 3 * i, mk3(0.0, 0.0, 0.0));
         st3(A.moment + 3 * i, mk3(0.0, 0.0, 0.0));
     }
+    if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) atomicOr(A.err, XDEM_ERR_NONFINITE);
+    if (A.drift) drift_sphere(A, S, i, ld4(A.pos + 4 * i), e, ld3(A.vel + 3 * i), a, av, mea);
 }
 
 // Kernel entry points of one precision (defined in xdem_step_f64.hip / xdem_step_mixed.hip).
@@ -518,7 +550,7 @@ hipError_t launch_pairs(const StepArgs&, bool multi, hipStream_t);
 CollideLaunch collide_launch_mixed();
 // xdem_grid.hip
 uint32_t bucket_table_size(uint32_t nbucket);   // bucket_count / bucket_start entries (+1 for start)
-hipError_t launch_integrate_hash(const StepArgs&, bool multi, hipStream_t);
+hipError_t launch_drift(const StepArgs&, bool multi, hipStream_t);
 hipError_t launch_grid(const StepArgs&, hipStream_t);
 hipError_t launch_finish(const StepArgs&, bool multi, hipStream_t);
 
