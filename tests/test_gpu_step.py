@@ -170,3 +170,26 @@ def test_dynamic_wall_wrench_matches_oracle():
     w = gpu.wrench()
     assert np.allclose(w[0], ref.wrench[0], rtol=1e-12, atol=1e-18)
     assert np.all(w[1:] == 0.0)
+
+
+@pytest.mark.parametrize("fuse", [False, True])
+def test_state_written_between_steps_matches_oracle(fuse):
+    """Writing the state after steps discards the drift the last step precomputed (and the
+    bucket counts it added): the following steps must follow the new state."""
+    scene = compressed_lattice()
+    gpu = DemContext(scene, precision="fp64", fuse_velocity=fuse)
+    ref = xdem_ref.RefSim(scene)
+    gpu.step(3)
+    ref.step(3)
+    rng = np.random.default_rng(11)
+    vel = ref.vel + rng.normal(0, 0.01, size=ref.vel.shape)
+    pos = ref.pos.copy()
+    pos[:, :3] += rng.normal(0, 2e-5, size=(scene.np, 3))
+    for name, val in (("vel", vel), ("pos", pos)):
+        gpu.write(name, val)
+        ref.write(name, val)
+    gpu.step(4)
+    ref.step(4)
+    for f in ("pos", "vel", "ev"):
+        assert rel(gpu.read(f), getattr(ref, f)) <= 1e-10, f
+    assert gpu.error_word() == 0

@@ -161,10 +161,7 @@ struct xdem_ctx {
     {
         hipError_t e;
         if (ev && (e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
-        if (!drift_valid) {
-            if ((e = launch_drift(args(), multi, stream)) != hipSuccess) return e;
-            drift_valid = true;
-        }
+        if (!drift_valid && (e = run_drift()) != hipSuccess) return e;
         set_pos_parity(spp ^ 1);   // the drifted positions become current
         StepArgs A = args();
         A.drift = fuse ? 1 : 0;    // split mode: k_finish computes the next drift
@@ -194,6 +191,19 @@ struct xdem_ctx {
         sv = p;
         vel = vel_buf[p];
         ev = ev_buf[p];
+    }
+
+    // Drift of the current state into the next buffers. A previous step may already have
+    // counted an (invalidated) drift into the bucket counts, so they are cleared first.
+    hipError_t run_drift()
+    {
+        const size_t nbt = bucket_table_size(nbucket);
+        hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nbt + 255) / 256)), dim3(256), 0, stream, bucket_count, 0u, nbt);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if ((e = launch_drift(args(), multi, stream)) != hipSuccess) return e;
+        drift_valid = true;
+        return hipSuccess;
     }
 
     void set_pos_parity(int p)
@@ -802,10 +812,7 @@ int xdem_ctx_graph_capture(xdem_ctx* c, unsigned int nsteps)
     CTX_CHECK(hipSetDevice(c->cfg.device));
     if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
-    if (!c->drift_valid) {   // the graph's steps start from a precomputed drift
-        CTX_CHECK(launch_drift(c->args(), c->multi, c->stream));
-        c->drift_valid = true;
-    }
+    if (!c->drift_valid) CTX_CHECK(c->run_drift());   // the graph's steps start from a precomputed drift
     CTX_CHECK(hipStreamSynchronize(c->stream));
     const int parity = c->cur, sparity = c->sv, pparity = c->spp;
     CTX_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

@@ -8,13 +8,12 @@
 // add); k_drift does the same when no previous step left it. Then:
 //   k_scan_reduce     \
 //   k_scan_parts       > exclusive prefix of the counts: start[b] = first sorted slot of
-//   k_scan_apply      /  bucket b, start[nbucket] = N; the counts are cleared on the way
-//   k_scatter         provisional slot start[key] + (atomic rank inside the bucket); the
-//                     counts are back to the occupancy afterwards
+//   k_scan_apply      /  bucket b, start[nbucket] = N
+//   k_scatter         provisional slot start[key] + (atomic countdown of the bucket), which
+//                     leaves every count at zero for the next step
 //   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
 //                     smaller id, i.e. exactly the reference's (hash, index) order; writes
-//                     the sorted id and the FP32 position copy the pair search reads, and
-//                     clears the bucket's count for the next step's drift
+//                     the sorted id and the FP32 position copy the pair search reads
 // The pair search k_pairs (xdem_step.hpp) is launched from here too: its FP64 pair
 // predicate must round exactly like the reference's.
 // The bucket of slot range [start[b], start[b+1]) needs no clearing between steps, and a
@@ -117,20 +116,20 @@ __global__ __launch_bounds__(kPartsBlock) void k_scan_parts(uint32_t* __restrict
         }
 }
 
-// start[b] = part[tile] + exclusive prefix inside the tile; the counts are zeroed.
-__global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ count,
+// start[b] = part[tile] + exclusive prefix inside the tile (the counts stay: k_scatter counts
+// them back down to zero).
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ count,
                                                        const uint32_t* __restrict__ part,
                                                        uint32_t* __restrict__ start)
 {
     __shared__ uint32_t wsum[kBlock / 64];
     const size_t base = (size_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint4* c = reinterpret_cast<uint4*>(count + base);
+    const uint4* c = reinterpret_cast<const uint4*>(count + base);
     uint32_t v[kScanItems];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const uint4 q = c[k];
         v[4 * k + 0] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
-        c[k] = make_uint4(0u, 0u, 0u, 0u);
     }
     uint32_t s = 0;
 #pragma unroll
@@ -156,7 +155,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(StepArgs A)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.np) return;
     const uint32_t key = A.keys[i];
-    A.slot_tmp[A.bucket_start[key] + atomicAdd(&A.bucket_count[key], 1u)] = i;
+    // counting the bucket back down gives each sphere a distinct slot and leaves the count
+    // at zero for the next step (the order inside the bucket is fixed by k_order_gather)
+    A.slot_tmp[A.bucket_start[key] + atomicSub(&A.bucket_count[key], 1u) - 1u] = i;
 }
 
 __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
@@ -170,7 +171,6 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     for (uint32_t m = s; m < e; m++) r += A.slot_tmp[m] < k ? 1u : 0u;
     const uint32_t f = s + r;
     A.svals[f] = k;
-    A.bucket_count[key] = 0u;   // for the next step's counts
     const double4 p = reinterpret_cast<const double4*>(A.pos)[k];
     const Grid& gr = A.scenes[A.scene_of ? A.scene_of[k] : 0u].grid;
     A.sposf[f] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), (float)p.w);

@@ -147,18 +147,28 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
                 total = len1 + (e2 - s2);
                 t = 0;
             };
+            // candidate m with FP32 copy f, in visiting order
+            auto test = [&](uint32_t m, float4 f) {
+                if (m == j) return;
+                const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
+                const float lim = f.w * 1.000001f + limi;
+                if (dx * dx + dy * dy + dz * dz > lim * lim) return;
+                if (n < lc) out[n] = A.svals[m];
+                n++;
+            };
             load_row(0);
             while (true) {
                 if (t < total) {
-                    const uint32_t m = t < len1 ? s1 + t : s2 + (t - len1);
-                    t++;
-                    if (m == j) continue;
-                    const float4 f = A.sposf[m];
-                    const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
-                    const float lim = f.w * 1.000001f + limi;
-                    if (dx * dx + dy * dy + dz * dz > lim * lim) continue;
-                    if (n < lc) out[n] = A.svals[m];
-                    n++;
+                    // two candidates per iteration (both loads in flight), tested in order
+                    const uint32_t m0 = t < len1 ? s1 + t : s2 + (t - len1);
+                    const bool two = t + 1u < total;
+                    const uint32_t t1 = t + 1u;
+                    const uint32_t m1 = two ? (t1 < len1 ? s1 + t1 : s2 + (t1 - len1)) : m0;
+                    t += two ? 2u : 1u;
+                    const float4 f0 = A.sposf[m0];
+                    const float4 f1 = A.sposf[m1];
+                    test(m0, f0);
+                    if (two) test(m1, f1);
                 } else {
                     if (++row == 9u) break;
                     load_row(row);
