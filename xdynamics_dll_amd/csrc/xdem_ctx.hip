@@ -69,6 +69,8 @@ struct xdem_ctx {
     uint32_t np = 0, nbucket = 0, key_bits = 1, cap = XDEM_MAX_P2P_COUNT, nwslot = 0;
     int tmax_persist = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;                       // the full contact pass
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;     // pair search done / full pass done
     CollideLaunch collide_fn = nullptr;
 
     double *pos = nullptr, *vel = nullptr, *acc = nullptr, *ep = nullptr, *ev = nullptr, *ea = nullptr;
@@ -77,7 +79,8 @@ struct xdem_ctx {
     SceneDev* d_scenes = nullptr;
     WallDev* d_walls = nullptr;
     uint32_t *keys = nullptr, *svals = nullptr, *slot_tmp = nullptr;
-    uint32_t *nbr_count = nullptr, *nbr = nullptr, *work = nullptr;
+    uint32_t *nbr_count = nullptr, *nbr = nullptr, *work = nullptr, *seg_count = nullptr;
+    uint32_t seg_cap = 0;
     uint32_t list_cap = 48;
     uint32_t *bucket_count = nullptr, *bucket_start = nullptr, *scan_part = nullptr;
     float4* sposf = nullptr;
@@ -140,6 +143,7 @@ struct xdem_ctx {
         A.keys = keys; A.svals = svals; A.slot_tmp = slot_tmp;
         A.bucket_count = bucket_count; A.bucket_start = bucket_start; A.scan_part = scan_part;
         A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap; A.work = work;
+        A.seg_count = seg_count; A.seg_cap = seg_cap;
         A.sposf = sposf;
         A.vel_out = vel; A.ev_out = ev;
         A.pos_next = pos_buf[spp ^ 1]; A.ep_next = ep_buf[spp ^ 1];
@@ -169,11 +173,19 @@ struct xdem_ctx {
             A.vel_out = vel_buf[sv ^ 1];
             A.ev_out = ev_buf[sv ^ 1];
         }
-        if ((e = launch_grid(A, stream)) != hipSuccess) return e;
+        if ((e = launch_grid(A, multi, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
         if ((e = launch_pairs(A, multi, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[2], stream)) != hipSuccess) return e;
-        if ((e = collide_fn(A, multi, fuse, model, cfg.quirks.gpu_force_law != 0, stream)) != hipSuccess) return e;
+        // the full pass (spheres k_pairs queued) on the second stream, concurrently with
+        // the lite pass (all others) on the first; the first stream waits for both
+        const bool gpu_law = cfg.quirks.gpu_force_law != 0;
+        if ((e = hipEventRecord(fork_ev, stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(stream2, fork_ev, 0)) != hipSuccess) return e;
+        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 1, stream2)) != hipSuccess) return e;
+        if ((e = hipEventRecord(join_ev, stream2)) != hipSuccess) return e;
+        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 0, stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(stream, join_ev, 0)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[3], stream)) != hipSuccess) return e;
         if (nwslot) {
             hipLaunchKernelGGL(k_wrench_reduce, dim3(nwslot * 6), dim3(256), 0, stream, wrench_pp, np, d_wslot_range,
@@ -270,7 +282,13 @@ int xdem_ctx_create(const xdem_ctx_config* cfg, xdem_ctx** out)
     if (c->cap > 64) { delete c; return XDEM_E_INVALID; }
     if (!c->cfg.grid_dim_max) c->cfg.grid_dim_max = 512;
     if (hipSetDevice(c->cfg.device) != hipSuccess) { delete c; return XDEM_E_HIP; }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return XDEM_E_HIP; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+        xdem_ctx_destroy(c);
+        return XDEM_E_HIP;
+    }
     *out = c;
     return XDEM_OK;
 }
@@ -284,6 +302,9 @@ int xdem_ctx_destroy(xdem_ctx* c)
     if (c->graph) hipGraphDestroy(c->graph);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     for (void* p : c->allocs) hipFree(p);
+    if (c->fork_ev) hipEventDestroy(c->fork_ev);
+    if (c->join_ev) hipEventDestroy(c->join_ev);
+    if (c->stream2) hipStreamDestroy(c->stream2);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return XDEM_OK;
@@ -436,7 +457,9 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->slot_tmp, sizeof(uint32_t) * n));
     c->list_cap = std::max(48u, 2u * c->cap);
     CTX_CHECK(c->alloc(&c->nbr_count, sizeof(uint32_t) * n));
-    CTX_CHECK(c->alloc(&c->work, sizeof(uint32_t) * n));
+    c->seg_cap = queue_segment_cap((uint32_t)n);
+    CTX_CHECK(c->alloc(&c->work, sizeof(uint32_t) * kQueueSegs * c->seg_cap));
+    CTX_CHECK(c->alloc(&c->seg_count, sizeof(uint32_t) * kQueueSegs * kSegStride));
     CTX_CHECK(c->alloc(&c->nbr, sizeof(uint32_t) * n * c->list_cap));
     const size_t nbt = bucket_table_size(c->nbucket);
     CTX_CHECK(c->alloc(&c->bucket_count, sizeof(uint32_t) * nbt));
@@ -507,6 +530,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(hipMemsetAsync(c->pl_tsd, 0, sizeof(double2) * n * XDEM_MAX_P2PL_COUNT, s));
     CTX_CHECK(hipMemsetAsync(c->err, 0, sizeof(uint32_t), s));
     CTX_CHECK(hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 4, s));
+    CTX_CHECK(hipMemsetAsync(c->seg_count, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
     if (c->nwslot) CTX_CHECK(hipMemsetAsync(c->wrench_out, 0, sizeof(double) * 6 * c->nwslot, s));
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nbt + 255) / 256)), dim3(256), 0, s, c->bucket_count, 0u, nbt);
     CTX_CHECK(hipGetLastError());

@@ -13,7 +13,8 @@
 //                     leaves every count at zero for the next step
 //   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
 //                     smaller id, i.e. exactly the reference's (hash, index) order; writes
-//                     the sorted id and the FP32 position copy the pair search reads
+//                     the sorted id and the FP32 position copy the pair search reads, with
+//                     the sphere's wall-reach flag (FP64 test) in the sign of the radius
 // The pair search k_pairs (xdem_step.hpp) is launched from here too: its FP64 pair
 // predicate must round exactly like the reference's.
 // The bucket of slot range [start[b], start[b+1]) needs no clearing between steps, and a
@@ -81,12 +82,14 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x)
 }
 
 // Exclusive scan of the block sums by one block; writes the grand total to *total. Also
-// clears the step's contact counters and the full-pass queue length.
+// clears the step's contact counters and the full-pass queue segment counters.
 __global__ __launch_bounds__(kPartsBlock) void k_scan_parts(uint32_t* __restrict__ part, uint32_t nparts,
                                                             uint32_t* __restrict__ total,
-                                                            unsigned long long* __restrict__ counters)
+                                                            unsigned long long* __restrict__ counters,
+                                                            uint32_t* __restrict__ seg_count)
 {
-    if (threadIdx.x < 3) counters[threadIdx.x] = 0ull;
+    if (threadIdx.x < 2) counters[threadIdx.x] = 0ull;
+    if (threadIdx.x < kQueueSegs) seg_count[threadIdx.x * kSegStride] = 0u;
     __shared__ uint32_t wsum[kPartsBlock / 64];
     const uint32_t per = (nparts + kPartsBlock - 1) / kPartsBlock;
     const uint32_t b0 = threadIdx.x * per;
@@ -160,6 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(StepArgs A)
     A.slot_tmp[A.bucket_start[key] + atomicSub(&A.bucket_count[key], 1u) - 1u] = i;
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,9 +175,18 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     for (uint32_t m = s; m < e; m++) r += A.slot_tmp[m] < k ? 1u : 0u;
     const uint32_t f = s + r;
     A.svals[f] = k;
-    const double4 p = reinterpret_cast<const double4*>(A.pos)[k];
-    const Grid& gr = A.scenes[A.scene_of ? A.scene_of[k] : 0u].grid;
-    A.sposf[f] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), (float)p.w);
+    const V4<double> p = ld4(A.pos + 4 * k);
+    const SceneDev& S = A.scenes[scene_index(A, k, MULTI)];
+    // exact FP64 reach test against every wall of the scene (wall_out_of_reach): a sphere
+    // with a wall in reach goes to the full contact pass
+    bool far = true;
+    for (uint32_t w = 0; w < S.nwall; w++) {
+        const WallDev& W = A.walls[S.wall_begin + w];
+        far = far & wall_out_of_reach(W.cc, W.info, p);
+    }
+    const Grid& gr = S.grid;
+    const float rf = (float)p.w;
+    A.sposf[f] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), far ? rf : -rf);
 }
 
 }  // namespace
@@ -199,18 +212,19 @@ hipError_t launch_pairs(const StepArgs& A, bool multi, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_grid(const StepArgs& A, hipStream_t s)
+hipError_t launch_grid(const StepArgs& A, bool multi, hipStream_t s)
 {
     const uint32_t nb = bucket_table_size(A.nbucket);
     const uint32_t ntile = nb / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part);
     hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kPartsBlock), 0, s, A.scan_part, ntile, A.bucket_start + nb,
-                       A.counters);
+                       A.counters, A.seg_count);
     hipLaunchKernelGGL(k_scan_apply, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part,
                        A.bucket_start);
     if (A.np) {
         hipLaunchKernelGGL(k_scatter, grid_for(A.np), dim3(kBlock), 0, s, A);
-        hipLaunchKernelGGL(k_order_gather, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+        if (multi) hipLaunchKernelGGL(k_order_gather<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+        else hipLaunchKernelGGL(k_order_gather<false>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
     }
     return hipGetLastError();
 }

@@ -16,6 +16,19 @@
 namespace xdem {
 
 constexpr uint32_t kInvalid = 0xffffffffu;
+constexpr uint32_t kFullPass = 0x80000000u;   // nbr_count flag: the sphere goes to the full contact pass
+// The full-pass queue is split in kQueueSegs segments, each with its own counter on its own
+// 128-byte line: the wave of sorted slots [64 w, 64 w + 64) appends to segment w mod
+// kQueueSegs, so the appends of different waves rarely hit the same counter.
+constexpr uint32_t kQueueSegs = 64;
+constexpr uint32_t kSegStride = 32;           // uint32 entries between two segment counters
+// Entries per segment for nsorted sorted slots launched in blocks of 256 (the most a
+// segment can receive: 64 spheres from each of its waves).
+inline uint32_t queue_segment_cap(uint32_t nsorted)
+{
+    const uint32_t nwaves = (nsorted + 255u) / 256u * 4u;
+    return (nwaves + kQueueSegs - 1u) / kQueueSegs * 64u;
+}
 constexpr int kMaxWallsPerScene = 16;
 
 struct SceneDev {
@@ -62,15 +75,17 @@ struct StepArgs {
     uint32_t* nbr_count;                     // near neighbours per sphere (k_pairs)
     uint32_t* nbr;                           // their sphere ids, list_cap per sphere
     float4* sposf;  // per sorted slot: position relative to the scene's grid origin and
-                    // radius, FP32 (the pair search reads only this)
+                    // radius, FP32 (the pair search reads only this); the radius is
+                    // negated when a wall is within reach of the sphere
     // sphere history (double-buffered) and wall history (in place)
     const uint32_t* pp_count_in; const uint32_t* pp_id_in; const double2* pp_tsd_in;
     uint32_t* pp_count_out; uint32_t* pp_id_out; double2* pp_tsd_out;
     uint32_t* pl_count; uint32_t* pl_id; double2* pl_tsd;
     double* wrench;          // [nwslot][6][np] per-sphere contributions to dynamic walls
     uint32_t* err;
-    unsigned long long* counters;  // [0] stored sphere contacts, [1] stored wall contacts,
-                                   // [2] spheres queued for the full contact pass
+    unsigned long long* counters;  // [0] stored sphere contacts, [1] stored wall contacts
+    uint32_t* seg_count;           // spheres per full-pass queue segment (kSegStride apart)
+    uint32_t seg_cap;              // entries per queue segment in work
     uint32_t np, nsorted, cap;
     uint32_t nwslot, nbucket;
     uint32_t list_cap;                       // neighbour-list entries per sorted slot
@@ -114,11 +129,12 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 {
     const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (j >= A.nsorted) return;
-    const uint32_t i = A.svals[j];
+    const uint32_t i = j < A.nsorted ? A.svals[j] : kInvalid;
     uint32_t n = 0;
+    bool full = false;
     if (i < A.np) {
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+        const float4 fi = A.sposf[j];
         if (S.has_pp) {
             // the sphere's bucket coordinates from its key (dims are powers of two)
             const Grid& G = S.grid;
@@ -126,11 +142,10 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
             const uint32_t bx = kb & G.mask_x;
             const uint32_t by = (kb >> __builtin_ctz(G.dim_x)) & G.mask_y;
             const uint32_t bz = kb >> __builtin_ctz(G.dim_xy);
-            const float4 fi = A.sposf[j];
             const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
             // lim = r_j (1 + 1e-6) + [(r_i + reach)(1 + 1e-6) + mg]: the same bound with the
             // sphere's own part hoisted (each FP32 rounding is far below the 1e-6 margin)
-            const float limi = ((fi.w + (float)S.cc.coh_reach) * 1.000001f) + mg;
+            const float limi = ((fabsf(fi.w) + (float)S.cc.coh_reach) * 1.000001f) + mg;
             const uint32_t lc = A.list_cap;
             uint32_t* out = A.nbr + (size_t)i * lc;
             const uint32_t bx0 = (bx - 1u) & G.mask_x;
@@ -151,7 +166,7 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
             auto test = [&](uint32_t m, float4 f) {
                 if (m == j) return;
                 const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
-                const float lim = f.w * 1.000001f + limi;
+                const float lim = fabsf(f.w) * 1.000001f + limi;
                 if (dx * dx + dy * dy + dz * dz > lim * lim) return;
                 if (n < lc) out[n] = A.svals[m];
                 n++;
@@ -175,8 +190,23 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
                 }
             }
         }
+        // the full contact pass takes spheres with listed neighbours or a wall in reach
+        // (k_order_gather marks the latter with the sign of the FP32 radius)
+        full = n != 0u || (__float_as_uint(fi.w) >> 31) != 0u;
+        A.nbr_count[i] = n | (full ? kFullPass : 0u);
     }
-    if (i < A.np) A.nbr_count[i] = n;
+    // queue for the full pass: one atomic per wave, issued by its first active lane, on the
+    // wave's segment counter
+    const unsigned long long mask = __ballot(full);
+    if (mask) {
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+        const uint32_t seg = (j >> 6) & (kQueueSegs - 1u);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&A.seg_count[seg * kSegStride], (uint32_t)__popcll(mask));
+        base = __shfl(base, (int)leader, 64);
+        if (full) A.work[(size_t)seg * A.seg_cap + base + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+    }
 }
 
 // Drift of the next step (impl:135-147; xIntegratorVV.cpp:66-100): Euler parameters and
@@ -310,7 +340,7 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
         if (old_count > cap) old_count = cap;
         V3<T> Ft = mk3(T(0), T(0), T(0));
         uint32_t n_new = 0, n_seen = 0;
-        const uint32_t nlist_all = A.nbr_count[i];
+        const uint32_t nlist_all = A.nbr_count[i] & ~kFullPass;
         const uint32_t lc = A.list_cap;
         if (nlist_all > lc) err |= XDEM_ERR_NBR_OVERFLOW;
         const uint32_t nlist = min(nlist_all, lc);
@@ -406,98 +436,100 @@ __device__ __forceinline__ void add_contact_counts(const StepArgs& A, unsigned l
     }
 }
 
-// Lite pass over all spheres. It completes every sphere with no listed neighbour and no
-// wall within reach (wall_out_of_reach): the full pass would add only zero sums, so the
-// lite pass applies the same tail to zero sums, and the walls' slot-count rule of
-// wall_no_contact through the scene's summary (the count drops to 0 if the scene has a wall
-// with id 0, impl:989). All loads are issued before the first branch, and the walls of a
-// single-scene launch are read from LDS. Other spheres are appended to A.work (one atomic
-// per wave; their order in the list does not affect any result).
+// Lite pass: every sphere k_pairs did not send to the full pass (no listed neighbour, no wall
+// within reach, wall_out_of_reach): the full pass would add only zero sums, so the lite pass
+// applies the same tail to zero sums, with the walls' slot-count rule of wall_no_contact
+// taken from the scene's summary (the count drops to 0 if the scene has a wall with id 0,
+// impl:989). All loads are issued before the first branch. It runs concurrently with the
+// full pass (the two touch disjoint spheres).
+
+// What the lite pass reads of one sphere, all issued together.
+struct LiteIn {
+    OwnState o;
+    uint32_t flags;    // nbr_count (kFullPass: the full pass owns the sphere)
+    uint32_t count0;   // wall slot count
+};
+
+template <bool FUSE>
+__device__ __forceinline__ void lite_load(const StepArgs& A, const SceneDev& S, uint32_t i, LiteIn& in)
+{
+    in.o = load_own<FUSE>(A, S, i);
+    in.flags = A.nbr_count[i];
+    in.count0 = S.nwall ? A.pl_count[i] : 0u;
+}
+
+template <bool FUSE>
+__device__ __forceinline__ void lite_sphere(const StepArgs& A, const SceneDev& S, uint32_t i, const LiteIn& in,
+                                            uint32_t& n_wall)
+{
+    if (in.flags & kFullPass) return;
+    uint32_t err = 0;
+    if (S.has_pp) A.pp_count_out[i] = 0u;
+    Accum tot;
+    tot.F = mk3(0.0, 0.0, 0.0);
+    tot.M = mk3(0.0, 0.0, 0.0);
+    tot.T = in.o.t0 + mk3(0.0, 0.0, 0.0);
+    tot.R = 0.0;
+    if (S.nwall) {
+        if (S.wall_dynamic)
+            for (uint32_t w = 0; w < S.nwall; w++) {
+                const int32_t slot = A.walls[S.wall_begin + w].wslot;
+                if (slot >= 0) {
+                    double* wr = A.wrench + (size_t)slot * 6 * A.np;
+#pragma unroll
+                    for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
+                }
+            }
+        const uint32_t count = S.wall_id0 ? 0u : min(in.count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
+        if (count != in.count0) A.pl_count[i] = count;
+        n_wall += count;
+    }
+    sphere_tail<FUSE>(A, S, i, in.o, tot, err);
+    if (err) atomicOr(A.err, err);
+}
+
+// The lite pass, one sphere per thread.
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide_lite(StepArgs A)
 {
-    __shared__ double wall_sh[kMaxWallsPerScene][8];   // xw, uw, |uw|^2, cohesion flag
-    if (!MULTI) {
-        const SceneDev& S0 = A.scenes[0];
-        const uint32_t nw = min(S0.nwall, (uint32_t)kMaxWallsPerScene);
-        if (threadIdx.x < nw) {
-            const WallDev& W = A.walls[S0.wall_begin + threadIdx.x];
-            const xdem_plane_info& pl = W.info;
-            wall_sh[threadIdx.x][0] = pl.xw.x; wall_sh[threadIdx.x][1] = pl.xw.y; wall_sh[threadIdx.x][2] = pl.xw.z;
-            wall_sh[threadIdx.x][3] = pl.uw.x; wall_sh[threadIdx.x][4] = pl.uw.y; wall_sh[threadIdx.x][5] = pl.uw.z;
-            wall_sh[threadIdx.x][6] = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
-            wall_sh[threadIdx.x][7] = W.cc.coh;
-        }
-        __syncthreads();
-    }
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n_pp = 0, n_wall = 0;
-    bool full = false;
+    uint32_t n_wall = 0;
     if (i < A.np) {
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-        const OwnState o = load_own<FUSE>(A, S, i);
-        const uint32_t nb = S.has_pp ? A.nbr_count[i] : 0u;
-        const uint32_t count0 = S.nwall ? A.pl_count[i] : 0u;
-        bool far = nb == 0u;
-        if (!MULTI && S.nwall <= kMaxWallsPerScene) {
-            for (uint32_t w = 0; w < S.nwall; w++) {
-                const double* q = wall_sh[w];
-                far = far & wall_out_of_reach_raw(q[7], mk3(q[0], q[1], q[2]), mk3(q[3], q[4], q[5]), q[6], o.pos);
-            }
-        } else {
-            for (uint32_t w = 0; w < S.nwall; w++) {
-                const WallDev& W = A.walls[S.wall_begin + w];
-                far = far & wall_out_of_reach(W.cc, W.info, o.pos);
-            }
-        }
-        if (far) {
-            uint32_t err = 0;
-            if (S.has_pp) A.pp_count_out[i] = 0u;
-            Accum tot;
-            tot.F = mk3(0.0, 0.0, 0.0);
-            tot.M = mk3(0.0, 0.0, 0.0);
-            tot.T = o.t0 + mk3(0.0, 0.0, 0.0);
-            tot.R = 0.0;
-            if (S.nwall) {
-                if (S.wall_dynamic)
-                    for (uint32_t w = 0; w < S.nwall; w++) {
-                        const int32_t slot = A.walls[S.wall_begin + w].wslot;
-                        if (slot >= 0) {
-                            double* wr = A.wrench + (size_t)slot * 6 * A.np;
-#pragma unroll
-                            for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
-                        }
-                    }
-                const uint32_t count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
-                if (count != count0) A.pl_count[i] = count;
-                n_wall = count;
-            }
-            sphere_tail<FUSE>(A, S, i, o, tot, err);
-            if (err) atomicOr(A.err, err);
-        } else {
-            full = true;
-        }
+        LiteIn in;
+        lite_load<FUSE>(A, S, i, in);
+        lite_sphere<FUSE>(A, S, i, in, n_wall);
     }
-    const unsigned long long mask = __ballot(full);
-    if (mask) {
-        const uint32_t lane = threadIdx.x & 63u;
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(&A.counters[2], (unsigned long long)__popcll(mask));
-        base = __shfl(base, 0, 64);
-        if (full) A.work[base + __popcll(mask & ((1ull << lane) - 1ull))] = i;
-    }
-    add_contact_counts(A, n_pp, n_wall);
+    add_contact_counts(A, 0u, n_wall);
 }
 
-// Full pass over the spheres the lite pass queued (grid-stride).
+// Full pass over the spheres k_pairs queued (grid-stride over the concatenated segments).
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide(StepArgs A)
 {
-    const uint32_t nwork = (uint32_t)A.counters[2];
+    // seg_end[s] = spheres in segments 0..s (inclusive prefix, one wave64 scan)
+    __shared__ uint32_t seg_end[kQueueSegs];
+    if (threadIdx.x < kQueueSegs) {
+        uint32_t x = A.seg_count[threadIdx.x * kSegStride];
+#pragma unroll
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (threadIdx.x >= off) x += y;
+        }
+        seg_end[threadIdx.x] = x;
+    }
+    __syncthreads();
+    const uint32_t nwork = seg_end[kQueueSegs - 1];
     unsigned long long c0 = 0, c1 = 0;
     for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwork; w += gridDim.x * blockDim.x) {
+        // segment of entry w: the first s with seg_end[s] > w
+        uint32_t sg = 0;
+#pragma unroll
+        for (uint32_t step = kQueueSegs / 2u; step > 0u; step >>= 1)
+            if (seg_end[sg + step - 1u] <= w) sg += step;
+        const uint32_t i = A.work[(size_t)sg * A.seg_cap + (w - (sg ? seg_end[sg - 1u] : 0u))];
         uint32_t n_pp = 0, n_wall = 0;
-        collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, A.work[w], n_pp, n_wall);
+        collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, i, n_pp, n_wall);
         c0 += n_pp;
         c1 += n_wall;
     }
@@ -551,17 +583,17 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
     if (A.drift) drift_sphere(A, S, i, ld4(A.pos + 4 * i), e, ld3(A.vel + 3 * i), a, av, mea);
 }
 
-// Kernel entry points of one precision (defined in xdem_step_f64.hip / xdem_step_mixed.hip).
-// Contact kernel of one precision (xdem_step_f64.hip / xdem_step_mixed.hip); the other
-// kernels are FP64 only and live in xdem_step_f64.hip.
-using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, hipStream_t);
+// Contact kernels of one precision (xdem_step_f64.hip / xdem_step_mixed.hip); pass 0 is the
+// lite pass, pass 1 the full pass. The other kernels are FP64 only.
+using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, int pass,
+                                     hipStream_t);
 CollideLaunch collide_launch_f64();
 hipError_t launch_pairs(const StepArgs&, bool multi, hipStream_t);
 CollideLaunch collide_launch_mixed();
 // xdem_grid.hip
 uint32_t bucket_table_size(uint32_t nbucket);   // bucket_count / bucket_start entries (+1 for start)
 hipError_t launch_drift(const StepArgs&, bool multi, hipStream_t);
-hipError_t launch_grid(const StepArgs&, hipStream_t);
+hipError_t launch_grid(const StepArgs&, bool multi, hipStream_t);
 hipError_t launch_finish(const StepArgs&, bool multi, hipStream_t);
 
 }  // namespace xdem

@@ -11,40 +11,41 @@ namespace {
 constexpr unsigned kBlock = 256;
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
-// Lite pass over every sphere, then the full pass over the spheres it queued (grid-stride
-// over at most kFullBlocks blocks; the queue length is read on the device).
+// pass 0: the lite pass over every sphere; pass 1: the full pass over the spheres k_pairs
+// queued (grid-stride over at most kFullBlocks blocks; the queue length is read on the
+// device). The two touch disjoint spheres and may run concurrently.
 constexpr unsigned kFullBlocks = 1024;
 
 template <bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
-void collide_passes(const StepArgs& A, hipStream_t s)
+void collide_passes(const StepArgs& A, int pass, hipStream_t s)
 {
     const dim3 g = grid_for(A.np), b(kBlock);
-    const dim3 gf(std::min(g.x, kFullBlocks));
-    hipLaunchKernelGGL((k_collide_lite<double, MULTI, FUSE, MODEL, GPU_LAW>), g, b, 0, s, A);
-    hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW>), gf, b, 0, s, A);
+    if (pass == 0) hipLaunchKernelGGL((k_collide_lite<double, MULTI, FUSE, MODEL, GPU_LAW>), g, b, 0, s, A);
+    else hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW>), dim3(std::min(g.x, kFullBlocks)), b, 0, s, A);
 }
 
 template <bool MULTI, bool FUSE>
-void collide_models(const StepArgs& A, int model, bool gpu_law, hipStream_t s)
+void collide_models(const StepArgs& A, int model, bool gpu_law, int pass, hipStream_t s)
 {
     if (model == 1) {
-        if (gpu_law) collide_passes<MULTI, FUSE, 1, true>(A, s);
-        else collide_passes<MULTI, FUSE, 1, false>(A, s);
+        if (gpu_law) collide_passes<MULTI, FUSE, 1, true>(A, pass, s);
+        else collide_passes<MULTI, FUSE, 1, false>(A, pass, s);
     } else {
-        if (gpu_law) collide_passes<MULTI, FUSE, 0, true>(A, s);
-        else collide_passes<MULTI, FUSE, 0, false>(A, s);
+        if (gpu_law) collide_passes<MULTI, FUSE, 0, true>(A, pass, s);
+        else collide_passes<MULTI, FUSE, 0, false>(A, pass, s);
     }
 }
 
-hipError_t collide_f64(const StepArgs& A, bool multi, bool fuse, int model, bool gpu_law, hipStream_t s)
+hipError_t collide_f64(const StepArgs& A, bool multi, bool fuse, int model, bool gpu_law, int pass,
+                       hipStream_t s)
 {
     if (A.np == 0) return hipSuccess;
     if (multi) {
-        if (fuse) collide_models<true, true>(A, model, gpu_law, s);
-        else collide_models<true, false>(A, model, gpu_law, s);
+        if (fuse) collide_models<true, true>(A, model, gpu_law, pass, s);
+        else collide_models<true, false>(A, model, gpu_law, pass, s);
     } else {
-        if (fuse) collide_models<false, true>(A, model, gpu_law, s);
-        else collide_models<false, false>(A, model, gpu_law, s);
+        if (fuse) collide_models<false, true>(A, model, gpu_law, pass, s);
+        else collide_models<false, false>(A, model, gpu_law, pass, s);
     }
     return hipGetLastError();
 }
