@@ -214,8 +214,8 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 // position and its count (a posted atomic add). Written to the second position/orientation
 // buffer, which the next step makes current. Same expressions, in the same order, as a
 // separate position-update pass would evaluate.
-__device__ __forceinline__ void drift_sphere(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
-                                             V4<double> e, V3<double> v, V3<double> ac, V4<double> d, V4<double> a)
+__device__ __forceinline__ void drift_orientation(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> e,
+                                                  V4<double> d, V4<double> a)
 {
     V4<double> ne;
     ne.x = e.x + S.dt * d.x + S.half2dt * a.x;
@@ -225,6 +225,11 @@ __device__ __forceinline__ void drift_sphere(const StepArgs& A, const SceneDev& 
     const double l = sqrt(ne.x * ne.x + ne.y * ne.y + ne.z * ne.z + ne.w * ne.w);
     ne.x = ne.x / l; ne.y = ne.y / l; ne.z = ne.z / l; ne.w = ne.w / l;
     st4(A.ep_next + 4 * i, ne);
+}
+
+__device__ __forceinline__ void drift_position(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
+                                               V3<double> v, V3<double> ac)
+{
     p.x += S.dt * v.x + S.half2dt * ac.x;
     p.y += S.dt * v.y + S.half2dt * ac.y;
     p.z += S.dt * v.z + S.half2dt * ac.z;
@@ -234,6 +239,13 @@ __device__ __forceinline__ void drift_sphere(const StepArgs& A, const SceneDev& 
     const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
     A.keys[i] = key;
     atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: k_scatter takes the rank
+}
+
+__device__ __forceinline__ void drift_sphere(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
+                                             V4<double> e, V3<double> v, V3<double> ac, V4<double> d, V4<double> a)
+{
+    drift_orientation(A, S, i, e, d, a);
+    drift_position(A, S, i, p, v, ac);
 }
 
 // A sphere's own state, loaded together before any branch (independent loads in flight at
@@ -503,6 +515,94 @@ __global__ __launch_bounds__(256) void k_collide_lite(StepArgs A)
     add_contact_counts(A, 0u, n_wall);
 }
 
+// The lite pass of a fused step, split by motion: k_lite_move does the translation of every
+// lite sphere (acceleration, velocity, next position and bucket key) and its contact
+// bookkeeping, k_lite_turn the rotation (carried rolling torque, Euler-parameter rates, next
+// orientation). They read and write disjoint fields, so each needs few registers and runs at
+// a high occupancy. Together they evaluate the expressions of sphere_tail with zero contact
+// sums, in the same order.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n_wall = 0;
+    if (i < A.np) {
+        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+        const uint32_t flags = A.nbr_count[i];
+        const uint32_t count0 = S.nwall ? A.pl_count[i] : 0u;
+        const V4<double> p = ld4(A.pos + 4 * i);
+        const V3<double> v0 = ld3(A.vel + 3 * i);
+        const V3<double> ao = ld3(A.acc + 3 * i);
+        const double m = A.mass[i];
+        if (!(flags & kFullPass)) {
+            if (S.has_pp) A.pp_count_out[i] = 0u;
+            if (S.nwall) {
+                if (S.wall_dynamic)
+                    for (uint32_t w = 0; w < S.nwall; w++) {
+                        const int32_t slot = A.walls[S.wall_begin + w].wslot;
+                        if (slot >= 0) {
+                            double* wr = A.wrench + (size_t)slot * 6 * A.np;
+#pragma unroll
+                            for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
+                        }
+                    }
+                const uint32_t count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
+                if (count != count0) A.pl_count[i] = count;
+                n_wall = count;
+            }
+            const double inv_m = 1.0 / m;
+            const V3<double> a = mk3(inv_m * (0.0 + m * S.gx), inv_m * (0.0 + m * S.gy), inv_m * (0.0 + m * S.gz));
+            const double hdt = 0.5 * S.dt;
+            V3<double> v = v0;
+            if (A.keep_velocity) {
+                if (A.vel_out != A.vel) st3(A.vel_out + 3 * i, v);
+            } else {
+                v = mk3(v.x + hdt * ao.x, v.y + hdt * ao.y, v.z + hdt * ao.z);
+                v = mk3(v.x + hdt * a.x, v.y + hdt * a.y, v.z + hdt * a.z);
+                st3(A.vel_out + 3 * i, v);
+            }
+            st3(A.acc + 3 * i, a);
+            if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) atomicOr(A.err, XDEM_ERR_NONFINITE);
+            if (A.drift) drift_position(A, S, i, p, v, a);
+        }
+    }
+    add_contact_counts(A, 0u, n_wall);
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.np) return;
+    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    const uint32_t flags = A.nbr_count[i];
+    const V4<double> e = ld4(A.ep + 4 * i);
+    V4<double> av = ld4(A.ev + 4 * i);
+    const V4<double> aa = ld4(A.ea + 4 * i);
+    const V3<double> inertia = ld3(A.inertia + 3 * i);
+    const V3<double> t0 = A.tmax_persist ? ld3(A.tmax + 3 * i) : mk3(0.0, 0.0, 0.0);
+    if (flags & kFullPass) return;
+    // no contact: the rolling sums are zero, so rolling_moment leaves the moment and the
+    // carried torque as they are
+    V3<double> M = mk3(0.0, 0.0, 0.0);
+    const V3<double> T = t0 + mk3(0.0, 0.0, 0.0);
+    if (A.tmax_persist) st3(A.tmax + 3 * i, T);
+    const V3<double> np_ = rot_local(M, e);
+    const V4<double> mea = uceom(inertia, e, av, np_);
+    const double hdt = 0.5 * S.dt;
+    if (A.keep_velocity) {
+        av.x = av.x + hdt * (aa.x + mea.x); av.y = av.y + hdt * (aa.y + mea.y);
+        av.z = av.z + hdt * (aa.z + mea.z); av.w = av.w + hdt * (aa.w + mea.w);
+    } else {
+        av.x = av.x + hdt * aa.x; av.y = av.y + hdt * aa.y; av.z = av.z + hdt * aa.z; av.w = av.w + hdt * aa.w;
+        av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y;
+        av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
+    }
+    st4(A.ev_out + 4 * i, av);
+    st4(A.ea + 4 * i, mea);
+    if (A.drift) drift_orientation(A, S, i, e, av, mea);
+}
+
 // Full pass over the spheres k_pairs queued (grid-stride over the concatenated segments).
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide(StepArgs A)
@@ -587,6 +687,7 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
 // lite pass, pass 1 the full pass. The other kernels are FP64 only.
 using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, int pass,
                                      hipStream_t);
+void launch_lite_fused(const StepArgs&, bool multi, hipStream_t);   // k_lite_move + k_lite_turn
 CollideLaunch collide_launch_f64();
 hipError_t launch_pairs(const StepArgs&, bool multi, hipStream_t);
 CollideLaunch collide_launch_mixed();

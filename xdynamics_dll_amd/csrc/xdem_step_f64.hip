@@ -11,17 +11,21 @@ namespace {
 constexpr unsigned kBlock = 256;
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
-// pass 0: the lite pass over every sphere; pass 1: the full pass over the spheres k_pairs
-// queued (grid-stride over at most kFullBlocks blocks; the queue length is read on the
-// device). The two touch disjoint spheres and may run concurrently.
+// pass 0: the lite pass over every sphere (two kernels in fused mode); pass 1: the full pass
+// over the spheres k_pairs queued (grid-stride over at most kFullBlocks blocks; the queue
+// length is read on the device). The two touch disjoint spheres and may run concurrently.
 constexpr unsigned kFullBlocks = 1024;
 
 template <bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 void collide_passes(const StepArgs& A, int pass, hipStream_t s)
 {
     const dim3 g = grid_for(A.np), b(kBlock);
-    if (pass == 0) hipLaunchKernelGGL((k_collide_lite<double, MULTI, FUSE, MODEL, GPU_LAW>), g, b, 0, s, A);
-    else hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW>), dim3(std::min(g.x, kFullBlocks)), b, 0, s, A);
+    if (pass == 0) {
+        if (FUSE) launch_lite_fused(A, MULTI, s);
+        else hipLaunchKernelGGL((k_collide_lite<double, MULTI, false, MODEL, GPU_LAW>), g, b, 0, s, A);
+    } else {
+        hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW>), dim3(std::min(g.x, kFullBlocks)), b, 0, s, A);
+    }
 }
 
 template <bool MULTI, bool FUSE>
@@ -53,6 +57,17 @@ hipError_t collide_f64(const StepArgs& A, bool multi, bool fuse, int model, bool
 }  // namespace
 
 CollideLaunch collide_launch_f64() { return collide_f64; }
+
+void launch_lite_fused(const StepArgs& A, bool multi, hipStream_t s)
+{
+    if (multi) {
+        hipLaunchKernelGGL(k_lite_move<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
+        hipLaunchKernelGGL(k_lite_turn<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    } else {
+        hipLaunchKernelGGL(k_lite_move<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+        hipLaunchKernelGGL(k_lite_turn<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    }
+}
 
 hipError_t launch_finish(const StepArgs& A, bool multi, hipStream_t s)
 {
