@@ -7,8 +7,9 @@
 // (drift_sphere), which also writes each sphere's bucket key and counts it (posted atomic
 // add); k_drift does the same when no previous step left it. Then:
 //   k_scan_reduce     \
-//   k_scan_parts       > exclusive prefix of the counts: start[b] = first sorted slot of
-//   k_scan_apply      /  bucket b, start[nbucket] = N
+//   (k_scan_parts)     > exclusive prefix of the counts: start[b] = first sorted slot of
+//   k_scan_apply      /  bucket b, start[nbucket] = N (tables of at most kDirectTiles tiles:
+//                        each k_scan_apply block sums the earlier tile sums itself)
 //   k_scatter         provisional slot start[key] + (atomic countdown of the bucket), which
 //                     leaves every count at zero for the next step
 //   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
@@ -29,6 +30,20 @@ constexpr unsigned kBlock = 256;
 constexpr unsigned kScanItems = 16;                   // per thread
 constexpr unsigned kScanTile = kBlock * kScanItems;   // 4096 buckets per block
 constexpr unsigned kPartsBlock = 1024;
+constexpr unsigned kDirectTiles = 2048;
+
+// Sum of x over the block (every thread gets it); red holds kBlock / 64 entries.
+__device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t* red)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    uint32_t t = 0;
+#pragma unroll
+    for (unsigned w = 0; w < kBlock / 64; w++) t += red[w];
+    return t;
+}
 
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
@@ -45,10 +60,17 @@ __global__ __launch_bounds__(kBlock) void k_drift(StepArgs A)
                  ld4(A.ev + 4 * i), ld4(A.ea + 4 * i));
 }
 
-// Sum of 16 consecutive buckets per thread, reduced per block of 4096 buckets.
+// Sum of 16 consecutive buckets per thread, reduced per block of 4096 buckets. Block 0 also
+// clears the step's contact counters and the full-pass queue segment counters.
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ count,
-                                                        uint32_t* __restrict__ part)
+                                                        uint32_t* __restrict__ part,
+                                                        unsigned long long* __restrict__ counters,
+                                                        uint32_t* __restrict__ seg_count)
 {
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 2) counters[threadIdx.x] = 0ull;
+        if (threadIdx.x < kQueueSegs) seg_count[threadIdx.x * kSegStride] = 0u;
+    }
     __shared__ uint32_t wsum[kBlock / 64];
     const uint4* c = reinterpret_cast<const uint4*>(count + (size_t)blockIdx.x * kScanTile) + threadIdx.x * 4;
     uint32_t s = 0;
@@ -81,15 +103,10 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x)
     return x;
 }
 
-// Exclusive scan of the block sums by one block; writes the grand total to *total. Also
-// clears the step's contact counters and the full-pass queue segment counters.
+// Exclusive scan of the block sums by one block; writes the grand total to *total.
 __global__ __launch_bounds__(kPartsBlock) void k_scan_parts(uint32_t* __restrict__ part, uint32_t nparts,
-                                                            uint32_t* __restrict__ total,
-                                                            unsigned long long* __restrict__ counters,
-                                                            uint32_t* __restrict__ seg_count)
+                                                            uint32_t* __restrict__ total)
 {
-    if (threadIdx.x < 2) counters[threadIdx.x] = 0ull;
-    if (threadIdx.x < kQueueSegs) seg_count[threadIdx.x * kSegStride] = 0u;
     __shared__ uint32_t wsum[kPartsBlock / 64];
     const uint32_t per = (nparts + kPartsBlock - 1) / kPartsBlock;
     const uint32_t b0 = threadIdx.x * per;
@@ -119,13 +136,26 @@ __global__ __launch_bounds__(kPartsBlock) void k_scan_parts(uint32_t* __restrict
         }
 }
 
-// start[b] = part[tile] + exclusive prefix inside the tile (the counts stay: k_scatter counts
-// them back down to zero).
+// start[b] = (sum of the earlier tiles) + exclusive prefix inside the tile (the counts stay:
+// k_scatter counts them back down to zero). DIRECT: part holds the tile sums and the block
+// adds up those of the earlier tiles itself; the last block writes the total to *total.
+// Otherwise part holds the exclusive prefix of the tile sums (k_scan_parts).
+template <bool DIRECT>
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ count,
                                                        const uint32_t* __restrict__ part,
-                                                       uint32_t* __restrict__ start)
+                                                       uint32_t* __restrict__ start,
+                                                       uint32_t* __restrict__ total)
 {
     __shared__ uint32_t wsum[kBlock / 64];
+    __shared__ uint32_t red[kBlock / 64];
+    uint32_t tile0;
+    if (DIRECT) {
+        uint32_t pre = 0;
+        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kBlock) pre += part[t];
+        tile0 = block_sum(pre, red);
+    } else {
+        tile0 = part[blockIdx.x];
+    }
     const size_t base = (size_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     const uint4* c = reinterpret_cast<const uint4*>(count + base);
     uint32_t v[kScanItems];
@@ -145,8 +175,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
     const uint32_t w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 63) wsum[w] = inc;
     __syncthreads();
-    uint32_t off = part[blockIdx.x] + inc - s;
+    uint32_t off = tile0 + inc - s;
     for (uint32_t k = 0; k < w; k++) off += wsum[k];
+    if (DIRECT && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total = off + s;
     uint4* o = reinterpret_cast<uint4*>(start + base);
 #pragma unroll
     for (int k = 0; k < 4; k++)
@@ -216,11 +247,16 @@ hipError_t launch_grid(const StepArgs& A, bool multi, hipStream_t s)
 {
     const uint32_t nb = bucket_table_size(A.nbucket);
     const uint32_t ntile = nb / kScanTile;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part);
-    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kPartsBlock), 0, s, A.scan_part, ntile, A.bucket_start + nb,
-                       A.counters, A.seg_count);
-    hipLaunchKernelGGL(k_scan_apply, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part,
-                       A.bucket_start);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part, A.counters,
+                       A.seg_count);
+    if (ntile <= kDirectTiles) {
+        hipLaunchKernelGGL(k_scan_apply<true>, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part,
+                           A.bucket_start, A.bucket_start + nb);
+    } else {
+        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kPartsBlock), 0, s, A.scan_part, ntile, A.bucket_start + nb);
+        hipLaunchKernelGGL(k_scan_apply<false>, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part,
+                           A.bucket_start, A.bucket_start + nb);
+    }
     if (A.np) {
         hipLaunchKernelGGL(k_scatter, grid_for(A.np), dim3(kBlock), 0, s, A);
         if (multi) hipLaunchKernelGGL(k_order_gather<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
