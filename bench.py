@@ -35,13 +35,16 @@ def step_bytes(cbar: float) -> float:
 
 
 def contact_kernel_bytes(cbar: float, fused: bool) -> float:
-    """Algorithmic bytes per sphere of the contact pass (k_collide_lite + k_collide).
-    Fused (contacts + rolling + velocity half): own state read pos 32, vel 24, acc 24,
-    ep 32, ev 32, ea 32, mass 8, inertia 24 (= 208); state write vel 24, acc 24, ev 32,
-    ea 32 (= 112); sphere and wall history: counts 2*8, entries 2*20*cbar.
-    Split (contacts only): read pos 32, vel 24, ep 32, ev 32, mass 8 (= 128); write force,
-    moment, rolling torque sum and magnitude 24+24+24+8 (= 80); history 16 + 40*cbar."""
-    return (336.0 if fused else 224.0) + 40.0 * cbar
+    """Algorithmic bytes per sphere of the contact pass.
+    Fused (k_lite_move + k_lite_turn for spheres without neighbours or walls in reach,
+    k_collide for the others: contacts, rolling, velocity update and the next step's drift):
+    own state read pos 32, vel 24, acc 24, ep 32, ev 32, ea 32, mass 8, inertia 24 (= 208);
+    state write pos 32, vel 24, acc 24, ep 32, ev 32, ea 32 (= 176); bucket key 4; sphere
+    and wall history: counts 2*8, entries 2*20*cbar.
+    Split (k_collide_lite + k_collide, contacts only): read pos 32, vel 24, ep 32, ev 32,
+    mass 8 (= 128); write force, moment, rolling torque sum and magnitude 24+24+24+8 (= 80);
+    history 16 + 40*cbar."""
+    return (404.0 if fused else 224.0) + 40.0 * cbar
 
 
 def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0):
@@ -138,7 +141,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_collide_dam_break_1m.js
 
 
 def measured_traffic(np_, fused):
-    """HBM bytes per step of the contact pass (k_collide_lite + k_collide) from the committed
+    """HBM bytes per step of the contact-pass kernels from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the ratio
     the same counter shows on a plain device copy of known size), or None when no summary
     matches this workload."""
@@ -205,7 +208,9 @@ def batch_main(args):
                    "parallelism": "single GPU, all scenes in one context"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "contact pass k_collide_lite + k_collide", "kernel_ms": k_ms,
+                     "kernel": "contact pass (k_lite_move + k_lite_turn, k_collide on a second stream)"
+                               if fuse else "contact pass (k_collide_lite, k_collide on a second stream)",
+                     "kernel_ms": k_ms,
                      "bytes_per_sphere": contact_kernel_bytes(cbar, fuse)},
         "mean_contacts_per_sphere": cbar,
         "grid_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
@@ -302,8 +307,9 @@ def main():
                    "nbucket": mixed["info"]["nbucket"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(n, fuse),
-                     "kernel": "contact pass k_collide_lite + k_collide (" +
-                               ("contacts, rolling, velocity update" if fuse else "contacts") + ")",
+                     "kernel": ("contact pass: k_lite_move + k_lite_turn, k_collide on a second stream "
+                                "(contacts, rolling, velocity update, next drift)" if fuse else
+                                "contact pass: k_collide_lite, k_collide on a second stream (contacts)"),
                      "kernel_ms": k_ms, "bytes_per_sphere": contact_kernel_bytes(cbar, fuse)},
         "step_gbps": step_gbps,
         "step_roofline_frac": step_gbps / HBM_PEAK_GBPS,
