@@ -47,15 +47,17 @@ def contact_kernel_bytes(cbar: float, fused: bool) -> float:
     return (404.0 if fused else 224.0) + 40.0 * cbar
 
 
-def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0):
+def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0, skin=0.0):
     import torch
     from xdynamics_dll_amd import DemContext
 
     ctx = DemContext(scene, precision=precision, p2p_capacity=capacity, fuse_velocity=fuse,
-                     buckets_per_sphere=buckets_per_sphere)
+                     buckets_per_sphere=buckets_per_sphere, skin=skin)
+    info0 = None
     ctx.step(warmup)
     ctx.synchronize()
     torch.cuda.synchronize()
+    info0 = ctx.info()
     ctx.set_timing(True)
     t0 = time.perf_counter()
     ctx.step(steps)
@@ -67,6 +69,7 @@ def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_s
     n_pp, n_wall = ctx.contact_count()
     err = ctx.error_word()
     info = ctx.info()
+    info["timed_list_builds"] = info["list_builds"] - info0["list_builds"]
     ctx.close()
     return {"seconds": t1 - t0, "timing": tm, "n_pp": n_pp, "n_wall": n_wall, "err": err, "info": info}
 
@@ -187,10 +190,10 @@ def batch_main(args):
     t_build = time.perf_counter() - t_build
     n = sum(s.np for s in scenes)
     fuse = not args.split
-    mixed = run_gpu(scenes, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere)
+    mixed = run_gpu(scenes, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin)
     value = n * args.steps / mixed["seconds"]
     f64 = run_gpu(scenes, "fp64", max(20, args.steps // 2), max(5, args.warmup // 2), args.capacity, fuse,
-                  args.buckets_per_sphere)
+                  args.buckets_per_sphere, args.skin)
     f64_value = n * max(20, args.steps // 2) / f64["seconds"]
     tm = mixed["timing"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
@@ -215,6 +218,9 @@ def batch_main(args):
         "mean_contacts_per_sphere": cbar,
         "grid_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
         "pair_search_ms_per_step": tm["pairs_ms"] / max(tm["steps"], 1),
+        "list_builds_timed": mixed["info"]["timed_list_builds"],
+        "list_spheres": mixed["info"]["list_spheres"],
+        "contact_spheres": mixed["info"]["contact_spheres"],
         "error_word": mixed["err"], "scene_build_s": t_build, "fp64_value": f64_value,
         "mixed_over_fp64": value / f64_value,
     }
@@ -250,6 +256,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--split", action="store_true",
                     help="rolling + velocity update in a kernel of their own (default: inside the contact pass)")
+    ap.add_argument("--skin", type=float, default=0.0,
+                    help="neighbour-list skin in m (0: 2.5%% of the smallest radius, <0: rebuild every step)")
     ap.add_argument("--buckets-per-sphere", type=int, default=0,
                     help="bucket budget of the grid per sphere (0 = library default)")
     ap.add_argument("--workload", choices=["dam_break", "batch"], default="dam_break",
@@ -279,7 +287,7 @@ def main():
     n = scene.np
 
     fuse = not args.split
-    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere)
+    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin)
     value = n * args.steps / mixed["seconds"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
@@ -317,12 +325,15 @@ def main():
         "grid_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
         "finish_ms_per_step": tm["finish_ms"] / max(tm["steps"], 1),
         "pair_search_ms_per_step": tm["pairs_ms"] / max(tm["steps"], 1),
+        "list_builds_timed": mixed["info"]["timed_list_builds"],
+        "list_spheres": mixed["info"]["list_spheres"],
+        "contact_spheres": mixed["info"]["contact_spheres"],
         "error_word": mixed["err"],
         "scene_build_s": t_build,
     }
     if not args.no_fp64:
         f64 = run_gpu(scene, "fp64", max(20, args.steps // 2), max(5, args.warmup // 2), args.capacity,
-                      fuse, args.buckets_per_sphere)
+                      fuse, args.buckets_per_sphere, args.skin)
         f64_value = n * max(20, args.steps // 2) / f64["seconds"]
         out["fp64_value"] = f64_value
         out["mixed_over_fp64"] = value / f64_value

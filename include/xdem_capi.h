@@ -108,6 +108,11 @@ typedef struct xdem_ctx_config {
     unsigned int buckets_per_sphere; /* bucket budget of a scene: the power of two at or above
                                         this times its sphere count (0 = 4); the largest axes
                                         are halved until the grid fits */
+    double skin;                 /* neighbour-list skin [m]: the grid and the lists are rebuilt
+                                    only on steps after a sphere has moved half of it since
+                                    the last build (or after a state write or a wall move);
+                                    0 = 2.5% of the scene's smallest radius, < 0 = no skin
+                                    (rebuilt every step). Results do not depend on it. */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */
@@ -135,6 +140,9 @@ typedef struct xdem_ctx_info {
     unsigned int p2p_capacity;
     unsigned long long device_bytes;
     unsigned long long steps_done;
+    unsigned long long list_builds;  /* steps that rebuilt the grid and the neighbour lists */
+    unsigned int list_spheres;       /* spheres with listed neighbours or a wall near (last build) */
+    unsigned int contact_spheres;    /* spheres of the last step's full contact pass */
 } xdem_ctx_info;
 
 int xdem_ctx_create(const xdem_ctx_config* cfg, xdem_ctx** out);

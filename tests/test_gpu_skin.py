@@ -1,0 +1,75 @@
+"""Neighbour lists with a skin (grid and lists rebuilt only after a sphere has moved half the
+skin) against lists rebuilt every step and against the oracle.
+
+The lists only select candidates; the pairs that pass the reference's predicate are visited
+in the reference's order (cell offset, then id), re-sorted when a sphere has changed cells
+since the build. So a context with a skin must give bit-identical states and histories
+(including the order of the history slots, which is the visiting order) to one without,
+and the slot order must equal the oracle's. The scene is a compressed polydisperse pack
+with fast random velocities, so spheres change cells and contacts form and break between
+list builds.
+"""
+import numpy as np
+import pytest
+
+import xdem_ref
+from scenes import random_pack
+from xdynamics_dll_amd import DemContext, SceneData
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "vel", "acc", "ep", "ev", "ea")
+
+
+def shaken_pack(n=2000, speed=1.0, seed=21):
+    s = random_pack(n=n, seed=seed)
+    rng = np.random.default_rng(seed)
+    vel = rng.normal(0.0, speed, size=(s.np, 3))
+    return SceneData.from_positions(s.pos, 2600.0, s.cp_pp, walls=list(zip(s.planes, s.plane_dbi)),
+                                    wall_cp=s.plane_cp[0], dt=5e-6, vel=vel)
+
+
+def run(scene, skin, precision, fuse, steps):
+    ctx = DemContext(scene, precision=precision, fuse_velocity=fuse, skin=skin)
+    ctx.step(steps)
+    out = {f: ctx.read(f) for f in FIELDS}
+    out["p2p"] = ctx.history("p2p")
+    out["plane"] = ctx.history("plane")
+    out["builds"] = ctx.info()["list_builds"]
+    out["err"] = ctx.error_word()
+    return out
+
+
+@pytest.mark.parametrize("precision,fuse", [("fp64", True), ("fp64", False), ("mixed", True)])
+def test_skin_lists_are_bit_identical_to_rebuilding_every_step(precision, fuse):
+    scene = shaken_pack()
+    steps = 120
+    a = run(scene, 1.0e-3, precision, fuse, steps)
+    b = run(scene, -1.0, precision, fuse, steps)
+    print("list builds with skin", a["builds"], "without", b["builds"], flush=True)
+    assert a["err"] == 0 and b["err"] == 0
+    assert b["builds"] == steps
+    assert 1 < a["builds"] < steps // 4
+    for f in FIELDS:
+        assert np.array_equal(a[f], b[f]), f
+    for h in ("p2p", "plane"):
+        for x, y in zip(a[h], b[h]):
+            assert np.array_equal(x, y), h
+
+
+def test_skin_history_slot_order_matches_oracle():
+    scene = shaken_pack(n=1500, seed=5)
+    gpu = DemContext(scene, precision="fp64", fuse_velocity=True, skin=1.0e-3)
+    ref = xdem_ref.RefSim(scene)
+    for n in (7, 33):
+        gpu.step(n)
+        ref.step(n)
+        c, ids, tsd = gpu.history("p2p")
+        assert np.array_equal(c, ref.pp_count)
+        for i in range(scene.np):
+            assert np.array_equal(ids[i, :c[i]], ref.pp_id[i, :c[i]]), i
+        for f in ("pos", "vel", "ep", "ev"):
+            g, r = gpu.read(f), getattr(ref, f)
+            assert float(np.abs(g - r).max() / np.abs(r).max()) <= 1e-10, (n, f)
+    assert gpu.info()["list_builds"] < 40
+    assert gpu.error_word() == 0

@@ -102,13 +102,14 @@ class Scene(C.Structure):
 class CtxConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("p2p_capacity", C.c_uint), ("quirks", Quirks),
                 ("block_size", C.c_uint), ("grid_dim_max", C.c_uint), ("fuse_velocity", C.c_int),
-                ("device", C.c_int), ("buckets_per_sphere", C.c_uint)]
+                ("device", C.c_int), ("buckets_per_sphere", C.c_uint), ("skin", C.c_double)]
 
 
 class CtxInfo(C.Structure):
     _fields_ = [("nscene", C.c_uint), ("np", C.c_uint), ("nsorted", C.c_uint), ("nplane_total", C.c_uint),
                 ("nbucket", C.c_uint), ("key_bits", C.c_uint), ("p2p_capacity", C.c_uint),
-                ("device_bytes", C.c_ulonglong), ("steps_done", C.c_ulonglong)]
+                ("device_bytes", C.c_ulonglong), ("steps_done", C.c_ulonglong),
+                ("list_builds", C.c_ulonglong), ("list_spheres", C.c_uint), ("contact_spheres", C.c_uint)]
 
 
 assert C.sizeof(DemParameters) == 144

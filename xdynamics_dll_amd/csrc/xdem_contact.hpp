@@ -140,24 +140,27 @@ struct WallSlots {
     }
 };
 
-// True when the sphere cannot touch the wall, for a wall without cohesion. Every contact
+// True when the sphere cannot touch the wall, for a wall without cohesion, even grown by
+// `skin` (a sphere that has moved less than the skin stays out of reach). Every contact
 // region of plane_detection needs (dp . uw)^2 < r^2 max(1, |uw|^2): corners and edges test
 // (dp . uw)^2 plus non-negative terms against r^2, the face needs |dp . uw| / |uw| < r.
 // Beyond that, with a 1e-6 margin for rounding, the outcome is the no-contact one.
 __device__ __forceinline__ bool wall_out_of_reach_raw(double coh, V3<double> xw, V3<double> uw, double uw2,
-                                                      V4<double> ipos)
+                                                      V4<double> ipos, double skin)
 {
     if (coh != 0.0) return false;
     const V3<double> dp = mk3(ipos.x, ipos.y, ipos.z) - xw;
     const double wz = dot(dp, uw);
-    const double r = ipos.w;
+    const double r = ipos.w + skin;
     return wz * wz >= (r * r) * fmax(1.0, uw2) * (1.0 + 1e-6);
 }
 
-__device__ __forceinline__ bool wall_out_of_reach(const ContactConst& cc, const xdem_plane_info& pl, V4<double> ipos)
+__device__ __forceinline__ bool wall_out_of_reach(const ContactConst& cc, const xdem_plane_info& pl, V4<double> ipos,
+                                                  double skin = 0.0)
 {
     const double uw2 = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
-    return wall_out_of_reach_raw(cc.coh, mk3(pl.xw.x, pl.xw.y, pl.xw.z), mk3(pl.uw.x, pl.uw.y, pl.uw.z), uw2, ipos);
+    return wall_out_of_reach_raw(cc.coh, mk3(pl.xw.x, pl.xw.y, pl.xw.z), mk3(pl.uw.x, pl.uw.y, pl.uw.z), uw2, ipos,
+                                 skin);
 }
 
 // The state update of wall_contact for a sphere out of the wall's reach: no force, the

@@ -80,10 +80,20 @@ struct xdem_ctx {
     WallDev* d_walls = nullptr;
     uint32_t *keys = nullptr, *svals = nullptr, *slot_tmp = nullptr;
     uint32_t *nbr_count = nullptr, *nbr = nullptr, *work = nullptr, *seg_count = nullptr;
+    uint32_t *work2 = nullptr, *seg_count2 = nullptr;
+    unsigned long long* pass_mask = nullptr;
     uint32_t seg_cap = 0;
     uint32_t list_cap = 48;
     uint32_t *bucket_count = nullptr, *bucket_start = nullptr, *scan_part = nullptr;
     float4* sposf = nullptr;
+    float4* pos_ref = nullptr;
+    // list rebuild flags, one per step parity: step s reads rebuild[rbp] (rbp flips at the
+    // start of every step) and its drift sets rebuild[rbp ^ 1] for step s + 1
+    uint32_t* rebuild = nullptr;
+    int rbp = 1;
+    uint32_t* dmax = nullptr;          // two sets of kQueueSegs counters, by rbp like rebuild
+    float *gap = nullptr, *disp = nullptr;
+    unsigned long long* n_rebuild = nullptr;
     // vel and ev: two buffers each in fused mode (the contact kernel writes the new values
     // into the other one while neighbours still read the old ones); vel/ev point at the
     // current one
@@ -143,8 +153,13 @@ struct xdem_ctx {
         A.keys = keys; A.svals = svals; A.slot_tmp = slot_tmp;
         A.bucket_count = bucket_count; A.bucket_start = bucket_start; A.scan_part = scan_part;
         A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap; A.work = work;
-        A.seg_count = seg_count; A.seg_cap = seg_cap;
-        A.sposf = sposf;
+        A.seg_count = seg_count; A.seg_cap = seg_cap; A.work2 = work2; A.seg_count2 = seg_count2;
+        A.pass_mask = pass_mask;
+        A.sposf = sposf; A.pos_ref = pos_ref;
+        A.rebuild_cur = rebuild + rbp; A.rebuild_next = rebuild + (rbp ^ 1); A.n_rebuild = n_rebuild;
+        A.gap = gap; A.disp = disp;
+        A.dmax_cur = dmax + (size_t)rbp * kQueueSegs * kSegStride;
+        A.dmax_next = dmax + (size_t)(rbp ^ 1) * kQueueSegs * kSegStride;
         A.vel_out = vel; A.ev_out = ev;
         A.pos_next = pos_buf[spp ^ 1]; A.ep_next = ep_buf[spp ^ 1];
         A.drift = 1;
@@ -165,6 +180,7 @@ struct xdem_ctx {
     {
         hipError_t e;
         if (ev && (e = hipEventRecord(ev[0], stream)) != hipSuccess) return e;
+        rbp ^= 1;
         if (!drift_valid && (e = run_drift()) != hipSuccess) return e;
         set_pos_parity(spp ^ 1);   // the drifted positions become current
         StepArgs A = args();
@@ -176,9 +192,10 @@ struct xdem_ctx {
         if ((e = launch_grid(A, multi, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
         if ((e = launch_pairs(A, multi, stream)) != hipSuccess) return e;
+        if ((e = launch_touch(A, multi, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[2], stream)) != hipSuccess) return e;
-        // the full pass (spheres k_pairs queued) on the second stream, concurrently with
-        // the lite pass (all others) on the first; the first stream waits for both
+        // the full pass (spheres k_touch queued) on the second stream, concurrently with the
+        // lite pass (all others) on the first; the first stream waits for both
         const bool gpu_law = cfg.quirks.gpu_force_law != 0;
         if ((e = hipEventRecord(fork_ev, stream)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(stream2, fork_ev, 0)) != hipSuccess) return e;
@@ -205,17 +222,27 @@ struct xdem_ctx {
         ev = ev_buf[p];
     }
 
-    // Drift of the current state into the next buffers. A previous step may already have
-    // counted an (invalidated) drift into the bucket counts, so they are cleared first.
+    // Drift of the current state into the next buffers (inside a step, after its rbp flip),
+    // and a list rebuild for this step: the lists' reference positions are those of an
+    // older state.
     hipError_t run_drift()
     {
-        const size_t nbt = bucket_table_size(nbucket);
-        hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nbt + 255) / 256)), dim3(256), 0, stream, bucket_count, 0u, nbt);
-        hipError_t e = hipGetLastError();
+        StepArgs A = args();
+        A.rebuild_next = nullptr;
+        A.dmax_next = nullptr;
+        hipError_t e = launch_drift(A, multi, stream);
         if (e != hipSuccess) return e;
-        if ((e = launch_drift(args(), multi, stream)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, rebuild + rbp, 1u, (size_t)1);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
         drift_valid = true;
         return hipSuccess;
+    }
+
+    // A list rebuild on the next step (between steps: a wall moved).
+    hipError_t force_rebuild()
+    {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, rebuild + (rbp ^ 1), 1u, (size_t)1);
+        return hipGetLastError();
     }
 
     void set_pos_parity(int p)
@@ -336,8 +363,18 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
     h.p_begin = c->np;
     h.wall_begin = (uint32_t)c->walls.size();
     c->np += s->np;
-    double rmax = 0.0;
-    for (unsigned i = 0; i < s->np; i++) rmax = std::max(rmax, h.pos[4 * i + 3]);
+    double rmax = 0.0, rmin = 1e300;
+    for (unsigned i = 0; i < s->np; i++) {
+        rmax = std::max(rmax, h.pos[4 * i + 3]);
+        rmin = std::min(rmin, h.pos[4 * i + 3]);
+    }
+    if (s->np == 0) rmin = 0.0;
+    // neighbour-list skin and bucket size: buckets hold every pair within 2 r_max + reach +
+    // skin in neighbouring buckets (the reference's cell size stays the one of the visiting
+    // order)
+    const ContactConst cc_pp = make_contact_const(s->cp_pp, 0.5 * rmax);
+    const double skin = c->cfg.skin > 0.0 ? c->cfg.skin : (c->cfg.skin == 0.0 ? 0.025 * rmin : 0.0);
+    const double bucket_cs = std::max(s->cell_size, 2.0 * rmax + cc_pp.coh_reach + skin) * (1.0 + 1e-9);
     // bucket grid: the reference's origin and cell size, power-of-two dims covering the
     // scene (wrapping beyond them, as the reference's 128^3 grid does)
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -353,7 +390,7 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
     const double wo[3] = {s->world_origin[0], s->world_origin[1], s->world_origin[2]};
     uint32_t dims[3];
     for (int a = 0; a < 3; a++) {
-        const double span = std::floor((hi[a] - wo[a]) / s->cell_size) - std::floor((lo[a] - wo[a]) / s->cell_size) + 3.0;
+        const double span = std::floor((hi[a] - wo[a]) / bucket_cs) - std::floor((lo[a] - wo[a]) / bucket_cs) + 3.0;
         uint32_t d = next_pow2((uint32_t)std::min(span, 1048576.0));
         d = std::max(4u, std::min(d, c->cfg.grid_dim_max));
         dims[a] = d;
@@ -380,7 +417,9 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
     D.half2dt = 0.5 * s->dt * s->dt;
     D.gx = s->gravity[0]; D.gy = s->gravity[1]; D.gz = s->gravity[2];
     D.grid.wo_x = wo[0]; D.grid.wo_y = wo[1]; D.grid.wo_z = wo[2];
-    D.grid.cs = s->cell_size;
+    D.grid.cs = bucket_cs;
+    D.ref_cs = s->cell_size;
+    D.skin = skin;
     D.grid.mask_x = dims[0] - 1; D.grid.mask_y = dims[1] - 1; D.grid.mask_z = dims[2] - 1;
     D.grid.dim_x = dims[0];
     D.grid.dim_xy = dims[0] * dims[1];
@@ -388,7 +427,7 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
     const unsigned long long nb = (unsigned long long)dims[0] * dims[1] * dims[2];
     if ((unsigned long long)c->nbucket + nb >= 0xffffffffull) return XDEM_E_INVALID;
     c->nbucket += (uint32_t)nb;
-    D.cc = make_contact_const(s->cp_pp, 0.5 * rmax);
+    D.cc = cc_pp;
     D.p_begin = h.p_begin;
     D.p_end = h.p_begin + s->np;
     D.wall_begin = h.wall_begin;
@@ -424,6 +463,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     if (c->committed) return XDEM_E_NOT_READY;
     if (c->scenes.empty()) return XDEM_E_INVALID;
     CTX_CHECK(hipSetDevice(c->cfg.device));
+    if (c->np >= (1u << kIdBits)) return XDEM_E_INVALID;   // ids packed below the visiting-order code
     c->multi = c->scenes.size() > 1;
     c->collide_fn = c->cfg.precision == XDEM_PRECISION_MIXED ? collide_launch_mixed() : collide_launch_f64();
     const size_t n = c->np;
@@ -455,17 +495,26 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->keys, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->svals, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->slot_tmp, sizeof(uint32_t) * n));
-    c->list_cap = std::max(48u, 2u * c->cap);
+    c->list_cap = std::min(64u, std::max(48u, 2u * c->cap));   // k_touch's pass mask has 64 bits
     CTX_CHECK(c->alloc(&c->nbr_count, sizeof(uint32_t) * n));
     c->seg_cap = queue_segment_cap((uint32_t)n);
     CTX_CHECK(c->alloc(&c->work, sizeof(uint32_t) * kQueueSegs * c->seg_cap));
     CTX_CHECK(c->alloc(&c->seg_count, sizeof(uint32_t) * kQueueSegs * kSegStride));
+    CTX_CHECK(c->alloc(&c->work2, sizeof(uint32_t) * kQueueSegs * c->seg_cap));
+    CTX_CHECK(c->alloc(&c->pass_mask, sizeof(unsigned long long) * n));
+    CTX_CHECK(c->alloc(&c->seg_count2, sizeof(uint32_t) * kQueueSegs * kSegStride));
     CTX_CHECK(c->alloc(&c->nbr, sizeof(uint32_t) * n * c->list_cap));
     const size_t nbt = bucket_table_size(c->nbucket);
     CTX_CHECK(c->alloc(&c->bucket_count, sizeof(uint32_t) * nbt));
     CTX_CHECK(c->alloc(&c->bucket_start, sizeof(uint32_t) * (nbt + 1)));
     CTX_CHECK(c->alloc(&c->scan_part, sizeof(uint32_t) * (nbt / 4096 + 1)));
     CTX_CHECK(c->alloc(&c->sposf, sizeof(float4) * n));
+    CTX_CHECK(c->alloc(&c->pos_ref, sizeof(float4) * n));
+    CTX_CHECK(c->alloc(&c->rebuild, sizeof(uint32_t) * 2));
+    CTX_CHECK(c->alloc(&c->dmax, sizeof(uint32_t) * 2 * kQueueSegs * kSegStride));
+    CTX_CHECK(c->alloc(&c->gap, sizeof(float) * n));
+    CTX_CHECK(c->alloc(&c->disp, sizeof(float) * n));
+    CTX_CHECK(c->alloc(&c->n_rebuild, sizeof(unsigned long long)));
     for (int b = 0; b < 2; b++) {
         CTX_CHECK(c->alloc(&c->pp_count[b], sizeof(uint32_t) * n));
         CTX_CHECK(c->alloc(&c->pp_id[b], sizeof(uint32_t) * n * c->cap));
@@ -530,7 +579,15 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(hipMemsetAsync(c->pl_tsd, 0, sizeof(double2) * n * XDEM_MAX_P2PL_COUNT, s));
     CTX_CHECK(hipMemsetAsync(c->err, 0, sizeof(uint32_t), s));
     CTX_CHECK(hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 4, s));
+    CTX_CHECK(hipMemsetAsync(c->n_rebuild, 0, sizeof(unsigned long long), s));
+    CTX_CHECK(hipMemsetAsync(c->dmax, 0, sizeof(uint32_t) * 2 * kQueueSegs * kSegStride, s));
+    CTX_CHECK(hipMemsetAsync(c->disp, 0, sizeof(float) * n, s));
+    CTX_CHECK(hipMemsetAsync(c->gap, 0, sizeof(float) * n, s));
+    CTX_CHECK(hipMemsetAsync(c->pos_ref, 0, sizeof(float4) * n, s));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, c->rebuild, 1u, (size_t)2);   // the first step builds
     CTX_CHECK(hipMemsetAsync(c->seg_count, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
+    CTX_CHECK(hipMemsetAsync(c->seg_count2, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
+    CTX_CHECK(hipMemsetAsync(c->nbr_count, 0, sizeof(uint32_t) * n, s));
     if (c->nwslot) CTX_CHECK(hipMemsetAsync(c->wrench_out, 0, sizeof(double) * 6 * c->nwslot, s));
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nbt + 255) / 256)), dim3(256), 0, s, c->bucket_count, 0u, nbt);
     CTX_CHECK(hipGetLastError());
@@ -787,6 +844,7 @@ int xdem_ctx_set_body_pose(xdem_ctx* c, unsigned int scene, unsigned int wall, c
     if (c->committed) {
         CTX_CHECK(hipSetDevice(c->cfg.device));
         CTX_CHECK(hipMemcpyAsync(c->d_walls + h.wall_begin + wall, &W, sizeof(WallDev), hipMemcpyHostToDevice, c->stream));
+        CTX_CHECK(c->force_rebuild());   // the wall-reach flags of the lists are for the old pose
         CTX_CHECK(hipStreamSynchronize(c->stream));
     }
     return XDEM_OK;
@@ -805,6 +863,21 @@ int xdem_ctx_info_get(xdem_ctx* c, xdem_ctx_info* out)
     out->p2p_capacity = c->cap;
     out->device_bytes = c->bytes;
     out->steps_done = c->steps_done;
+    if (c->committed) {
+        CTX_CHECK(hipSetDevice(c->cfg.device));
+        CTX_CHECK(hipMemcpyAsync(&out->list_builds, c->n_rebuild, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                 c->stream));
+        std::vector<uint32_t> q1(kQueueSegs * kSegStride), q2(kQueueSegs * kSegStride);
+        CTX_CHECK(hipMemcpyAsync(q1.data(), c->seg_count, sizeof(uint32_t) * q1.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        CTX_CHECK(hipMemcpyAsync(q2.data(), c->seg_count2, sizeof(uint32_t) * q2.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        CTX_CHECK(hipStreamSynchronize(c->stream));
+        for (uint32_t k = 0; k < kQueueSegs; k++) {
+            out->list_spheres += q1[k * kSegStride];
+            out->contact_spheres += q2[k * kSegStride];
+        }
+    }
     return XDEM_OK;
 }
 
@@ -838,13 +911,14 @@ int xdem_ctx_graph_capture(xdem_ctx* c, unsigned int nsteps)
     if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
     if (!c->drift_valid) CTX_CHECK(c->run_drift());   // the graph's steps start from a precomputed drift
     CTX_CHECK(hipStreamSynchronize(c->stream));
-    const int parity = c->cur, sparity = c->sv, pparity = c->spp;
+    const int parity = c->cur, sparity = c->sv, pparity = c->spp, rparity = c->rbp;
     CTX_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (unsigned k = 0; k < nsteps && e == hipSuccess; k++) e = c->full_step(nullptr);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     c->cur = parity;
+    c->rbp = rparity;
     c->set_state_parity(sparity);
     c->set_pos_parity(pparity);
     c->drift_valid = true;

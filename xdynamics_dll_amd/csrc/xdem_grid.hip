@@ -1,11 +1,15 @@
-// xdem_grid.hip -- the bucket grid of the fused step (xdem_ctx path).
+// xdem_grid.hip -- the bucket grid and neighbour lists of the fused step (xdem_ctx path).
 //
 // The reference sorts (hash, index) pairs with a radix sort every step and then finds the
 // first and last sorted position of every cell (def:94-124; impl:317-360). Here the grid is
-// a counting sort over a dense bucket table. The drift of a step (position and orientation
-// update, impl:135-147) is computed at the end of the previous step's velocity update
-// (drift_sphere), which also writes each sphere's bucket key and counts it (posted atomic
-// add); k_drift does the same when no previous step left it. Then:
+// a counting sort over a dense bucket table, rebuilt together with the neighbour lists only
+// on the steps that need it: the lists hold every pair within reach plus a skin, and the
+// drift of each step (drift_position) flags the next step for a rebuild once a sphere has
+// moved half the skin since the last build. Every kernel below returns at once on the other
+// steps. The drift of a step (position and orientation update, impl:135-147) is computed at
+// the end of the previous step's velocity update; k_drift does it when no previous step
+// left it. A rebuild is:
+//   k_count           bucket key of every sphere and its count (posted atomic add)
 //   k_scan_reduce     \
 //   (k_scan_parts)     > exclusive prefix of the counts: start[b] = first sorted slot of
 //   k_scan_apply      /  bucket b, start[nbucket] = N (tables of at most kDirectTiles tiles:
@@ -15,12 +19,15 @@
 //   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
 //                     smaller id, i.e. exactly the reference's (hash, index) order; writes
 //                     the sorted id and the FP32 position copy the pair search reads, with
-//                     the sphere's wall-reach flag (FP64 test) in the sign of the radius
-// The pair search k_pairs (xdem_step.hpp) is launched from here too: its FP64 pair
-// predicate must round exactly like the reference's.
+//                     the sphere's wall-reach flag (FP64 test, reach + skin) in the sign of
+//                     the radius; the same copy by sphere id is the reference position of
+//                     the half-skin test; the wall gap of the sphere (k_touch)
+// The pair search k_pairs and k_order_lists (xdem_step.hpp) are launched from here too.
 // The bucket of slot range [start[b], start[b+1]) needs no clearing between steps, and a
 // row of three consecutive buckets is the single range [start[b], start[b+3]).
 // Built with -ffp-contract=off: the drift is bit-identical to the reference's.
+#include <algorithm>
+
 #include "xdem_step.hpp"
 
 namespace xdem {
@@ -60,17 +67,39 @@ __global__ __launch_bounds__(kBlock) void k_drift(StepArgs A)
                  ld4(A.ev + 4 * i), ld4(A.ea + 4 * i));
 }
 
+// First kernel of every step: clears the step's contact counters, full-pass queue and the
+// next step's rebuild flag. On a rebuild: the bucket key of every sphere and its count.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_count(StepArgs A)
+{
+    if (blockIdx.x == 0 && threadIdx.x < 2) A.counters[threadIdx.x] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) {
+        A.seg_count2[threadIdx.x * kSegStride] = 0u;
+        A.dmax_next[threadIdx.x * kSegStride] = 0u;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *A.rebuild_next = 0u;
+    if (!*A.rebuild_cur) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *A.n_rebuild += 1ull;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.np) return;
+    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    const V4<double> p = ld4(A.pos + 4 * i);
+    int gx, gy, gz;
+    grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
+    const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
+    A.keys[i] = key;
+    atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: k_scatter takes the rank
+}
+
 // Sum of 16 consecutive buckets per thread, reduced per block of 4096 buckets. Block 0 also
-// clears the step's contact counters and the full-pass queue segment counters.
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ count,
+// clears the full-pass queue segment counters.
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ gate,
+                                                        const uint32_t* __restrict__ count,
                                                         uint32_t* __restrict__ part,
-                                                        unsigned long long* __restrict__ counters,
                                                         uint32_t* __restrict__ seg_count)
 {
-    if (blockIdx.x == 0) {
-        if (threadIdx.x < 2) counters[threadIdx.x] = 0ull;
-        if (threadIdx.x < kQueueSegs) seg_count[threadIdx.x * kSegStride] = 0u;
-    }
+    if (!*gate) return;
+    if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) seg_count[threadIdx.x * kSegStride] = 0u;
     __shared__ uint32_t wsum[kBlock / 64];
     const uint4* c = reinterpret_cast<const uint4*>(count + (size_t)blockIdx.x * kScanTile) + threadIdx.x * 4;
     uint32_t s = 0;
@@ -104,9 +133,11 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x)
 }
 
 // Exclusive scan of the block sums by one block; writes the grand total to *total.
-__global__ __launch_bounds__(kPartsBlock) void k_scan_parts(uint32_t* __restrict__ part, uint32_t nparts,
+__global__ __launch_bounds__(kPartsBlock) void k_scan_parts(const uint32_t* __restrict__ gate,
+                                                            uint32_t* __restrict__ part, uint32_t nparts,
                                                             uint32_t* __restrict__ total)
 {
+    if (!*gate) return;
     __shared__ uint32_t wsum[kPartsBlock / 64];
     const uint32_t per = (nparts + kPartsBlock - 1) / kPartsBlock;
     const uint32_t b0 = threadIdx.x * per;
@@ -141,11 +172,13 @@ __global__ __launch_bounds__(kPartsBlock) void k_scan_parts(uint32_t* __restrict
 // adds up those of the earlier tiles itself; the last block writes the total to *total.
 // Otherwise part holds the exclusive prefix of the tile sums (k_scan_parts).
 template <bool DIRECT>
-__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ count,
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ gate,
+                                                       const uint32_t* __restrict__ count,
                                                        const uint32_t* __restrict__ part,
                                                        uint32_t* __restrict__ start,
                                                        uint32_t* __restrict__ total)
 {
+    if (!*gate) return;
     __shared__ uint32_t wsum[kBlock / 64];
     __shared__ uint32_t red[kBlock / 64];
     uint32_t tile0;
@@ -186,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
 
 __global__ __launch_bounds__(kBlock) void k_scatter(StepArgs A)
 {
+    if (!*A.rebuild_cur) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.np) return;
     const uint32_t key = A.keys[i];
@@ -197,6 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(StepArgs A)
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
 {
+    if (!*A.rebuild_cur) return;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= A.nsorted) return;
     const uint32_t k = A.slot_tmp[j];
@@ -208,16 +243,27 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     A.svals[f] = k;
     const V4<double> p = ld4(A.pos + 4 * k);
     const SceneDev& S = A.scenes[scene_index(A, k, MULTI)];
-    // exact FP64 reach test against every wall of the scene (wall_out_of_reach): a sphere
-    // with a wall in reach goes to the full contact pass
+    // exact FP64 reach test against every wall of the scene (wall_out_of_reach, radius
+    // grown by the skin): a sphere with a wall in reach goes to the full contact pass; and
+    // the least displacement that could bring a wall into reach, |dp . uw| / max(1, |uw|) -
+    // r (1 + 1e-6), the gap k_touch compares with (any cohesive wall: none)
     bool far = true;
+    double wgap = 3.0e38;
     for (uint32_t w = 0; w < S.nwall; w++) {
         const WallDev& W = A.walls[S.wall_begin + w];
-        far = far & wall_out_of_reach(W.cc, W.info, p);
+        far = far & wall_out_of_reach(W.cc, W.info, p, S.skin);
+        const xdem_plane_info& pl = W.info;
+        const double uw2 = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
+        const double wz = (p.x - pl.xw.x) * pl.uw.x + (p.y - pl.xw.y) * pl.uw.y + (p.z - pl.xw.z) * pl.uw.z;
+        const double g = W.cc.coh != 0.0 ? -1.0 : fabs(wz) / sqrt(fmax(1.0, uw2)) - p.w * (1.0 + 1e-6) - 1e-9;
+        wgap = fmin(wgap, g);
     }
+    A.gap[k] = __double2float_rd(wgap);
     const Grid& gr = S.grid;
     const float rf = (float)p.w;
-    A.sposf[f] = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), far ? rf : -rf);
+    const float4 q = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), rf);
+    A.pos_ref[k] = q;
+    A.sposf[f] = make_float4(q.x, q.y, q.z, far ? rf : -rf);
 }
 
 }  // namespace
@@ -238,8 +284,21 @@ hipError_t launch_drift(const StepArgs& A, bool multi, hipStream_t s)
 hipError_t launch_pairs(const StepArgs& A, bool multi, hipStream_t s)
 {
     if (A.nsorted == 0) return hipSuccess;
-    if (multi) hipLaunchKernelGGL(k_pairs<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-    else hipLaunchKernelGGL(k_pairs<false>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+    if (multi) {
+        hipLaunchKernelGGL(k_pairs<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+        hipLaunchKernelGGL(k_order_lists<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    } else {
+        hipLaunchKernelGGL(k_pairs<false>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+        hipLaunchKernelGGL(k_order_lists<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_touch(const StepArgs& A, bool multi, hipStream_t s)
+{
+    const dim3 g(kQueueSegs * kQueueChunks);
+    if (multi) hipLaunchKernelGGL(k_touch<true>, g, dim3(kBlock), 0, s, A);
+    else hipLaunchKernelGGL(k_touch<false>, g, dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 
@@ -247,14 +306,19 @@ hipError_t launch_grid(const StepArgs& A, bool multi, hipStream_t s)
 {
     const uint32_t nb = bucket_table_size(A.nbucket);
     const uint32_t ntile = nb / kScanTile;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part, A.counters,
+    const uint32_t* gate = A.rebuild_cur;
+    // the first kernel also runs when np is 0: it clears the step's counters and flag
+    if (multi) hipLaunchKernelGGL(k_count<true>, grid_for(std::max(A.np, 1u)), dim3(kBlock), 0, s, A);
+    else hipLaunchKernelGGL(k_count<false>, grid_for(std::max(A.np, 1u)), dim3(kBlock), 0, s, A);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(ntile), dim3(kBlock), 0, s, gate, A.bucket_count, A.scan_part,
                        A.seg_count);
     if (ntile <= kDirectTiles) {
-        hipLaunchKernelGGL(k_scan_apply<true>, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part,
+        hipLaunchKernelGGL(k_scan_apply<true>, dim3(ntile), dim3(kBlock), 0, s, gate, A.bucket_count, A.scan_part,
                            A.bucket_start, A.bucket_start + nb);
     } else {
-        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kPartsBlock), 0, s, A.scan_part, ntile, A.bucket_start + nb);
-        hipLaunchKernelGGL(k_scan_apply<false>, dim3(ntile), dim3(kBlock), 0, s, A.bucket_count, A.scan_part,
+        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kPartsBlock), 0, s, gate, A.scan_part, ntile,
+                           A.bucket_start + nb);
+        hipLaunchKernelGGL(k_scan_apply<false>, dim3(ntile), dim3(kBlock), 0, s, gate, A.bucket_count, A.scan_part,
                            A.bucket_start, A.bucket_start + nb);
     }
     if (A.np) {

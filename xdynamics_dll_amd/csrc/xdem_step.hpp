@@ -16,7 +16,11 @@
 namespace xdem {
 
 constexpr uint32_t kInvalid = 0xffffffffu;
-constexpr uint32_t kFullPass = 0x80000000u;   // nbr_count flag: the sphere goes to the full contact pass
+// nbr_count of a sphere: the length of its neighbour list and three flags
+constexpr uint32_t kCandidate = 0x80000000u;  // listed neighbours or a wall within reach + skin (build)
+constexpr uint32_t kWallNear = 0x40000000u;   // a wall within reach + skin (build)
+constexpr uint32_t kTouch = 0x20000000u;      // this step: a pair passes or a wall is in reach (k_touch)
+constexpr uint32_t kCountMask = 0x1fffffffu;
 // The full-pass queue is split in kQueueSegs segments, each with its own counter on its own
 // 128-byte line: the wave of sorted slots [64 w, 64 w + 64) appends to segment w mod
 // kQueueSegs, so the appends of different waves rarely hit the same counter.
@@ -42,6 +46,9 @@ struct SceneDev {
     uint32_t wall_id0;        // a wall of the scene has plane id 0 (it resets the wall slot count)
     uint32_t wall_dynamic;    // a wall of the scene is a dynamic body (per-sphere wrench output)
     uint32_t pad_;
+    double skin;              // neighbour-list skin (0: the lists are rebuilt every step)
+    double ref_cs;            // the reference's cell size (visiting order of the contacts);
+                              // grid.cs, the bucket size, is larger by the skin and reach
 };
 
 struct WallDev {
@@ -71,12 +78,24 @@ struct StepArgs {
     uint32_t* scan_part;                     // per 4096-bucket tile sums / offsets
     uint32_t* slot_tmp;                      // sphere id per provisional slot
     uint32_t* svals;                         // sphere id per sorted slot
-    uint32_t* work;                          // spheres for the full contact pass
-    uint32_t* nbr_count;                     // near neighbours per sphere (k_pairs)
+    uint32_t* work;                          // candidate spheres (k_pairs), by queue segment
+    uint32_t* work2;                         // spheres of this step's full contact pass (k_touch)
+    unsigned long long* pass_mask;           // per sphere of work2: list entries that pass (k_touch)
+    uint32_t* nbr_count;                     // listed neighbours per sphere | kCandidate, kWallNear, kTouch
     uint32_t* nbr;                           // their sphere ids, list_cap per sphere
     float4* sposf;  // per sorted slot: position relative to the scene's grid origin and
                     // radius, FP32 (the pair search reads only this); the radius is
-                    // negated when a wall is within reach of the sphere
+                    // negated when a wall is within reach (+ skin) of the sphere
+    float4* pos_ref;                         // per sphere: sposf at the last list build
+    uint32_t* rebuild_cur;                   // nonzero: this step rebuilds grid and lists
+    uint32_t* rebuild_next;                  // set when a sphere moved half the skin (or null)
+    float* gap;                              // per sphere: lower bound of the distance its listed
+                                             // pairs and near walls lack to pass (last build)
+    float* disp;                             // per sphere: upper bound of its displacement since
+                                             // the last build (the drift of the previous step)
+    uint32_t* dmax_cur;                      // largest disp of this step (bits of a float, per
+    uint32_t* dmax_next;                     // queue segment); the drift writes the next one's
+    unsigned long long* n_rebuild;           // list builds so far
     // sphere history (double-buffered) and wall history (in place)
     const uint32_t* pp_count_in; const uint32_t* pp_id_in; const double2* pp_tsd_in;
     uint32_t* pp_count_out; uint32_t* pp_id_out; double2* pp_tsd_out;
@@ -84,8 +103,9 @@ struct StepArgs {
     double* wrench;          // [nwslot][6][np] per-sphere contributions to dynamic walls
     uint32_t* err;
     unsigned long long* counters;  // [0] stored sphere contacts, [1] stored wall contacts
-    uint32_t* seg_count;           // spheres per full-pass queue segment (kSegStride apart)
-    uint32_t seg_cap;              // entries per queue segment in work
+    uint32_t* seg_count;           // spheres per candidate queue segment (kSegStride apart)
+    uint32_t* seg_count2;          // spheres per full-pass queue segment (kSegStride apart)
+    uint32_t seg_cap;              // entries per queue segment in work and work2
     uint32_t np, nsorted, cap;
     uint32_t nwslot, nbucket;
     uint32_t list_cap;                       // neighbour-list entries per sorted slot
@@ -109,29 +129,33 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
     return x * per + min(x, extra) + r;
 }
 
-// Pair search: for the sphere in sorted slot j, the ids of every neighbour that can pass
-// the reference's pair predicate (cdist > 0, or inside the JKR separation distance of a
-// cohesive pair), in the reference's visiting order: 27 buckets in (z, y, x) order, spheres
-// of a bucket in id order. The three buckets of an x-row are consecutive keys, or two runs
-// when the row wraps (the buckets up to the end of the row, then those from its start).
-// One flat loop walks all nine rows, so each lane advances through its own candidates
-// without waiting for the other lanes row by row. Only an FP32 test runs here (few
-// registers, many waves per SIMD to hide the latency of the candidate loads); k_collide
-// applies the exact FP64 predicate to the list, in the same order.
+// Pair search (steps that rebuild the lists): for the sphere in sorted slot j, the ids of
+// every neighbour within r_i + r_j + reach + skin, where reach bounds the JKR separation
+// distance of the scene's pairs: every pair that can pass the reference's predicate
+// (cdist > 0, or inside the JKR separation distance of a cohesive pair) before any sphere
+// has moved half the skin. Buckets are at least 2 r_max + reach + skin wide, so the 27
+// buckets around the sphere's hold them all. The three buckets of an x-row are consecutive
+// keys, or two runs when the row wraps (the buckets up to the end of the row, then those
+// from its start). One flat loop walks all nine rows, so each lane advances through its own
+// candidates without waiting for the other lanes row by row. Only an FP32 test runs here
+// (few registers, many waves per SIMD to hide the latency of the candidate loads);
+// k_order_lists puts each list in the reference's visiting order and k_collide applies the
+// exact FP64 predicate to it every step.
 //
-// FP32 test: a candidate is dropped only when its FP32 distance exceeds r_i + r_j + reach
-// by more than the rounding of the FP32 coordinates (|x| 2^-24 per coordinate and operand,
-// bounded by mg) and of the FP32 radius sum (factor 1 + 1e-6), so every pair the exact test
-// accepts is in the list. reach bounds the JKR separation distance of the scene's pairs.
-// The traversal works on bucket coordinates, i.e. the cell coordinates masked to the
-// grid, exactly as grid_bucket(grid_cell(position)) of the reference's traversal.
+// FP32 test: a candidate is dropped only when its FP32 distance exceeds r_i + r_j + reach +
+// skin by more than the rounding of the FP32 coordinates (|x| 2^-24 per coordinate and
+// operand, bounded by mg) and of the FP32 radius sum (factor 1 + 1e-6).
+// The traversal works on bucket coordinates, i.e. the bucket-grid cell coordinates masked to
+// the grid, exactly as grid_bucket(grid_cell(position)).
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 {
+    if (!*A.rebuild_cur) return;
     const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const uint32_t i = j < A.nsorted ? A.svals[j] : kInvalid;
     uint32_t n = 0;
     bool full = false;
+    float pgap = 3.0e38f;
     if (i < A.np) {
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
         const float4 fi = A.sposf[j];
@@ -145,7 +169,8 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
             const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
             // lim = r_j (1 + 1e-6) + [(r_i + reach)(1 + 1e-6) + mg]: the same bound with the
             // sphere's own part hoisted (each FP32 rounding is far below the 1e-6 margin)
-            const float limi = ((fabsf(fi.w) + (float)S.cc.coh_reach) * 1.000001f) + mg;
+            const float limi = ((fabsf(fi.w) + (float)(S.cc.coh_reach + S.skin)) * 1.000001f) + mg;
+            const float limr = ((fabsf(fi.w) + (float)S.cc.coh_reach) * 1.000001f) + 2.0f * mg + 1e-9f;
             const uint32_t lc = A.list_cap;
             uint32_t* out = A.nbr + (size_t)i * lc;
             const uint32_t bx0 = (bx - 1u) & G.mask_x;
@@ -162,12 +187,16 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
                 total = len1 + (e2 - s2);
                 t = 0;
             };
-            // candidate m with FP32 copy f, in visiting order
+            // candidate m with FP32 copy f, in visiting order; pgap: the least FP32 distance
+            // a listed pair lacks to pass, less the rounding bound
             auto test = [&](uint32_t m, float4 f) {
                 if (m == j) return;
                 const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
-                const float lim = fabsf(f.w) * 1.000001f + limi;
-                if (dx * dx + dy * dy + dz * dz > lim * lim) return;
+                const float rj = fabsf(f.w) * 1.000001f;
+                const float lim = rj + limi;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 > lim * lim) return;
+                pgap = fminf(pgap, sqrtf(d2) - (rj + limr));
                 if (n < lc) out[n] = A.svals[m];
                 n++;
             };
@@ -190,10 +219,12 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
                 }
             }
         }
-        // the full contact pass takes spheres with listed neighbours or a wall in reach
-        // (k_order_gather marks the latter with the sign of the FP32 radius)
-        full = n != 0u || (__float_as_uint(fi.w) >> 31) != 0u;
-        A.nbr_count[i] = n | (full ? kFullPass : 0u);
+        // candidates: spheres with listed neighbours or a wall in reach + skin (k_order_gather
+        // marks the latter with the sign of the FP32 radius); k_touch tests them every step
+        const bool wall_near = (__float_as_uint(fi.w) >> 31) != 0u;
+        full = n != 0u || wall_near;
+        A.nbr_count[i] = n | (full ? kCandidate : 0u) | (wall_near ? kWallNear : 0u);
+        if (n) A.gap[i] = fminf(A.gap[i], pgap);
     }
     // queue for the full pass: one atomic per wave, issued by its first active lane, on the
     // wave's segment counter
@@ -210,10 +241,10 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 }
 
 // Drift of the next step (impl:135-147; xIntegratorVV.cpp:66-100): Euler parameters and
-// position from the state the velocity update just produced, the bucket key of the new
-// position and its count (a posted atomic add). Written to the second position/orientation
-// buffer, which the next step makes current. Same expressions, in the same order, as a
-// separate position-update pass would evaluate.
+// position from the state the velocity update just produced, written to the second
+// position/orientation buffer, which the next step makes current; and the half-skin test
+// that decides whether the next step rebuilds the grid and the neighbour lists. Same
+// expressions, in the same order, as a separate position-update pass would evaluate.
 __device__ __forceinline__ void drift_orientation(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> e,
                                                   V4<double> d, V4<double> a)
 {
@@ -227,25 +258,51 @@ __device__ __forceinline__ void drift_orientation(const StepArgs& A, const Scene
     st4(A.ep_next + 4 * i, ne);
 }
 
-__device__ __forceinline__ void drift_position(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
-                                               V3<double> v, V3<double> ac)
+__device__ __forceinline__ float drift_position(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
+                                                V3<double> v, V3<double> ac)
 {
     p.x += S.dt * v.x + S.half2dt * ac.x;
     p.y += S.dt * v.y + S.half2dt * ac.y;
     p.z += S.dt * v.z + S.half2dt * ac.z;
     st4(A.pos_next + 4 * i, p);
-    int gx, gy, gz;
-    grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
-    const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
-    A.keys[i] = key;
-    atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: k_scatter takes the rank
+    // half-skin test: the lists of the last build hold every pair within reach as long as no
+    // sphere has moved half the skin since; the rounding of the FP32 reference position
+    // (bounded as in k_pairs) is taken off the threshold
+    // and the displacement bound k_touch compares with the gaps of the lists
+    float dsp = 0.0f;
+    if (A.rebuild_next) {
+        const float4 r = A.pos_ref[i];
+        const double dx = (p.x - S.grid.wo_x) - (double)r.x;
+        const double dy = (p.y - S.grid.wo_y) - (double)r.y;
+        const double dz = (p.z - S.grid.wo_z) - (double)r.z;
+        const double eps = 2.4e-7 * (fabs((double)r.x) + fabs((double)r.y) + fabs((double)r.z)) + 1e-9;
+        const double lim = 0.5 * S.skin - eps;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (!(lim > 0.0 && d2 < lim * lim)) *A.rebuild_next = 1u;
+        dsp = __double2float_ru(sqrt(d2) + eps);
+        if (!(dsp < 3.0e38f)) dsp = 3.0e38f;   // a non-finite position: the largest bound
+        A.disp[i] = dsp;
+    }
+    return dsp;
 }
 
-__device__ __forceinline__ void drift_sphere(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
-                                             V4<double> e, V3<double> v, V3<double> ac, V4<double> d, V4<double> a)
+__device__ __forceinline__ float drift_sphere(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
+                                              V4<double> e, V3<double> v, V3<double> ac, V4<double> d, V4<double> a)
 {
     drift_orientation(A, S, i, e, d, a);
-    drift_position(A, S, i, p, v, ac);
+    return drift_position(A, S, i, p, v, ac);
+}
+
+// Wave-wide max of the displacement bounds into the segment counter of the wave (floats >= 0
+// compare like their bits); every lane of the wave calls it.
+__device__ __forceinline__ void add_disp_max(const StepArgs& A, float d)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) d = fmaxf(d, __shfl_xor(d, off, 64));
+    if ((threadIdx.x & 63) == 0 && d > 0.0f && A.dmax_next) {
+        const uint32_t w = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) & (kQueueSegs - 1u);
+        atomicMax(&A.dmax_next[w * kSegStride], __float_as_uint(d));
+    }
 }
 
 // A sphere's own state, loaded together before any branch (independent loads in flight at
@@ -285,12 +342,181 @@ __device__ __forceinline__ OwnState load_own(const StepArgs& A, const SceneDev& 
     return o;
 }
 
+// ---------------------------------------------------------------- reference visiting order
+// The reference visits the neighbours of sphere i cell by cell, z, y, x offsets -1..1 in
+// that nesting, spheres of a cell by id (impl:613-733 over the sorted cell table; the stable
+// sort of (hash, index), def:94-124). A pair that passes its predicate lies in neighbouring
+// cells, so its place in that order is the key (code << 27 | id) with code =
+// (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1) of the reference's cells (impl:54-72 without the
+// 128 wrap, which only aliases cells too far apart to pass); code 27 marks a sphere whose
+// cell is not a neighbour (the reference never visits it). Sphere ids stay below 2^27.
+constexpr uint32_t kIdBits = 27;
+constexpr uint32_t kIdMask = (1u << kIdBits) - 1u;
+constexpr uint32_t kNotVisited = 27u;
+
+__device__ __forceinline__ void ref_cell(const SceneDev& S, V4<double> p, int& cx, int& cy, int& cz)
+{
+    cx = (int)floor((p.x - S.grid.wo_x) / S.ref_cs);
+    cy = (int)floor((p.y - S.grid.wo_y) / S.ref_cs);
+    cz = (int)floor((p.z - S.grid.wo_z) / S.ref_cs);
+}
+
+__device__ __forceinline__ uint32_t visit_code(const SceneDev& S, int ix, int iy, int iz, V4<double> pj)
+{
+    int jx, jy, jz;
+    ref_cell(S, pj, jx, jy, jz);
+    const int dx = jx - ix, dy = jy - iy, dz = jz - iz;
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return kNotVisited;
+    return (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+}
+
+// Puts a neighbour list in the reference's visiting order at the current positions: codes
+// are packed above the ids, the list is insertion-sorted in place (it is short and this
+// thread's own), then the codes are removed. k_order_lists does it for every list after a
+// build; between builds k_collide checks the order of the pairs that pass and sorts the list
+// again when a neighbour has changed cells.
+__device__ __forceinline__ void order_list(const StepArgs& A, const SceneDev& S, uint32_t* lst, uint32_t n, int ix,
+                                           int iy, int iz)
+{
+    for (uint32_t t = 0; t < n; t++) {
+        const uint32_t k = lst[t];
+        lst[t] = (visit_code(S, ix, iy, iz, ld4(A.pos + 4 * (size_t)k)) << kIdBits) | k;
+    }
+    for (uint32_t t = 1; t < n; t++) {
+        const uint32_t x = lst[t];
+        uint32_t u = t;
+        while (u > 0u && lst[u - 1u] > x) {
+            lst[u] = lst[u - 1u];
+            u--;
+        }
+        lst[u] = x;
+    }
+    for (uint32_t t = 0; t < n; t++) lst[t] &= kIdMask;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_order_lists(StepArgs A)
+{
+    if (!*A.rebuild_cur) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.np) return;
+    const uint32_t n = min(A.nbr_count[i] & kCountMask, A.list_cap);
+    if (n < 2u) return;
+    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    int ix, iy, iz;
+    ref_cell(S, ld4(A.pos + 4 * i), ix, iy, iz);
+    order_list(A, S, A.nbr + (size_t)i * A.list_cap, n, ix, iy, iz);
+}
+
+// The reference's pair predicate: overlap, or inside the JKR separation depth of a cohesive
+// pair (impl:671-718).
+__device__ __forceinline__ bool pair_passes(const ContactConst& cc, double ir, double jr, const PairGeom& g)
+{
+    if (g.cdist > 0) return true;
+    if (!(cc.coh_reach > 0.0 && -g.cdist < cc.coh_reach)) return false;
+    const double coh_s = cohesion_limit_depth(cc, jr ? (ir * jr) / (ir + jr) : ir);
+    return fabs(g.cdist) < fabs(coh_s);
+}
+
+// Every step, for the candidates of the last build: does a listed pair pass the exact
+// predicate (in neighbouring reference cells), or is a wall in reach (wall_out_of_reach)?
+// Those spheres get kTouch and go to this step's full pass (queue work2, appended within the
+// candidate's queue segment, so a segment never holds more than seg_cap), with a bit mask of
+// the passing list entries (lists hold at most 64); the others are completed by the lite
+// pass, as the full pass would add only zero sums for them. The list is sorted again
+// (order_list) when the keys of its passing pairs are not increasing, i.e. a sphere has
+// changed cells since the build. Candidates whose gap exceeds their displacement plus the
+// largest displacement since the build cannot touch and are not tested. Block b walks
+// segment b mod kQueueSegs.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_touch(StepArgs A)
+{
+    const uint32_t seg = blockIdx.x % kQueueSegs;
+    const uint32_t nchunk = gridDim.x / kQueueSegs;
+    const uint32_t cnt = A.seg_count[seg * kSegStride];
+    const uint32_t lane = threadIdx.x & 63u;
+    // no sphere has moved more than dm since the build (0 on a build step)
+    const bool built = *A.rebuild_cur != 0u;
+    __shared__ float dm_sh;
+    if (threadIdx.x < 64) {
+        float d = built ? 0.0f : __uint_as_float(A.dmax_cur[threadIdx.x * kSegStride]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) d = fmaxf(d, __shfl_xor(d, off, 64));
+        if (threadIdx.x == 0) dm_sh = d;
+    }
+    __syncthreads();
+    const float dm = dm_sh;
+    for (uint32_t base = (blockIdx.x / kQueueSegs) * blockDim.x; base < cnt; base += nchunk * blockDim.x) {
+        const uint32_t e = base + threadIdx.x;
+        bool touch = false;
+        uint32_t i = 0;
+        if (e < cnt) {
+            i = A.work[(size_t)seg * A.seg_cap + e];
+            const uint32_t fl = A.nbr_count[i];
+            // a pair can pass only once the two spheres have moved its gap together, a wall
+            // come into reach only once the sphere has moved its gap
+            const float di = built ? 0.0f : A.disp[i];
+            const bool may = !(A.gap[i] > (di + dm) * 1.0001f + 1e-9f);
+            const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+            const V4<double> p = ld4(A.pos + 4 * i);
+            if (may && (fl & kWallNear))
+                for (uint32_t w = 0; w < S.nwall; w++) {
+                    const WallDev& W = A.walls[S.wall_begin + w];
+                    touch = touch || !wall_out_of_reach(W.cc, W.info, p);
+                }
+            const uint32_t n = may ? min(fl & kCountMask, A.list_cap) : 0u;
+            if (n) {
+                int ix, iy, iz;
+                ref_cell(S, p, ix, iy, iz);
+                uint32_t* lst = A.nbr + (size_t)i * A.list_cap;
+                uint32_t prev = 0;
+                bool have = false, ordered = true;
+                unsigned long long m = 0;
+                for (uint32_t t = 0; t < n; t++) {
+                    const uint32_t k = lst[t];
+                    const V4<double> pj = ld4(A.pos + 4 * (size_t)k);
+                    if (!pair_passes(S.cc, p.w, pj.w, pair_geom(p, pj))) continue;
+                    const uint32_t code = visit_code(S, ix, iy, iz, pj);
+                    if (code == kNotVisited) continue;
+                    m |= 1ull << t;
+                    const uint32_t key = (code << kIdBits) | k;
+                    if (have && key <= prev) ordered = false;
+                    prev = key;
+                    have = true;
+                }
+                if (!ordered) {
+                    // sort, then find the passing entries at their new places
+                    order_list(A, S, lst, n, ix, iy, iz);
+                    m = 0;
+                    for (uint32_t t = 0; t < n; t++) {
+                        const V4<double> pj = ld4(A.pos + 4 * (size_t)lst[t]);
+                        if (pair_passes(S.cc, p.w, pj.w, pair_geom(p, pj)) &&
+                            visit_code(S, ix, iy, iz, pj) != kNotVisited)
+                            m |= 1ull << t;
+                    }
+                }
+                touch = touch || m != 0ull;
+                A.pass_mask[i] = m;
+            }
+            A.nbr_count[i] = (fl & ~kTouch) | (touch ? kTouch : 0u);
+        }
+        const unsigned long long mask = __ballot(touch);
+        if (mask) {
+            const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+            uint32_t b2 = 0;
+            if (lane == leader) b2 = atomicAdd(&A.seg_count2[seg * kSegStride], (uint32_t)__popcll(mask));
+            b2 = __shfl(b2, (int)leader, 64);
+            if (touch) A.work2[(size_t)seg * A.seg_cap + b2 + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+        }
+    }
+}
+
 // End of a sphere's contact pass. Fused mode: rolling resistance (impl:2420-2454) and the
 // velocity and orientation-rate update (xIntegratorVV.cpp:131-152; impl:192-222). Split
 // mode: the sums are stored for k_finish.
 template <bool FUSE>
 __device__ __forceinline__ void sphere_tail(const StepArgs& A, const SceneDev& S, uint32_t i, const OwnState& o,
-                                            Accum tot, uint32_t& err)
+                                            Accum tot, uint32_t& err, float& dmx)
 {
     if (FUSE) {
         const V4<double> e = o.ep;
@@ -323,7 +549,7 @@ __device__ __forceinline__ void sphere_tail(const StepArgs& A, const SceneDev& S
         st3(A.acc + 3 * i, a);
         st4(A.ea + 4 * i, mea);
         if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) err |= XDEM_ERR_NONFINITE;
-        if (A.drift) drift_sphere(A, S, i, o.pos, e, v, a, av, mea);
+        if (A.drift) dmx = fmaxf(dmx, drift_sphere(A, S, i, o.pos, e, v, a, av, mea));
     } else {
         st3(A.force + 3 * i, tot.F);
         st3(A.moment + 3 * i, tot.M);
@@ -336,7 +562,8 @@ __device__ __forceinline__ void sphere_tail(const StepArgs& A, const SceneDev& S
 // the exact FP64 predicate; every wall of the scene in id order; then sphere_tail. The
 // sphere's own state and outputs are read and written by id.
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
-__device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, uint32_t& n_pp, uint32_t& n_wall)
+__device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, uint32_t& n_pp, uint32_t& n_wall,
+                                               float& dmx)
 {
     const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
     OwnState o = load_own<false>(A, S, i);
@@ -352,13 +579,17 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
         if (old_count > cap) old_count = cap;
         V3<T> Ft = mk3(T(0), T(0), T(0));
         uint32_t n_new = 0, n_seen = 0;
-        const uint32_t nlist_all = A.nbr_count[i] & ~kFullPass;
+        const uint32_t nlist_all = A.nbr_count[i] & kCountMask;
         const uint32_t lc = A.list_cap;
         if (nlist_all > lc) err |= XDEM_ERR_NBR_OVERFLOW;
         const uint32_t nlist = min(nlist_all, lc);
         const uint32_t* lst = A.nbr + (size_t)i * lc;
-        const double reach = S.cc.coh_reach;
-        for (uint32_t t = 0; t < nlist; t++) {
+        // the listed pairs that pass (k_touch's mask), in list order, which is the reference's
+        // visiting order (k_order_lists, and k_touch when a sphere has changed cells)
+        unsigned long long m = nlist ? A.pass_mask[i] : 0ull;
+        while (m) {
+            const uint32_t t = (uint32_t)__ffsll((long long)m) - 1u;
+            m &= m - 1ull;
             const uint32_t k = lst[t];
             const V4<double> jpos = ld4(A.pos + 4 * (size_t)k);
             const PairGeom g = pair_geom(o.pos, jpos);
@@ -376,11 +607,7 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
                 sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, o.pos, o.vel, iom, o.mass, jpos, jvel, jom,
                                                   A.mass[k], g, sd0, sd1, Ft, pp);
             } else {
-                if (!(reach > 0.0 && -g.cdist < reach)) continue;
-                const double jr = jpos.w, ir = o.pos.w;
-                const double coh_s = cohesion_limit_depth(S.cc, jr ? (ir * jr) / (ir + jr) : ir);
-                if (!(fabs(g.cdist) < fabs(coh_s))) continue;
-                sphere_cohesion(S.cc, ir, jr, g, pp);
+                sphere_cohesion(S.cc, o.pos.w, jpos.w, g, pp);
             }
             n_seen++;
             if (n_new < cap) {
@@ -430,7 +657,7 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
     }
 
     if (FUSE) load_own_fused(A, S, i, o);
-    sphere_tail<FUSE>(A, S, i, o, tot, err);
+    sphere_tail<FUSE>(A, S, i, o, tot, err, dmx);
     if (err) atomicOr(A.err, err);
 }
 
@@ -448,17 +675,17 @@ __device__ __forceinline__ void add_contact_counts(const StepArgs& A, unsigned l
     }
 }
 
-// Lite pass: every sphere k_pairs did not send to the full pass (no listed neighbour, no wall
-// within reach, wall_out_of_reach): the full pass would add only zero sums, so the lite pass
-// applies the same tail to zero sums, with the walls' slot-count rule of wall_no_contact
-// taken from the scene's summary (the count drops to 0 if the scene has a wall with id 0,
-// impl:989). All loads are issued before the first branch. It runs concurrently with the
-// full pass (the two touch disjoint spheres).
+// Lite pass: every sphere outside this step's full pass (no listed pair passes, no wall in
+// reach): the full pass would add only zero sums, so the lite pass applies the same tail to
+// zero sums, with the walls' slot-count rule of wall_no_contact taken from the scene's
+// summary (the count drops to 0 if the scene has a wall with id 0, impl:989). All loads are
+// issued before the first branch. It runs concurrently with the full pass (the two touch
+// disjoint spheres).
 
 // What the lite pass reads of one sphere, all issued together.
 struct LiteIn {
     OwnState o;
-    uint32_t flags;    // nbr_count (kFullPass: the full pass owns the sphere)
+    uint32_t flags;    // nbr_count (kTouch: the full pass owns the sphere this step)
     uint32_t count0;   // wall slot count
 };
 
@@ -472,9 +699,9 @@ __device__ __forceinline__ void lite_load(const StepArgs& A, const SceneDev& S, 
 
 template <bool FUSE>
 __device__ __forceinline__ void lite_sphere(const StepArgs& A, const SceneDev& S, uint32_t i, const LiteIn& in,
-                                            uint32_t& n_wall)
+                                            uint32_t skip, uint32_t& n_wall)
 {
-    if (in.flags & kFullPass) return;
+    if (in.flags & skip) return;
     uint32_t err = 0;
     if (S.has_pp) A.pp_count_out[i] = 0u;
     Accum tot;
@@ -496,37 +723,46 @@ __device__ __forceinline__ void lite_sphere(const StepArgs& A, const SceneDev& S
         if (count != in.count0) A.pl_count[i] = count;
         n_wall += count;
     }
-    sphere_tail<FUSE>(A, S, i, in.o, tot, err);
+    float dmx = 0.0f;   // the split lite pass computes no drift
+    sphere_tail<FUSE>(A, S, i, in.o, tot, err, dmx);
     if (err) atomicOr(A.err, err);
 }
 
-// The lite pass, one sphere per thread.
+// The lite kernels run over every sphere outside this step's full pass (skip kTouch), one
+// per thread.
+template <class F>
+__device__ __forceinline__ void lite_spheres(const StepArgs& A, F&& body)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A.np) body(i, kTouch);
+}
+
+// The lite pass of a split step (contact sums only).
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide_lite(StepArgs A)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n_wall = 0;
-    if (i < A.np) {
+    lite_spheres(A, [&](uint32_t i, uint32_t skip) {
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
         LiteIn in;
         lite_load<FUSE>(A, S, i, in);
-        lite_sphere<FUSE>(A, S, i, in, n_wall);
-    }
+        lite_sphere<FUSE>(A, S, i, in, skip, n_wall);
+    });
     add_contact_counts(A, 0u, n_wall);
 }
 
 // The lite pass of a fused step, split by motion: k_lite_move does the translation of every
-// lite sphere (acceleration, velocity, next position and bucket key) and its contact
-// bookkeeping, k_lite_turn the rotation (carried rolling torque, Euler-parameter rates, next
-// orientation). They read and write disjoint fields, so each needs few registers and runs at
-// a high occupancy. Together they evaluate the expressions of sphere_tail with zero contact
-// sums, in the same order.
+// lite sphere (acceleration, velocity, next position) and its contact bookkeeping,
+// k_lite_turn the rotation (carried rolling torque, Euler-parameter rates, next orientation).
+// They read and write disjoint fields, so each needs few registers and runs at a high
+// occupancy. Together they evaluate the expressions of sphere_tail with zero contact sums,
+// in the same order.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n_wall = 0;
-    if (i < A.np) {
+    float dmx = 0.0f;
+    lite_spheres(A, [&](uint32_t i, uint32_t skip) {
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
         const uint32_t flags = A.nbr_count[i];
         const uint32_t count0 = S.nwall ? A.pl_count[i] : 0u;
@@ -534,83 +770,83 @@ __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
         const V3<double> v0 = ld3(A.vel + 3 * i);
         const V3<double> ao = ld3(A.acc + 3 * i);
         const double m = A.mass[i];
-        if (!(flags & kFullPass)) {
-            if (S.has_pp) A.pp_count_out[i] = 0u;
-            if (S.nwall) {
-                if (S.wall_dynamic)
-                    for (uint32_t w = 0; w < S.nwall; w++) {
-                        const int32_t slot = A.walls[S.wall_begin + w].wslot;
-                        if (slot >= 0) {
-                            double* wr = A.wrench + (size_t)slot * 6 * A.np;
+        if (flags & skip) return;
+        if (S.has_pp) A.pp_count_out[i] = 0u;
+        if (S.nwall) {
+            if (S.wall_dynamic)
+                for (uint32_t w = 0; w < S.nwall; w++) {
+                    const int32_t slot = A.walls[S.wall_begin + w].wslot;
+                    if (slot >= 0) {
+                        double* wr = A.wrench + (size_t)slot * 6 * A.np;
 #pragma unroll
-                            for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
-                        }
+                        for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
                     }
-                const uint32_t count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
-                if (count != count0) A.pl_count[i] = count;
-                n_wall = count;
-            }
-            const double inv_m = 1.0 / m;
-            const V3<double> a = mk3(inv_m * (0.0 + m * S.gx), inv_m * (0.0 + m * S.gy), inv_m * (0.0 + m * S.gz));
-            const double hdt = 0.5 * S.dt;
-            V3<double> v = v0;
-            if (A.keep_velocity) {
-                if (A.vel_out != A.vel) st3(A.vel_out + 3 * i, v);
-            } else {
-                v = mk3(v.x + hdt * ao.x, v.y + hdt * ao.y, v.z + hdt * ao.z);
-                v = mk3(v.x + hdt * a.x, v.y + hdt * a.y, v.z + hdt * a.z);
-                st3(A.vel_out + 3 * i, v);
-            }
-            st3(A.acc + 3 * i, a);
-            if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) atomicOr(A.err, XDEM_ERR_NONFINITE);
-            if (A.drift) drift_position(A, S, i, p, v, a);
+                }
+            const uint32_t count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
+            if (count != count0) A.pl_count[i] = count;
+            n_wall += count;
         }
-    }
+        const double inv_m = 1.0 / m;
+        const V3<double> a = mk3(inv_m * (0.0 + m * S.gx), inv_m * (0.0 + m * S.gy), inv_m * (0.0 + m * S.gz));
+        const double hdt = 0.5 * S.dt;
+        V3<double> v = v0;
+        if (A.keep_velocity) {
+            if (A.vel_out != A.vel) st3(A.vel_out + 3 * i, v);
+        } else {
+            v = mk3(v.x + hdt * ao.x, v.y + hdt * ao.y, v.z + hdt * ao.z);
+            v = mk3(v.x + hdt * a.x, v.y + hdt * a.y, v.z + hdt * a.z);
+            st3(A.vel_out + 3 * i, v);
+        }
+        st3(A.acc + 3 * i, a);
+        if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) atomicOr(A.err, XDEM_ERR_NONFINITE);
+        if (A.drift) dmx = fmaxf(dmx, drift_position(A, S, i, p, v, a));
+    });
     add_contact_counts(A, 0u, n_wall);
+    add_disp_max(A, dmx);
 }
 
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.np) return;
-    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-    const uint32_t flags = A.nbr_count[i];
-    const V4<double> e = ld4(A.ep + 4 * i);
-    V4<double> av = ld4(A.ev + 4 * i);
-    const V4<double> aa = ld4(A.ea + 4 * i);
-    const V3<double> inertia = ld3(A.inertia + 3 * i);
-    const V3<double> t0 = A.tmax_persist ? ld3(A.tmax + 3 * i) : mk3(0.0, 0.0, 0.0);
-    if (flags & kFullPass) return;
-    // no contact: the rolling sums are zero, so rolling_moment leaves the moment and the
-    // carried torque as they are
-    V3<double> M = mk3(0.0, 0.0, 0.0);
-    const V3<double> T = t0 + mk3(0.0, 0.0, 0.0);
-    if (A.tmax_persist) st3(A.tmax + 3 * i, T);
-    const V3<double> np_ = rot_local(M, e);
-    const V4<double> mea = uceom(inertia, e, av, np_);
-    const double hdt = 0.5 * S.dt;
-    if (A.keep_velocity) {
-        av.x = av.x + hdt * (aa.x + mea.x); av.y = av.y + hdt * (aa.y + mea.y);
-        av.z = av.z + hdt * (aa.z + mea.z); av.w = av.w + hdt * (aa.w + mea.w);
-    } else {
-        av.x = av.x + hdt * aa.x; av.y = av.y + hdt * aa.y; av.z = av.z + hdt * aa.z; av.w = av.w + hdt * aa.w;
-        av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y;
-        av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
-    }
-    st4(A.ev_out + 4 * i, av);
-    st4(A.ea + 4 * i, mea);
-    if (A.drift) drift_orientation(A, S, i, e, av, mea);
+    lite_spheres(A, [&](uint32_t i, uint32_t skip) {
+        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+        const uint32_t flags = A.nbr_count[i];
+        const V4<double> e = ld4(A.ep + 4 * i);
+        V4<double> av = ld4(A.ev + 4 * i);
+        const V4<double> aa = ld4(A.ea + 4 * i);
+        const V3<double> inertia = ld3(A.inertia + 3 * i);
+        const V3<double> t0 = A.tmax_persist ? ld3(A.tmax + 3 * i) : mk3(0.0, 0.0, 0.0);
+        if (flags & skip) return;
+        // no contact: the rolling sums are zero, so rolling_moment leaves the moment and the
+        // carried torque as they are
+        V3<double> M = mk3(0.0, 0.0, 0.0);
+        const V3<double> T = t0 + mk3(0.0, 0.0, 0.0);
+        if (A.tmax_persist) st3(A.tmax + 3 * i, T);
+        const V3<double> np_ = rot_local(M, e);
+        const V4<double> mea = uceom(inertia, e, av, np_);
+        const double hdt = 0.5 * S.dt;
+        if (A.keep_velocity) {
+            av.x = av.x + hdt * (aa.x + mea.x); av.y = av.y + hdt * (aa.y + mea.y);
+            av.z = av.z + hdt * (aa.z + mea.z); av.w = av.w + hdt * (aa.w + mea.w);
+        } else {
+            av.x = av.x + hdt * aa.x; av.y = av.y + hdt * aa.y; av.z = av.z + hdt * aa.z; av.w = av.w + hdt * aa.w;
+            av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y;
+            av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
+        }
+        st4(A.ev_out + 4 * i, av);
+        st4(A.ea + 4 * i, mea);
+        if (A.drift) drift_orientation(A, S, i, e, av, mea);
+    });
 }
 
-// Full pass over the spheres k_pairs queued (grid-stride over the concatenated segments).
+// Full pass over the spheres k_touch queued (grid-stride over the concatenated segments).
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide(StepArgs A)
 {
     // seg_end[s] = spheres in segments 0..s (inclusive prefix, one wave64 scan)
     __shared__ uint32_t seg_end[kQueueSegs];
     if (threadIdx.x < kQueueSegs) {
-        uint32_t x = A.seg_count[threadIdx.x * kSegStride];
+        uint32_t x = A.seg_count2[threadIdx.x * kSegStride];
 #pragma unroll
         for (uint32_t off = 1; off < 64u; off <<= 1) {
             const uint32_t y = __shfl_up(x, off, 64);
@@ -621,19 +857,21 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
     __syncthreads();
     const uint32_t nwork = seg_end[kQueueSegs - 1];
     unsigned long long c0 = 0, c1 = 0;
+    float dmx = 0.0f;
     for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwork; w += gridDim.x * blockDim.x) {
         // segment of entry w: the first s with seg_end[s] > w
         uint32_t sg = 0;
 #pragma unroll
         for (uint32_t step = kQueueSegs / 2u; step > 0u; step >>= 1)
             if (seg_end[sg + step - 1u] <= w) sg += step;
-        const uint32_t i = A.work[(size_t)sg * A.seg_cap + (w - (sg ? seg_end[sg - 1u] : 0u))];
+        const uint32_t i = A.work2[(size_t)sg * A.seg_cap + (w - (sg ? seg_end[sg - 1u] : 0u))];
         uint32_t n_pp = 0, n_wall = 0;
-        collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, i, n_pp, n_wall);
+        collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, i, n_pp, n_wall, dmx);
         c0 += n_pp;
         c1 += n_wall;
     }
     add_contact_counts(A, c0, c1);
+    add_disp_max(A, dmx);
 }
 
 // Second half of a split step: rolling resistance and the velocity update from the
@@ -642,45 +880,48 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) void k_finish(StepArgs A)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.np) return;
-    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-    const V4<double> e = ld4(A.ep + 4 * i);
-    V4<double> av = ld4(A.ev + 4 * i);
-    V3<double> M = ld3(A.moment + 3 * i);
-    const double J = A.flat_inertia ? A.inertia[i + 2 * S.p_begin] : A.inertia[3 * i];
-    if (rolling_moment(S.dt, J, omega_of(e, av), ld3(A.tmax + 3 * i), A.rres[i], M)) {
-        st3(A.tmax + 3 * i, mk3(0.0, 0.0, 0.0));
-        A.rres[i] = 0.0;
+    float dmx = 0.0f;
+    if (i < A.np) {
+        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+        const V4<double> e = ld4(A.ep + 4 * i);
+        V4<double> av = ld4(A.ev + 4 * i);
+        V3<double> M = ld3(A.moment + 3 * i);
+        const double J = A.flat_inertia ? A.inertia[i + 2 * S.p_begin] : A.inertia[3 * i];
+        if (rolling_moment(S.dt, J, omega_of(e, av), ld3(A.tmax + 3 * i), A.rres[i], M)) {
+            st3(A.tmax + 3 * i, mk3(0.0, 0.0, 0.0));
+            A.rres[i] = 0.0;
+        }
+        const double m = A.mass[i];
+        const V3<double> F = ld3(A.force + 3 * i);
+        const V4<double> aa = ld4(A.ea + 4 * i);
+        const V3<double> np_ = rot_local(M, e);
+        const V4<double> mea = uceom(ld3(A.inertia + 3 * i), e, av, np_);
+        const double inv_m = 1.0 / m;
+        const V3<double> a = mk3(inv_m * (F.x + m * S.gx), inv_m * (F.y + m * S.gy), inv_m * (F.z + m * S.gz));
+        const double hdt = 0.5 * S.dt;
+        if (A.keep_velocity) {
+            av.x = av.x + hdt * (aa.x + mea.x); av.y = av.y + hdt * (aa.y + mea.y);
+            av.z = av.z + hdt * (aa.z + mea.z); av.w = av.w + hdt * (aa.w + mea.w);
+        } else {
+            V3<double> v = ld3(A.vel + 3 * i);
+            const V3<double> ao = ld3(A.acc + 3 * i);
+            v = mk3(v.x + hdt * ao.x, v.y + hdt * ao.y, v.z + hdt * ao.z);
+            av.x = av.x + hdt * aa.x; av.y = av.y + hdt * aa.y; av.z = av.z + hdt * aa.z; av.w = av.w + hdt * aa.w;
+            v = mk3(v.x + hdt * a.x, v.y + hdt * a.y, v.z + hdt * a.z);
+            av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y; av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
+            st3(A.vel + 3 * i, v);
+        }
+        st4(A.ev + 4 * i, av);
+        st3(A.acc + 3 * i, a);
+        st4(A.ea + 4 * i, mea);
+        if (A.zero_force) {
+            st3(A.force + 3 * i, mk3(0.0, 0.0, 0.0));
+            st3(A.moment + 3 * i, mk3(0.0, 0.0, 0.0));
+        }
+        if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) atomicOr(A.err, XDEM_ERR_NONFINITE);
+        if (A.drift) dmx = drift_sphere(A, S, i, ld4(A.pos + 4 * i), e, ld3(A.vel + 3 * i), a, av, mea);
     }
-    const double m = A.mass[i];
-    const V3<double> F = ld3(A.force + 3 * i);
-    const V4<double> aa = ld4(A.ea + 4 * i);
-    const V3<double> np_ = rot_local(M, e);
-    const V4<double> mea = uceom(ld3(A.inertia + 3 * i), e, av, np_);
-    const double inv_m = 1.0 / m;
-    const V3<double> a = mk3(inv_m * (F.x + m * S.gx), inv_m * (F.y + m * S.gy), inv_m * (F.z + m * S.gz));
-    const double hdt = 0.5 * S.dt;
-    if (A.keep_velocity) {
-        av.x = av.x + hdt * (aa.x + mea.x); av.y = av.y + hdt * (aa.y + mea.y);
-        av.z = av.z + hdt * (aa.z + mea.z); av.w = av.w + hdt * (aa.w + mea.w);
-    } else {
-        V3<double> v = ld3(A.vel + 3 * i);
-        const V3<double> ao = ld3(A.acc + 3 * i);
-        v = mk3(v.x + hdt * ao.x, v.y + hdt * ao.y, v.z + hdt * ao.z);
-        av.x = av.x + hdt * aa.x; av.y = av.y + hdt * aa.y; av.z = av.z + hdt * aa.z; av.w = av.w + hdt * aa.w;
-        v = mk3(v.x + hdt * a.x, v.y + hdt * a.y, v.z + hdt * a.z);
-        av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y; av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
-        st3(A.vel + 3 * i, v);
-    }
-    st4(A.ev + 4 * i, av);
-    st3(A.acc + 3 * i, a);
-    st4(A.ea + 4 * i, mea);
-    if (A.zero_force) {
-        st3(A.force + 3 * i, mk3(0.0, 0.0, 0.0));
-        st3(A.moment + 3 * i, mk3(0.0, 0.0, 0.0));
-    }
-    if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) atomicOr(A.err, XDEM_ERR_NONFINITE);
-    if (A.drift) drift_sphere(A, S, i, ld4(A.pos + 4 * i), e, ld3(A.vel + 3 * i), a, av, mea);
+    add_disp_max(A, dmx);
 }
 
 // Contact kernels of one precision (xdem_step_f64.hip / xdem_step_mixed.hip); pass 0 is the
@@ -688,6 +929,7 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
 using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, int pass,
                                      hipStream_t);
 void launch_lite_fused(const StepArgs&, bool multi, hipStream_t);   // k_lite_move + k_lite_turn
+constexpr unsigned kQueueChunks = 16;   // blocks per queue segment of k_touch
 CollideLaunch collide_launch_f64();
 hipError_t launch_pairs(const StepArgs&, bool multi, hipStream_t);
 CollideLaunch collide_launch_mixed();
@@ -695,6 +937,7 @@ CollideLaunch collide_launch_mixed();
 uint32_t bucket_table_size(uint32_t nbucket);   // bucket_count / bucket_start entries (+1 for start)
 hipError_t launch_drift(const StepArgs&, bool multi, hipStream_t);
 hipError_t launch_grid(const StepArgs&, bool multi, hipStream_t);
+hipError_t launch_touch(const StepArgs&, bool multi, hipStream_t);   // k_touch, every step
 hipError_t launch_finish(const StepArgs&, bool multi, hipStream_t);
 
 }  // namespace xdem

@@ -11,20 +11,22 @@ namespace {
 constexpr unsigned kBlock = 256;
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
-// pass 0: the lite pass over every sphere (two kernels in fused mode); pass 1: the full pass
-// over the spheres k_pairs queued (grid-stride over at most kFullBlocks blocks; the queue
-// length is read on the device). The two touch disjoint spheres and may run concurrently.
-constexpr unsigned kFullBlocks = 1024;
+// pass 0: the lite pass over every sphere outside this step's full pass (two kernels in
+// fused mode); pass 1: the full pass over the spheres k_touch queued (grid-stride over at
+// most kFullBlocks blocks, one per CU: the kernel runs one wave per SIMD; the queue length
+// is read on the device). The two touch disjoint spheres and may run concurrently.
+constexpr unsigned kFullBlocks = 256;
 
 template <bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 void collide_passes(const StepArgs& A, int pass, hipStream_t s)
 {
     const dim3 g = grid_for(A.np), b(kBlock);
-    if (pass == 0) {
-        if (FUSE) launch_lite_fused(A, MULTI, s);
-        else hipLaunchKernelGGL((k_collide_lite<float, MULTI, false, MODEL, GPU_LAW>), g, b, 0, s, A);
-    } else {
+    if (pass == 1) {
         hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, MODEL, GPU_LAW>), dim3(std::min(g.x, kFullBlocks)), b, 0, s, A);
+    } else if (FUSE) {
+        launch_lite_fused(A, MULTI, s);
+    } else {
+        hipLaunchKernelGGL((k_collide_lite<float, MULTI, false, MODEL, GPU_LAW>), g, b, 0, s, A);
     }
 }
 

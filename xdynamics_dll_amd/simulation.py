@@ -18,7 +18,7 @@ from .scene import PARITY_QUIRKS, SceneData
 class DemContext:
     def __init__(self, scenes, precision: str = "fp64", p2p_capacity: int = 12, quirks: dict | None = None,
                  grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False,
-                 buckets_per_sphere: int = 0):
+                 buckets_per_sphere: int = 0, skin: float = 0.0):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -33,6 +33,7 @@ class DemContext:
         cfg.fuse_velocity = 1 if fuse_velocity else 0
         cfg.device = device
         cfg.buckets_per_sphere = buckets_per_sphere
+        cfg.skin = skin   # 0: 2.5% of the smallest radius; < 0: grid and lists rebuilt every step
         self.precision = precision
         self.capacity = p2p_capacity
         self._h = C.c_void_p()
