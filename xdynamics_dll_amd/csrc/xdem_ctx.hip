@@ -86,7 +86,6 @@ struct xdem_ctx {
     uint32_t list_cap = 48;
     uint32_t *bucket_count = nullptr, *bucket_start = nullptr, *scan_part = nullptr;
     float4* sposf = nullptr;
-    float4* pos_ref = nullptr;
     // list rebuild flags, one per step parity: step s reads rebuild[rbp] (rbp flips at the
     // start of every step) and its drift sets rebuild[rbp ^ 1] for step s + 1
     uint32_t* rebuild = nullptr;
@@ -155,7 +154,7 @@ struct xdem_ctx {
         A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap; A.work = work;
         A.seg_count = seg_count; A.seg_cap = seg_cap; A.work2 = work2; A.seg_count2 = seg_count2;
         A.pass_mask = pass_mask;
-        A.sposf = sposf; A.pos_ref = pos_ref;
+        A.sposf = sposf;
         A.rebuild_cur = rebuild + rbp; A.rebuild_next = rebuild + (rbp ^ 1); A.n_rebuild = n_rebuild;
         A.gap = gap; A.disp = disp;
         A.dmax_cur = dmax + (size_t)rbp * kQueueSegs * kSegStride;
@@ -509,7 +508,6 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->bucket_start, sizeof(uint32_t) * (nbt + 1)));
     CTX_CHECK(c->alloc(&c->scan_part, sizeof(uint32_t) * (nbt / 4096 + 1)));
     CTX_CHECK(c->alloc(&c->sposf, sizeof(float4) * n));
-    CTX_CHECK(c->alloc(&c->pos_ref, sizeof(float4) * n));
     CTX_CHECK(c->alloc(&c->rebuild, sizeof(uint32_t) * 2));
     CTX_CHECK(c->alloc(&c->dmax, sizeof(uint32_t) * 2 * kQueueSegs * kSegStride));
     CTX_CHECK(c->alloc(&c->gap, sizeof(float) * n));
@@ -583,7 +581,6 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(hipMemsetAsync(c->dmax, 0, sizeof(uint32_t) * 2 * kQueueSegs * kSegStride, s));
     CTX_CHECK(hipMemsetAsync(c->disp, 0, sizeof(float) * n, s));
     CTX_CHECK(hipMemsetAsync(c->gap, 0, sizeof(float) * n, s));
-    CTX_CHECK(hipMemsetAsync(c->pos_ref, 0, sizeof(float4) * n, s));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, c->rebuild, 1u, (size_t)2);   // the first step builds
     CTX_CHECK(hipMemsetAsync(c->seg_count, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
     CTX_CHECK(hipMemsetAsync(c->seg_count2, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));

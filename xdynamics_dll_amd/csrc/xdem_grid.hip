@@ -20,9 +20,10 @@
 //                     smaller id, i.e. exactly the reference's (hash, index) order; writes
 //                     the sorted id and the FP32 position copy the pair search reads, with
 //                     the sphere's wall-reach flag (FP64 test, reach + skin) in the sign of
-//                     the radius; the same copy by sphere id is the reference position of
-//                     the half-skin test; the wall gap of the sphere (k_touch)
-// The pair search k_pairs and k_order_lists (xdem_step.hpp) are launched from here too.
+//                     the radius; the wall gap of the sphere (k_touch); restarts the
+//                     sphere's path length of the half-skin test
+// The pair search k_pairs and the per-step pair test k_touch (xdem_step.hpp) are launched
+// from here too.
 // The bucket of slot range [start[b], start[b+1]) needs no clearing between steps, and a
 // row of three consecutive buckets is the single range [start[b], start[b+3]).
 // Built with -ffp-contract=off: the drift is bit-identical to the reference's.
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     const Grid& gr = S.grid;
     const float rf = (float)p.w;
     const float4 q = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), rf);
-    A.pos_ref[k] = q;
+    A.disp[k] = 0.0f;   // the path length of the half-skin test starts here
     A.sposf[f] = make_float4(q.x, q.y, q.z, far ? rf : -rf);
 }
 
@@ -284,13 +285,8 @@ hipError_t launch_drift(const StepArgs& A, bool multi, hipStream_t s)
 hipError_t launch_pairs(const StepArgs& A, bool multi, hipStream_t s)
 {
     if (A.nsorted == 0) return hipSuccess;
-    if (multi) {
-        hipLaunchKernelGGL(k_pairs<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-        hipLaunchKernelGGL(k_order_lists<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
-    } else {
-        hipLaunchKernelGGL(k_pairs<false>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-        hipLaunchKernelGGL(k_order_lists<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
-    }
+    if (multi) hipLaunchKernelGGL(k_pairs<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
+    else hipLaunchKernelGGL(k_pairs<false>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
     return hipGetLastError();
 }
 

@@ -86,13 +86,12 @@ struct StepArgs {
     float4* sposf;  // per sorted slot: position relative to the scene's grid origin and
                     // radius, FP32 (the pair search reads only this); the radius is
                     // negated when a wall is within reach (+ skin) of the sphere
-    float4* pos_ref;                         // per sphere: sposf at the last list build
     uint32_t* rebuild_cur;                   // nonzero: this step rebuilds grid and lists
     uint32_t* rebuild_next;                  // set when a sphere moved half the skin (or null)
     float* gap;                              // per sphere: lower bound of the distance its listed
                                              // pairs and near walls lack to pass (last build)
     float* disp;                             // per sphere: upper bound of its displacement since
-                                             // the last build (the drift of the previous step)
+                                             // the last build (path length, summed by the drift)
     uint32_t* dmax_cur;                      // largest disp of this step (bits of a float, per
     uint32_t* dmax_next;                     // queue segment); the drift writes the next one's
     unsigned long long* n_rebuild;           // list builds so far
@@ -138,9 +137,9 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
 // keys, or two runs when the row wraps (the buckets up to the end of the row, then those
 // from its start). One flat loop walks all nine rows, so each lane advances through its own
 // candidates without waiting for the other lanes row by row. Only an FP32 test runs here
-// (few registers, many waves per SIMD to hide the latency of the candidate loads);
-// k_order_lists puts each list in the reference's visiting order and k_collide applies the
-// exact FP64 predicate to it every step.
+// (few registers, many waves per SIMD to hide the latency of the candidate loads); every
+// step k_touch applies the exact FP64 predicate to the lists and puts a list in the
+// reference's visiting order when its passing pairs are not in that order.
 //
 // FP32 test: a candidate is dropped only when its FP32 distance exceeds r_i + r_j + reach +
 // skin by more than the rounding of the FP32 coordinates (|x| 2^-24 per coordinate and
@@ -261,26 +260,23 @@ __device__ __forceinline__ void drift_orientation(const StepArgs& A, const Scene
 __device__ __forceinline__ float drift_position(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
                                                 V3<double> v, V3<double> ac)
 {
+    const V4<double> p0 = p;
     p.x += S.dt * v.x + S.half2dt * ac.x;
     p.y += S.dt * v.y + S.half2dt * ac.y;
     p.z += S.dt * v.z + S.half2dt * ac.z;
     st4(A.pos_next + 4 * i, p);
     // half-skin test: the lists of the last build hold every pair within reach as long as no
-    // sphere has moved half the skin since; the rounding of the FP32 reference position
-    // (bounded as in k_pairs) is taken off the threshold
-    // and the displacement bound k_touch compares with the gaps of the lists
+    // sphere has moved half the skin since. The bound is the path length since the build
+    // (disp, set to 0 by the build), each step's length rounded up; k_touch compares it with
+    // the gaps of the lists.
     float dsp = 0.0f;
     if (A.rebuild_next) {
-        const float4 r = A.pos_ref[i];
-        const double dx = (p.x - S.grid.wo_x) - (double)r.x;
-        const double dy = (p.y - S.grid.wo_y) - (double)r.y;
-        const double dz = (p.z - S.grid.wo_z) - (double)r.z;
-        const double eps = 2.4e-7 * (fabs((double)r.x) + fabs((double)r.y) + fabs((double)r.z)) + 1e-9;
-        const double lim = 0.5 * S.skin - eps;
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        if (!(lim > 0.0 && d2 < lim * lim)) *A.rebuild_next = 1u;
-        dsp = __double2float_ru(sqrt(d2) + eps);
-        if (!(dsp < 3.0e38f)) dsp = 3.0e38f;   // a non-finite position: the largest bound
+        const double dx = p.x - p0.x, dy = p.y - p0.y, dz = p.z - p0.z;
+        dsp = __double2float_ru((double)A.disp[i] + sqrt(dx * dx + dy * dy + dz * dz) * (1.0 + 1e-6) + 1e-12);
+        if (!(dsp < 0.5f * (float)S.skin)) {
+            *A.rebuild_next = 1u;
+            if (!(dsp < 3.0e38f)) dsp = 3.0e38f;   // a non-finite position: the largest bound
+        }
         A.disp[i] = dsp;
     }
     return dsp;
@@ -372,9 +368,9 @@ __device__ __forceinline__ uint32_t visit_code(const SceneDev& S, int ix, int iy
 
 // Puts a neighbour list in the reference's visiting order at the current positions: codes
 // are packed above the ids, the list is insertion-sorted in place (it is short and this
-// thread's own), then the codes are removed. k_order_lists does it for every list after a
-// build; between builds k_collide checks the order of the pairs that pass and sorts the list
-// again when a neighbour has changed cells.
+// thread's own), then the codes are removed. k_touch does it when the pairs of a list that
+// pass are not in that order (after a build, the lists are in bucket order; later, a sphere
+// may have changed cells).
 __device__ __forceinline__ void order_list(const StepArgs& A, const SceneDev& S, uint32_t* lst, uint32_t n, int ix,
                                            int iy, int iz)
 {
@@ -392,20 +388,6 @@ __device__ __forceinline__ void order_list(const StepArgs& A, const SceneDev& S,
         lst[u] = x;
     }
     for (uint32_t t = 0; t < n; t++) lst[t] &= kIdMask;
-}
-
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_order_lists(StepArgs A)
-{
-    if (!*A.rebuild_cur) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.np) return;
-    const uint32_t n = min(A.nbr_count[i] & kCountMask, A.list_cap);
-    if (n < 2u) return;
-    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-    int ix, iy, iz;
-    ref_cell(S, ld4(A.pos + 4 * i), ix, iy, iz);
-    order_list(A, S, A.nbr + (size_t)i * A.list_cap, n, ix, iy, iz);
 }
 
 // The reference's pair predicate: overlap, or inside the JKR separation depth of a cohesive
@@ -584,8 +566,8 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
         if (nlist_all > lc) err |= XDEM_ERR_NBR_OVERFLOW;
         const uint32_t nlist = min(nlist_all, lc);
         const uint32_t* lst = A.nbr + (size_t)i * lc;
-        // the listed pairs that pass (k_touch's mask), in list order, which is the reference's
-        // visiting order (k_order_lists, and k_touch when a sphere has changed cells)
+        // the listed pairs that pass (k_touch's mask), in list order, which k_touch has made
+        // the reference's visiting order
         unsigned long long m = nlist ? A.pass_mask[i] : 0ull;
         while (m) {
             const uint32_t t = (uint32_t)__ffsll((long long)m) - 1u;
