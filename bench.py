@@ -47,7 +47,7 @@ def contact_kernel_bytes(cbar: float, fused: bool) -> float:
     return (404.0 if fused else 224.0) + 40.0 * cbar
 
 
-def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0, skin=0.0):
+def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0, skin=0.0, graph=0):
     import torch
     from xdynamics_dll_amd import DemContext
 
@@ -56,9 +56,16 @@ def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_s
     ctx.step(warmup)
     ctx.synchronize()
     torch.cuda.synchronize()
+    if graph:
+        ctx.graph_capture(graph)   # graph steps recorded once; replays do the timed steps
+        ctx.synchronize()
     info0 = ctx.info()
     t0 = time.perf_counter()
-    ctx.step(steps)
+    if graph:
+        ctx.graph_launch(steps // graph)
+        ctx.step(steps % graph)
+    else:
+        ctx.step(steps)
     ctx.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -193,10 +200,10 @@ def batch_main(args):
     t_build = time.perf_counter() - t_build
     n = sum(s.np for s in scenes)
     fuse = not args.split
-    mixed = run_gpu(scenes, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin)
+    mixed = run_gpu(scenes, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin, args.graph)
     value = n * args.steps / mixed["seconds"]
     f64 = run_gpu(scenes, "fp64", max(20, args.steps // 2), max(5, args.warmup // 2), args.capacity, fuse,
-                  args.buckets_per_sphere, args.skin)
+                  args.buckets_per_sphere, args.skin, args.graph)
     f64_value = n * max(20, args.steps // 2) / f64["seconds"]
     tm = mixed["timing"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
@@ -259,6 +266,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--split", action="store_true",
                     help="rolling + velocity update in a kernel of their own (default: inside the contact pass)")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="replay the timed steps from a HIP graph of this many steps (even; 0: eager)")
     ap.add_argument("--skin", type=float, default=0.0,
                     help="neighbour-list skin in m (0: 2.5%% of the smallest radius, <0: rebuild every step)")
     ap.add_argument("--buckets-per-sphere", type=int, default=0,
@@ -290,7 +299,7 @@ def main():
     n = scene.np
 
     fuse = not args.split
-    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin)
+    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin, args.graph)
     value = n * args.steps / mixed["seconds"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
@@ -336,7 +345,7 @@ def main():
     }
     if not args.no_fp64:
         f64 = run_gpu(scene, "fp64", max(20, args.steps // 2), max(5, args.warmup // 2), args.capacity,
-                      fuse, args.buckets_per_sphere, args.skin)
+                      fuse, args.buckets_per_sphere, args.skin, args.graph)
         f64_value = n * max(20, args.steps // 2) / f64["seconds"]
         out["fp64_value"] = f64_value
         out["mixed_over_fp64"] = value / f64_value
