@@ -583,7 +583,6 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(hipMemsetAsync(c->gap, 0, sizeof(float) * n, s));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, c->rebuild, 1u, (size_t)2);   // the first step builds
     CTX_CHECK(hipMemsetAsync(c->seg_count, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
-    CTX_CHECK(hipMemsetAsync(c->scan_part, 0, sizeof(uint32_t) * (nbt / 4096 + 1), s));
     CTX_CHECK(hipMemsetAsync(c->seg_count2, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
     CTX_CHECK(hipMemsetAsync(c->nbr_count, 0, sizeof(uint32_t) * n, s));
     if (c->nwslot) CTX_CHECK(hipMemsetAsync(c->wrench_out, 0, sizeof(double) * 6 * c->nwslot, s));

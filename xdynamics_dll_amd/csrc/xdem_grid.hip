@@ -9,14 +9,13 @@
 // steps. The drift of a step (position and orientation update, impl:135-147) is computed at
 // the end of the previous step's velocity update; k_drift does it when no previous step
 // left it. A rebuild is:
-//   k_count           bucket key of every sphere, its count and the count of its 4096-bucket
-//                     tile (posted atomic adds; one per wave for the tile when the wave's
-//                     spheres share it)
-//   (k_scan_parts)    \ exclusive prefix of the counts: start[b] = first sorted slot of
+//   k_count           bucket key of every sphere and its count (posted atomic add)
+//   k_scan_reduce     \
+//   (k_scan_parts)     > exclusive prefix of the counts: start[b] = first sorted slot of
 //   k_scan_apply      /  bucket b, start[nbucket] = N (tables of at most kDirectTiles tiles:
-//                        each k_scan_apply block sums the earlier tile counts itself)
+//                        each k_scan_apply block sums the earlier tile sums itself)
 //   k_scatter         provisional slot start[key] + (atomic countdown of the bucket), which
-//                     leaves every count at zero for the next build; clears the tile counts
+//                     leaves every count at zero for the next step
 //   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
 //                     smaller id, i.e. exactly the reference's (hash, index) order; writes
 //                     the sorted id and the FP32 position copy the pair search reads, with
@@ -81,33 +80,44 @@ __global__ __launch_bounds__(kBlock) void k_count(StepArgs A)
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *A.rebuild_next = 0u;
     if (!*A.rebuild_cur) return;
-    if (blockIdx.x == 0) {
-        if (threadIdx.x == 0) *A.n_rebuild += 1ull;
-        if (threadIdx.x < kQueueSegs) A.seg_count[threadIdx.x * kSegStride] = 0u;   // candidate queue
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *A.n_rebuild += 1ull;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = i < A.np;
-    uint32_t tile = 0xffffffffu;
-    if (active) {
-        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-        const V4<double> p = ld4(A.pos + 4 * i);
-        int gx, gy, gz;
-        grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
-        const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
-        A.keys[i] = key;
-        atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: k_scatter takes the rank
-        tile = key / kScanTile;
+    if (i >= A.np) return;
+    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    const V4<double> p = ld4(A.pos + 4 * i);
+    int gx, gy, gz;
+    grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
+    const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
+    A.keys[i] = key;
+    atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: k_scatter takes the rank
+}
+
+// Sum of 16 consecutive buckets per thread, reduced per block of 4096 buckets. Block 0 also
+// clears the full-pass queue segment counters.
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ gate,
+                                                        const uint32_t* __restrict__ count,
+                                                        uint32_t* __restrict__ part,
+                                                        uint32_t* __restrict__ seg_count)
+{
+    if (!*gate) return;
+    if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) seg_count[threadIdx.x * kSegStride] = 0u;
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint4* c = reinterpret_cast<const uint4*>(count + (size_t)blockIdx.x * kScanTile) + threadIdx.x * 4;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint4 v = c[k];
+        s += v.x + v.y + v.z + v.w;
     }
-    // tile counts: the spheres of a wave that share the first active lane's tile add
-    // together, the others one by one
-    const unsigned long long act = __ballot(active);
-    if (act) {
-        const uint32_t lane = threadIdx.x & 63u;
-        const uint32_t first = (uint32_t)__ffsll((long long)act) - 1u;
-        const uint32_t t0 = __shfl(tile, (int)first, 64);
-        const unsigned long long same = __ballot(active && tile == t0);
-        if (lane == first) atomicAdd(&A.scan_part[t0], (uint32_t)__popcll(same));
-        if (active && tile != t0) atomicAdd(&A.scan_part[tile], 1u);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (unsigned w = 0; w < kBlock / 64; w++) t += wsum[w];
+        part[blockIdx.x] = t;
     }
 }
 
@@ -208,11 +218,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
         o[k] = make_uint4(off + v[4 * k], off + v[4 * k + 1], off + v[4 * k + 2], off + v[4 * k + 3]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(StepArgs A, uint32_t ntile)
+__global__ __launch_bounds__(kBlock) void k_scatter(StepArgs A)
 {
     if (!*A.rebuild_cur) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ntile) A.scan_part[i] = 0u;   // tile counts, for the next build
     if (i >= A.np) return;
     const uint32_t key = A.keys[i];
     // counting the bucket back down gives each sphere a distinct slot and leaves the count
@@ -297,6 +306,8 @@ hipError_t launch_grid(const StepArgs& A, bool multi, hipStream_t s)
     // the first kernel also runs when np is 0: it clears the step's counters and flag
     if (multi) hipLaunchKernelGGL(k_count<true>, grid_for(std::max(A.np, 1u)), dim3(kBlock), 0, s, A);
     else hipLaunchKernelGGL(k_count<false>, grid_for(std::max(A.np, 1u)), dim3(kBlock), 0, s, A);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(ntile), dim3(kBlock), 0, s, gate, A.bucket_count, A.scan_part,
+                       A.seg_count);
     if (ntile <= kDirectTiles) {
         hipLaunchKernelGGL(k_scan_apply<true>, dim3(ntile), dim3(kBlock), 0, s, gate, A.bucket_count, A.scan_part,
                            A.bucket_start, A.bucket_start + nb);
@@ -307,7 +318,7 @@ hipError_t launch_grid(const StepArgs& A, bool multi, hipStream_t s)
                            A.bucket_start, A.bucket_start + nb);
     }
     if (A.np) {
-        hipLaunchKernelGGL(k_scatter, grid_for(std::max(A.np, ntile)), dim3(kBlock), 0, s, A, ntile);
+        hipLaunchKernelGGL(k_scatter, grid_for(A.np), dim3(kBlock), 0, s, A);
         if (multi) hipLaunchKernelGGL(k_order_gather<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
         else hipLaunchKernelGGL(k_order_gather<false>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
     }
