@@ -206,8 +206,11 @@ __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_
     bool stored = false, overflow = false;
     if (cdist > 0) {
         ws.load();
-        for (uint32_t s = 0; s < XDEM_MAX_P2PL_COUNT; s++)
-            if (ws.id[s] == k) { sd0 = ws.s0[s]; sd1 = ws.s1[s]; break; }
+        // the first slot with this wall's id (constant indices: the slots stay in registers)
+        bool found = false;
+#pragma unroll
+        for (int s = 0; s < XDEM_MAX_P2PL_COUNT; s++)
+            if (!found && ws.id[s] == k) { sd0 = ws.s0[s]; sd1 = ws.s1[s]; found = true; }
         const V3<double> dcpr = cpt - ip3;
         V4<double> be = {dbi[XDEM_E0], dbi[XDEM_E1], dbi[XDEM_E2], dbi[XDEM_E3]};
         V4<double> bd = {dbi[XDEM_ED0], dbi[XDEM_ED1], dbi[XDEM_ED2], dbi[XDEM_ED3]};
@@ -239,7 +242,9 @@ __device__ __forceinline__ bool wall_contact(const ContactConst& cc, const xdem_
     if (stored) {
         if (n_new < XDEM_MAX_P2PL_COUNT) {
             ws.load();
-            ws.id[n_new] = k; ws.s0[n_new] = sd0; ws.s1[n_new] = sd1;
+#pragma unroll
+            for (int s = 0; s < XDEM_MAX_P2PL_COUNT; s++)
+                if ((uint32_t)s == n_new) { ws.id[s] = k; ws.s0[s] = sd0; ws.s1[s] = sd1; }
             ws.dirty = 1;
             n_new++;
         } else {
