@@ -36,15 +36,15 @@ def step_bytes(cbar: float) -> float:
 
 def contact_kernel_bytes(cbar: float, fused: bool) -> float:
     """Algorithmic bytes per sphere of the contact pass.
-    Fused (k_lite_move + k_lite_turn for spheres without neighbours or walls in reach,
-    k_collide for the others: contacts, rolling, velocity update and the next step's drift):
-    own state read pos 32, vel 24, acc 24, ep 32, ev 32, ea 32, mass 8, inertia 24 (= 208);
-    state write pos 32, vel 24, acc 24, ep 32, ev 32, ea 32 (= 176); bucket key 4; sphere
-    and wall history: counts 2*8, entries 2*20*cbar.
+    Fused (k_lite_move + k_lite_turn for the spheres outside this step's full pass, k_collide
+    for the others: contacts, rolling, velocity update and the next step's drift): own state
+    read pos 32, vel 24, acc 24, ep 32, ev 32, ea 32, mass 8, inertia 24 (= 208); state write
+    pos 32, vel 24, acc 24, ep 32, ev 32, ea 32 (= 176); list flags 2*4, path length 2*4;
+    sphere and wall history: counts 2*4, entries 2*20*cbar.
     Split (k_collide_lite + k_collide, contacts only): read pos 32, vel 24, ep 32, ev 32,
     mass 8 (= 128); write force, moment, rolling torque sum and magnitude 24+24+24+8 (= 80);
     history 16 + 40*cbar."""
-    return (404.0 if fused else 224.0) + 40.0 * cbar
+    return (408.0 if fused else 224.0) + 40.0 * cbar
 
 
 def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0, skin=0.0, graph=0):
@@ -269,7 +269,7 @@ def main():
     ap.add_argument("--graph", type=int, default=0,
                     help="replay the timed steps from a HIP graph of this many steps (even; 0: eager)")
     ap.add_argument("--skin", type=float, default=0.0,
-                    help="neighbour-list skin in m (0: 2.5%% of the smallest radius, <0: rebuild every step)")
+                    help="neighbour-list skin in m (0: adaptive, <0: rebuild every step)")
     ap.add_argument("--buckets-per-sphere", type=int, default=0,
                     help="bucket budget of the grid per sphere (0 = library default)")
     ap.add_argument("--workload", choices=["dam_break", "batch"], default="dam_break",

@@ -110,9 +110,12 @@ typedef struct xdem_ctx_config {
                                         are halved until the grid fits */
     double skin;                 /* neighbour-list skin [m]: the grid and the lists are rebuilt
                                     only on steps after a sphere has moved half of it since
-                                    the last build (or after a state write or a wall move);
-                                    0 = 2.5% of the scene's smallest radius, < 0 = no skin
-                                    (rebuilt every step). Results do not depend on it. */
+                                    the last build (or after a state write or a wall move).
+                                    0 = adaptive: between 1.25% and 20% of the scene's
+                                    smallest radius, starting at 2.5%, doubled when the lists
+                                    lasted fewer than 16 steps and halved when they lasted
+                                    more than 64; < 0 = no skin (rebuilt every step).
+                                    Results do not depend on it. */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */

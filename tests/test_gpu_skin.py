@@ -40,16 +40,18 @@ def run(scene, skin, precision, fuse, steps):
     return out
 
 
-@pytest.mark.parametrize("precision,fuse", [("fp64", True), ("fp64", False), ("mixed", True)])
-def test_skin_lists_are_bit_identical_to_rebuilding_every_step(precision, fuse):
+@pytest.mark.parametrize("precision,fuse,skin", [("fp64", True, 1.0e-3), ("fp64", False, 1.0e-3),
+                                                 ("mixed", True, 1.0e-3), ("fp64", True, 0.0)])
+def test_skin_lists_are_bit_identical_to_rebuilding_every_step(precision, fuse, skin):
+    """A fixed 1 mm skin, and the default adaptive skin (its scale changes between builds)."""
     scene = shaken_pack()
     steps = 120
-    a = run(scene, 1.0e-3, precision, fuse, steps)
+    a = run(scene, skin, precision, fuse, steps)
     b = run(scene, -1.0, precision, fuse, steps)
     print("list builds with skin", a["builds"], "without", b["builds"], flush=True)
     assert a["err"] == 0 and b["err"] == 0
     assert b["builds"] == steps
-    assert 1 < a["builds"] < steps // 4
+    assert 1 < a["builds"] < (steps // 4 if skin > 0 else steps)
     for f in FIELDS:
         assert np.array_equal(a[f], b[f]), f
     for h in ("p2p", "plane"):

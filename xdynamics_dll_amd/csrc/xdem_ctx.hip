@@ -93,6 +93,7 @@ struct xdem_ctx {
     uint32_t* dmax = nullptr;          // two sets of kQueueSegs counters, by rbp like rebuild
     float *gap = nullptr, *disp = nullptr;
     unsigned long long* n_rebuild = nullptr;
+    ListCtl* lctl = nullptr;
     // vel and ev: two buffers each in fused mode (the contact kernel writes the new values
     // into the other one while neighbours still read the old ones); vel/ev point at the
     // current one
@@ -156,6 +157,7 @@ struct xdem_ctx {
         A.pass_mask = pass_mask;
         A.sposf = sposf;
         A.rebuild_cur = rebuild + rbp; A.rebuild_next = rebuild + (rbp ^ 1); A.n_rebuild = n_rebuild;
+        A.lctl = lctl;
         A.gap = gap; A.disp = disp;
         A.dmax_cur = dmax + (size_t)rbp * kQueueSegs * kSegStride;
         A.dmax_next = dmax + (size_t)(rbp ^ 1) * kQueueSegs * kSegStride;
@@ -372,7 +374,8 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
     // skin in neighbouring buckets (the reference's cell size stays the one of the visiting
     // order)
     const ContactConst cc_pp = make_contact_const(s->cp_pp, 0.5 * rmax);
-    const double skin = c->cfg.skin > 0.0 ? c->cfg.skin : (c->cfg.skin == 0.0 ? 0.025 * rmin : 0.0);
+    // default: adaptive, up to 20% of the smallest radius, starting at 2.5%
+    const double skin = c->cfg.skin > 0.0 ? c->cfg.skin : (c->cfg.skin == 0.0 ? 0.2 * rmin : 0.0);
     const double bucket_cs = std::max(s->cell_size, 2.0 * rmax + cc_pp.coh_reach + skin) * (1.0 + 1e-9);
     // bucket grid: the reference's origin and cell size, power-of-two dims covering the
     // scene (wrapping beyond them, as the reference's 128^3 grid does)
@@ -509,6 +512,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->scan_part, sizeof(uint32_t) * (nbt / 4096 + 1)));
     CTX_CHECK(c->alloc(&c->sposf, sizeof(float4) * n));
     CTX_CHECK(c->alloc(&c->rebuild, sizeof(uint32_t) * 2));
+    CTX_CHECK(c->alloc(&c->lctl, sizeof(ListCtl)));
     CTX_CHECK(c->alloc(&c->dmax, sizeof(uint32_t) * 2 * kQueueSegs * kSegStride));
     CTX_CHECK(c->alloc(&c->gap, sizeof(float) * n));
     CTX_CHECK(c->alloc(&c->disp, sizeof(float) * n));
@@ -578,6 +582,15 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(hipMemsetAsync(c->err, 0, sizeof(uint32_t), s));
     CTX_CHECK(hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * 4, s));
     CTX_CHECK(hipMemsetAsync(c->n_rebuild, 0, sizeof(unsigned long long), s));
+    {
+        ListCtl L;
+        std::memset(&L, 0, sizeof(L));
+        L.adaptive = c->cfg.skin == 0.0 ? 1u : 0u;
+        L.scale = L.adaptive ? 0.125f : 1.0f;
+        L.min_scale = L.adaptive ? 0.0625f : 1.0f;
+        CTX_CHECK(hipStreamSynchronize(s));
+        CTX_CHECK(hipMemcpy(c->lctl, &L, sizeof(L), hipMemcpyHostToDevice));
+    }
     CTX_CHECK(hipMemsetAsync(c->dmax, 0, sizeof(uint32_t) * 2 * kQueueSegs * kSegStride, s));
     CTX_CHECK(hipMemsetAsync(c->disp, 0, sizeof(float) * n, s));
     CTX_CHECK(hipMemsetAsync(c->gap, 0, sizeof(float) * n, s));

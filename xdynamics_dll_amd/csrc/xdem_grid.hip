@@ -78,7 +78,20 @@ __global__ __launch_bounds__(kBlock) void k_count(StepArgs A)
         A.seg_count2[threadIdx.x * kSegStride] = 0u;
         A.dmax_next[threadIdx.x * kSegStride] = 0u;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *A.rebuild_next = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *A.rebuild_next = 0u;
+        ListCtl& L = *A.lctl;
+        L.step += 1u;
+        if (*A.rebuild_cur) {
+            // skin of the lists this build makes, from how long the last ones lasted
+            const uint32_t lasted = L.step - L.last_build;
+            if (L.adaptive && L.last_build != 0u) {
+                if (lasted < 16u) L.scale = fminf(1.0f, 2.0f * L.scale);
+                else if (lasted > 64u) L.scale = fmaxf(L.min_scale, 0.5f * L.scale);
+            }
+            L.last_build = L.step;
+        }
+    }
     if (!*A.rebuild_cur) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) *A.n_rebuild += 1ull;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -252,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     double wgap = 3.0e38;
     for (uint32_t w = 0; w < S.nwall; w++) {
         const WallDev& W = A.walls[S.wall_begin + w];
-        far = far & wall_out_of_reach(W.cc, W.info, p, S.skin);
+        far = far & wall_out_of_reach(W.cc, W.info, p, S.skin * A.lctl->scale);
         const xdem_plane_info& pl = W.info;
         const double uw2 = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
         const double wz = (p.x - pl.xw.x) * pl.uw.x + (p.y - pl.xw.y) * pl.uw.y + (p.z - pl.xw.z) * pl.uw.z;

@@ -35,6 +35,16 @@ inline uint32_t queue_segment_cap(uint32_t nsorted)
 }
 constexpr int kMaxWallsPerScene = 16;
 
+// Neighbour-list skin control (device memory, one per context). Every step counts `step`;
+// on a list build with `adaptive` set, the skin scale doubles when the lists lasted fewer
+// than 16 steps and halves when they lasted more than 64 (within [min_scale, 1]). The scale
+// changes only at the start of a build, before anything uses it, so each list generation
+// has one skin throughout.
+struct ListCtl {
+    float scale, min_scale;
+    uint32_t step, last_build, adaptive, pad_;
+};
+
 struct SceneDev {
     double dt, half2dt;
     double gx, gy, gz;
@@ -46,7 +56,8 @@ struct SceneDev {
     uint32_t wall_id0;        // a wall of the scene has plane id 0 (it resets the wall slot count)
     uint32_t wall_dynamic;    // a wall of the scene is a dynamic body (per-sphere wrench output)
     uint32_t pad_;
-    double skin;              // neighbour-list skin (0: the lists are rebuilt every step)
+    double skin;              // largest neighbour-list skin (0: the lists are rebuilt every step);
+                              // the lists use skin * ListCtl::scale
     double ref_cs;            // the reference's cell size (visiting order of the contacts);
                               // grid.cs, the bucket size, is larger by the skin and reach
 };
@@ -88,6 +99,7 @@ struct StepArgs {
                     // negated when a wall is within reach (+ skin) of the sphere
     uint32_t* rebuild_cur;                   // nonzero: this step rebuilds grid and lists
     uint32_t* rebuild_next;                  // set when a sphere moved half the skin (or null)
+    ListCtl* lctl;                           // skin scale of the current lists
     float* gap;                              // per sphere: lower bound of the distance its listed
                                              // pairs and near walls lack to pass (last build)
     float* disp;                             // per sphere: upper bound of its displacement since
@@ -168,7 +180,7 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
             const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
             // lim = r_j (1 + 1e-6) + [(r_i + reach)(1 + 1e-6) + mg]: the same bound with the
             // sphere's own part hoisted (each FP32 rounding is far below the 1e-6 margin)
-            const float limi = ((fabsf(fi.w) + (float)(S.cc.coh_reach + S.skin)) * 1.000001f) + mg;
+            const float limi = ((fabsf(fi.w) + (float)(S.cc.coh_reach + S.skin * A.lctl->scale)) * 1.000001f) + mg;
             const float limr = ((fabsf(fi.w) + (float)S.cc.coh_reach) * 1.000001f) + 2.0f * mg + 1e-9f;
             const uint32_t lc = A.list_cap;
             uint32_t* out = A.nbr + (size_t)i * lc;
@@ -273,7 +285,7 @@ __device__ __forceinline__ float drift_position(const StepArgs& A, const SceneDe
     if (A.rebuild_next) {
         const double dx = p.x - p0.x, dy = p.y - p0.y, dz = p.z - p0.z;
         dsp = __double2float_ru((double)A.disp[i] + sqrt(dx * dx + dy * dy + dz * dz) * (1.0 + 1e-6) + 1e-12);
-        if (!(dsp < 0.5f * (float)S.skin)) {
+        if (!(dsp < 0.5f * (float)(S.skin * A.lctl->scale))) {
             *A.rebuild_next = 1u;
             if (!(dsp < 3.0e38f)) dsp = 3.0e38f;   // a non-finite position: the largest bound
         }

@@ -33,7 +33,7 @@ class DemContext:
         cfg.fuse_velocity = 1 if fuse_velocity else 0
         cfg.device = device
         cfg.buckets_per_sphere = buckets_per_sphere
-        cfg.skin = skin   # 0: 2.5% of the smallest radius; < 0: grid and lists rebuilt every step
+        cfg.skin = skin   # 0: adaptive (1.25-20% of the smallest radius); < 0: lists rebuilt every step
         self.precision = precision
         self.capacity = p2p_capacity
         self._h = C.c_void_p()
