@@ -14,7 +14,7 @@ import pytest
 
 import xdem_ref
 from scenes import random_pack
-from xdynamics_dll_amd import DemContext, SceneData
+from xdynamics_dll_amd import DemContext, SceneData, box_walls, contact_property
 
 pytestmark = pytest.mark.gpu
 
@@ -74,4 +74,39 @@ def test_skin_history_slot_order_matches_oracle():
             g, r = gpu.read(f), getattr(ref, f)
             assert float(np.abs(g - r).max() / np.abs(r).max()) <= 1e-10, (n, f)
     assert gpu.info()["list_builds"] < 40
+    assert gpu.error_word() == 0
+
+
+def bouncing_spheres(n_side=12, seed=8):
+    """Spheres falling onto the floor with sideways velocity: no spin in flight, spin from
+    the friction of the impact, free spinning flight after the bounce."""
+    rng = np.random.default_rng(seed)
+    g = (np.arange(n_side) + 0.5) * 3.0e-3
+    x, y = np.meshgrid(g, g, indexing="ij")
+    r = 1.0e-3
+    z = r + rng.uniform(5e-6, 4e-4, size=x.size)
+    pos = np.stack([x.ravel(), y.ravel(), z, np.full(x.size, r)], axis=1)
+    vel = np.stack([rng.normal(0, 0.3, x.size), rng.normal(0, 0.3, x.size), -rng.uniform(0.3, 1.0, x.size)], axis=1)
+    cp = contact_property(1e7, 1e7, 0.3, 0.3, 0.6, 0.5, 0.02, 0.0)
+    w = box_walls(n_side * 3.0e-3, n_side * 3.0e-3, 0.03)
+    return SceneData.from_positions(pos, 2500.0, cp, walls=w, dt=1e-5, vel=vel)
+
+
+def test_spin_stable_spheres_through_impact_match_oracle():
+    """Rotation writes are skipped for spheres whose rotation state does not change; the
+    spheres here go from no spin (skipped) through a frictional impact (full pass) to
+    spinning flight (written every step) and must follow the oracle throughout."""
+    scene = bouncing_spheres()
+    gpu = DemContext(scene, precision="fp64", fuse_velocity=True)
+    ref = xdem_ref.RefSim(scene)
+    spun = False
+    for n in (20, 60, 220):
+        gpu.step(n)
+        ref.step(n)
+        for f in ("pos", "vel", "ep", "ev", "ea"):
+            g, r = gpu.read(f), getattr(ref, f)
+            s = np.abs(r).max()
+            assert s == 0 or float(np.abs(g - r).max() / s) <= 1e-10, (n, f)
+        spun = spun or bool(np.abs(ref.ev).max() > 0)
+    assert spun
     assert gpu.error_word() == 0

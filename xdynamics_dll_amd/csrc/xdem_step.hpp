@@ -20,7 +20,8 @@ constexpr uint32_t kInvalid = 0xffffffffu;
 constexpr uint32_t kCandidate = 0x80000000u;  // listed neighbours or a wall within reach + skin (build)
 constexpr uint32_t kWallNear = 0x40000000u;   // a wall within reach + skin (build)
 constexpr uint32_t kTouch = 0x20000000u;      // this step: a pair passes or a wall is in reach (k_touch)
-constexpr uint32_t kCountMask = 0x1fffffffu;
+constexpr uint32_t kSpinStable = 0x10000000u; // both ev and ep buffers hold the sphere's current values
+constexpr uint32_t kCountMask = 0x0fffffffu;
 // The full-pass queue is split in kQueueSegs segments, each with its own counter on its own
 // 128-byte line: the wave of sorted slots [64 w, 64 w + 64) appends to segment w mod
 // kQueueSegs, so the appends of different waves rarely hit the same counter.
@@ -92,7 +93,8 @@ struct StepArgs {
     uint32_t* work;                          // candidate spheres (k_pairs), by queue segment
     uint32_t* work2;                         // spheres of this step's full contact pass (k_touch)
     unsigned long long* pass_mask;           // per sphere of work2: list entries that pass (k_touch)
-    uint32_t* nbr_count;                     // listed neighbours per sphere | kCandidate, kWallNear, kTouch
+    uint32_t* nbr_count;                     // listed neighbours per sphere | kCandidate, kWallNear, kTouch,
+                                             // kSpinStable
     uint32_t* nbr;                           // their sphere ids, list_cap per sphere
     float4* sposf;  // per sorted slot: position relative to the scene's grid origin and
                     // radius, FP32 (the pair search reads only this); the radius is
@@ -256,8 +258,7 @@ __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 // position/orientation buffer, which the next step makes current; and the half-skin test
 // that decides whether the next step rebuilds the grid and the neighbour lists. Same
 // expressions, in the same order, as a separate position-update pass would evaluate.
-__device__ __forceinline__ void drift_orientation(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> e,
-                                                  V4<double> d, V4<double> a)
+__device__ __forceinline__ V4<double> next_orientation(const SceneDev& S, V4<double> e, V4<double> d, V4<double> a)
 {
     V4<double> ne;
     ne.x = e.x + S.dt * d.x + S.half2dt * a.x;
@@ -266,7 +267,19 @@ __device__ __forceinline__ void drift_orientation(const StepArgs& A, const Scene
     ne.w = e.w + S.dt * d.w + S.half2dt * a.w;
     const double l = sqrt(ne.x * ne.x + ne.y * ne.y + ne.z * ne.z + ne.w * ne.w);
     ne.x = ne.x / l; ne.y = ne.y / l; ne.z = ne.z / l; ne.w = ne.w / l;
-    st4(A.ep_next + 4 * i, ne);
+    return ne;
+}
+
+__device__ __forceinline__ void drift_orientation(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> e,
+                                                  V4<double> d, V4<double> a)
+{
+    st4(A.ep_next + 4 * i, next_orientation(S, e, d, a));
+}
+
+__device__ __forceinline__ bool same_bits(V4<double> a, V4<double> b)
+{
+    return __double_as_longlong(a.x) == __double_as_longlong(b.x) && __double_as_longlong(a.y) == __double_as_longlong(b.y) &&
+           __double_as_longlong(a.z) == __double_as_longlong(b.z) && __double_as_longlong(a.w) == __double_as_longlong(b.w);
 }
 
 __device__ __forceinline__ float drift_position(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
@@ -492,7 +505,8 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
                 touch = touch || m != 0ull;
                 A.pass_mask[i] = m;
             }
-            A.nbr_count[i] = (fl & ~kTouch) | (touch ? kTouch : 0u);
+            // a touched sphere's rotation is written by the full pass: no longer known stable
+            A.nbr_count[i] = touch ? ((fl & ~kSpinStable) | kTouch) : (fl & ~kTouch);
         }
         const unsigned long long mask = __ballot(touch);
         if (mask) {
@@ -806,7 +820,7 @@ __global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
         const uint32_t flags = A.nbr_count[i];
         const V4<double> e = ld4(A.ep + 4 * i);
-        V4<double> av = ld4(A.ev + 4 * i);
+        const V4<double> ev0 = ld4(A.ev + 4 * i);
         const V4<double> aa = ld4(A.ea + 4 * i);
         const V3<double> inertia = ld3(A.inertia + 3 * i);
         const V3<double> t0 = A.tmax_persist ? ld3(A.tmax + 3 * i) : mk3(0.0, 0.0, 0.0);
@@ -816,6 +830,7 @@ __global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
         V3<double> M = mk3(0.0, 0.0, 0.0);
         const V3<double> T = t0 + mk3(0.0, 0.0, 0.0);
         if (A.tmax_persist) st3(A.tmax + 3 * i, T);
+        V4<double> av = ev0;
         const V3<double> np_ = rot_local(M, e);
         const V4<double> mea = uceom(inertia, e, av, np_);
         const double hdt = 0.5 * S.dt;
@@ -827,9 +842,21 @@ __global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
             av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y;
             av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
         }
-        st4(A.ev_out + 4 * i, av);
-        st4(A.ea + 4 * i, mea);
-        if (A.drift) drift_orientation(A, S, i, e, av, mea);
+        if (!same_bits(mea, aa)) st4(A.ea + 4 * i, mea);
+        if (!A.drift) {
+            st4(A.ev_out + 4 * i, av);
+            return;
+        }
+        // a sphere whose rotation state does not change (no spin, no torque) keeps the same
+        // bits in both buffers of ev and ep: once both hold them, nothing is written
+        const V4<double> ne = next_orientation(S, e, av, mea);
+        const bool same = same_bits(av, ev0) && same_bits(ne, e);
+        const bool stable = (flags & kSpinStable) != 0u;
+        if (!(same && stable)) {
+            st4(A.ev_out + 4 * i, av);
+            st4(A.ep_next + 4 * i, ne);
+        }
+        if (same != stable) A.nbr_count[i] = same ? (flags | kSpinStable) : (flags & ~kSpinStable);
     });
 }
 
