@@ -813,18 +813,23 @@ __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
     add_disp_max(A, dmx);
 }
 
+// A lite sphere's rotation update depends only on ep, ev, ea, inertia and the carried
+// rolling torque. When it leaves them all unchanged (no spin, no torque) it is at a fixed
+// point: kSpinStable is set once both buffers of ev and ep hold those bits, and from then on
+// the sphere is skipped (nothing read, nothing written) until k_touch or a list build clears
+// the flag.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
 {
     lite_spheres(A, [&](uint32_t i, uint32_t skip) {
-        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
         const uint32_t flags = A.nbr_count[i];
+        if (flags & (skip | kSpinStable)) return;
+        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
         const V4<double> e = ld4(A.ep + 4 * i);
         const V4<double> ev0 = ld4(A.ev + 4 * i);
         const V4<double> aa = ld4(A.ea + 4 * i);
         const V3<double> inertia = ld3(A.inertia + 3 * i);
         const V3<double> t0 = A.tmax_persist ? ld3(A.tmax + 3 * i) : mk3(0.0, 0.0, 0.0);
-        if (flags & skip) return;
         // no contact: the rolling sums are zero, so rolling_moment leaves the moment and the
         // carried torque as they are
         V3<double> M = mk3(0.0, 0.0, 0.0);
@@ -842,21 +847,16 @@ __global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
             av.x = av.x + hdt * mea.x; av.y = av.y + hdt * mea.y;
             av.z = av.z + hdt * mea.z; av.w = av.w + hdt * mea.w;
         }
-        if (!same_bits(mea, aa)) st4(A.ea + 4 * i, mea);
-        if (!A.drift) {
-            st4(A.ev_out + 4 * i, av);
-            return;
-        }
-        // a sphere whose rotation state does not change (no spin, no torque) keeps the same
-        // bits in both buffers of ev and ep: once both hold them, nothing is written
+        const bool ea_same = same_bits(mea, aa);
+        if (!ea_same) st4(A.ea + 4 * i, mea);
+        st4(A.ev_out + 4 * i, av);
+        if (!A.drift) return;
         const V4<double> ne = next_orientation(S, e, av, mea);
-        const bool same = same_bits(av, ev0) && same_bits(ne, e);
-        const bool stable = (flags & kSpinStable) != 0u;
-        if (!(same && stable)) {
-            st4(A.ev_out + 4 * i, av);
-            st4(A.ep_next + 4 * i, ne);
-        }
-        if (same != stable) A.nbr_count[i] = same ? (flags | kSpinStable) : (flags & ~kSpinStable);
+        st4(A.ep_next + 4 * i, ne);
+        const bool t_same = !A.tmax_persist || (__double_as_longlong(T.x) == __double_as_longlong(t0.x) &&
+                                                __double_as_longlong(T.y) == __double_as_longlong(t0.y) &&
+                                                __double_as_longlong(T.z) == __double_as_longlong(t0.z));
+        if (ea_same && t_same && same_bits(av, ev0) && same_bits(ne, e)) A.nbr_count[i] = flags | kSpinStable;
     });
 }
 
