@@ -202,9 +202,9 @@ def batch_main(args):
     fuse = not args.split
     mixed = run_gpu(scenes, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin, args.graph)
     value = n * args.steps / mixed["seconds"]
-    f64 = run_gpu(scenes, "fp64", max(20, args.steps // 2), max(5, args.warmup // 2), args.capacity, fuse,
+    f64 = run_gpu(scenes, "fp64", args.steps, args.warmup, args.capacity, fuse,
                   args.buckets_per_sphere, args.skin, args.graph)
-    f64_value = n * max(20, args.steps // 2) / f64["seconds"]
+    f64_value = n * args.steps / f64["seconds"]
     tm = mixed["timing"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     k_ms = tm["contact_ms"] / max(tm["steps"], 1)
@@ -266,8 +266,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--split", action="store_true",
                     help="rolling + velocity update in a kernel of their own (default: inside the contact pass)")
-    ap.add_argument("--graph", type=int, default=0,
-                    help="replay the timed steps from a HIP graph of this many steps (even; 0: eager)")
+    ap.add_argument("--graph", type=int, default=50,
+                    help="replay the timed steps from a HIP graph of this many steps (even; 0: eager launches)")
     ap.add_argument("--skin", type=float, default=0.0,
                     help="neighbour-list skin in m (0: adaptive, <0: rebuild every step)")
     ap.add_argument("--buckets-per-sphere", type=int, default=0,
@@ -344,9 +344,9 @@ def main():
         "scene_build_s": t_build,
     }
     if not args.no_fp64:
-        f64 = run_gpu(scene, "fp64", max(20, args.steps // 2), max(5, args.warmup // 2), args.capacity,
+        f64 = run_gpu(scene, "fp64", args.steps, args.warmup, args.capacity,
                       fuse, args.buckets_per_sphere, args.skin, args.graph)
-        f64_value = n * max(20, args.steps // 2) / f64["seconds"]
+        f64_value = n * args.steps / f64["seconds"]
         out["fp64_value"] = f64_value
         out["mixed_over_fp64"] = value / f64_value
         out["force_check"] = force_error(scene, args.capacity)
