@@ -116,6 +116,13 @@ typedef struct xdem_ctx_config {
                                     lasted fewer than 16 steps and halved when they lasted
                                     more than 64; < 0 = no skin (rebuilt every step).
                                     Results do not depend on it. */
+    int step_graph;              /* > 0: xdem_ctx_step replays a HIP graph of this many steps
+                                    (made even) for runs at least that long, which saves the
+                                    host's launch time when steps are short; the graph runs the
+                                    full and lite contact passes one after the other, so it
+                                    loses their overlap when many spheres are in contact.
+                                    0 (default): every kernel launched from the host. Results
+                                    do not depend on it. */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */

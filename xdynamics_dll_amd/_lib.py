@@ -102,7 +102,8 @@ class Scene(C.Structure):
 class CtxConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("p2p_capacity", C.c_uint), ("quirks", Quirks),
                 ("block_size", C.c_uint), ("grid_dim_max", C.c_uint), ("fuse_velocity", C.c_int),
-                ("device", C.c_int), ("buckets_per_sphere", C.c_uint), ("skin", C.c_double)]
+                ("device", C.c_int), ("buckets_per_sphere", C.c_uint), ("skin", C.c_double),
+                ("step_graph", C.c_int)]
 
 
 class CtxInfo(C.Structure):

@@ -124,6 +124,11 @@ struct xdem_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     unsigned graph_steps = 0;
+    // xdem_ctx_step's own graph of auto_steps steps (replayed when the parities match)
+    hipGraph_t agraph = nullptr;
+    hipGraphExec_t agexec = nullptr;
+    unsigned auto_steps = 0;
+    int apar[4] = {0, 0, 0, 0};
     int graph_parity = 0, graph_sparity = 0, graph_pparity = 0;
 
     bool timing = false;
@@ -308,6 +313,7 @@ int xdem_ctx_create(const xdem_ctx_config* cfg, xdem_ctx** out)
     if (c->cfg.precision != XDEM_PRECISION_FP64 && c->cfg.precision != XDEM_PRECISION_MIXED) { delete c; return XDEM_E_INVALID; }
     c->cap = c->cfg.p2p_capacity ? c->cfg.p2p_capacity : XDEM_MAX_P2P_COUNT;
     if (c->cap > 64) { delete c; return XDEM_E_INVALID; }
+    c->auto_steps = c->cfg.step_graph > 0 ? (unsigned)(c->cfg.step_graph + 1) & ~1u : 0u;
     if (!c->cfg.grid_dim_max) c->cfg.grid_dim_max = 512;
     if (hipSetDevice(c->cfg.device) != hipSuccess) { delete c; return XDEM_E_HIP; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -328,6 +334,8 @@ int xdem_ctx_destroy(xdem_ctx* c)
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
+    if (c->agexec) hipGraphExecDestroy(c->agexec);
+    if (c->agraph) hipGraphDestroy(c->agraph);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     for (void* p : c->allocs) hipFree(p);
     if (c->fork_ev) hipEventDestroy(c->fork_ev);
@@ -606,6 +614,38 @@ int xdem_ctx_commit(xdem_ctx* c)
     return XDEM_OK;
 }
 
+// Records nsteps steps (even) into a HIP graph without running them: the drift must be valid
+// (it is computed first otherwise); the parities (history, state, position, rebuild flag)
+// the graph's buffers assume are returned in par and restored afterwards.
+static int capture_steps(xdem_ctx* c, unsigned nsteps, hipGraph_t* graph, hipGraphExec_t* exec, int par[4])
+{
+    if (!c->drift_valid) CTX_CHECK(c->run_drift());   // the graph's steps start from a precomputed drift
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    par[0] = c->cur; par[1] = c->sv; par[2] = c->spp; par[3] = c->rbp;
+    CTX_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (unsigned k = 0; k < nsteps && e == hipSuccess; k++) e = c->full_step(nullptr);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    c->cur = par[0];
+    c->rbp = par[3];
+    c->set_state_parity(par[1]);
+    c->set_pos_parity(par[2]);
+    c->drift_valid = true;
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return XDEM_E_HIP;
+    }
+    *graph = g;
+    CTX_CHECK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return XDEM_OK;
+}
+
+static bool parities_match(const xdem_ctx* c, const int par[4])
+{
+    return c->cur == par[0] && c->sv == par[1] && c->spp == par[2] && c->rbp == par[3] && c->drift_valid;
+}
+
 static int run_steps(xdem_ctx* c, unsigned nsteps)
 {
     const bool time_it = c->timing;
@@ -618,10 +658,27 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
             c->events.push_back(e);
         }
     }
-    for (unsigned k = 0; k < nsteps; k++) {
+    // with step_graph set, untimed runs of at least auto_steps steps replay a graph of that
+    // many steps (launching the kernels one by one can cost more host time than the GPU
+    // spends on a sparse step)
+    const unsigned K = c->auto_steps;
+    for (unsigned k = 0; k < nsteps;) {
+        if (!time_it && K && nsteps - k >= K && c->drift_valid) {
+            if (!c->agexec) {
+                const int rc = capture_steps(c, K, &c->agraph, &c->agexec, c->apar);
+                if (rc != XDEM_OK) return rc;
+            }
+            if (parities_match(c, c->apar)) {
+                CTX_CHECK(hipGraphLaunch(c->agexec, c->stream));
+                c->steps_done += K;
+                k += K;
+                continue;
+            }
+        }
         hipEvent_t* ev = (time_it && k < kmax) ? &c->events[kEventsPerStep * k] : nullptr;
         CTX_CHECK(c->full_step(ev));
         c->steps_done++;
+        k++;
     }
     if (time_it) {
         CTX_CHECK(hipStreamSynchronize(c->stream));
@@ -919,28 +976,12 @@ int xdem_ctx_graph_capture(xdem_ctx* c, unsigned int nsteps)
     CTX_CHECK(hipSetDevice(c->cfg.device));
     if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
-    if (!c->drift_valid) CTX_CHECK(c->run_drift());   // the graph's steps start from a precomputed drift
-    CTX_CHECK(hipStreamSynchronize(c->stream));
-    const int parity = c->cur, sparity = c->sv, pparity = c->spp, rparity = c->rbp;
-    CTX_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    for (unsigned k = 0; k < nsteps && e == hipSuccess; k++) e = c->full_step(nullptr);
-    hipGraph_t g = nullptr;
-    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-    c->cur = parity;
-    c->rbp = rparity;
-    c->set_state_parity(sparity);
-    c->set_pos_parity(pparity);
-    c->drift_valid = true;
-    c->graph_parity = parity;
-    c->graph_sparity = sparity;
-    c->graph_pparity = pparity;
-    if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) hipGraphDestroy(g);
-        return XDEM_E_HIP;
-    }
-    c->graph = g;
-    CTX_CHECK(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    int par[4];
+    const int rc = capture_steps(c, nsteps, &c->graph, &c->gexec, par);
+    if (rc != XDEM_OK) return rc;
+    c->graph_parity = par[0];
+    c->graph_sparity = par[1];
+    c->graph_pparity = par[2];
     c->graph_steps = nsteps;
     return XDEM_OK;
 }
