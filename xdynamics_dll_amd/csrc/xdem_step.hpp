@@ -585,6 +585,11 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
         const uint32_t hb = i * cap;
         uint32_t old_count = A.pp_count_in[i];
         if (old_count > cap) old_count = cap;
+        // the history ids in 16-byte loads when the slots allow (cap a multiple of 4); the
+        // first four are loaded once for all pairs
+        const bool vec = (cap & 3u) == 0u;
+        const uint4* hid4 = reinterpret_cast<const uint4*>(A.pp_id_in + hb);
+        const uint4 h0 = (vec && old_count) ? hid4[0] : make_uint4(0u, 0u, 0u, 0u);
         V3<T> Ft = mk3(T(0), T(0), T(0));
         uint32_t n_new = 0, n_seen = 0;
         const uint32_t nlist_all = A.nbr_count[i] & kCountMask;
@@ -603,12 +608,24 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
             const PairGeom g = pair_geom(o.pos, jpos);
             double sd0 = 0.0, sd1 = 0.0;
             if (g.cdist > 0) {
-                for (uint32_t s = 0; s < old_count; s++)
-                    if (A.pp_id_in[hb + s] == k) {
-                        const double2 h = A.pp_tsd_in[hb + s];
-                        sd0 = h.x; sd1 = h.y;
-                        break;
+                // the first slot of the previous step holding k
+                uint32_t slot = kInvalid;
+                if (vec) {
+                    for (uint32_t c = 0; c < old_count && slot == kInvalid; c += 4u) {
+                        const uint4 q = c ? hid4[c / 4u] : h0;
+                        if (q.x == k) slot = c;
+                        else if (c + 1u < old_count && q.y == k) slot = c + 1u;
+                        else if (c + 2u < old_count && q.z == k) slot = c + 2u;
+                        else if (c + 3u < old_count && q.w == k) slot = c + 3u;
                     }
+                } else {
+                    for (uint32_t s = 0; s < old_count; s++)
+                        if (A.pp_id_in[hb + s] == k) { slot = s; break; }
+                }
+                if (slot != kInvalid) {
+                    const double2 h = A.pp_tsd_in[hb + slot];
+                    sd0 = h.x; sd1 = h.y;
+                }
                 const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
                 const V3<double> jom = omega_of(ld4(A.ep + 4 * (size_t)k), ld4(A.ev + 4 * (size_t)k));
                 if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
