@@ -21,7 +21,11 @@ constexpr uint32_t kCandidate = 0x80000000u;  // listed neighbours or a wall wit
 constexpr uint32_t kWallNear = 0x40000000u;   // a wall within reach + skin (build)
 constexpr uint32_t kTouch = 0x20000000u;      // this step: a pair passes or a wall is in reach (k_touch)
 constexpr uint32_t kSpinStable = 0x10000000u; // both ev and ep buffers hold the sphere's current values
-constexpr uint32_t kCountMask = 0x0fffffffu;
+constexpr uint32_t kAccStable = 0x08000000u;  // acc holds the sphere's contact-free acceleration
+constexpr uint32_t kNoContacts = 0x04000000u; // both p2p count buffers and the wall count hold 0, the
+                                              // dynamic-wall wrench slots hold zeros
+constexpr uint32_t kLiteStable = kSpinStable | kAccStable | kNoContacts;   // set by the lite pass
+constexpr uint32_t kCountMask = 0x03ffffffu;
 // The full-pass queue is split in kQueueSegs segments, each with its own counter on its own
 // 128-byte line: the wave of sorted slots [64 w, 64 w + 64) appends to segment w mod
 // kQueueSegs, so the appends of different waves rarely hit the same counter.
@@ -94,7 +98,7 @@ struct StepArgs {
     uint32_t* work2;                         // spheres of this step's full contact pass (k_touch)
     unsigned long long* pass_mask;           // per sphere of work2: list entries that pass (k_touch)
     uint32_t* nbr_count;                     // listed neighbours per sphere | kCandidate, kWallNear, kTouch,
-                                             // kSpinStable
+                                             // kSpinStable, kAccStable, kNoContacts
     uint32_t* nbr;                           // their sphere ids, list_cap per sphere
     float4* sposf;  // per sorted slot: position relative to the scene's grid origin and
                     // radius, FP32 (the pair search reads only this); the radius is
@@ -280,6 +284,12 @@ __device__ __forceinline__ bool same_bits(V4<double> a, V4<double> b)
 {
     return __double_as_longlong(a.x) == __double_as_longlong(b.x) && __double_as_longlong(a.y) == __double_as_longlong(b.y) &&
            __double_as_longlong(a.z) == __double_as_longlong(b.z) && __double_as_longlong(a.w) == __double_as_longlong(b.w);
+}
+
+__device__ __forceinline__ bool same_bits(V3<double> a, V3<double> b)
+{
+    return __double_as_longlong(a.x) == __double_as_longlong(b.x) && __double_as_longlong(a.y) == __double_as_longlong(b.y) &&
+           __double_as_longlong(a.z) == __double_as_longlong(b.z);
 }
 
 __device__ __forceinline__ float drift_position(const StepArgs& A, const SceneDev& S, uint32_t i, V4<double> p,
@@ -506,7 +516,7 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
                 A.pass_mask[i] = m;
             }
             // a touched sphere's rotation is written by the full pass: no longer known stable
-            A.nbr_count[i] = touch ? ((fl & ~kSpinStable) | kTouch) : (fl & ~kTouch);
+            A.nbr_count[i] = touch ? ((fl & ~kLiteStable) | kTouch) : (fl & ~kTouch);
         }
         const unsigned long long mask = __ballot(touch);
         if (mask) {
@@ -790,29 +800,47 @@ __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
     lite_spheres(A, [&](uint32_t i, uint32_t skip) {
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
         const uint32_t flags = A.nbr_count[i];
-        const uint32_t count0 = S.nwall ? A.pl_count[i] : 0u;
+        const bool settled = flags & kNoContacts;
+        const bool acc_kept = flags & kAccStable;
+        const uint32_t count0 = (S.nwall && !settled) ? A.pl_count[i] : 0u;
+        const uint32_t pp_in = (S.has_pp && !settled) ? A.pp_count_in[i] : 0u;
         const V4<double> p = ld4(A.pos + 4 * i);
         const V3<double> v0 = ld3(A.vel + 3 * i);
         const V3<double> ao = ld3(A.acc + 3 * i);
-        const double m = A.mass[i];
+        const double m = acc_kept ? 0.0 : A.mass[i];
         if (flags & skip) return;
-        if (S.has_pp) A.pp_count_out[i] = 0u;
-        if (S.nwall) {
-            if (S.wall_dynamic)
-                for (uint32_t w = 0; w < S.nwall; w++) {
-                    const int32_t slot = A.walls[S.wall_begin + w].wslot;
-                    if (slot >= 0) {
-                        double* wr = A.wrench + (size_t)slot * 6 * A.np;
+        uint32_t nf = flags;
+        if (!settled) {
+            // contact bookkeeping of a sphere without contacts: empty p2p history, the walls'
+            // slot-count rule, zero dynamic-wall wrench; once all of it holds zeros (the p2p
+            // count in both buffers) nothing changes until the sphere is touched again
+            if (S.has_pp) A.pp_count_out[i] = 0u;
+            uint32_t count = 0u;
+            if (S.nwall) {
+                if (S.wall_dynamic)
+                    for (uint32_t w = 0; w < S.nwall; w++) {
+                        const int32_t slot = A.walls[S.wall_begin + w].wslot;
+                        if (slot >= 0) {
+                            double* wr = A.wrench + (size_t)slot * 6 * A.np;
 #pragma unroll
-                        for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
+                            for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
+                        }
                     }
-                }
-            const uint32_t count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
-            if (count != count0) A.pl_count[i] = count;
-            n_wall += count;
+                count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
+                if (count != count0) A.pl_count[i] = count;
+                n_wall += count;
+            }
+            if (pp_in == 0u && count == 0u) nf |= kNoContacts;
         }
-        const double inv_m = 1.0 / m;
-        const V3<double> a = mk3(inv_m * (0.0 + m * S.gx), inv_m * (0.0 + m * S.gy), inv_m * (0.0 + m * S.gz));
+        V3<double> a = ao;
+        if (!acc_kept) {
+            const double inv_m = 1.0 / m;
+            a = mk3(inv_m * (0.0 + m * S.gx), inv_m * (0.0 + m * S.gy), inv_m * (0.0 + m * S.gz));
+            const bool finite = isfinite(a.x) && isfinite(a.y) && isfinite(a.z);
+            if (!finite) atomicOr(A.err, XDEM_ERR_NONFINITE);
+            if (!same_bits(a, ao)) st3(A.acc + 3 * i, a);
+            else if (finite) nf |= kAccStable;   // the next steps compute the same bits
+        }
         const double hdt = 0.5 * S.dt;
         V3<double> v = v0;
         if (A.keep_velocity) {
@@ -822,8 +850,7 @@ __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
             v = mk3(v.x + hdt * a.x, v.y + hdt * a.y, v.z + hdt * a.z);
             st3(A.vel_out + 3 * i, v);
         }
-        st3(A.acc + 3 * i, a);
-        if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) atomicOr(A.err, XDEM_ERR_NONFINITE);
+        if (nf != flags) A.nbr_count[i] = nf;
         if (A.drift) dmx = fmaxf(dmx, drift_position(A, S, i, p, v, a));
     });
     add_contact_counts(A, 0u, n_wall);
