@@ -450,6 +450,10 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
 {
     const uint32_t seg = blockIdx.x % kQueueSegs;
     const uint32_t nchunk = gridDim.x / kQueueSegs;
+    const uint32_t* wq = A.work + (size_t)seg * A.seg_cap;
+    uint32_t base = (blockIdx.x / kQueueSegs) * blockDim.x;
+    // the first entry is read together with the segment's count
+    uint32_t i_next = base + threadIdx.x < A.seg_cap ? wq[base + threadIdx.x] : 0u;
     const uint32_t cnt = A.seg_count[seg * kSegStride];
     const uint32_t lane = threadIdx.x & 63u;
     // no sphere has moved more than dm since the build (0 on a build step)
@@ -463,12 +467,12 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
     }
     __syncthreads();
     const float dm = dm_sh;
-    for (uint32_t base = (blockIdx.x / kQueueSegs) * blockDim.x; base < cnt; base += nchunk * blockDim.x) {
+    for (; base < cnt; base += nchunk * blockDim.x) {
         const uint32_t e = base + threadIdx.x;
         bool touch = false;
-        uint32_t i = 0;
+        const uint32_t i = i_next;
+        if (e + nchunk * blockDim.x < cnt) i_next = wq[e + nchunk * blockDim.x];
         if (e < cnt) {
-            i = A.work[(size_t)seg * A.seg_cap + e];
             const uint32_t fl = A.nbr_count[i];
             // a pair can pass only once the two spheres have moved its gap together, a wall
             // come into reach only once the sphere has moved its gap
@@ -584,7 +588,15 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
                                                float& dmx)
 {
     const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    // the sphere's own state, list flags and wall slot count, issued together (the fields
+    // of the velocity update are read just before the tail: fewer registers live across
+    // the pairs)
     OwnState o = load_own<false>(A, S, i);
+    const uint32_t flags = A.nbr_count[i];
+    WallSlots ws;
+    const uint32_t count0 = S.nwall ? A.pl_count[i] : 0u;
+    ws.init(count0, A.pl_id + XDEM_MAX_P2PL_COUNT * i,
+            reinterpret_cast<const double*>(A.pl_tsd + XDEM_MAX_P2PL_COUNT * i));
     const V3<double> iom = omega_of(o.ep, o.ev);
     uint32_t err = 0;
 
@@ -602,7 +614,7 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
         const uint4 h0 = (vec && old_count) ? hid4[0] : make_uint4(0u, 0u, 0u, 0u);
         V3<T> Ft = mk3(T(0), T(0), T(0));
         uint32_t n_new = 0, n_seen = 0;
-        const uint32_t nlist_all = A.nbr_count[i] & kCountMask;
+        const uint32_t nlist_all = flags & kCountMask;
         const uint32_t lc = A.list_cap;
         if (nlist_all > lc) err |= XDEM_ERR_NBR_OVERFLOW;
         const uint32_t nlist = min(nlist_all, lc);
@@ -610,37 +622,41 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
         // the listed pairs that pass (k_touch's mask), in list order, which k_touch has made
         // the reference's visiting order
         unsigned long long m = nlist ? A.pass_mask[i] : 0ull;
+        // each pair's list entry is read one pair ahead, and all of the neighbour's state in
+        // one go, so a pair costs one memory round trip
+        uint32_t k_next = m ? lst[__ffsll((long long)m) - 1] : 0u;
         while (m) {
-            const uint32_t t = (uint32_t)__ffsll((long long)m) - 1u;
             m &= m - 1ull;
-            const uint32_t k = lst[t];
+            const uint32_t k = k_next;
+            if (m) k_next = lst[__ffsll((long long)m) - 1];
             const V4<double> jpos = ld4(A.pos + 4 * (size_t)k);
+            const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
+            const V4<double> jep = ld4(A.ep + 4 * (size_t)k);
+            const V4<double> jev = ld4(A.ev + 4 * (size_t)k);
+            const double jmass = A.mass[k];
+            // the first slot of the previous step holding k
+            uint32_t slot = kInvalid;
+            if (vec) {
+                for (uint32_t c = 0; c < old_count && slot == kInvalid; c += 4u) {
+                    const uint4 q = c ? hid4[c / 4u] : h0;
+                    if (q.x == k) slot = c;
+                    else if (c + 1u < old_count && q.y == k) slot = c + 1u;
+                    else if (c + 2u < old_count && q.z == k) slot = c + 2u;
+                    else if (c + 3u < old_count && q.w == k) slot = c + 3u;
+                }
+            } else {
+                for (uint32_t s = 0; s < old_count; s++)
+                    if (A.pp_id_in[hb + s] == k) { slot = s; break; }
+            }
+            const double2 h = slot != kInvalid ? A.pp_tsd_in[hb + slot] : make_double2(0.0, 0.0);
             const PairGeom g = pair_geom(o.pos, jpos);
             double sd0 = 0.0, sd1 = 0.0;
             if (g.cdist > 0) {
-                // the first slot of the previous step holding k
-                uint32_t slot = kInvalid;
-                if (vec) {
-                    for (uint32_t c = 0; c < old_count && slot == kInvalid; c += 4u) {
-                        const uint4 q = c ? hid4[c / 4u] : h0;
-                        if (q.x == k) slot = c;
-                        else if (c + 1u < old_count && q.y == k) slot = c + 1u;
-                        else if (c + 2u < old_count && q.z == k) slot = c + 2u;
-                        else if (c + 3u < old_count && q.w == k) slot = c + 3u;
-                    }
-                } else {
-                    for (uint32_t s = 0; s < old_count; s++)
-                        if (A.pp_id_in[hb + s] == k) { slot = s; break; }
-                }
-                if (slot != kInvalid) {
-                    const double2 h = A.pp_tsd_in[hb + slot];
-                    sd0 = h.x; sd1 = h.y;
-                }
-                const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
-                const V3<double> jom = omega_of(ld4(A.ep + 4 * (size_t)k), ld4(A.ev + 4 * (size_t)k));
+                sd0 = h.x; sd1 = h.y;
+                const V3<double> jom = omega_of(jep, jev);
                 if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
                 sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, o.pos, o.vel, iom, o.mass, jpos, jvel, jom,
-                                                  A.mass[k], g, sd0, sd1, Ft, pp);
+                                                  jmass, g, sd0, sd1, Ft, pp);
             } else {
                 sphere_cohesion(S.cc, o.pos.w, jpos.w, g, pp);
             }
@@ -665,10 +681,6 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
     if (S.nwall) {
         // wall slots are read only when a wall needs them and written back only
         // when one changed (the values in memory are the same either way)
-        WallSlots ws;
-        const uint32_t count0 = A.pl_count[i];
-        ws.init(count0, A.pl_id + XDEM_MAX_P2PL_COUNT * i,
-                reinterpret_cast<const double*>(A.pl_tsd + XDEM_MAX_P2PL_COUNT * i));
         for (uint32_t w = 0; w < S.nwall; w++) {
             const WallDev& W = A.walls[S.wall_begin + w];
             double bf[6] = {0, 0, 0, 0, 0, 0};
@@ -904,36 +916,27 @@ __global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
     });
 }
 
-// Full pass over the spheres k_touch queued (grid-stride over the concatenated segments).
+// Full pass over the spheres k_touch queued: block b takes queue segment b mod kQueueSegs,
+// its entries chunk b / kQueueSegs, ... strided by the chunks per segment. The first entry's
+// sphere id is read together with the segment's count.
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide(StepArgs A)
 {
-    // seg_end[s] = spheres in segments 0..s (inclusive prefix, one wave64 scan)
-    __shared__ uint32_t seg_end[kQueueSegs];
-    if (threadIdx.x < kQueueSegs) {
-        uint32_t x = A.seg_count2[threadIdx.x * kSegStride];
-#pragma unroll
-        for (uint32_t off = 1; off < 64u; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off, 64);
-            if (threadIdx.x >= off) x += y;
-        }
-        seg_end[threadIdx.x] = x;
-    }
-    __syncthreads();
-    const uint32_t nwork = seg_end[kQueueSegs - 1];
+    const uint32_t sg = blockIdx.x % kQueueSegs;
+    const uint32_t stride = gridDim.x / kQueueSegs * blockDim.x;
+    const uint32_t* q = A.work2 + (size_t)sg * A.seg_cap;
+    uint32_t e = blockIdx.x / kQueueSegs * blockDim.x + threadIdx.x;
+    uint32_t i = e < A.seg_cap ? q[e] : 0u;
+    const uint32_t cnt = A.seg_count2[sg * kSegStride];
     unsigned long long c0 = 0, c1 = 0;
     float dmx = 0.0f;
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwork; w += gridDim.x * blockDim.x) {
-        // segment of entry w: the first s with seg_end[s] > w
-        uint32_t sg = 0;
-#pragma unroll
-        for (uint32_t step = kQueueSegs / 2u; step > 0u; step >>= 1)
-            if (seg_end[sg + step - 1u] <= w) sg += step;
-        const uint32_t i = A.work2[(size_t)sg * A.seg_cap + (w - (sg ? seg_end[sg - 1u] : 0u))];
+    while (e < cnt) {
         uint32_t n_pp = 0, n_wall = 0;
         collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, i, n_pp, n_wall, dmx);
         c0 += n_pp;
         c1 += n_wall;
+        e += stride;
+        if (e < cnt) i = q[e];
     }
     add_contact_counts(A, c0, c1);
     add_disp_max(A, dmx);

@@ -12,17 +12,23 @@ constexpr unsigned kBlock = 256;
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
 // pass 0: the lite pass over every sphere outside this step's full pass (two kernels in
-// fused mode); pass 1: the full pass over the spheres k_touch queued (grid-stride over at
-// most kFullBlocks blocks, one per CU: the kernel runs one wave per SIMD; the queue length
-// is read on the device). The two touch disjoint spheres and may run concurrently.
+// fused mode); pass 1: the full pass over the spheres k_touch queued (kQueueSegs blocks per
+// chunk of each queue segment, at most kFullBlocks in all, one per CU: the kernel runs one
+// wave per SIMD; the queue length is read on the device). The two touch disjoint spheres
+// and may run concurrently.
 constexpr unsigned kFullBlocks = 256;
+inline dim3 full_grid(unsigned np)
+{
+    const unsigned chunks = std::max(1u, std::min(grid_for(np).x, kFullBlocks) / kQueueSegs);
+    return dim3(chunks * kQueueSegs);
+}
 
 template <bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 void collide_passes(const StepArgs& A, int pass, hipStream_t s)
 {
     const dim3 g = grid_for(A.np), b(kBlock);
     if (pass == 1) {
-        hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW>), dim3(std::min(g.x, kFullBlocks)), b, 0, s, A);
+        hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW>), full_grid(A.np), b, 0, s, A);
     } else if (FUSE) {
         launch_lite_fused(A, MULTI, s);
     } else {
