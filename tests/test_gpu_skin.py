@@ -132,3 +132,29 @@ def test_step_graph_replay_is_bit_identical_to_eager_launches():
         assert np.array_equal(out[0][f], out[1][f]), f
     for x, y in zip(out[0]["p2p"], out[1]["p2p"]):
         assert np.array_equal(x, y)
+
+
+def test_bucket_table_size_does_not_change_results():
+    """The bucket grid only selects candidates: a table of 16M buckets (4096 scan tiles,
+    scanned in two levels) gives the same bits as the default one (one level) over steps
+    that all rebuild the lists."""
+    s = random_pack(n=3000, seed=4)
+    vel = np.random.default_rng(4).normal(0.0, 1.0, size=(s.np, 3))
+    # a 2 x 1 x 1 m box around the 4 x 4 x 8 cm pack: the grid covers the box
+    scene = SceneData.from_positions(s.pos, 2600.0, s.cp_pp, walls=box_walls(2.0, 1.0, 1.0), wall_cp=s.plane_cp[0],
+                                     dt=5e-6, vel=vel)
+    out = []
+    for bps in (0, 8192):
+        ctx = DemContext(scene, precision="fp64", fuse_velocity=True, buckets_per_sphere=bps, skin=-1.0)
+        ctx.step(60)
+        info = ctx.info()
+        out.append({f: ctx.read(f) for f in FIELDS} | {"p2p": ctx.history("p2p"), "info": info})
+        assert ctx.error_word() == 0
+    print("buckets", out[0]["info"]["nbucket"], out[1]["info"]["nbucket"],
+          "builds", out[0]["info"]["list_builds"], out[1]["info"]["list_builds"], flush=True)
+    assert out[0]["info"]["nbucket"] <= 2048 * 4096 < out[1]["info"]["nbucket"]
+    assert out[1]["info"]["list_builds"] > 2
+    for f in FIELDS:
+        assert np.array_equal(out[0][f], out[1][f]), f
+    for x, y in zip(out[0]["p2p"], out[1]["p2p"]):
+        assert np.array_equal(x, y)

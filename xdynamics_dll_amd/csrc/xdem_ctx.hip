@@ -81,6 +81,8 @@ struct xdem_ctx {
     uint32_t *keys = nullptr, *svals = nullptr, *slot_tmp = nullptr;
     uint32_t *nbr_count = nullptr, *nbr = nullptr, *work = nullptr, *seg_count = nullptr;
     uint32_t *work2 = nullptr, *seg_count2 = nullptr;
+    uint32_t* grid_bar = nullptr;                        // k_rebuild's barrier counter
+    uint32_t rebuild_blocks = 0;                         // k_rebuild's grid
     unsigned long long* pass_mask = nullptr;
     uint32_t seg_cap = 0;
     uint32_t list_cap = 48;
@@ -159,6 +161,7 @@ struct xdem_ctx {
         A.bucket_count = bucket_count; A.bucket_start = bucket_start; A.scan_part = scan_part;
         A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap; A.work = work;
         A.seg_count = seg_count; A.seg_cap = seg_cap; A.work2 = work2; A.seg_count2 = seg_count2;
+        A.grid_bar = grid_bar; A.rebuild_blocks = rebuild_blocks;
         A.pass_mask = pass_mask;
         A.sposf = sposf;
         A.rebuild_cur = rebuild + rbp; A.rebuild_next = rebuild + (rbp ^ 1); A.n_rebuild = n_rebuild;
@@ -513,6 +516,9 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->work2, sizeof(uint32_t) * kQueueSegs * c->seg_cap));
     CTX_CHECK(c->alloc(&c->pass_mask, sizeof(unsigned long long) * n));
     CTX_CHECK(c->alloc(&c->seg_count2, sizeof(uint32_t) * kQueueSegs * kSegStride));
+    CTX_CHECK(c->alloc(&c->grid_bar, sizeof(uint32_t) * kSegStride * (1 + kBarFlags)));
+    c->rebuild_blocks = rebuild_grid_size(c->multi);
+    if (c->rebuild_blocks == 0) return XDEM_E_HIP;
     CTX_CHECK(c->alloc(&c->nbr, sizeof(uint32_t) * n * c->list_cap));
     const size_t nbt = bucket_table_size(c->nbucket);
     CTX_CHECK(c->alloc(&c->bucket_count, sizeof(uint32_t) * nbt));
@@ -605,6 +611,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, c->rebuild, 1u, (size_t)2);   // the first step builds
     CTX_CHECK(hipMemsetAsync(c->seg_count, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
     CTX_CHECK(hipMemsetAsync(c->seg_count2, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
+    CTX_CHECK(hipMemsetAsync(c->grid_bar, 0, sizeof(uint32_t) * kSegStride * (1 + kBarFlags), s));
     CTX_CHECK(hipMemsetAsync(c->nbr_count, 0, sizeof(uint32_t) * n, s));
     if (c->nwslot) CTX_CHECK(hipMemsetAsync(c->wrench_out, 0, sizeof(double) * 6 * c->nwslot, s));
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nbt + 255) / 256)), dim3(256), 0, s, c->bucket_count, 0u, nbt);

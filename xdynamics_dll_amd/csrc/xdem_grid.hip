@@ -8,15 +8,17 @@
 // moved half the skin since the last build. Every kernel below returns at once on the other
 // steps. The drift of a step (position and orientation update, impl:135-147) is computed at
 // the end of the previous step's velocity update; k_drift does it when no previous step
-// left it. A rebuild is:
-//   k_count           bucket key of every sphere and its count (posted atomic add)
-//   k_scan_reduce     \
-//   (k_scan_parts)     > exclusive prefix of the counts: start[b] = first sorted slot of
-//   k_scan_apply      /  bucket b, start[nbucket] = N (tables of at most kDirectTiles tiles:
-//                        each k_scan_apply block sums the earlier tile sums itself)
-//   k_scatter         provisional slot start[key] + (atomic countdown of the bucket), which
-//                     leaves every count at zero for the next step
-//   k_order_gather    final slot = start[key] + number of spheres of the same bucket with a
+// left it. A rebuild is one kernel, k_rebuild, in phases separated by grid barriers (every
+// block is resident; the kernel's first lines, on every step, clear the step's counters,
+// and the steps that do not rebuild return right after them):
+//   count             bucket key of every sphere and its count (posted atomic add)
+//   scan              exclusive prefix of the counts, tile sums then tiles: start[b] =
+//                     first sorted slot of bucket b, start[nbucket] = N (each tile adds up
+//                     the earlier tile sums itself up to kDirectTiles tiles, else one block
+//                     scans them first)
+//   scatter           provisional slot start[key] + (atomic countdown of the bucket), which
+//                     leaves every count at zero for the next build
+//   order + gather    final slot = start[key] + number of spheres of the same bucket with a
 //                     smaller id, i.e. exactly the reference's (hash, index) order; writes
 //                     the sorted id and the FP32 position copy the pair search reads, with
 //                     the sphere's wall-reach flag (FP64 test, reach + skin) in the sign of
@@ -37,7 +39,6 @@ namespace {
 constexpr unsigned kBlock = 256;
 constexpr unsigned kScanItems = 16;                   // per thread
 constexpr unsigned kScanTile = kBlock * kScanItems;   // 4096 buckets per block
-constexpr unsigned kPartsBlock = 1024;
 constexpr unsigned kDirectTiles = 2048;
 
 // Sum of x over the block (every thread gets it); red holds kBlock / 64 entries.
@@ -68,70 +69,120 @@ __global__ __launch_bounds__(kBlock) void k_drift(StepArgs A)
                  ld4(A.ev + 4 * i), ld4(A.ea + 4 * i));
 }
 
-// First kernel of every step: clears the step's contact counters, full-pass queue and the
-// next step's rebuild flag. On a rebuild: the bucket key of every sphere and its count.
-template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_count(StepArgs A)
+// k_rebuild's phases hand data to blocks on other XCDs, whose L2 caches are not coherent
+// with each other. Everything one phase writes for a later one is stored and loaded with
+// agent-scope atomic accesses (written through to memory, read from it), so the barrier
+// needs no cache write-back or invalidation. Loads of data a phase may have left stale are
+// also checked against the array bounds (XDEM_ERR_GRID_SYNC instead of a wild access).
+__device__ __forceinline__ uint32_t ld_co(const uint32_t* p)
 {
-    if (blockIdx.x == 0 && threadIdx.x < 2) A.counters[threadIdx.x] = 0ull;
-    if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_co(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every step, block 0: clears the step's contact counters, full-pass queue and the next
+// step's rebuild flag, and counts the step for the adaptive skin (on a rebuild: the skin of
+// the lists it makes, from how long the last ones lasted).
+__device__ __forceinline__ void step_prologue(const StepArgs& A)
+{
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x < 2) A.counters[threadIdx.x] = 0ull;
+    if (threadIdx.x < kQueueSegs) {
         A.seg_count2[threadIdx.x * kSegStride] = 0u;
         A.dmax_next[threadIdx.x * kSegStride] = 0u;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (threadIdx.x == 0) {
         *A.rebuild_next = 0u;
         ListCtl& L = *A.lctl;
         L.step += 1u;
         if (*A.rebuild_cur) {
-            // skin of the lists this build makes, from how long the last ones lasted
             const uint32_t lasted = L.step - L.last_build;
             if (L.adaptive && L.last_build != 0u) {
-                if (lasted < 16u) L.scale = fminf(1.0f, 2.0f * L.scale);
-                else if (lasted > 64u) L.scale = fmaxf(L.min_scale, 0.5f * L.scale);
+                float sc = L.scale;
+                if (lasted < 16u) sc = fminf(1.0f, 2.0f * sc);
+                else if (lasted > 64u) sc = fmaxf(L.min_scale, 0.5f * sc);
+                st_co(reinterpret_cast<uint32_t*>(&L.scale), __float_as_uint(sc));   // read by order_gather_slot
             }
             L.last_build = L.step;
+            *A.n_rebuild += 1ull;
         }
     }
-    if (!*A.rebuild_cur) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *A.n_rebuild += 1ull;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.np) return;
+}
+
+// Barrier of all blocks of k_rebuild (all resident at once: the grid is sized by the
+// occupancy). Every wave waits for its stores to complete (they are written through), then
+// thread 0 of each block arrives on the counter grid_bar[0]; the block whose arrival
+// completes the gen-th round writes gen to kBarFlags release flags (one per 128-byte line,
+// one wave store); every block waits on flag blockIdx mod kBarFlags (a few blocks per
+// address, so the waits do not crowd the counter). k_touch, the next kernel of the step,
+// sets the counter and the flags back to zero. The wait is bounded (kSyncSpins tries of
+// ~1 us): beyond it the barrier flags XDEM_ERR_GRID_SYNC and returns false, and the block
+// leaves the kernel, so the kernel always ends.
+constexpr uint32_t kSyncSpins = 1u << 20;
+
+__device__ __forceinline__ bool grid_sync(const StepArgs& A, uint32_t& gen)
+{
+    __shared__ uint32_t passed;
+    gen += 1u;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t last = 0;
+        if (threadIdx.x == 0) last = atomicAdd(A.grid_bar, 1u) + 1u == gen * gridDim.x ? 1u : 0u;
+        last = __shfl(last, 0, 64);
+        if (last) st_co(A.grid_bar + kSegStride * (1u + threadIdx.x), gen);
+        if (threadIdx.x == 0) {
+            const uint32_t* flag = A.grid_bar + kSegStride * (1u + blockIdx.x % kBarFlags);
+            uint32_t tries = 0;
+            bool ok = false;
+            while (true) {
+                ok = ld_co(flag) >= gen;
+                if (ok || ++tries == kSyncSpins) break;
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (!ok) atomicOr(A.err, XDEM_ERR_GRID_SYNC);
+            passed = ok ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+    return passed != 0u;
+}
+
+// Bucket key of sphere i and its count (posted atomic add).
+template <bool MULTI>
+__device__ __forceinline__ void count_sphere(const StepArgs& A, uint32_t i)
+{
     const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
     const V4<double> p = ld4(A.pos + 4 * i);
     int gx, gy, gz;
     grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
     const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
-    A.keys[i] = key;
-    atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: k_scatter takes the rank
+    st_co(A.keys + i, key);
+    atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: the scatter takes the rank
 }
 
-// Sum of 16 consecutive buckets per thread, reduced per block of 4096 buckets. Block 0 also
-// clears the full-pass queue segment counters.
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ gate,
-                                                        const uint32_t* __restrict__ count,
-                                                        uint32_t* __restrict__ part,
-                                                        uint32_t* __restrict__ seg_count)
+// Sum of tile t (4096 buckets) into part[t]; lane-contiguous loads (coalesced).
+__device__ __forceinline__ void scan_reduce_tile(const uint32_t* count, uint32_t* part, uint32_t t)
 {
-    if (!*gate) return;
-    if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) seg_count[threadIdx.x * kSegStride] = 0u;
     __shared__ uint32_t wsum[kBlock / 64];
-    const uint4* c = reinterpret_cast<const uint4*>(count + (size_t)blockIdx.x * kScanTile) + threadIdx.x * 4;
+    const uint32_t* c = count + (size_t)t * kScanTile + threadIdx.x;
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const uint4 v = c[k];
-        s += v.x + v.y + v.z + v.w;
-    }
+    for (int k = 0; k < (int)kScanItems; k++) s += ld_co(c + k * kBlock);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t t = 0;
+        uint32_t u = 0;
 #pragma unroll
-        for (unsigned w = 0; w < kBlock / 64; w++) t += wsum[w];
-        part[blockIdx.x] = t;
+        for (unsigned w = 0; w < kBlock / 64; w++) u += wsum[w];
+        st_co(part + t, u);
     }
+    __syncthreads();
 }
 
 // Inclusive scan of one value per lane across a wave64.
@@ -146,77 +197,78 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x)
     return x;
 }
 
-// Exclusive scan of the block sums by one block; writes the grand total to *total.
-__global__ __launch_bounds__(kPartsBlock) void k_scan_parts(const uint32_t* __restrict__ gate,
-                                                            uint32_t* __restrict__ part, uint32_t nparts,
-                                                            uint32_t* __restrict__ total)
+// Exclusive scan of the tile sums by one block (tables of more than kDirectTiles tiles);
+// writes the grand total to *total.
+__device__ __forceinline__ void scan_parts_block(uint32_t* part, uint32_t nparts, uint32_t* total)
 {
-    if (!*gate) return;
-    __shared__ uint32_t wsum[kPartsBlock / 64];
-    const uint32_t per = (nparts + kPartsBlock - 1) / kPartsBlock;
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint32_t per = (nparts + kBlock - 1) / kBlock;
     const uint32_t b0 = threadIdx.x * per;
     uint32_t s = 0;
     for (uint32_t k = 0; k < per; k++)
-        if (b0 + k < nparts) s += part[b0 + k];
+        if (b0 + k < nparts) s += ld_co(part + b0 + k);
     const uint32_t inc = wave_inclusive_scan(s);
     const uint32_t w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 63) wsum[w] = inc;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t run = 0;
-        for (unsigned k = 0; k < kPartsBlock / 64; k++) {
+        for (unsigned k = 0; k < kBlock / 64; k++) {
             const uint32_t t = wsum[k];
             wsum[k] = run;
             run += t;
         }
-        *total = run;
+        st_co(total, run);
     }
     __syncthreads();
     uint32_t run = wsum[w] + inc - s;
     for (uint32_t k = 0; k < per; k++)
         if (b0 + k < nparts) {
-            const uint32_t t = part[b0 + k];
-            part[b0 + k] = run;
+            const uint32_t t = ld_co(part + b0 + k);
+            st_co(part + b0 + k, run);
             run += t;
         }
+    __syncthreads();
 }
 
-// start[b] = (sum of the earlier tiles) + exclusive prefix inside the tile (the counts stay:
-// k_scatter counts them back down to zero). DIRECT: part holds the tile sums and the block
-// adds up those of the earlier tiles itself; the last block writes the total to *total.
-// Otherwise part holds the exclusive prefix of the tile sums (k_scan_parts).
+// start[b] of tile t = (sum of the earlier tiles) + exclusive prefix inside the tile (the
+// counts stay: the scatter counts them back down to zero). DIRECT: part holds the tile sums
+// and the block adds up those of the earlier tiles itself; the last tile writes the total
+// to *total. Otherwise part holds the exclusive prefix of the tile sums (scan_parts_block).
 template <bool DIRECT>
-__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ gate,
-                                                       const uint32_t* __restrict__ count,
-                                                       const uint32_t* __restrict__ part,
-                                                       uint32_t* __restrict__ start,
-                                                       uint32_t* __restrict__ total)
+__device__ __forceinline__ void scan_apply_tile(const uint32_t* count, const uint32_t* part, uint32_t* start,
+                                                uint32_t* total, uint32_t t, uint32_t ntile)
 {
-    if (!*gate) return;
     __shared__ uint32_t wsum[kBlock / 64];
     __shared__ uint32_t red[kBlock / 64];
     uint32_t tile0;
     if (DIRECT) {
         uint32_t pre = 0;
-        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kBlock) pre += part[t];
+        for (uint32_t u = threadIdx.x; u < t; u += kBlock) pre += ld_co(part + u);
         tile0 = block_sum(pre, red);
     } else {
-        tile0 = part[blockIdx.x];
+        tile0 = ld_co(part + t);
     }
-    const size_t base = (size_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    const uint4* c = reinterpret_cast<const uint4*>(count + base);
+    // the tile through LDS: coalesced loads and stores, kScanItems consecutive buckets per
+    // thread for the serial part (one pad word per kScanItems: conflict-free rows)
+    __shared__ uint32_t tile[kScanTile + kScanTile / kScanItems];
+    const size_t base = (size_t)t * kScanTile;
+#pragma unroll
+    for (int k = 0; k < (int)kScanItems; k++) {
+        const uint32_t q = k * kBlock + threadIdx.x;
+        tile[q + q / kScanItems] = ld_co(count + base + q);
+    }
+    __syncthreads();
+    uint32_t* row = tile + threadIdx.x * (kScanItems + 1);
     uint32_t v[kScanItems];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const uint4 q = c[k];
-        v[4 * k + 0] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
-    }
+    for (int k = 0; k < (int)kScanItems; k++) v[k] = row[k];
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < (int)kScanItems; k++) {
-        const uint32_t t = v[k];
+        const uint32_t u = v[k];
         v[k] = s;
-        s += t;
+        s += u;
     }
     const uint32_t inc = wave_inclusive_scan(s);
     const uint32_t w = threadIdx.x >> 6;
@@ -224,36 +276,42 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restric
     __syncthreads();
     uint32_t off = tile0 + inc - s;
     for (uint32_t k = 0; k < w; k++) off += wsum[k];
-    if (DIRECT && blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) *total = off + s;
-    uint4* o = reinterpret_cast<uint4*>(start + base);
+    if (DIRECT && t == ntile - 1 && threadIdx.x == kBlock - 1) st_co(total, off + s);
 #pragma unroll
-    for (int k = 0; k < 4; k++)
-        o[k] = make_uint4(off + v[4 * k], off + v[4 * k + 1], off + v[4 * k + 2], off + v[4 * k + 3]);
+    for (int k = 0; k < (int)kScanItems; k++) row[k] = off + v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < (int)kScanItems; k++) {
+        const uint32_t q = k * kBlock + threadIdx.x;
+        st_co(start + base + q, tile[q + q / kScanItems]);
+    }
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(StepArgs A)
+// Provisional slot of sphere i: counting the bucket back down gives each sphere a distinct
+// slot and leaves the count at zero for the next build (the order inside the bucket is
+// fixed by order_gather_slot).
+__device__ __forceinline__ void scatter_sphere(const StepArgs& A, uint32_t i)
 {
-    if (!*A.rebuild_cur) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.np) return;
-    const uint32_t key = A.keys[i];
-    // counting the bucket back down gives each sphere a distinct slot and leaves the count
-    // at zero for the next step (the order inside the bucket is fixed by k_order_gather)
-    A.slot_tmp[A.bucket_start[key] + atomicSub(&A.bucket_count[key], 1u) - 1u] = i;
+    const uint32_t key = ld_co(A.keys + i);
+    if (key >= A.nbucket) { atomicOr(A.err, XDEM_ERR_GRID_SYNC); return; }
+    const uint32_t slot = ld_co(A.bucket_start + key) + atomicSub(&A.bucket_count[key], 1u) - 1u;
+    if (slot < A.np) st_co(A.slot_tmp + slot, i);
+    else atomicOr(A.err, XDEM_ERR_GRID_SYNC);
 }
 
 template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
+__device__ __forceinline__ void order_gather_slot(const StepArgs& A, uint32_t j)
 {
-    if (!*A.rebuild_cur) return;
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= A.nsorted) return;
-    const uint32_t k = A.slot_tmp[j];
-    const uint32_t key = A.keys[k];
-    const uint32_t s = A.bucket_start[key], e = A.bucket_start[key + 1];
+    const uint32_t k = ld_co(A.slot_tmp + j);
+    if (k >= A.np) { atomicOr(A.err, XDEM_ERR_GRID_SYNC); return; }
+    const uint32_t key = ld_co(A.keys + k);
+    if (key >= A.nbucket) { atomicOr(A.err, XDEM_ERR_GRID_SYNC); return; }
+    const uint32_t s = ld_co(A.bucket_start + key), e = min(ld_co(A.bucket_start + key + 1), A.nsorted);
     uint32_t r = 0;
-    for (uint32_t m = s; m < e; m++) r += A.slot_tmp[m] < k ? 1u : 0u;
+    for (uint32_t m = s; m < e; m++) r += ld_co(A.slot_tmp + m) < k ? 1u : 0u;
     const uint32_t f = s + r;
+    if (f >= A.nsorted) { atomicOr(A.err, XDEM_ERR_GRID_SYNC); return; }
     A.svals[f] = k;
     const V4<double> p = ld4(A.pos + 4 * k);
     const SceneDev& S = A.scenes[scene_index(A, k, MULTI)];
@@ -261,11 +319,12 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     // grown by the skin): a sphere with a wall in reach goes to the full contact pass; and
     // the least displacement that could bring a wall into reach, |dp . uw| / max(1, |uw|) -
     // r (1 + 1e-6), the gap k_touch compares with (any cohesive wall: none)
+    const float scale = __uint_as_float(ld_co(reinterpret_cast<const uint32_t*>(&A.lctl->scale)));
     bool far = true;
     double wgap = 3.0e38;
     for (uint32_t w = 0; w < S.nwall; w++) {
         const WallDev& W = A.walls[S.wall_begin + w];
-        far = far & wall_out_of_reach(W.cc, W.info, p, S.skin * A.lctl->scale);
+        far = far & wall_out_of_reach(W.cc, W.info, p, S.skin * scale);
         const xdem_plane_info& pl = W.info;
         const double uw2 = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
         const double wz = (p.x - pl.xw.x) * pl.uw.x + (p.y - pl.xw.y) * pl.uw.y + (p.z - pl.xw.z) * pl.uw.z;
@@ -278,6 +337,38 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(StepArgs A)
     const float4 q = make_float4((float)(p.x - gr.wo_x), (float)(p.y - gr.wo_y), (float)(p.z - gr.wo_z), rf);
     A.disp[k] = 0.0f;   // the path length of the half-skin test starts here
     A.sposf[f] = make_float4(q.x, q.y, q.z, far ? rf : -rf);
+}
+
+// First kernel of every step: the step's counters (block 0), then, on a rebuild only, the
+// bucket grid in four phases separated by grid barriers (grid-stride loops; the other
+// steps return at once, so a step that does not rebuild costs one launch, not six).
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_rebuild(StepArgs A, uint32_t ntile)
+{
+    step_prologue(A);
+    if (!*A.rebuild_cur) return;
+    const uint32_t G = gridDim.x, b = blockIdx.x;
+    const uint32_t stride = G * kBlock;
+    uint32_t gen = 0;
+    if (b == 0 && threadIdx.x < kQueueSegs) A.seg_count[threadIdx.x * kSegStride] = 0u;   // candidate queue
+    for (uint32_t i = b * kBlock + threadIdx.x; i < A.np; i += stride) count_sphere<MULTI>(A, i);
+    if (!grid_sync(A, gen)) return;
+    for (uint32_t t = b; t < ntile; t += G) scan_reduce_tile(A.bucket_count, A.scan_part, t);
+    if (!grid_sync(A, gen)) return;
+    uint32_t* total = A.bucket_start + (size_t)ntile * kScanTile;
+    if (ntile <= kDirectTiles) {
+        for (uint32_t t = b; t < ntile; t += G)
+            scan_apply_tile<true>(A.bucket_count, A.scan_part, A.bucket_start, total, t, ntile);
+    } else {
+        if (b == 0) scan_parts_block(A.scan_part, ntile, total);
+        if (!grid_sync(A, gen)) return;
+        for (uint32_t t = b; t < ntile; t += G)
+            scan_apply_tile<false>(A.bucket_count, A.scan_part, A.bucket_start, total, t, ntile);
+    }
+    if (!grid_sync(A, gen)) return;
+    for (uint32_t i = b * kBlock + threadIdx.x; i < A.np; i += stride) scatter_sphere(A, i);
+    if (!grid_sync(A, gen)) return;
+    for (uint32_t j = b * kBlock + threadIdx.x; j < A.nsorted; j += stride) order_gather_slot<MULTI>(A, j);
 }
 
 }  // namespace
@@ -313,29 +404,28 @@ hipError_t launch_touch(const StepArgs& A, bool multi, hipStream_t s)
 
 hipError_t launch_grid(const StepArgs& A, bool multi, hipStream_t s)
 {
-    const uint32_t nb = bucket_table_size(A.nbucket);
-    const uint32_t ntile = nb / kScanTile;
-    const uint32_t* gate = A.rebuild_cur;
-    // the first kernel also runs when np is 0: it clears the step's counters and flag
-    if (multi) hipLaunchKernelGGL(k_count<true>, grid_for(std::max(A.np, 1u)), dim3(kBlock), 0, s, A);
-    else hipLaunchKernelGGL(k_count<false>, grid_for(std::max(A.np, 1u)), dim3(kBlock), 0, s, A);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(ntile), dim3(kBlock), 0, s, gate, A.bucket_count, A.scan_part,
-                       A.seg_count);
-    if (ntile <= kDirectTiles) {
-        hipLaunchKernelGGL(k_scan_apply<true>, dim3(ntile), dim3(kBlock), 0, s, gate, A.bucket_count, A.scan_part,
-                           A.bucket_start, A.bucket_start + nb);
-    } else {
-        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kPartsBlock), 0, s, gate, A.scan_part, ntile,
-                           A.bucket_start + nb);
-        hipLaunchKernelGGL(k_scan_apply<false>, dim3(ntile), dim3(kBlock), 0, s, gate, A.bucket_count, A.scan_part,
-                           A.bucket_start, A.bucket_start + nb);
-    }
-    if (A.np) {
-        hipLaunchKernelGGL(k_scatter, grid_for(A.np), dim3(kBlock), 0, s, A);
-        if (multi) hipLaunchKernelGGL(k_order_gather<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-        else hipLaunchKernelGGL(k_order_gather<false>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-    }
+    const uint32_t ntile = bucket_table_size(A.nbucket) / kScanTile;
+    if (A.rebuild_blocks == 0) return hipErrorInvalidValue;
+    const dim3 g(A.rebuild_blocks);
+    if (multi) hipLaunchKernelGGL(k_rebuild<true>, g, dim3(kBlock), 0, s, A, ntile);
+    else hipLaunchKernelGGL(k_rebuild<false>, g, dim3(kBlock), 0, s, A, ntile);
     return hipGetLastError();
+}
+
+uint32_t rebuild_grid_size(bool multi)
+{
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    const hipError_t e = multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<true>, kBlock, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<false>, kBlock, 0);
+    if (e != hipSuccess || ncu <= 0 || per <= 0) return 0;
+    // every block resident at once (the barriers need it), at most 4 per CU (the barrier's
+    // counter takes one atomic per block), a multiple of 8 (blocks b and b + G are on the
+    // same XCD)
+    const uint32_t g = std::min((uint32_t)ncu * (uint32_t)per, 4u * (uint32_t)ncu) & ~7u;
+    return g;
 }
 
 }  // namespace xdem
