@@ -47,6 +47,34 @@ def contact_kernel_bytes(cbar: float, fused: bool) -> float:
     return (408.0 if fused else 224.0) + 40.0 * cbar
 
 
+# Algorithmic bytes per sphere of k_lite_move (the translation of every sphere outside the
+# step's full pass): the translational part of the per-sphere state traffic, read pos 32,
+# vel 24, acc 24, mass 8 and write pos 32, vel 24, acc 24.
+LITE_MOVE_BYTES = 168.0
+
+
+def roofline(tm, n, cbar, fuse, info, traffic):
+    """Roofline object of the dominant kernel. Fused mode: k_lite_move, timed with HIP events
+    recorded on the context's stream right before and after it; its units are the spheres
+    outside the full pass (n - contact_spheres of the last step). Split mode: the contact
+    pass (both streams, fork to join)."""
+    steps = max(tm["steps"], 1)
+    if fuse:
+        k_ms = tm["lite_move_ms"] / steps
+        units = n - info["contact_spheres"]
+        achieved = LITE_MOVE_BYTES * units / (k_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                "kernel": "k_lite_move (translation of the spheres outside the full contact pass)",
+                "kernel_ms": k_ms, "bytes_per_unit": LITE_MOVE_BYTES, "units": units}
+    k_ms = tm["contact_ms"] / steps
+    achieved = contact_kernel_bytes(cbar, False) * n / (k_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "kernel": "contact pass: k_collide_lite, k_collide on a second stream (contacts)",
+            "kernel_ms": k_ms, "bytes_per_unit": contact_kernel_bytes(cbar, False), "units": n}
+
+
 def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0, skin=0.0, graph=0):
     import torch
     from xdynamics_dll_amd import DemContext
@@ -125,7 +153,8 @@ def force_error(scene, capacity, nsteps=40):
                    for i in np.nonzero(cnt)[0])
 
     out = {"mixed_force_err": mixed_err, "fp64_force_err": fp64_err, "pairs_exact_mixed": same_pairs(b),
-           "pairs_exact_fp64": same_pairs(c), "contacts_checked": int(r.pp_count.sum()), "state_step": nsteps}
+           "pairs_exact_fp64": same_pairs(c), "contacts_checked": int(r.pp_count.sum()), "state_step": nsteps,
+           "error_words": [b.error_word(), c.error_word()]}
     b.close()
     c.close()
     return out
@@ -150,20 +179,21 @@ def cpu_baseline(scene, capacity, budget_s=12.0):
                       f"FP64 C restatement, OpenMP over spheres"}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_collide_dam_break_1m.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_lite_move_dam_break_1m.json")
 
 
 def measured_traffic(np_, fused):
-    """HBM bytes per step of the contact-pass kernels from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the ratio
-    the same counter shows on a plain device copy of known size), or None when no summary
-    matches this workload."""
+    """HBM bytes per dispatch of k_lite_move from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the ratio the same counter shows
+    on a plain device copy of known size), or None when no summary matches this workload."""
+    if not fused:
+        return None
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    k = t.get("fused" if fused else "split")
+    k = t.get("k_lite_move")
     if not k or t.get("np") != np_:
         return None
     return (k["fetch_kib"] * t["copy_scale_fetch"] + k["write_kib"] * t["copy_scale_write"]) * 1024.0
@@ -207,8 +237,6 @@ def batch_main(args):
     f64_value = n * args.steps / f64["seconds"]
     tm = mixed["timing"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
-    k_ms = tm["contact_ms"] / max(tm["steps"], 1)
-    achieved = contact_kernel_bytes(cbar, fuse) * n / (k_ms * 1e-3) / 1e9
     out = {
         "metric": "particle-steps/s, batched independent scenes",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -219,15 +247,11 @@ def batch_main(args):
                                "dt = min(1e-5 s, 0.2 Rayleigh time) per scene",
                    "np_total": n, "nscene": args.nscene, "nbucket": mixed["info"]["nbucket"],
                    "parallelism": "single GPU, all scenes in one context"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "contact pass (k_lite_move + k_lite_turn, k_collide on a second stream)"
-                               if fuse else "contact pass (k_collide_lite, k_collide on a second stream)",
-                     "kernel_ms": k_ms,
-                     "bytes_per_sphere": contact_kernel_bytes(cbar, fuse)},
+        "roofline": roofline(tm, n, cbar, fuse, mixed["info"], None),
         "mean_contacts_per_sphere": cbar,
-        "grid_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
-        "pair_search_ms_per_step": tm["pairs_ms"] / max(tm["steps"], 1),
+        "contact_pass_ms_per_step": tm["contact_ms"] / max(tm["steps"], 1),
+        "rebuild_kernel_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
+        "touch_ms_per_step": tm["touch_ms"] / max(tm["steps"], 1),
         "list_builds_timed": mixed["info"]["timed_list_builds"],
         "list_spheres": mixed["info"]["list_spheres"],
         "contact_spheres": mixed["info"]["contact_spheres"],
@@ -303,9 +327,6 @@ def main():
     value = n * args.steps / mixed["seconds"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
-    k_ms = tm["contact_ms"] / max(tm["steps"], 1)
-    k_bytes = contact_kernel_bytes(cbar, fuse) * n
-    achieved = k_bytes / (k_ms * 1e-3) / 1e9
     step_gbps = step_bytes(cbar) * n * args.steps / mixed["seconds"] / 1e9
 
     out = {
@@ -325,18 +346,14 @@ def main():
                                "5 walls, mixed FP32 contact arithmetic (FP64 positions/history/sums)",
                    "np": n, "seed": SEED, "p2p_capacity": args.capacity, "parallelism": "single GPU",
                    "nbucket": mixed["info"]["nbucket"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(n, fuse),
-                     "kernel": ("contact pass: k_lite_move + k_lite_turn, k_collide on a second stream "
-                                "(contacts, rolling, velocity update, next drift)" if fuse else
-                                "contact pass: k_collide_lite, k_collide on a second stream (contacts)"),
-                     "kernel_ms": k_ms, "bytes_per_sphere": contact_kernel_bytes(cbar, fuse)},
+        "roofline": roofline(tm, n, cbar, fuse, mixed["info"], measured_traffic(n, fuse)),
         "step_gbps": step_gbps,
         "step_roofline_frac": step_gbps / HBM_PEAK_GBPS,
         "mean_contacts_per_sphere": cbar,
-        "grid_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
+        "contact_pass_ms_per_step": tm["contact_ms"] / max(tm["steps"], 1),
+        "rebuild_kernel_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
+        "touch_ms_per_step": tm["touch_ms"] / max(tm["steps"], 1),
         "finish_ms_per_step": tm["finish_ms"] / max(tm["steps"], 1),
-        "pair_search_ms_per_step": tm["pairs_ms"] / max(tm["steps"], 1),
         "list_builds_timed": mixed["info"]["timed_list_builds"],
         "list_spheres": mixed["info"]["list_spheres"],
         "contact_spheres": mixed["info"]["contact_spheres"],

@@ -202,13 +202,16 @@ int xdem_ctx_stream(xdem_ctx* ctx, void** out);
 int xdem_ctx_graph_capture(xdem_ctx* ctx, unsigned int nsteps);
 /* Launches the captured graph `nlaunch` times. */
 int xdem_ctx_graph_launch(xdem_ctx* ctx, unsigned int nlaunch);
-/* Per-kernel time of the last xdem_ctx_step call, measured with HIP events on the
- * context's stream when timing is on (sums over the steps timed, at most 2048):
- * out[0] = contact kernel (k_collide) ms, out[1] = grid (position update, bucket sort,
- * sorted copies) ms, out[2] = number of steps timed, out[3] = rolling + velocity kernel
- * ms (0 when fused into the contact kernel), out[4] = pair search (k_pairs) ms. */
+/* Phase times of the last xdem_ctx_step call, measured with HIP events on the context's
+ * stream when timing is on (sums over the steps timed, at most 2048; steps are then
+ * launched one by one, never from a graph):
+ * out[0] = contact passes (after k_touch until both streams joined) ms, out[1] = k_rebuild
+ * (the step's counters; bucket grid, sorted copies and neighbour lists on a rebuild) ms,
+ * out[2] = number of steps timed, out[3] = rolling + velocity kernel ms (0 when fused
+ * into the contact passes), out[4] = per-step pair test (k_touch) ms, out[5] = k_lite_move
+ * ms (fused mode; 0 otherwise). */
 int xdem_ctx_set_timing(xdem_ctx* ctx, int on);
-int xdem_ctx_timing(xdem_ctx* ctx, double out[5]);
+int xdem_ctx_timing(xdem_ctx* ctx, double out[6]);
 
 #ifdef __cplusplus
 }

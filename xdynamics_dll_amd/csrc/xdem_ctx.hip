@@ -135,7 +135,7 @@ struct xdem_ctx {
 
     bool timing = false;
     std::vector<hipEvent_t> events;  // kEventsPerStep per timed step
-    double t_contact = 0.0, t_grid = 0.0, t_finish = 0.0, t_pairs = 0.0;
+    double t_contact = 0.0, t_grid = 0.0, t_finish = 0.0, t_pairs = 0.0, t_lite_move = 0.0;
     unsigned t_steps = 0;
     unsigned long long steps_done = 0;
 
@@ -210,7 +210,18 @@ struct xdem_ctx {
         if ((e = hipStreamWaitEvent(stream2, fork_ev, 0)) != hipSuccess) return e;
         if ((e = collide_fn(A, multi, fuse, model, gpu_law, 1, stream2)) != hipSuccess) return e;
         if ((e = hipEventRecord(join_ev, stream2)) != hipSuccess) return e;
-        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 0, stream)) != hipSuccess) return e;
+        if (ev && fuse) {
+            // timed steps: k_lite_move bracketed by its own events
+            if ((e = hipEventRecord(ev[5], stream)) != hipSuccess) return e;
+            launch_lite_fused(A, multi, stream, 1);
+            if ((e = hipEventRecord(ev[6], stream)) != hipSuccess) return e;
+            launch_lite_fused(A, multi, stream, 2);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        } else {
+            if (ev && ((e = hipEventRecord(ev[5], stream)) != hipSuccess || (e = hipEventRecord(ev[6], stream)) != hipSuccess))
+                return e;
+            if ((e = collide_fn(A, multi, fuse, model, gpu_law, 0, stream)) != hipSuccess) return e;
+        }
         if ((e = hipStreamWaitEvent(stream, join_ev, 0)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[3], stream)) != hipSuccess) return e;
         if (nwslot) {
@@ -289,7 +300,7 @@ struct xdem_ctx {
     }
 };
 
-constexpr unsigned kEventsPerStep = 5;
+constexpr unsigned kEventsPerStep = 7;
 
 static int hip_status(hipError_t e) { return e == hipSuccess ? XDEM_OK : XDEM_E_HIP; }
 
@@ -693,18 +704,21 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
         c->t_grid = 0.0;
         c->t_finish = 0.0;
         c->t_pairs = 0.0;
+        c->t_lite_move = 0.0;
         c->t_steps = std::min(nsteps, kmax);
         for (unsigned k = 0; k < c->t_steps; k++) {
             const hipEvent_t* ev = &c->events[kEventsPerStep * k];
-            float a = 0.f, p = 0.f, b = 0.f, f = 0.f;
+            float a = 0.f, p = 0.f, b = 0.f, f = 0.f, m = 0.f;
             CTX_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
             CTX_CHECK(hipEventElapsedTime(&p, ev[1], ev[2]));
             CTX_CHECK(hipEventElapsedTime(&b, ev[2], ev[3]));
             CTX_CHECK(hipEventElapsedTime(&f, ev[3], ev[4]));
+            CTX_CHECK(hipEventElapsedTime(&m, ev[5], ev[6]));
             c->t_grid += a;
             c->t_pairs += p;
             c->t_contact += b;
             c->t_finish += f;
+            c->t_lite_move += m;
         }
     }
     return XDEM_OK;
@@ -1016,9 +1030,10 @@ int xdem_ctx_set_timing(xdem_ctx* c, int on)
     return XDEM_OK;
 }
 
-int xdem_ctx_timing(xdem_ctx* c, double out[5])
+int xdem_ctx_timing(xdem_ctx* c, double out[6])
 {
     if (!c || !out) return XDEM_E_INVALID;
+    out[5] = c->t_lite_move;
     out[0] = c->t_contact;
     out[1] = c->t_grid;
     out[2] = (double)c->t_steps;

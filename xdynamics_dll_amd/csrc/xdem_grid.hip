@@ -421,10 +421,12 @@ uint32_t rebuild_grid_size(bool multi)
     const hipError_t e = multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<true>, kBlock, 0)
                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<false>, kBlock, 0);
     if (e != hipSuccess || ncu <= 0 || per <= 0) return 0;
-    // every block resident at once (the barriers need it), at most 4 per CU (the barrier's
-    // counter takes one atomic per block), a multiple of 8 (blocks b and b + G are on the
-    // same XCD)
-    const uint32_t g = std::min((uint32_t)ncu * (uint32_t)per, 4u * (uint32_t)ncu) & ~7u;
+    // every block resident at once (the barriers need it): two per CU at most, and at most
+    // a third of the occupancy limit, so that the blocks also find room next to other
+    // kernels running on the device (up to three contexts' k_rebuild at once: rebuilds
+    // waiting on each other's CU slots could only end by the barrier's time limit); a
+    // multiple of 8 (blocks b and b + G are on the same XCD)
+    const uint32_t g = std::min((uint32_t)ncu * (uint32_t)per / 3u, 2u * (uint32_t)ncu) & ~7u;
     return g;
 }
 

@@ -1002,7 +1002,8 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
 // lite pass, pass 1 the full pass. The other kernels are FP64 only.
 using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, int pass,
                                      hipStream_t);
-void launch_lite_fused(const StepArgs&, bool multi, hipStream_t);   // k_lite_move + k_lite_turn
+// which: 0 = k_lite_move then k_lite_turn, 1 = k_lite_move only, 2 = k_lite_turn only
+void launch_lite_fused(const StepArgs&, bool multi, hipStream_t, int which = 0);
 constexpr unsigned kQueueChunks = 16;   // blocks per queue segment of k_touch
 CollideLaunch collide_launch_f64();
 hipError_t launch_pairs(const StepArgs&, bool multi, hipStream_t);

@@ -66,14 +66,16 @@ hipError_t collide_f64(const StepArgs& A, bool multi, bool fuse, int model, bool
 
 CollideLaunch collide_launch_f64() { return collide_f64; }
 
-void launch_lite_fused(const StepArgs& A, bool multi, hipStream_t s)
+void launch_lite_fused(const StepArgs& A, bool multi, hipStream_t s, int which)
 {
-    if (multi) {
-        hipLaunchKernelGGL(k_lite_move<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
-        hipLaunchKernelGGL(k_lite_turn<true>, grid_for(A.np), dim3(kBlock), 0, s, A);
-    } else {
-        hipLaunchKernelGGL(k_lite_move<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
-        hipLaunchKernelGGL(k_lite_turn<false>, grid_for(A.np), dim3(kBlock), 0, s, A);
+    const dim3 g = grid_for(A.np), b(kBlock);
+    if (which != 2) {
+        if (multi) hipLaunchKernelGGL(k_lite_move<true>, g, b, 0, s, A);
+        else hipLaunchKernelGGL(k_lite_move<false>, g, b, 0, s, A);
+    }
+    if (which != 1) {
+        if (multi) hipLaunchKernelGGL(k_lite_turn<true>, g, b, 0, s, A);
+        else hipLaunchKernelGGL(k_lite_turn<false>, g, b, 0, s, A);
     }
 }
 

@@ -124,10 +124,10 @@ class DemContext:
         _lib.check(self.L.xdem_ctx_set_timing(self._h, 1 if on else 0), "xdem_ctx_set_timing")
 
     def timing(self):
-        out = (C.c_double * 5)()
+        out = (C.c_double * 6)()
         _lib.check(self.L.xdem_ctx_timing(self._h, out), "xdem_ctx_timing")
         return {"contact_ms": out[0], "grid_ms": out[1], "steps": int(out[2]), "finish_ms": out[3],
-                "pairs_ms": out[4]}
+                "touch_ms": out[4], "lite_move_ms": out[5]}
 
     def close(self) -> None:
         if self._h:
