@@ -81,7 +81,7 @@ struct xdem_ctx {
     uint32_t *keys = nullptr, *svals = nullptr, *slot_tmp = nullptr;
     uint32_t *nbr_count = nullptr, *nbr = nullptr, *work = nullptr, *seg_count = nullptr;
     uint32_t *work2 = nullptr, *seg_count2 = nullptr;
-    uint32_t* grid_bar = nullptr;                        // k_rebuild's barrier counter
+    uint32_t* grid_bar = nullptr;                        // k_rebuild's phase cursors and flags
     uint32_t rebuild_blocks = 0;                         // k_rebuild's grid
     unsigned long long* pass_mask = nullptr;
     uint32_t seg_cap = 0;
@@ -527,7 +527,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->work2, sizeof(uint32_t) * kQueueSegs * c->seg_cap));
     CTX_CHECK(c->alloc(&c->pass_mask, sizeof(unsigned long long) * n));
     CTX_CHECK(c->alloc(&c->seg_count2, sizeof(uint32_t) * kQueueSegs * kSegStride));
-    CTX_CHECK(c->alloc(&c->grid_bar, sizeof(uint32_t) * kSegStride * (1 + kBarFlags)));
+    CTX_CHECK(c->alloc(&c->grid_bar, sizeof(uint32_t) * kSegStride * kBarSlots));
     c->rebuild_blocks = rebuild_grid_size(c->multi);
     if (c->rebuild_blocks == 0) return XDEM_E_HIP;
     CTX_CHECK(c->alloc(&c->nbr, sizeof(uint32_t) * n * c->list_cap));
@@ -622,7 +622,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, c->rebuild, 1u, (size_t)2);   // the first step builds
     CTX_CHECK(hipMemsetAsync(c->seg_count, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
     CTX_CHECK(hipMemsetAsync(c->seg_count2, 0, sizeof(uint32_t) * kQueueSegs * kSegStride, s));
-    CTX_CHECK(hipMemsetAsync(c->grid_bar, 0, sizeof(uint32_t) * kSegStride * (1 + kBarFlags), s));
+    CTX_CHECK(hipMemsetAsync(c->grid_bar, 0, sizeof(uint32_t) * kSegStride * kBarSlots, s));
     CTX_CHECK(hipMemsetAsync(c->nbr_count, 0, sizeof(uint32_t) * n, s));
     if (c->nwslot) CTX_CHECK(hipMemsetAsync(c->wrench_out, 0, sizeof(double) * 6 * c->nwslot, s));
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nbt + 255) / 256)), dim3(256), 0, s, c->bucket_count, 0u, nbt);

@@ -84,9 +84,8 @@ __device__ __forceinline__ void st_co(uint32_t* p, uint32_t v)
 }
 
 // Every step, block 0: clears the step's contact counters, full-pass queue and the next
-// step's rebuild flag, and counts the step for the adaptive skin (on a rebuild: the skin of
-// the lists it makes, from how long the last ones lasted).
-__device__ __forceinline__ void step_prologue(const StepArgs& A)
+// step's rebuild flag.
+__device__ __forceinline__ void step_counters(const StepArgs& A)
 {
     if (blockIdx.x != 0) return;
     if (threadIdx.x < 2) A.counters[threadIdx.x] = 0ull;
@@ -94,74 +93,78 @@ __device__ __forceinline__ void step_prologue(const StepArgs& A)
         A.seg_count2[threadIdx.x * kSegStride] = 0u;
         A.dmax_next[threadIdx.x * kSegStride] = 0u;
     }
-    if (threadIdx.x == 0) {
-        *A.rebuild_next = 0u;
-        ListCtl& L = *A.lctl;
-        L.step += 1u;
-        if (*A.rebuild_cur) {
-            const uint32_t lasted = L.step - L.last_build;
-            if (L.adaptive && L.last_build != 0u) {
-                float sc = L.scale;
-                if (lasted < 16u) sc = fminf(1.0f, 2.0f * sc);
-                else if (lasted > 64u) sc = fmaxf(L.min_scale, 0.5f * sc);
-                st_co(reinterpret_cast<uint32_t*>(&L.scale), __float_as_uint(sc));   // read by order_gather_slot
-            }
-            L.last_build = L.step;
-            *A.n_rebuild += 1ull;
-        }
-    }
+    if (threadIdx.x == 0) *A.rebuild_next = 0u;
 }
 
-// Barrier of all blocks of k_rebuild (all resident at once: the grid is sized by the
-// occupancy). Every wave waits for its stores to complete (they are written through), then
-// thread 0 of each block arrives on the counter grid_bar[0]; the block whose arrival
-// completes the gen-th round writes gen to kBarFlags release flags (one per 128-byte line,
-// one wave store); every block waits on flag blockIdx mod kBarFlags (a few blocks per
-// address, so the waits do not crowd the counter). k_touch, the next kernel of the step,
-// sets the counter and the flags back to zero. The wait is bounded (kSyncSpins tries of
-// ~1 us): beyond it the barrier flags XDEM_ERR_GRID_SYNC and returns false, and the block
-// leaves the kernel, so the kernel always ends.
-constexpr uint32_t kSyncSpins = 1u << 20;
-
-__device__ __forceinline__ bool grid_sync(const StepArgs& A, uint32_t& gen)
+// Every step, one thread: counts the step for the adaptive skin; on a rebuild, sets the skin
+// of the lists it makes from how long the last ones lasted (read by the order phase).
+__device__ __forceinline__ void list_ctl_update(const StepArgs& A, bool rebuild)
 {
-    __shared__ uint32_t passed;
-    gen += 1u;
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        uint32_t last = 0;
-        if (threadIdx.x == 0) last = atomicAdd(A.grid_bar, 1u) + 1u == gen * gridDim.x ? 1u : 0u;
-        last = __shfl(last, 0, 64);
-        if (last) st_co(A.grid_bar + kSegStride * (1u + threadIdx.x), gen);
-        if (threadIdx.x == 0) {
-            const uint32_t* flag = A.grid_bar + kSegStride * (1u + blockIdx.x % kBarFlags);
-            uint32_t tries = 0;
-            bool ok = false;
-            while (true) {
-                ok = ld_co(flag) >= gen;
-                if (ok || ++tries == kSyncSpins) break;
-                __builtin_amdgcn_s_sleep(2);
-            }
-            if (!ok) atomicOr(A.err, XDEM_ERR_GRID_SYNC);
-            passed = ok ? 1u : 0u;
+    ListCtl& L = *A.lctl;
+    L.step += 1u;
+    if (!rebuild) return;
+    const uint32_t lasted = L.step - L.last_build;
+    if (L.adaptive && L.last_build != 0u) {
+        float sc = L.scale;
+        if (lasted < 16u) sc = fminf(1.0f, 2.0f * sc);
+        else if (lasted > 64u) sc = fmaxf(L.min_scale, 0.5f * sc);
+        st_co(reinterpret_cast<uint32_t*>(&L.scale), __float_as_uint(sc));
+    }
+    L.last_build = L.step;
+    *A.n_rebuild += 1ull;
+}
+
+// The phases of a rebuild hand out their work in units (kUnit spheres, or one scan tile)
+// from an atomic cursor per phase; a block claims units until none is left, then waits for
+// the phase to be complete: every unit done, counted on a second counter per phase. The
+// block that finishes the last unit writes the phase's number to kBarFlags release flags
+// (one per 128-byte line, one wave store), and each block waits on flag blockIdx mod
+// kBarFlags. A block only ever waits for units that running blocks have claimed, so the
+// phases complete whether or not every block of the grid is resident (other kernels or
+// other contexts may hold the device). Every wave waits for its stores to complete (they are
+// written through) before its unit is counted. k_touch, the next kernel of the step, sets the
+// cursors, counters and flags back to zero. The wait is bounded (kSyncSpins tries of ~1 us):
+// beyond it XDEM_ERR_GRID_SYNC is set and the block leaves the kernel, so it always ends.
+constexpr uint32_t kSyncSpins = 1u << 20;
+constexpr uint32_t kUnit = 4 * kBlock;
+
+template <class F>
+__device__ __forceinline__ bool run_phase(const StepArgs& A, uint32_t k, uint32_t nunits, F&& body)
+{
+    if (nunits == 0) return true;
+    __shared__ uint32_t claim, passed;
+    uint32_t* cursor = A.grid_bar + kSegStride * k;
+    uint32_t* done = A.grid_bar + kSegStride * (kRebuildPhases + k);
+    while (true) {
+        __syncthreads();   // every thread has read the last claim
+        if (threadIdx.x == 0) claim = atomicAdd(cursor, 1u);
+        __syncthreads();
+        const uint32_t u = claim;
+        if (u >= nunits) break;
+        body(u);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            uint32_t last = 0;
+            if (threadIdx.x == 0) last = atomicAdd(done, 1u) + 1u == nunits ? 1u : 0u;
+            last = __shfl(last, 0, 64);
+            if (last) st_co(A.grid_bar + kSegStride * (2 * kRebuildPhases + threadIdx.x), k + 1u);
         }
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t* flag = A.grid_bar + kSegStride * (2 * kRebuildPhases + blockIdx.x % kBarFlags);
+        uint32_t tries = 0;
+        bool ok = false;
+        while (true) {
+            ok = ld_co(flag) >= k + 1u;
+            if (ok || ++tries == kSyncSpins) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (!ok) atomicOr(A.err, XDEM_ERR_GRID_SYNC);
+        passed = ok ? 1u : 0u;
     }
     __syncthreads();
     return passed != 0u;
-}
-
-// Bucket key of sphere i and its count (posted atomic add).
-template <bool MULTI>
-__device__ __forceinline__ void count_sphere(const StepArgs& A, uint32_t i)
-{
-    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-    const V4<double> p = ld4(A.pos + 4 * i);
-    int gx, gy, gz;
-    grid_cell(S.grid, p.x, p.y, p.z, gx, gy, gz);
-    const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
-    st_co(A.keys + i, key);
-    atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: the scatter takes the rank
 }
 
 // Sum of tile t (4096 buckets) into part[t]; lane-contiguous loads (coalesced).
@@ -237,7 +240,7 @@ __device__ __forceinline__ void scan_parts_block(uint32_t* part, uint32_t nparts
 // to *total. Otherwise part holds the exclusive prefix of the tile sums (scan_parts_block).
 template <bool DIRECT>
 __device__ __forceinline__ void scan_apply_tile(const uint32_t* count, const uint32_t* part, uint32_t* start,
-                                                uint32_t* total, uint32_t t, uint32_t ntile)
+                                                uint32_t* total, uint32_t t, uint32_t ntile, uint32_t* tile)
 {
     __shared__ uint32_t wsum[kBlock / 64];
     __shared__ uint32_t red[kBlock / 64];
@@ -249,9 +252,9 @@ __device__ __forceinline__ void scan_apply_tile(const uint32_t* count, const uin
     } else {
         tile0 = ld_co(part + t);
     }
-    // the tile through LDS: coalesced loads and stores, kScanItems consecutive buckets per
-    // thread for the serial part (one pad word per kScanItems: conflict-free rows)
-    __shared__ uint32_t tile[kScanTile + kScanTile / kScanItems];
+    // the tile through LDS (tile: kScanTile + kScanTile / kScanItems words): coalesced loads
+    // and stores, kScanItems consecutive buckets per thread for the serial part (one pad
+    // word per kScanItems: conflict-free rows)
     const size_t base = (size_t)t * kScanTile;
 #pragma unroll
     for (int k = 0; k < (int)kScanItems; k++) {
@@ -286,6 +289,30 @@ __device__ __forceinline__ void scan_apply_tile(const uint32_t* count, const uin
         st_co(start + base + q, tile[q + q / kScanItems]);
     }
     __syncthreads();
+}
+
+// Count phase unit u: spheres u * kUnit + r * kBlock + thread, r < 4, their positions read
+// together.
+template <bool MULTI>
+__device__ __forceinline__ void count_unit(const StepArgs& A, uint32_t u)
+{
+    V4<double> p[4];
+    uint32_t idx[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        idx[r] = u * kUnit + r * kBlock + threadIdx.x;
+        p[r] = ld4(A.pos + 4 * (size_t)min(idx[r], A.np - 1u));
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        if (idx[r] >= A.np) continue;
+        const SceneDev& S = A.scenes[scene_index(A, idx[r], MULTI)];
+        int gx, gy, gz;
+        grid_cell(S.grid, p[r].x, p[r].y, p[r].z, gx, gy, gz);
+        const uint32_t key = grid_bucket(S.grid, gx, gy, gz);
+        st_co(A.keys + idx[r], key);
+        atomicAdd(&A.bucket_count[key], 1u);   // result not waited for: the scatter takes the rank
+    }
 }
 
 // Provisional slot of sphere i: counting the bucket back down gives each sphere a distinct
@@ -340,35 +367,48 @@ __device__ __forceinline__ void order_gather_slot(const StepArgs& A, uint32_t j)
 }
 
 // First kernel of every step: the step's counters (block 0), then, on a rebuild only, the
-// bucket grid in four phases separated by grid barriers (grid-stride loops; the other
-// steps return at once, so a step that does not rebuild costs one launch, not six).
+// bucket grid in phases (work units claimed from per-phase cursors; the other steps return
+// at once, so a step that does not rebuild costs one launch, not six).
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_rebuild(StepArgs A, uint32_t ntile)
 {
-    step_prologue(A);
-    if (!*A.rebuild_cur) return;
-    const uint32_t G = gridDim.x, b = blockIdx.x;
-    const uint32_t stride = G * kBlock;
-    uint32_t gen = 0;
-    if (b == 0 && threadIdx.x < kQueueSegs) A.seg_count[threadIdx.x * kSegStride] = 0u;   // candidate queue
-    for (uint32_t i = b * kBlock + threadIdx.x; i < A.np; i += stride) count_sphere<MULTI>(A, i);
-    if (!grid_sync(A, gen)) return;
-    for (uint32_t t = b; t < ntile; t += G) scan_reduce_tile(A.bucket_count, A.scan_part, t);
-    if (!grid_sync(A, gen)) return;
-    uint32_t* total = A.bucket_start + (size_t)ntile * kScanTile;
-    if (ntile <= kDirectTiles) {
-        for (uint32_t t = b; t < ntile; t += G)
-            scan_apply_tile<true>(A.bucket_count, A.scan_part, A.bucket_start, total, t, ntile);
-    } else {
-        if (b == 0) scan_parts_block(A.scan_part, ntile, total);
-        if (!grid_sync(A, gen)) return;
-        for (uint32_t t = b; t < ntile; t += G)
-            scan_apply_tile<false>(A.bucket_count, A.scan_part, A.bucket_start, total, t, ntile);
+    step_counters(A);
+    const bool rebuild = *A.rebuild_cur != 0u;
+    if (!rebuild) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) list_ctl_update(A, false);
+        return;
     }
-    if (!grid_sync(A, gen)) return;
-    for (uint32_t i = b * kBlock + threadIdx.x; i < A.np; i += stride) scatter_sphere(A, i);
-    if (!grid_sync(A, gen)) return;
-    for (uint32_t j = b * kBlock + threadIdx.x; j < A.nsorted; j += stride) order_gather_slot<MULTI>(A, j);
+    if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) A.seg_count[threadIdx.x * kSegStride] = 0u;   // candidate queue
+    const uint32_t sp_units = (A.np + kUnit - 1) / kUnit;
+    if (!run_phase(A, 0, sp_units, [&](uint32_t u) {
+            if (u == 0 && threadIdx.x == 0) list_ctl_update(A, true);   // before the order phase reads the skin
+            count_unit<MULTI>(A, u);
+        }))
+        return;
+    if (!run_phase(A, 1, ntile, [&](uint32_t t) { scan_reduce_tile(A.bucket_count, A.scan_part, t); })) return;
+    __shared__ uint32_t tile[kScanTile + kScanTile / kScanItems];
+    uint32_t* total = A.bucket_start + (size_t)ntile * kScanTile;
+    const bool direct = ntile <= kDirectTiles;
+    if (!run_phase(A, 2, direct ? 0u : 1u, [&](uint32_t) { scan_parts_block(A.scan_part, ntile, total); })) return;
+    if (!run_phase(A, 3, ntile, [&](uint32_t t) {
+            if (direct) scan_apply_tile<true>(A.bucket_count, A.scan_part, A.bucket_start, total, t, ntile, tile);
+            else scan_apply_tile<false>(A.bucket_count, A.scan_part, A.bucket_start, total, t, ntile, tile);
+        }))
+        return;
+    if (!run_phase(A, 4, sp_units, [&](uint32_t u) {
+            for (uint32_t r = 0; r < 4; r++) {
+                const uint32_t i = u * kUnit + r * kBlock + threadIdx.x;
+                if (i < A.np) scatter_sphere(A, i);
+            }
+        }))
+        return;
+    const uint32_t sl_units = (A.nsorted + kUnit - 1) / kUnit;
+    run_phase(A, 5, sl_units, [&](uint32_t u) {
+        for (uint32_t r = 0; r < 4; r++) {
+            const uint32_t j = u * kUnit + r * kBlock + threadIdx.x;
+            if (j < A.nsorted) order_gather_slot<MULTI>(A, j);
+        }
+    });
 }
 
 }  // namespace
@@ -421,12 +461,9 @@ uint32_t rebuild_grid_size(bool multi)
     const hipError_t e = multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<true>, kBlock, 0)
                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<false>, kBlock, 0);
     if (e != hipSuccess || ncu <= 0 || per <= 0) return 0;
-    // every block resident at once (the barriers need it): two per CU at most, and at most
-    // a third of the occupancy limit, so that the blocks also find room next to other
-    // kernels running on the device (up to three contexts' k_rebuild at once: rebuilds
-    // waiting on each other's CU slots could only end by the barrier's time limit); a
-    // multiple of 8 (blocks b and b + G are on the same XCD)
-    const uint32_t g = std::min((uint32_t)ncu * (uint32_t)per / 3u, 2u * (uint32_t)ncu) & ~7u;
+    // two blocks per CU (the phases do not need them all resident: work goes to the blocks
+    // that run; more blocks only add claims on the phase cursors: measured slower)
+    const uint32_t g = std::min((uint32_t)ncu * (uint32_t)per, 2u * (uint32_t)ncu);
     return g;
 }
 

@@ -31,7 +31,9 @@ constexpr uint32_t kCountMask = 0x03ffffffu;
 // kQueueSegs, so the appends of different waves rarely hit the same counter.
 constexpr uint32_t kQueueSegs = 64;
 constexpr uint32_t kSegStride = 32;           // uint32 entries between two segment counters
-constexpr uint32_t kBarFlags = 64;            // release flags of k_rebuild's grid barrier
+constexpr uint32_t kBarFlags = 64;            // release flags of k_rebuild's phases
+constexpr uint32_t kRebuildPhases = 6;        // k_rebuild: count, reduce, parts, apply, scatter, order
+constexpr uint32_t kBarSlots = 2 * kRebuildPhases + kBarFlags;   // cursors, done counters, flags
 // Entries per segment for nsorted sorted slots launched in blocks of 256 (the most a
 // segment can receive: 64 spheres from each of its waves).
 inline uint32_t queue_segment_cap(uint32_t nsorted)
@@ -124,9 +126,9 @@ struct StepArgs {
     uint32_t* seg_count;           // spheres per candidate queue segment (kSegStride apart)
     uint32_t* seg_count2;          // spheres per full-pass queue segment (kSegStride apart)
     uint32_t seg_cap;              // entries per queue segment in work and work2
-    uint32_t* grid_bar;            // k_rebuild's barrier: counter, kBarFlags release flags, each
-                                   // on its own 128-byte line (zero between steps)
-    uint32_t rebuild_blocks;       // k_rebuild's grid: blocks that are all resident at once
+    uint32_t* grid_bar;            // k_rebuild's phase cursors, done counters and release flags,
+                                   // kBarSlots, each on its own 128-byte line (zero between steps)
+    uint32_t rebuild_blocks;       // k_rebuild's grid
     uint32_t np, nsorted, cap;
     uint32_t nwslot, nbucket;
     uint32_t list_cap;                       // neighbour-list entries per sorted slot
@@ -460,8 +462,8 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
     uint32_t i_next = base + threadIdx.x < A.seg_cap ? wq[base + threadIdx.x] : 0u;
     const uint32_t cnt = A.seg_count[seg * kSegStride];
     const uint32_t lane = threadIdx.x & 63u;
-    // k_rebuild's barrier counter and release flags start at zero
-    if (blockIdx.x == 0 && threadIdx.x <= kBarFlags) A.grid_bar[kSegStride * threadIdx.x] = 0u;
+    // k_rebuild's phase cursors, counters and release flags start at zero
+    if (blockIdx.x == 0 && threadIdx.x < kBarSlots) A.grid_bar[kSegStride * threadIdx.x] = 0u;
     // no sphere has moved more than dm since the build (0 on a build step)
     const bool built = *A.rebuild_cur != 0u;
     __shared__ float dm_sh;
