@@ -112,8 +112,8 @@ typedef struct xdem_quirks {
 #define XDEM_ERR_MESH_LIST      0x10u /* more than 20 triangle candidates of one kind */
 #define XDEM_ERR_NBR_OVERFLOW   0x20u /* more near neighbours than the pair-search list holds
                                          (step layer; the contacts beyond it are missed) */
-#define XDEM_ERR_GRID_SYNC      0x40u /* a grid barrier of the list rebuild waited ~1 s and gave
-                                         up (step layer; the step's lists are incomplete) */
+#define XDEM_ERR_GRID_SYNC      0x40u /* a phase of the list rebuild did not complete within ~1 s
+                                         (step layer; the step's lists are incomplete) */
 
 /* Status codes returned by every C entry point. */
 #define XDEM_OK              0

@@ -8,9 +8,9 @@
 // moved half the skin since the last build. Every kernel below returns at once on the other
 // steps. The drift of a step (position and orientation update, impl:135-147) is computed at
 // the end of the previous step's velocity update; k_drift does it when no previous step
-// left it. A rebuild is one kernel, k_rebuild, in phases separated by grid barriers (every
-// block is resident; the kernel's first lines, on every step, clear the step's counters,
-// and the steps that do not rebuild return right after them):
+// left it. A rebuild is one kernel, k_rebuild, in phases (each waits until the previous one
+// is complete; the kernel's first lines, on every step, clear the step's counters, and the
+// steps that do not rebuild return right after them):
 //   count             bucket key of every sphere and its count (posted atomic add)
 //   scan              exclusive prefix of the counts, tile sums then tiles: start[b] =
 //                     first sorted slot of bucket b, start[nbucket] = N (each tile adds up
@@ -71,8 +71,8 @@ __global__ __launch_bounds__(kBlock) void k_drift(StepArgs A)
 
 // k_rebuild's phases hand data to blocks on other XCDs, whose L2 caches are not coherent
 // with each other. Everything one phase writes for a later one is stored and loaded with
-// agent-scope atomic accesses (written through to memory, read from it), so the barrier
-// needs no cache write-back or invalidation. Loads of data a phase may have left stale are
+// agent-scope atomic accesses (written through to memory, read from it), so the wait
+// between phases needs no cache write-back or invalidation. Loads of data a phase may have left stale are
 // also checked against the array bounds (XDEM_ERR_GRID_SYNC instead of a wild access).
 __device__ __forceinline__ uint32_t ld_co(const uint32_t* p)
 {
