@@ -118,11 +118,15 @@ typedef struct xdem_ctx_config {
                                     Results do not depend on it. */
     int step_graph;              /* > 0: xdem_ctx_step replays a HIP graph of this many steps
                                     (made even) for runs at least that long, which saves the
-                                    host's launch time when steps are short; the graph runs the
-                                    full and lite contact passes one after the other, so it
-                                    loses their overlap when many spheres are in contact.
-                                    0 (default): every kernel launched from the host. Results
-                                    do not depend on it. */
+                                    host's launch time and the gaps between kernels when steps
+                                    are short (the full and lite contact passes stay concurrent
+                                    branches). 0 (default): every kernel launched from the host.
+                                    Results do not depend on it. */
+    unsigned int rebuild_wait_limit; /* tries (~1 us each) a block of the list rebuild waits for
+                                    a phase to complete before it gives up, sets
+                                    XDEM_ERR_GRID_SYNC and leaves the kernel (0 = 2^20, about
+                                    a second); the step's lists are then not rebuilt. Small
+                                    values exist to test that path. */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */

@@ -180,3 +180,22 @@ def test_contexts_stepping_concurrently_match_sequential_runs():
     for c in ctxs[:2]:
         for f in FIELDS:
             assert np.array_equal(c.read(f), ref[f]), f
+
+
+def test_rebuild_wait_limit_flags_the_error_and_every_kernel_ends():
+    """With a wait limit of one try, the blocks of the list rebuild that find no work unit left
+    give up at once instead of waiting for the phase: the step sets XDEM_ERR_GRID_SYNC, keeps
+    the lists of the last build, and every kernel still ends (the partial build reads and
+    writes in bounds). The same scene with the default limit sets nothing."""
+    from xdynamics_dll_amd._lib import ERR_GRID_SYNC
+
+    scene = shaken_pack(n=1500, seed=13)
+    bad = DemContext(scene, precision="fp64", fuse_velocity=True, skin=-1.0, rebuild_wait_limit=1)
+    bad.step(5)
+    bad.synchronize()
+    assert bad.error_word() & ERR_GRID_SYNC
+    assert np.isfinite(bad.read("pos")).all()
+    bad.close()
+    good = DemContext(scene, precision="fp64", fuse_velocity=True, skin=-1.0)
+    good.step(5)
+    assert good.error_word() == 0

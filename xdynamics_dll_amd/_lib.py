@@ -24,6 +24,7 @@ ERR_MESH_OVERFLOW = 0x4
 ERR_NONFINITE = 0x8
 ERR_MESH_LIST = 0x10
 ERR_NBR_OVERFLOW = 0x20
+ERR_GRID_SYNC = 0x40
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -103,7 +104,7 @@ class CtxConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("p2p_capacity", C.c_uint), ("quirks", Quirks),
                 ("block_size", C.c_uint), ("grid_dim_max", C.c_uint), ("fuse_velocity", C.c_int),
                 ("device", C.c_int), ("buckets_per_sphere", C.c_uint), ("skin", C.c_double),
-                ("step_graph", C.c_int)]
+                ("step_graph", C.c_int), ("rebuild_wait_limit", C.c_uint)]
 
 
 class CtxInfo(C.Structure):

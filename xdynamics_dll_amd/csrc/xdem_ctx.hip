@@ -161,7 +161,7 @@ struct xdem_ctx {
         A.bucket_count = bucket_count; A.bucket_start = bucket_start; A.scan_part = scan_part;
         A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap; A.work = work;
         A.seg_count = seg_count; A.seg_cap = seg_cap; A.work2 = work2; A.seg_count2 = seg_count2;
-        A.grid_bar = grid_bar; A.rebuild_blocks = rebuild_blocks;
+        A.grid_bar = grid_bar; A.rebuild_blocks = rebuild_blocks; A.rebuild_wait_limit = cfg.rebuild_wait_limit;
         A.pass_mask = pass_mask;
         A.sposf = sposf;
         A.rebuild_cur = rebuild + rbp; A.rebuild_next = rebuild + (rbp ^ 1); A.n_rebuild = n_rebuild;

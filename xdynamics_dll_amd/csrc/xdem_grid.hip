@@ -123,8 +123,10 @@ __device__ __forceinline__ void list_ctl_update(const StepArgs& A, bool rebuild)
 // phases complete whether or not every block of the grid is resident (other kernels or
 // other contexts may hold the device). Every wave waits for its stores to complete (they are
 // written through) before its unit is counted. k_touch, the next kernel of the step, sets the
-// cursors, counters and flags back to zero. The wait is bounded (kSyncSpins tries of ~1 us):
-// beyond it XDEM_ERR_GRID_SYNC is set and the block leaves the kernel, so it always ends.
+// cursors, counters and flags back to zero. The wait is bounded (kSyncSpins tries of ~1 us,
+// or the context's rebuild_wait_limit): beyond it XDEM_ERR_GRID_SYNC and the failed word are
+// set and the block leaves the kernel, so it always ends; every index a later phase reads
+// from an earlier one is bounds-checked, so a partial build reads and writes in bounds.
 constexpr uint32_t kSyncSpins = 1u << 20;
 constexpr uint32_t kUnit = 4 * kBlock;
 
@@ -155,12 +157,16 @@ __device__ __forceinline__ bool run_phase(const StepArgs& A, uint32_t k, uint32_
         const uint32_t* flag = A.grid_bar + kSegStride * (2 * kRebuildPhases + blockIdx.x % kBarFlags);
         uint32_t tries = 0;
         bool ok = false;
+        const uint32_t limit = A.rebuild_wait_limit ? A.rebuild_wait_limit : kSyncSpins;
         while (true) {
             ok = ld_co(flag) >= k + 1u;
-            if (ok || ++tries == kSyncSpins) break;
+            if (ok || ++tries >= limit) break;
             __builtin_amdgcn_s_sleep(2);
         }
-        if (!ok) atomicOr(A.err, XDEM_ERR_GRID_SYNC);
+        if (!ok) {
+            atomicOr(A.err, XDEM_ERR_GRID_SYNC);
+            st_co(A.grid_bar + kSegStride * kRebuildFailed, 1u);
+        }
         passed = ok ? 1u : 0u;
     }
     __syncthreads();

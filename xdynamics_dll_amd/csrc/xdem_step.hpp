@@ -33,7 +33,10 @@ constexpr uint32_t kQueueSegs = 64;
 constexpr uint32_t kSegStride = 32;           // uint32 entries between two segment counters
 constexpr uint32_t kBarFlags = 64;            // release flags of k_rebuild's phases
 constexpr uint32_t kRebuildPhases = 6;        // k_rebuild: count, reduce, parts, apply, scatter, order
-constexpr uint32_t kBarSlots = 2 * kRebuildPhases + kBarFlags;   // cursors, done counters, flags
+// k_rebuild's slots: phase cursors, done counters, release flags, and a word set when a phase
+// gave up waiting (k_pairs then leaves the lists of the last build as they are)
+constexpr uint32_t kRebuildFailed = 2 * kRebuildPhases + kBarFlags;
+constexpr uint32_t kBarSlots = kRebuildFailed + 1;
 // Entries per segment for nsorted sorted slots launched in blocks of 256 (the most a
 // segment can receive: 64 spheres from each of its waves).
 inline uint32_t queue_segment_cap(uint32_t nsorted)
@@ -129,6 +132,7 @@ struct StepArgs {
     uint32_t* grid_bar;            // k_rebuild's phase cursors, done counters and release flags,
                                    // kBarSlots, each on its own 128-byte line (zero between steps)
     uint32_t rebuild_blocks;       // k_rebuild's grid
+    uint32_t rebuild_wait_limit;   // tries before a k_rebuild block gives up waiting on a phase
     uint32_t np, nsorted, cap;
     uint32_t nwslot, nbucket;
     uint32_t list_cap;                       // neighbour-list entries per sorted slot
@@ -173,7 +177,9 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 {
-    if (!*A.rebuild_cur) return;
+    // no build this step, or one whose phases did not complete (the sorted copies may be
+    // partial: the lists of the last build stay)
+    if (!*A.rebuild_cur || A.grid_bar[kSegStride * kRebuildFailed]) return;
     const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const uint32_t i = j < A.nsorted ? A.svals[j] : kInvalid;
     uint32_t n = 0;

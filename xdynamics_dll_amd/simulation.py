@@ -18,7 +18,8 @@ from .scene import PARITY_QUIRKS, SceneData
 class DemContext:
     def __init__(self, scenes, precision: str = "fp64", p2p_capacity: int = 12, quirks: dict | None = None,
                  grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False,
-                 buckets_per_sphere: int = 0, skin: float = 0.0, step_graph: int = 0):
+                 buckets_per_sphere: int = 0, skin: float = 0.0, step_graph: int = 0,
+                 rebuild_wait_limit: int = 0):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -35,6 +36,7 @@ class DemContext:
         cfg.buckets_per_sphere = buckets_per_sphere
         cfg.skin = skin   # 0: adaptive (1.25-20% of the smallest radius); < 0: lists rebuilt every step
         cfg.step_graph = step_graph   # > 0: step() replays HIP graphs of that many steps; 0: eager launches
+        cfg.rebuild_wait_limit = rebuild_wait_limit   # 0: ~1 s before a rebuild phase gives up
         self.precision = precision
         self.capacity = p2p_capacity
         self._h = C.c_void_p()
