@@ -200,7 +200,6 @@ struct xdem_ctx {
         }
         if ((e = launch_grid(A, multi, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
-        if ((e = launch_pairs(A, multi, stream)) != hipSuccess) return e;
         if ((e = launch_touch(A, multi, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[2], stream)) != hipSuccess) return e;
         // the full pass (spheres k_touch queued) on the second stream, concurrently with the

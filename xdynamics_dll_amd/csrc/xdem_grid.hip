@@ -24,8 +24,8 @@
 //                     the sphere's wall-reach flag (FP64 test, reach + skin) in the sign of
 //                     the radius; the wall gap of the sphere (k_touch); restarts the
 //                     sphere's path length of the half-skin test
-// The pair search k_pairs and the per-step pair test k_touch (xdem_step.hpp) are launched
-// from here too.
+// The per-step pair test k_touch (xdem_step.hpp), which also does the pair search on a
+// build step, is launched from here too.
 // The bucket of slot range [start[b], start[b+1]) needs no clearing between steps, and a
 // row of three consecutive buckets is the single range [start[b], start[b+3]).
 // Built with -ffp-contract=off: the drift is bit-identical to the reference's.
@@ -432,16 +432,9 @@ hipError_t launch_drift(const StepArgs& A, bool multi, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_pairs(const StepArgs& A, bool multi, hipStream_t s)
-{
-    if (A.nsorted == 0) return hipSuccess;
-    if (multi) hipLaunchKernelGGL(k_pairs<true>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-    else hipLaunchKernelGGL(k_pairs<false>, grid_for(A.nsorted), dim3(kBlock), 0, s, A);
-    return hipGetLastError();
-}
-
 hipError_t launch_touch(const StepArgs& A, bool multi, hipStream_t s)
 {
+    // kQueueChunks blocks per queue segment (a build step loops over the sorted slots)
     const dim3 g(kQueueSegs * kQueueChunks);
     if (multi) hipLaunchKernelGGL(k_touch<true>, g, dim3(kBlock), 0, s, A);
     else hipLaunchKernelGGL(k_touch<false>, g, dim3(kBlock), 0, s, A);

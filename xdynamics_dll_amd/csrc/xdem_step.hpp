@@ -34,7 +34,7 @@ constexpr uint32_t kSegStride = 32;           // uint32 entries between two segm
 constexpr uint32_t kBarFlags = 64;            // release flags of k_rebuild's phases
 constexpr uint32_t kRebuildPhases = 6;        // k_rebuild: count, reduce, parts, apply, scatter, order
 // k_rebuild's slots: phase cursors, done counters, release flags, and a word set when a phase
-// gave up waiting (k_pairs then leaves the lists of the last build as they are)
+// gave up waiting (k_touch then leaves the lists of the last build as they are)
 constexpr uint32_t kRebuildFailed = 2 * kRebuildPhases + kBarFlags;
 constexpr uint32_t kBarSlots = kRebuildFailed + 1;
 // Entries per segment for nsorted sorted slots launched in blocks of 256 (the most a
@@ -100,7 +100,7 @@ struct StepArgs {
     uint32_t* scan_part;                     // per 4096-bucket tile sums / offsets
     uint32_t* slot_tmp;                      // sphere id per provisional slot
     uint32_t* svals;                         // sphere id per sorted slot
-    uint32_t* work;                          // candidate spheres (k_pairs), by queue segment
+    uint32_t* work;                          // candidate spheres (pair search), by queue segment
     uint32_t* work2;                         // spheres of this step's full contact pass (k_touch)
     unsigned long long* pass_mask;           // per sphere of work2: list entries that pass (k_touch)
     uint32_t* nbr_count;                     // listed neighbours per sphere | kCandidate, kWallNear, kTouch,
@@ -175,98 +175,78 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
 // The traversal works on bucket coordinates, i.e. the bucket-grid cell coordinates masked to
 // the grid, exactly as grid_bucket(grid_cell(position)).
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_pairs(StepArgs A)
+__device__ __forceinline__ uint32_t pairs_slot(const StepArgs& A, uint32_t j, uint32_t i)
 {
-    // no build this step, or one whose phases did not complete (the sorted copies may be
-    // partial: the lists of the last build stay)
-    if (!*A.rebuild_cur || A.grid_bar[kSegStride * kRebuildFailed]) return;
-    const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const uint32_t i = j < A.nsorted ? A.svals[j] : kInvalid;
     uint32_t n = 0;
-    bool full = false;
     float pgap = 3.0e38f;
-    if (i < A.np) {
-        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-        const float4 fi = A.sposf[j];
-        if (S.has_pp) {
-            // the sphere's bucket coordinates from its key (dims are powers of two)
-            const Grid& G = S.grid;
-            const uint32_t kb = A.keys[i] - G.base;
-            const uint32_t bx = kb & G.mask_x;
-            const uint32_t by = (kb >> __builtin_ctz(G.dim_x)) & G.mask_y;
-            const uint32_t bz = kb >> __builtin_ctz(G.dim_xy);
-            const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
-            // lim = r_j (1 + 1e-6) + [(r_i + reach)(1 + 1e-6) + mg]: the same bound with the
-            // sphere's own part hoisted (each FP32 rounding is far below the 1e-6 margin)
-            const float limi = ((fabsf(fi.w) + (float)(S.cc.coh_reach + S.skin * A.lctl->scale)) * 1.000001f) + mg;
-            const float limr = ((fabsf(fi.w) + (float)S.cc.coh_reach) * 1.000001f) + 2.0f * mg + 1e-9f;
-            const uint32_t lc = A.list_cap;
-            uint32_t* out = A.nbr + (size_t)i * lc;
-            const uint32_t bx0 = (bx - 1u) & G.mask_x;
-            const uint32_t n1 = min(3u, G.dim_x - bx0);   // buckets before the wrap
-            uint32_t row = 0, t = 0, s1 = 0, len1 = 0, s2 = 0, total = 0;
-            auto load_row = [&](uint32_t r) {
-                const uint32_t rb = G.base + ((bz + r / 3u - 1u) & G.mask_z) * G.dim_xy +
-                                    ((by + r % 3u - 1u) & G.mask_y) * G.dim_x;
-                s1 = A.bucket_start[rb + bx0];
-                const uint32_t e1 = A.bucket_start[rb + bx0 + n1];
-                s2 = A.bucket_start[rb];
-                const uint32_t e2 = A.bucket_start[rb + 3 - n1];
-                len1 = e1 - s1;
-                total = len1 + (e2 - s2);
-                t = 0;
-            };
-            // candidate m with FP32 copy f, in visiting order; pgap: the least FP32 distance
-            // a listed pair lacks to pass, less the rounding bound
-            auto test = [&](uint32_t m, float4 f) {
-                if (m == j) return;
-                const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
-                const float rj = fabsf(f.w) * 1.000001f;
-                const float lim = rj + limi;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                if (d2 > lim * lim) return;
-                pgap = fminf(pgap, sqrtf(d2) - (rj + limr));
-                if (n < lc) out[n] = A.svals[m];
-                n++;
-            };
-            load_row(0);
-            while (true) {
-                if (t < total) {
-                    // two candidates per iteration (both loads in flight), tested in order
-                    const uint32_t m0 = t < len1 ? s1 + t : s2 + (t - len1);
-                    const bool two = t + 1u < total;
-                    const uint32_t t1 = t + 1u;
-                    const uint32_t m1 = two ? (t1 < len1 ? s1 + t1 : s2 + (t1 - len1)) : m0;
-                    t += two ? 2u : 1u;
-                    const float4 f0 = A.sposf[m0];
-                    const float4 f1 = A.sposf[m1];
-                    test(m0, f0);
-                    if (two) test(m1, f1);
-                } else {
-                    if (++row == 9u) break;
-                    load_row(row);
-                }
+    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    const float4 fi = A.sposf[j];
+    if (S.has_pp) {
+        // the sphere's bucket coordinates from its key (dims are powers of two)
+        const Grid& G = S.grid;
+        const uint32_t kb = A.keys[i] - G.base;
+        const uint32_t bx = kb & G.mask_x;
+        const uint32_t by = (kb >> __builtin_ctz(G.dim_x)) & G.mask_y;
+        const uint32_t bz = kb >> __builtin_ctz(G.dim_xy);
+        const float mg = 2.4e-7f * (fabsf(fi.x) + fabsf(fi.y) + fabsf(fi.z)) + 1e-9f;
+        // lim = r_j (1 + 1e-6) + [(r_i + reach)(1 + 1e-6) + mg]: the same bound with the
+        // sphere's own part hoisted (each FP32 rounding is far below the 1e-6 margin)
+        const float limi = ((fabsf(fi.w) + (float)(S.cc.coh_reach + S.skin * A.lctl->scale)) * 1.000001f) + mg;
+        const float limr = ((fabsf(fi.w) + (float)S.cc.coh_reach) * 1.000001f) + 2.0f * mg + 1e-9f;
+        const uint32_t lc = A.list_cap;
+        uint32_t* out = A.nbr + (size_t)i * lc;
+        const uint32_t bx0 = (bx - 1u) & G.mask_x;
+        const uint32_t n1 = min(3u, G.dim_x - bx0);   // buckets before the wrap
+        uint32_t row = 0, t = 0, s1 = 0, len1 = 0, s2 = 0, total = 0;
+        auto load_row = [&](uint32_t r) {
+            const uint32_t rb = G.base + ((bz + r / 3u - 1u) & G.mask_z) * G.dim_xy +
+                                ((by + r % 3u - 1u) & G.mask_y) * G.dim_x;
+            s1 = A.bucket_start[rb + bx0];
+            const uint32_t e1 = A.bucket_start[rb + bx0 + n1];
+            s2 = A.bucket_start[rb];
+            const uint32_t e2 = A.bucket_start[rb + 3 - n1];
+            len1 = e1 - s1;
+            total = len1 + (e2 - s2);
+            t = 0;
+        };
+        // candidate m with FP32 copy f, in visiting order; pgap: the least FP32 distance
+        // a listed pair lacks to pass, less the rounding bound
+        auto test = [&](uint32_t m, float4 f) {
+            if (m == j) return;
+            const float dx = f.x - fi.x, dy = f.y - fi.y, dz = f.z - fi.z;
+            const float rj = fabsf(f.w) * 1.000001f;
+            const float lim = rj + limi;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > lim * lim) return;
+            pgap = fminf(pgap, sqrtf(d2) - (rj + limr));
+            if (n < lc) out[n] = A.svals[m];
+            n++;
+        };
+        load_row(0);
+        while (true) {
+            if (t < total) {
+                // two candidates per iteration (both loads in flight), tested in order
+                const uint32_t m0 = t < len1 ? s1 + t : s2 + (t - len1);
+                const bool two = t + 1u < total;
+                const uint32_t t1 = t + 1u;
+                const uint32_t m1 = two ? (t1 < len1 ? s1 + t1 : s2 + (t1 - len1)) : m0;
+                t += two ? 2u : 1u;
+                const float4 f0 = A.sposf[m0];
+                const float4 f1 = A.sposf[m1];
+                test(m0, f0);
+                if (two) test(m1, f1);
+            } else {
+                if (++row == 9u) break;
+                load_row(row);
             }
         }
-        // candidates: spheres with listed neighbours or a wall in reach + skin (k_order_gather
-        // marks the latter with the sign of the FP32 radius); k_touch tests them every step
-        const bool wall_near = (__float_as_uint(fi.w) >> 31) != 0u;
-        full = n != 0u || wall_near;
-        A.nbr_count[i] = n | (full ? kCandidate : 0u) | (wall_near ? kWallNear : 0u);
-        if (n) A.gap[i] = fminf(A.gap[i], pgap);
     }
-    // queue for the full pass: one atomic per wave, issued by its first active lane, on the
-    // wave's segment counter
-    const unsigned long long mask = __ballot(full);
-    if (mask) {
-        const uint32_t lane = threadIdx.x & 63u;
-        const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
-        const uint32_t seg = (j >> 6) & (kQueueSegs - 1u);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&A.seg_count[seg * kSegStride], (uint32_t)__popcll(mask));
-        base = __shfl(base, (int)leader, 64);
-        if (full) A.work[(size_t)seg * A.seg_cap + base + __popcll(mask & ((1ull << lane) - 1ull))] = i;
-    }
+    // candidates: spheres with listed neighbours or a wall in reach + skin (the order phase
+    // marks the latter with the sign of the FP32 radius); k_touch tests them every step
+    const bool wall_near = (__float_as_uint(fi.w) >> 31) != 0u;
+    const bool full = n != 0u || wall_near;
+    if (n) A.gap[i] = fminf(A.gap[i], pgap);
+    return n | (full ? kCandidate : 0u) | (wall_near ? kWallNear : 0u);
 }
 
 // Drift of the next step (impl:135-147; xIntegratorVV.cpp:66-100): Euler parameters and
@@ -447,19 +427,111 @@ __device__ __forceinline__ bool pair_passes(const ContactConst& cc, double ir, d
     return fabs(g.cdist) < fabs(coh_s);
 }
 
-// Every step, for the candidates of the last build: does a listed pair pass the exact
-// predicate (in neighbouring reference cells), or is a wall in reach (wall_out_of_reach)?
-// Those spheres get kTouch and go to this step's full pass (queue work2, appended within the
-// candidate's queue segment, so a segment never holds more than seg_cap), with a bit mask of
-// the passing list entries (lists hold at most 64); the others are completed by the lite
-// pass, as the full pass would add only zero sums for them. The list is sorted again
-// (order_list) when the keys of its passing pairs are not increasing, i.e. a sphere has
-// changed cells since the build. Candidates whose gap exceeds their displacement plus the
-// largest displacement since the build cannot touch and are not tested. Block b walks
-// segment b mod kQueueSegs.
+// Exact test of candidate i with list flags fl: walls in reach, the listed pairs that pass
+// (in the reference's visiting order, the list re-sorted when broken), their mask. built: the
+// lists were made this step (no displacement since); dm: the largest displacement since the
+// build.
+template <bool MULTI>
+__device__ __forceinline__ bool touch_sphere(const StepArgs& A, uint32_t i, uint32_t fl, bool built, float dm)
+{
+    bool touch = false;
+    // a pair can pass only once the two spheres have moved its gap together, a wall come
+    // into reach only once the sphere has moved its gap
+    const float di = built ? 0.0f : A.disp[i];
+    const bool may = !(A.gap[i] > (di + dm) * 1.0001f + 1e-9f);
+    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    const V4<double> p = ld4(A.pos + 4 * i);
+    if (may && (fl & kWallNear))
+        for (uint32_t w = 0; w < S.nwall; w++) {
+            const WallDev& W = A.walls[S.wall_begin + w];
+            touch = touch || !wall_out_of_reach(W.cc, W.info, p);
+        }
+    const uint32_t n = may ? min(fl & kCountMask, A.list_cap) : 0u;
+    if (n) {
+        int ix, iy, iz;
+        ref_cell(S, p, ix, iy, iz);
+        uint32_t* lst = A.nbr + (size_t)i * A.list_cap;
+        uint32_t prev = 0;
+        bool have = false, ordered = true;
+        unsigned long long m = 0;
+        for (uint32_t t = 0; t < n; t++) {
+            const uint32_t k = lst[t];
+            const V4<double> pj = ld4(A.pos + 4 * (size_t)k);
+            if (!pair_passes(S.cc, p.w, pj.w, pair_geom(p, pj))) continue;
+            const uint32_t code = visit_code(S, ix, iy, iz, pj);
+            if (code == kNotVisited) continue;
+            m |= 1ull << t;
+            const uint32_t key = (code << kIdBits) | k;
+            if (have && key <= prev) ordered = false;
+            prev = key;
+            have = true;
+        }
+        if (!ordered) {
+            // sort, then find the passing entries at their new places
+            order_list(A, S, lst, n, ix, iy, iz);
+            m = 0;
+            for (uint32_t t = 0; t < n; t++) {
+                const V4<double> pj = ld4(A.pos + 4 * (size_t)lst[t]);
+                if (pair_passes(S.cc, p.w, pj.w, pair_geom(p, pj)) && visit_code(S, ix, iy, iz, pj) != kNotVisited)
+                    m |= 1ull << t;
+            }
+        }
+        touch = touch || m != 0ull;
+        A.pass_mask[i] = m;
+    }
+    return touch;
+}
+
+// Appends the lanes with `on` set to queue segment seg (one atomic per wave, by its first
+// active lane).
+__device__ __forceinline__ void queue_append(uint32_t* q, uint32_t* count, uint32_t seg, uint32_t cap, bool on,
+                                             uint32_t i)
+{
+    const unsigned long long mask = __ballot(on);
+    if (!mask) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+    uint32_t b = 0;
+    if (lane == leader) b = atomicAdd(&count[seg * kSegStride], (uint32_t)__popcll(mask));
+    b = __shfl(b, (int)leader, 64);
+    if (on) q[(size_t)seg * cap + b + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+}
+
+// Every step, the spheres of this step's full pass. On a build step (the rebuild kernel's
+// phases completed), one thread per sorted slot: the pair search for the slot's sphere
+// (pairs_slot: its list, list gap and candidate flags, the candidate queue work), then the
+// exact test of a candidate on the list it has just written. On the other steps, the
+// candidates of the last build from their queue (block b walks segment b mod kQueueSegs):
+// candidates whose gap exceeds their displacement plus the largest displacement since the
+// build cannot touch and are not tested. Touched spheres get kTouch, the pass mask, and a
+// place in the full-pass queue work2 (the segment of their candidate queue entry, so a
+// segment never holds more than seg_cap); the others are completed by the lite pass, as the
+// full pass would add only zero sums for them. The grid has a multiple of kQueueSegs
+// blocks.
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_touch(StepArgs A)
 {
+    const bool built = *A.rebuild_cur != 0u;
+    if (built && A.grid_bar[kSegStride * kRebuildFailed] == 0u) {
+        // virtual blocks of 256 sorted slots, block vb on the XCD of physical block vb mod
+        // gridDim (gridDim is a multiple of 8)
+        const uint32_t nvb = (A.nsorted + blockDim.x - 1) / blockDim.x;
+        for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+            const uint32_t j = xcd_block(vb, nvb) * blockDim.x + threadIdx.x;
+            const uint32_t i = j < A.nsorted ? A.svals[j] : kInvalid;
+            bool full = false, touch = false;
+            if (i < A.np) {
+                const uint32_t fl = pairs_slot<MULTI>(A, j, i);
+                full = (fl & kCandidate) != 0u;
+                if (full) touch = touch_sphere<MULTI>(A, i, fl, true, 0.0f);
+                A.nbr_count[i] = touch ? (fl | kTouch) : fl;
+            }
+            const uint32_t seg = (j >> 6) & (kQueueSegs - 1u);
+            queue_append(A.work, A.seg_count, seg, A.seg_cap, full, i);
+            queue_append(A.work2, A.seg_count2, seg, A.seg_cap, touch, i);
+        }
+        return;
+    }
     const uint32_t seg = blockIdx.x % kQueueSegs;
     const uint32_t nchunk = gridDim.x / kQueueSegs;
     const uint32_t* wq = A.work + (size_t)seg * A.seg_cap;
@@ -467,11 +539,8 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
     // the first entry is read together with the segment's count
     uint32_t i_next = base + threadIdx.x < A.seg_cap ? wq[base + threadIdx.x] : 0u;
     const uint32_t cnt = A.seg_count[seg * kSegStride];
-    const uint32_t lane = threadIdx.x & 63u;
-    // k_rebuild's phase cursors, counters and release flags start at zero
-    if (blockIdx.x == 0 && threadIdx.x < kBarSlots) A.grid_bar[kSegStride * threadIdx.x] = 0u;
+    if (base >= cnt) return;
     // no sphere has moved more than dm since the build (0 on a build step)
-    const bool built = *A.rebuild_cur != 0u;
     __shared__ float dm_sh;
     if (threadIdx.x < 64) {
         float d = built ? 0.0f : __uint_as_float(A.dmax_cur[threadIdx.x * kSegStride]);
@@ -488,62 +557,12 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
         if (e + nchunk * blockDim.x < cnt) i_next = wq[e + nchunk * blockDim.x];
         if (e < cnt) {
             const uint32_t fl = A.nbr_count[i];
-            // a pair can pass only once the two spheres have moved its gap together, a wall
-            // come into reach only once the sphere has moved its gap
-            const float di = built ? 0.0f : A.disp[i];
-            const bool may = !(A.gap[i] > (di + dm) * 1.0001f + 1e-9f);
-            const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-            const V4<double> p = ld4(A.pos + 4 * i);
-            if (may && (fl & kWallNear))
-                for (uint32_t w = 0; w < S.nwall; w++) {
-                    const WallDev& W = A.walls[S.wall_begin + w];
-                    touch = touch || !wall_out_of_reach(W.cc, W.info, p);
-                }
-            const uint32_t n = may ? min(fl & kCountMask, A.list_cap) : 0u;
-            if (n) {
-                int ix, iy, iz;
-                ref_cell(S, p, ix, iy, iz);
-                uint32_t* lst = A.nbr + (size_t)i * A.list_cap;
-                uint32_t prev = 0;
-                bool have = false, ordered = true;
-                unsigned long long m = 0;
-                for (uint32_t t = 0; t < n; t++) {
-                    const uint32_t k = lst[t];
-                    const V4<double> pj = ld4(A.pos + 4 * (size_t)k);
-                    if (!pair_passes(S.cc, p.w, pj.w, pair_geom(p, pj))) continue;
-                    const uint32_t code = visit_code(S, ix, iy, iz, pj);
-                    if (code == kNotVisited) continue;
-                    m |= 1ull << t;
-                    const uint32_t key = (code << kIdBits) | k;
-                    if (have && key <= prev) ordered = false;
-                    prev = key;
-                    have = true;
-                }
-                if (!ordered) {
-                    // sort, then find the passing entries at their new places
-                    order_list(A, S, lst, n, ix, iy, iz);
-                    m = 0;
-                    for (uint32_t t = 0; t < n; t++) {
-                        const V4<double> pj = ld4(A.pos + 4 * (size_t)lst[t]);
-                        if (pair_passes(S.cc, p.w, pj.w, pair_geom(p, pj)) &&
-                            visit_code(S, ix, iy, iz, pj) != kNotVisited)
-                            m |= 1ull << t;
-                    }
-                }
-                touch = touch || m != 0ull;
-                A.pass_mask[i] = m;
-            }
-            // a touched sphere's rotation is written by the full pass: no longer known stable
+            touch = touch_sphere<MULTI>(A, i, fl, built, dm);
+            // a touched sphere's rotation, acceleration and contact counts are written by the
+            // full pass: no longer known stable
             A.nbr_count[i] = touch ? ((fl & ~kLiteStable) | kTouch) : (fl & ~kTouch);
         }
-        const unsigned long long mask = __ballot(touch);
-        if (mask) {
-            const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
-            uint32_t b2 = 0;
-            if (lane == leader) b2 = atomicAdd(&A.seg_count2[seg * kSegStride], (uint32_t)__popcll(mask));
-            b2 = __shfl(b2, (int)leader, 64);
-            if (touch) A.work2[(size_t)seg * A.seg_cap + b2 + __popcll(mask & ((1ull << lane) - 1ull))] = i;
-        }
+        queue_append(A.work2, A.seg_count2, seg, A.seg_cap, touch, i);
     }
 }
 
@@ -594,7 +613,7 @@ __device__ __forceinline__ void sphere_tail(const StepArgs& A, const SceneDev& S
     }
 }
 
-// Full contact pass of sphere i: the near neighbours k_pairs listed, in their order, under
+// Full contact pass of sphere i: the near neighbours the pair search listed, in their order, under
 // the exact FP64 predicate; every wall of the scene in id order; then sphere_tail. The
 // sphere's own state and outputs are read and written by id.
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
@@ -789,6 +808,14 @@ __device__ __forceinline__ void lite_sphere(const StepArgs& A, const SceneDev& S
     if (err) atomicOr(A.err, err);
 }
 
+// k_rebuild's phase cursors, counters, release flags and failed word start at zero: cleared
+// by block 0 of the lite pass (after k_touch has read the failed word, before the next step's
+// k_rebuild).
+__device__ __forceinline__ void reset_rebuild_slots(const StepArgs& A)
+{
+    if (blockIdx.x == 0 && threadIdx.x < kBarSlots) A.grid_bar[kSegStride * threadIdx.x] = 0u;
+}
+
 // The lite kernels run over every sphere outside this step's full pass (skip kTouch), one
 // per thread.
 template <class F>
@@ -802,6 +829,7 @@ __device__ __forceinline__ void lite_spheres(const StepArgs& A, F&& body)
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide_lite(StepArgs A)
 {
+    reset_rebuild_slots(A);
     uint32_t n_wall = 0;
     lite_spheres(A, [&](uint32_t i, uint32_t skip) {
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
@@ -821,6 +849,7 @@ __global__ __launch_bounds__(256) void k_collide_lite(StepArgs A)
 template <bool MULTI>
 __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
 {
+    reset_rebuild_slots(A);
     uint32_t n_wall = 0;
     float dmx = 0.0f;
     lite_spheres(A, [&](uint32_t i, uint32_t skip) {
@@ -1014,14 +1043,13 @@ using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int
 void launch_lite_fused(const StepArgs&, bool multi, hipStream_t, int which = 0);
 constexpr unsigned kQueueChunks = 16;   // blocks per queue segment of k_touch
 CollideLaunch collide_launch_f64();
-hipError_t launch_pairs(const StepArgs&, bool multi, hipStream_t);
 CollideLaunch collide_launch_mixed();
 // xdem_grid.hip
 uint32_t bucket_table_size(uint32_t nbucket);   // bucket_count / bucket_start entries (+1 for start)
 hipError_t launch_drift(const StepArgs&, bool multi, hipStream_t);
 hipError_t launch_grid(const StepArgs&, bool multi, hipStream_t);   // k_rebuild, every step
 uint32_t rebuild_grid_size(bool multi);   // k_rebuild's grid on the current device (0: error)
-hipError_t launch_touch(const StepArgs&, bool multi, hipStream_t);   // k_touch, every step
+hipError_t launch_touch(const StepArgs&, bool multi, hipStream_t);   // k_touch (with the pair search), every step
 hipError_t launch_finish(const StepArgs&, bool multi, hipStream_t);
 
 }  // namespace xdem
