@@ -161,9 +161,9 @@ def test_bucket_table_size_does_not_change_results():
 
 
 def test_contexts_stepping_concurrently_match_sequential_runs():
-    """Three contexts whose steps are enqueued back to back (their rebuild kernels, each with
-    grid barriers, run concurrently on the device) give the same bits as one context alone,
-    with no barrier time-out in the error word."""
+    """Three contexts whose steps are enqueued back to back (their rebuild kernels, each
+    waiting between its phases, run concurrently on the device) give the same bits as one
+    context alone, with no time-out in the error word."""
     scene = shaken_pack(n=1500, seed=12)
     alone = DemContext(scene, precision="fp64", fuse_velocity=True, skin=-1.0)
     alone.step(30)

@@ -164,10 +164,11 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
 // buckets around the sphere's hold them all. The three buckets of an x-row are consecutive
 // keys, or two runs when the row wraps (the buckets up to the end of the row, then those
 // from its start). One flat loop walks all nine rows, so each lane advances through its own
-// candidates without waiting for the other lanes row by row. Only an FP32 test runs here
-// (few registers, many waves per SIMD to hide the latency of the candidate loads); every
-// step k_touch applies the exact FP64 predicate to the lists and puts a list in the
-// reference's visiting order when its passing pairs are not in that order.
+// candidates without waiting for the other lanes row by row. Only an FP32 test runs here;
+// every step k_touch (whose build-step branch calls this) applies the exact FP64 predicate to
+// the lists and puts a list in the reference's visiting order when its passing pairs are not
+// in that order. Returns the sphere's list length and candidate flags; writes the list and
+// lowers the sphere's gap to the least FP32 distance a listed pair lacks to pass.
 //
 // FP32 test: a candidate is dropped only when its FP32 distance exceeds r_i + r_j + reach +
 // skin by more than the rounding of the FP32 coordinates (|x| 2^-24 per coordinate and
