@@ -343,18 +343,19 @@ int xdem_ctx_create(const xdem_ctx_config* cfg, xdem_ctx** out)
 int xdem_ctx_destroy(xdem_ctx* c)
 {
     if (!c) return XDEM_OK;
-    hipSetDevice(c->cfg.device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->gexec) hipGraphExecDestroy(c->gexec);
-    if (c->graph) hipGraphDestroy(c->graph);
-    if (c->agexec) hipGraphExecDestroy(c->agexec);
-    if (c->agraph) hipGraphDestroy(c->agraph);
-    for (hipEvent_t e : c->events) hipEventDestroy(e);
-    for (void* p : c->allocs) hipFree(p);
-    if (c->fork_ev) hipEventDestroy(c->fork_ev);
-    if (c->join_ev) hipEventDestroy(c->join_ev);
-    if (c->stream2) hipStreamDestroy(c->stream2);
-    if (c->stream) hipStreamDestroy(c->stream);
+    // teardown: a failing call leaves nothing to undo, so the codes are not checked
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);   // the second stream joins into it
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->agexec) (void)hipGraphExecDestroy(c->agexec);
+    if (c->agraph) (void)hipGraphDestroy(c->agraph);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return XDEM_OK;
 }
@@ -650,7 +651,7 @@ static int capture_steps(xdem_ctx* c, unsigned nsteps, hipGraph_t* graph, hipGra
     c->set_pos_parity(par[2]);
     c->drift_valid = true;
     if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);   // already failing: report the capture error
         return XDEM_E_HIP;
     }
     *graph = g;
@@ -994,8 +995,9 @@ int xdem_ctx_graph_capture(xdem_ctx* c, unsigned int nsteps)
     if (!c->committed) return XDEM_E_NOT_READY;
     if (nsteps == 0 || (nsteps & 1u)) return XDEM_E_INVALID;  // even: history buffers return to the same parity
     CTX_CHECK(hipSetDevice(c->cfg.device));
-    if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-    if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+    // the old graph is replaced; destroying it cannot fail in a way the caller could act on
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
     int par[4];
     const int rc = capture_steps(c, nsteps, &c->graph, &c->gexec, par);
     if (rc != XDEM_OK) return rc;
