@@ -115,9 +115,10 @@ __device__ __forceinline__ void list_ctl_update(const StepArgs& A, bool rebuild)
 }
 
 // The phases of a rebuild hand out their work in units (kUnit spheres, or one scan tile)
-// from an atomic cursor per phase; a block claims units until none is left, then waits for
-// the phase to be complete: every unit done, counted on a second counter per phase. The
-// block that finishes the last unit writes the phase's number to kBarFlags release flags
+// from atomic cursors (kClaimGroups ranges of units per phase, each with its own cursor);
+// a block claims units until none is left, then waits for the phase to be complete: every
+// unit done, counted per range, and the completed ranges counted per phase. The block that
+// finishes the last unit writes the phase's number to kBarFlags release flags
 // (one per 128-byte line, one wave store), and each block waits on flag blockIdx mod
 // kBarFlags. A block only ever waits for units that running blocks have claimed, so the
 // phases complete whether or not every block of the grid is resident (other kernels or
@@ -135,26 +136,37 @@ __device__ __forceinline__ bool run_phase(const StepArgs& A, uint32_t k, uint32_
 {
     if (nunits == 0) return true;
     __shared__ uint32_t claim, passed;
-    uint32_t* cursor = A.grid_bar + kSegStride * k;
-    uint32_t* done = A.grid_bar + kSegStride * (kRebuildPhases + k);
-    while (true) {
+    // kClaimGroups equal ranges of units, each with its own cursor and done counter (fewer
+    // atomics per address); a block starts on range blockIdx mod kClaimGroups and moves on to
+    // the next range once one is exhausted
+    uint32_t nonempty = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < kClaimGroups; c++) nonempty += (c + 1u) * nunits / kClaimGroups > c * nunits / kClaimGroups;
+    uint32_t c = blockIdx.x % kClaimGroups, tried = 0;
+    while (tried < kClaimGroups) {
+        const uint32_t lo = c * nunits / kClaimGroups, hi = (c + 1u) * nunits / kClaimGroups;
         __syncthreads();   // every thread has read the last claim
-        if (threadIdx.x == 0) claim = atomicAdd(cursor, 1u);
+        if (threadIdx.x == 0) claim = lo < hi ? lo + atomicAdd(A.grid_bar + kSegStride * bar_cursor(k, c), 1u) : hi;
         __syncthreads();
         const uint32_t u = claim;
-        if (u >= nunits) break;
+        if (u >= hi) {
+            c = (c + 1u) % kClaimGroups;
+            tried++;
+            continue;
+        }
         body(u);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (threadIdx.x < 64) {
             uint32_t last = 0;
-            if (threadIdx.x == 0) last = atomicAdd(done, 1u) + 1u == nunits ? 1u : 0u;
+            if (threadIdx.x == 0 && atomicAdd(A.grid_bar + kSegStride * bar_done(k, c), 1u) + 1u == hi - lo)
+                last = atomicAdd(A.grid_bar + kSegStride * bar_groups(k), 1u) + 1u == nonempty ? 1u : 0u;
             last = __shfl(last, 0, 64);
-            if (last) st_co(A.grid_bar + kSegStride * (2 * kRebuildPhases + threadIdx.x), k + 1u);
+            if (last) st_co(A.grid_bar + kSegStride * bar_flag(threadIdx.x), k + 1u);
         }
     }
     if (threadIdx.x == 0) {
-        const uint32_t* flag = A.grid_bar + kSegStride * (2 * kRebuildPhases + blockIdx.x % kBarFlags);
+        const uint32_t* flag = A.grid_bar + kSegStride * bar_flag(blockIdx.x % kBarFlags);
         uint32_t tries = 0;
         bool ok = false;
         const uint32_t limit = A.rebuild_wait_limit ? A.rebuild_wait_limit : kSyncSpins;

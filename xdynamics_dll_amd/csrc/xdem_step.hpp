@@ -33,9 +33,15 @@ constexpr uint32_t kQueueSegs = 64;
 constexpr uint32_t kSegStride = 32;           // uint32 entries between two segment counters
 constexpr uint32_t kBarFlags = 64;            // release flags of k_rebuild's phases
 constexpr uint32_t kRebuildPhases = 6;        // k_rebuild: count, reduce, parts, apply, scatter, order
-// k_rebuild's slots: phase cursors, done counters, release flags, and a word set when a phase
-// gave up waiting (k_touch then leaves the lists of the last build as they are)
-constexpr uint32_t kRebuildFailed = 2 * kRebuildPhases + kBarFlags;
+constexpr uint32_t kClaimGroups = 8;          // unit ranges per phase, each with a cursor and a counter
+// k_rebuild's slots (each on its own 128-byte line): per phase and range a cursor and a done
+// counter, per phase a count of completed ranges, the release flags, and a word set when a
+// phase gave up waiting (k_touch then leaves the lists of the last build as they are)
+__host__ __device__ constexpr uint32_t bar_cursor(uint32_t k, uint32_t c) { return k * kClaimGroups + c; }
+__host__ __device__ constexpr uint32_t bar_done(uint32_t k, uint32_t c) { return (kRebuildPhases + k) * kClaimGroups + c; }
+__host__ __device__ constexpr uint32_t bar_groups(uint32_t k) { return 2 * kRebuildPhases * kClaimGroups + k; }
+__host__ __device__ constexpr uint32_t bar_flag(uint32_t f) { return 2 * kRebuildPhases * kClaimGroups + kRebuildPhases + f; }
+constexpr uint32_t kRebuildFailed = bar_flag(kBarFlags);
 constexpr uint32_t kBarSlots = kRebuildFailed + 1;
 // Entries per segment for nsorted sorted slots launched in blocks of 256 (the most a
 // segment can receive: 64 spheres from each of its waves).
@@ -814,7 +820,8 @@ __device__ __forceinline__ void lite_sphere(const StepArgs& A, const SceneDev& S
 // k_rebuild).
 __device__ __forceinline__ void reset_rebuild_slots(const StepArgs& A)
 {
-    if (blockIdx.x == 0 && threadIdx.x < kBarSlots) A.grid_bar[kSegStride * threadIdx.x] = 0u;
+    if (blockIdx.x == 0)
+        for (uint32_t t = threadIdx.x; t < kBarSlots; t += blockDim.x) A.grid_bar[kSegStride * t] = 0u;
 }
 
 // The lite kernels run over every sphere outside this step's full pass (skip kTouch), one
