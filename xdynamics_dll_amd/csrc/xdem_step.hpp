@@ -519,7 +519,10 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) void k_touch(StepArgs A)
 {
     const bool built = *A.rebuild_cur != 0u;
-    if (built && A.grid_bar[kSegStride * kRebuildFailed] == 0u) {
+    const bool failed = built && A.grid_bar[kSegStride * kRebuildFailed] != 0u;
+    // a build whose phases did not complete is tried again on the next step
+    if (failed && A.rebuild_next && blockIdx.x == 0 && threadIdx.x == 0) *A.rebuild_next = 1u;
+    if (built && !failed) {
         // virtual blocks of 256 sorted slots, block vb on the XCD of physical block vb mod
         // gridDim (gridDim is a multiple of 8)
         const uint32_t nvb = (A.nsorted + blockDim.x - 1) / blockDim.x;
