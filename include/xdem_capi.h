@@ -44,6 +44,21 @@ int xdem_get_body_force_buffers(double** fx, double** fy, double** fz, double** 
 /* Device error word of the launcher layer (XDEM_ERR_* bits); clear != 0 resets it. */
 int xdem_get_error_word(unsigned int* out, int clear);
 
+/* Launcher workspace (no counterpart in the reference, whose launchers take thrust
+ * temporaries, def:107-109, def:290). The launchers never allocate: the error word, the
+ * sort buffers of xdem_cu_reorderDataAndFindCellStart, the partials of xdem_reduction and the
+ * body-force buffers used when none are registered live in a workspace for launches of up to
+ * n elements. xdem_setDEMSymbolicParameter reserves one for np + nsphere elements (an
+ * init-time allocation); xdem_reserve_workspace grows the library's for larger launches, and
+ * xdem_set_workspace hands the library a caller-owned device buffer instead (256-byte aligned,
+ * at least xdem_workspace_size(n) bytes; xdem_set_workspace(NULL, 0, 0) detaches it, after
+ * which the caller may free it; the next init call reserves a library workspace). A launch
+ * beyond the workspace returns XDEM_E_INVALID (XDEM_E_NOT_READY without one) and sets
+ * XDEM_ERR_WORKSPACE in the error word. */
+int xdem_workspace_size(unsigned int n, unsigned long long* bytes);
+int xdem_reserve_workspace(unsigned int n);
+int xdem_set_workspace(void* d_ptr, unsigned long long bytes, unsigned int n);
+
 /* vv_update_position (decl:282-283) */
 int xdem_vv_update_position(double* pos, double* ep, double* vel, double* ev, double* acc, double* ea,
                             unsigned int np);
@@ -123,10 +138,11 @@ typedef struct xdem_ctx_config {
                                     branches). 0 (default): every kernel launched from the host.
                                     Results do not depend on it. */
     unsigned int rebuild_wait_limit; /* tries (~1 us each) a block of the list rebuild waits for
-                                    a phase to complete before it gives up, sets
-                                    XDEM_ERR_GRID_SYNC and leaves the kernel (0 = 2^20, about
-                                    a second); the step's lists are then not rebuilt. Small
-                                    values exist to test that path. */
+                                    a phase to complete before it gives up, sets the
+                                    XDEM_ERR_GRID_SYNC warning and leaves the kernel (0 = 2^20,
+                                    about a second); the blocks that stay complete the build,
+                                    so results do not change. Small values exist to test that
+                                    path. */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */

@@ -112,8 +112,16 @@ typedef struct xdem_quirks {
 #define XDEM_ERR_MESH_LIST      0x10u /* more than 20 triangle candidates of one kind */
 #define XDEM_ERR_NBR_OVERFLOW   0x20u /* more near neighbours than the pair-search list holds
                                          (step layer; the contacts beyond it are missed) */
-#define XDEM_ERR_GRID_SYNC      0x40u /* a phase of the list rebuild did not complete within ~1 s
-                                         (step layer; the step's lists are incomplete) */
+#define XDEM_ERR_GRID_SYNC      0x40u /* warning (step layer): a block of the list rebuild stopped
+                                         waiting for a phase (~1 s, or the configured limit); the
+                                         blocks that stayed complete the build, results unchanged */
+#define XDEM_ERR_BUILD_INCOMPLETE 0x80u /* (step layer) a list rebuild did not complete; the step ran
+                                         without contacts. Not expected: the rebuild's phases always
+                                         complete with the blocks that stay */
+#define XDEM_ERR_WORKSPACE      0x100u /* (launcher layer) a launch needed more scratch than the
+                                         workspace reserved for it; the launch did not run */
+#define XDEM_ERR_MESH_NEAR      0x200u /* (step layer) more triangles within reach of a sphere than its
+                                         triangle list holds (the contacts beyond it are missed) */
 
 /* Status codes returned by every C entry point. */
 #define XDEM_OK              0

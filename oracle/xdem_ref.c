@@ -938,6 +938,35 @@ void ref_update_mesh_object_data(const double* dbi, const double* vList, double*
     }
 }
 
+/* ------------------------------------------- wall pose (xParticlePlaneContact.cpp:96-110) */
+/* Local corners of a wall at its pose (xPlaneObject::define, xPlaneObject.cpp:45-48). */
+void ref_plane_local_points(const ref_plane_info* pl, const double* dbi, double out[12])
+{
+    v3 bp = mk3(dbi[REF_PX], dbi[REF_PY], dbi[REF_PZ]);
+    v4 e = {dbi[REF_E0], dbi[REF_E1], dbi[REF_E2], dbi[REF_E3]};
+    const double* c[4] = {pl->xw, pl->w2, pl->w3, pl->w4};
+    for (int q = 0; q < 4; q++) st3(out + 3 * q, to_local(sub3(ld3(c[q]), bp), e));
+}
+
+/* Corners, edge vectors, lengths and unit vectors of the wall at the pose dbi. */
+void ref_plane_update(const double local[12], const double* dbi, unsigned int id, ref_plane_info* out)
+{
+    v3 bp = mk3(dbi[REF_PX], dbi[REF_PY], dbi[REF_PZ]);
+    v4 e = {dbi[REF_E0], dbi[REF_E1], dbi[REF_E2], dbi[REF_E3]};
+    v3 xw = add3(bp, to_global(ld3(local + 0), e));
+    v3 w2 = add3(bp, to_global(ld3(local + 3), e));
+    v3 w3 = add3(bp, to_global(ld3(local + 6), e));
+    v3 w4 = add3(bp, to_global(ld3(local + 9), e));
+    v3 pa = sub3(w2, xw), pb = sub3(w4, xw);
+    double l1 = sqrt(dot3(pa, pa)), l2 = sqrt(dot3(pb, pb));
+    v3 u1 = div3(pa, l1), u2 = div3(pb, l2);
+    out->id = id;
+    out->l1 = l1; out->l2 = l2;
+    st3(out->xw, xw); st3(out->w2, w2); st3(out->w3, w3); st3(out->w4, w4);
+    st3(out->pa, pa); st3(out->pb, pb);
+    st3(out->u1, u1); st3(out->u2, u2); st3(out->uw, cross3(u1, u2));
+}
+
 double ref_reduction(const double* in, unsigned int n)
 {
     double s = 0.0;

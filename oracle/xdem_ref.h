@@ -154,6 +154,11 @@ void ref_update_mesh_object_data(const double* dbi, const double* vList, double*
                                  const double* dlocal, ref_triangle_info* dpi,
                                  unsigned int ntriangle);
 
+/* xPlaneObject::define's local corners A(e)^T (corner - position) (xPlaneObject.cpp:45-48),
+ * and xParticlePlaneContact::update (xParticlePlaneContact.cpp:96-110): the wall at pose dbi. */
+void ref_plane_local_points(const ref_plane_info* pl, const double* dbi, double out[12]);
+void ref_plane_update(const double local[12], const double* dbi, unsigned int id, ref_plane_info* out);
+
 /* def:285-317: sum of n doubles, left to right. */
 double ref_reduction(const double* in, unsigned int n);
 

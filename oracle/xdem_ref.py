@@ -58,6 +58,8 @@ def lib() -> C.CDLL:
             "ref_cohesion_force": ([D, D, P, D], D),
             "ref_fit_sphere_to_triangle": ([P, P, P, D, P], None),
             "ref_set_pair_magnitude_output": ([P], None),
+            "ref_plane_local_points": ([P, P, P], None),
+            "ref_plane_update": ([P, P, U, P], None),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -127,6 +129,21 @@ class RefSim:
         self.bf = [np.zeros(n) for _ in range(6)]
         self.wrench = np.zeros((len(self.planes), 6))
         self.overflow = {"p2p": 0, "plane": 0}
+
+    def set_plane_pose(self, k: int, dbi) -> None:
+        """Moves wall k to pose dbi (15 doubles) from its local corners at the first pose
+        (xParticlePlaneContact::update)."""
+        if not hasattr(self, "_plane_local"):
+            self._plane_local = []
+            for j, pl in enumerate(self.planes):
+                loc = np.zeros(12)
+                self.L.ref_plane_local_points(C.byref(pl), _p(self.plane_dbi[j]), _p(loc))
+                self._plane_local.append(loc)
+        d = np.ascontiguousarray(dbi, dtype=np.float64).copy()
+        pl = type(self.planes[k])()
+        self.L.ref_plane_update(_p(self._plane_local[k]), _p(d), self.planes[k].id, C.byref(pl))
+        self.planes[k] = pl
+        self.plane_dbi[k] = d
 
     def write(self, name: str, values) -> None:
         getattr(self, name)[...] = np.asarray(values, dtype=np.float64).reshape(getattr(self, name).shape)

@@ -7,7 +7,8 @@ from conftest import load_fixture
 from xdynamics_dll_amd import SceneData, box_walls, contact_property, make_plane
 
 
-def compressed_lattice(n_side=6, pitch=1.98e-3, seed=7, coh=0.0, walls=True, rfric=0.01):
+def compressed_lattice(n_side=6, pitch=1.98e-3, seed=7, coh=0.0, walls=True, rfric=0.01, contact_model=1,
+                       sratio=0.0):
     """n_side^3 spheres (r = 1 mm) closer than a diameter: every lattice neighbour overlaps,
     the bottom layer overlaps the floor; random velocities drive the tangential law."""
     rng = np.random.default_rng(seed)
@@ -15,9 +16,9 @@ def compressed_lattice(n_side=6, pitch=1.98e-3, seed=7, coh=0.0, walls=True, rfr
     x, y, z = np.meshgrid(g + 2e-3, g + 2e-3, g, indexing="ij")
     pos = np.stack([x.ravel(), y.ravel(), z.ravel(), np.full(x.size, 1.0e-3)], axis=1)
     vel = rng.uniform(-0.05, 0.05, size=(pos.shape[0], 3))
-    cp = contact_property(1e7, 1e7, 0.3, 0.3, 0.3, 0.3, rfric, coh)
+    cp = contact_property(1e7, 1e7, 0.3, 0.3, 0.3, 0.3, rfric, coh, sratio)
     w = box_walls(0.004 + n_side * pitch, 0.004 + n_side * pitch, 0.03) if walls else ()
-    return SceneData.from_positions(pos, 2500.0, cp, walls=w, dt=1e-5, vel=vel)
+    return SceneData.from_positions(pos, 2500.0, cp, walls=w, dt=1e-5, vel=vel, contact_model=contact_model)
 
 
 def sphere_drop_scene(coh=0.0):

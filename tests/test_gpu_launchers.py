@@ -144,6 +144,7 @@ def test_reorder_sort_is_stable_and_exact(n, ncell):
     d_hash, d_index = dev(keys), dev(index)
     d_cs, d_ce = dev(np.zeros(ncell, np.uint32)), dev(np.zeros(ncell, np.uint32))
     d_sorted = dev(np.zeros(n, np.uint32))
+    call("xdem_reserve_workspace", n)   # set-up: the launchers themselves never allocate
     call("xdem_cu_reorderDataAndFindCellStart", ptr(d_hash), ptr(d_index), ptr(d_cs), ptr(d_ce), ptr(d_sorted), n,
          ncell)
     torch.cuda.synchronize()
@@ -167,6 +168,42 @@ def test_reduction_fp64():
         a = rng.normal(size=n)
         out = C.c_double()
         t = dev(a)
+        call("xdem_reserve_workspace", 1)
         call("xdem_reduction", ptr(t), n, C.byref(out))
         exact = float(np.sum(a.astype(np.longdouble)))
         assert abs(out.value - exact) <= 1e-13 * np.abs(a).sum()
+
+
+def test_caller_owned_workspace_and_the_workspace_bound():
+    """The sort runs in a caller-owned workspace of exactly xdem_workspace_size(n) bytes and
+    gives the same result as in the library's; a launch beyond the workspace is refused with
+    XDEM_E_INVALID and XDEM_ERR_WORKSPACE, and leaves its arrays untouched."""
+    n, ncell = 5000, 1 << 12
+    rng = np.random.default_rng(11)
+    keys = rng.integers(0, ncell, n).astype(np.uint32)
+    order = np.argsort(keys, kind="stable")
+    size = C.c_ulonglong()
+    call("xdem_workspace_size", n, C.byref(size))
+    ws = torch.empty(size.value + 256, dtype=torch.uint8, device="cuda")
+    base = (ws.data_ptr() + 255) // 256 * 256
+    L = _lib.lib()
+    assert L.xdem_set_workspace(C.c_void_p(base), size.value - 1, n) == _lib.STATUS_INVALID
+    call("xdem_set_workspace", C.c_void_p(base), size.value, n)
+    try:
+        d_hash, d_index = dev(keys), dev(np.arange(n, dtype=np.uint32))
+        d_cs, d_ce, d_sorted = dev(np.zeros(ncell, np.uint32)), dev(np.zeros(ncell, np.uint32)), dev(np.zeros(n, np.uint32))
+        call("xdem_cu_reorderDataAndFindCellStart", ptr(d_hash), ptr(d_index), ptr(d_cs), ptr(d_ce), ptr(d_sorted),
+             n, ncell)
+        torch.cuda.synchronize()
+        assert np.array_equal(host(d_sorted, np.uint32), order.astype(np.uint32))
+        # one element more than the workspace holds: refused, nothing written
+        big = dev(np.zeros(n + 1, np.uint32))
+        rc = L.xdem_cu_reorderDataAndFindCellStart(ptr(big), ptr(big), ptr(d_cs), ptr(d_ce), ptr(big), n + 1, ncell)
+        assert rc == _lib.STATUS_INVALID
+        w = C.c_uint()
+        call("xdem_get_error_word", C.byref(w), 1)
+        assert w.value == _lib.ERR_WORKSPACE
+        assert np.array_equal(host(d_sorted, np.uint32), order.astype(np.uint32))
+    finally:
+        call("xdem_set_workspace", None, 0, 0)
+    del ws

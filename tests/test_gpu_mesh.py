@@ -88,7 +88,9 @@ def test_mesh_contact_matches_oracle(coh, gpu_law):
                                            nptr(r_ce), C.byref(cp), n, nt, CAP)
     # --- GPU, through the launchers
     gq = _lib.Quirks(gpu_law, 0, 1, 1)
-    call("xdem_setDEMSymbolicParameter", C.byref(prm))
+    gprm = scene.dem_parameters()
+    gprm.nsphere = nt   # the launchers' workspace covers the particles and the proxy spheres
+    call("xdem_setDEMSymbolicParameter", C.byref(gprm))
     call("xdem_set_quirks", C.byref(gq))
     d_vlist, d_dlocal, d_dbi = dev(vlist), dev(dlocal), dev(dbi)
     g_sph0 = np.zeros((nt, 4)); g_sph0[:, 3] = prox[:, 3]

@@ -521,3 +521,242 @@ def test_mesh_face_contact_is_hertz_and_balances_the_body():
     # x and y only carry the rounding of the closest point (barycentric), ~1e-14 relative
     assert np.allclose(F[0], [0.0, 0.0, fz], rtol=1e-12, atol=1e-12 * fz)
     assert np.allclose([bf[0][0], bf[1][0], bf[2][0]], -F[0], rtol=1e-12, atol=1e-12 * fz)
+
+
+# ------------------------------------------------------------------ mesh contact selection
+def _mesh_collide(tris, sphere, vel=(0.0, 0.0, 0.0), ev=(0.0, 0.0, 0.0, 0.0), body=None, cp=None):
+    """Runs the oracle's sphere-triangle pass (impl:1930-2109) for one sphere against the
+    given global triangles (T, 3, 3). Every proxy centre is put at the sphere's centre, so the
+    27-cell scan visits the triangles in index order. Returns (force, moment, count, ids,
+    body force/moment)."""
+    from mesh_scenes import triangle_infos
+
+    tris = np.asarray(tris, dtype=np.float64)
+    nt = len(tris)
+    pos = np.array([sphere], dtype=np.float64)
+    cp = cp if cp is not None else contact_property(EI, EJ, NUI, NUJ, REST, FRIC, 0.0)
+    s = SceneData.from_positions(pos, 2500.0, cp, walls=(), dt=1e-5)
+    prm = s.dem_parameters()
+    q = xdem_ref.quirks_struct({})
+    tri = triangle_infos(nt)
+    for t in range(nt):
+        for name, v in zip("PQR", tris[t]):
+            setattr(tri[t], name, _lib.double3(*v))
+    dbi = np.zeros(15) if body is None else np.asarray(body, dtype=np.float64).copy()
+    if body is None:
+        dbi[0] = 5.0
+        dbi[7] = 1.0
+    ncell = 128 ** 3
+    sph = np.tile(np.r_[pos[0, :3], 0.0], (nt, 1))
+    h = np.zeros(1 + nt, np.uint32); idx = np.zeros(1 + nt, np.uint32)
+    L.ref_calculate_hash_and_index(C.byref(prm), ptr(h), ptr(idx), ptr(pos), 0, 1)
+    L.ref_calculate_hash_and_index(C.byref(prm), ptr(h), ptr(idx), ptr(sph), 1, nt)
+    cs = np.zeros(ncell, np.uint32); ce = np.zeros(ncell, np.uint32); srt = np.zeros(1 + nt, np.uint32)
+    L.ref_reorder_data_and_find_cell_start(ptr(h), ptr(idx), ptr(cs), ptr(ce), ptr(srt), 1 + nt, ncell)
+    F = np.zeros((1, 3)); M = np.zeros((1, 3)); T = np.zeros((1, 3)); R = np.zeros(1)
+    cnt = np.zeros(1, np.uint32); ids = np.zeros((1, 5), np.uint32); tsd = np.zeros((1, 5, 2))
+    bf = [np.zeros(1) for _ in range(6)]
+    v3 = np.array([vel], dtype=np.float64)
+    ep = np.array([[1.0, 0.0, 0.0, 0.0]])
+    evv = np.array([ev], dtype=np.float64)
+    L.ref_particle_polygon_collision(C.byref(prm), C.byref(q), C.addressof(tri), ptr(dbi), *(ptr(b) for b in bf),
+                                     ptr(pos), ptr(ep), ptr(v3), ptr(evv), ptr(F), ptr(M), ptr(s.mass), ptr(T), ptr(R),
+                                     ptr(cnt), ptr(ids), ptr(tsd), ptr(srt), ptr(cs), ptr(ce), C.byref(cp), 1, nt, 5)
+    return F[0], M[0], int(cnt[0]), ids[0, : cnt[0]].copy(), np.array([b[0] for b in bf])
+
+
+R_T, KN_T = 1e-3, (4.0 / 3.0) * ESTAR * math.sqrt(1e-3)   # sphere radius and its Hertz stiffness on a mesh
+
+
+def test_ridge_gives_one_edge_contact():
+    """Two triangles meeting in a ridge along x (a roof falling away at 45 degrees on both
+    sides). A sphere straight above the ridge is closest to the shared edge in both triangles
+    (the edge region, not the face), so each yields an edge candidate at the same point; the
+    second equals the first within 1e-9 and is dropped (checkOverlab, impl:1729-1746,
+    2019-2030): one contact, the Hertz force k_n delta^1.5 along +z, delta = r - h."""
+    s, o = 4e-3, np.array([0.013, -0.007, 0.002])
+    ridge_a, ridge_b = o, o + (s, 0.0, 0.0)
+    tris = [[ridge_a, ridge_b, o + (s / 2, -s, -s)], [ridge_b, ridge_a, o + (s / 2, s, -s)]]
+    h = R_T - 3e-6
+    F, M, n, ids, _ = _mesh_collide(tris, (o[0] + s / 2, o[1], o[2] + h, R_T))
+    assert n == 1 and ids[0] == 0
+    fz = KN_T * (R_T - h) ** 1.5
+    assert np.allclose(F, [0.0, 0.0, fz], rtol=1e-12, atol=1e-12 * fz)
+    assert np.abs(M).max() <= 1e-12 * fz * R_T   # a normal force through the centre has no moment
+
+
+def test_apex_of_a_fan_of_six_gives_one_vertex_contact():
+    """A convex pyramid of six triangles sharing the apex: a sphere straight above the apex is
+    in the apex's vertex region of every triangle, so six vertex candidates at the same point;
+    all but the first are dropped as duplicates. One contact along +z with delta = r - h."""
+    for apex in ((0.011, 0.023, 0.005), (-0.02, 0.004, 0.031)):
+        F, n, ids, fz = _apex_fan(np.array(apex))
+        assert n == 1 and ids[0] == 0
+        assert np.allclose(F, [0.0, 0.0, fz], rtol=1e-12, atol=1e-12 * fz)
+
+
+def _apex_fan(apex):
+    ring = [apex + np.array([3e-3 * math.cos(k * math.pi / 3), 3e-3 * math.sin(k * math.pi / 3), -2e-3])
+            for k in range(6)]
+    tris = [[apex, ring[k], ring[(k + 1) % 6]] for k in range(6)]
+    h = R_T - 2e-6
+    F, _, n, ids, _ = _mesh_collide(tris, (apex[0], apex[1], apex[2] + h, R_T))
+    return F, n, ids, KN_T * (R_T - h) ** 1.5
+
+
+def test_vertex_and_edge_contacts_at_the_origin_are_dropped():
+    """Reference quirk of the duplicate test (impl:1983-1984, 2015-2026): the previous edge and
+    vertex contact points start at (0, 0, 0), so the first edge or vertex candidate within
+    1e-9 m of the world origin counts as a duplicate and is dropped. The apex fan with its apex
+    at the origin has no contact at all; face contacts there are kept."""
+    F, n, _, _ = _apex_fan(np.zeros(3))
+    assert n == 0 and not F.any()
+
+
+def test_face_contact_hides_the_neighbours_edge_contact():
+    """Two coplanar triangles sharing the diagonal of a square. A sphere whose projection lies
+    inside the first triangle, 0.1 mm from the shared edge, touches the first in its face and
+    the second at the shared edge (distance sqrt(h^2 + d^2)). Edges apply only when no face
+    does (impl:2037-2100): one contact, the face one, k_n (r - h)^1.5 along the face normal."""
+    s, o = 4e-3, np.array([0.01, 0.02, 0.003])
+    a, b, c, d = (o + q for q in ((0.0, 0.0, 0.0), (s, 0.0, 0.0), (s, s, 0.0), (0.0, s, 0.0)))
+    tris = [[a, b, d], [b, c, d]]
+    dist = 2e-5
+    off = dist / math.sqrt(2.0)
+    p = o + (s / 2 - off, s / 2 - off, 0.0)   # inside the first, 20 um from the diagonal b-d
+    h = R_T - 4e-6
+    assert math.hypot(h, dist) < R_T          # the neighbour's edge is within reach too
+    F, M, n, ids, _ = _mesh_collide(tris, (p[0], p[1], p[2] + h, R_T))
+    assert n == 1 and ids[0] == 0
+    fz = KN_T * (R_T - h) ** 1.5
+    assert np.allclose(F, [0.0, 0.0, fz], rtol=1e-12, atol=1e-12 * fz)
+
+
+def test_projection_exactly_on_a_shared_edge_is_a_face_of_both_triangles():
+    """Reference quirk of closestPtPointTriangle (impl:1655-1705): the face test runs last and
+    accepts va, vb, vc >= 0, so a projection exactly on an edge (one barycentric weight exactly
+    0) is a face contact, overriding the edge region. With coordinates exact in binary the
+    weight is exactly 0 for both triangles sharing the edge (va = d3 d6 - d5 d4 = L^4/4 - L^4/4
+    for the first, vb = 0 for the second), so both apply as faces: twice the Hertz force."""
+    Lq = 2.0 ** -8
+    tris = [[(0.0, 0.0, 0.0), (Lq, 0.0, 0.0), (0.0, Lq, 0.0)], [(Lq, 0.0, 0.0), (Lq, Lq, 0.0), (0.0, Lq, 0.0)]]
+    h = R_T - 2.0 ** -18
+    F, _, n, ids, _ = _mesh_collide(tris, (Lq / 2, Lq / 2, h, R_T))
+    assert n == 2 and list(ids) == [0, 1]
+    fz = KN_T * (R_T - h) ** 1.5
+    assert np.allclose(F, [0.0, 0.0, 2.0 * fz], rtol=1e-12, atol=1e-12 * fz)
+
+
+def test_mesh_pose_update_is_the_rigid_transform():
+    """ref_update_mesh_object_data (impl:2544-2601): every vertex and proxy centre is the body
+    position plus A(e) times its local coordinates, A the rotation of the Euler parameters
+    (checked against scipy's quaternion rotation, computed independently)."""
+    from scipy.spatial.transform import Rotation
+
+    from mesh_scenes import body_pose, fit_proxies, terrain, triangle_infos
+
+    tris = terrain(nx=4)
+    nt = len(tris)
+    prox = fit_proxies(tris)
+    dbi = body_pose(seed=8)
+    e = dbi[7:11]
+    Rm = Rotation.from_quat([e[1], e[2], e[3], e[0]]).as_matrix()   # scipy: (x, y, z, w)
+    vlist = np.ascontiguousarray(tris.reshape(nt, 9))
+    dlocal = np.ascontiguousarray(prox[:, :3])
+    sph = np.zeros((nt, 4)); sph[:, 3] = prox[:, 3]
+    tri = triangle_infos(nt)
+    L.ref_update_mesh_object_data(ptr(dbi), ptr(vlist), ptr(sph), ptr(dlocal), C.addressof(tri), nt)
+    got = np.frombuffer(bytes(tri), dtype=np.float64).reshape(nt, 10)[:, 1:].reshape(nt, 3, 3)
+    expect = tris @ Rm.T + dbi[1:4]
+    assert np.allclose(got, expect, rtol=0, atol=1e-15)
+    assert np.allclose(sph[:, :3], prox[:, :3] @ Rm.T + dbi[1:4], rtol=0, atol=1e-15)
+    assert np.array_equal(sph[:, 3], prox[:, 3])
+
+
+# ------------------------------------------------------------------ moment assembly
+def test_sphere_pair_moments_are_lever_arm_cross_force():
+    """One pair with sliding and spin: the moment on each sphere is (its radius times the unit
+    vector towards the other) cross its contact force, so M_i / r_i = M_j / r_j = u x F_i
+    (impl:700-703). The forces come from the oracle; the lever arms from the geometry."""
+    ri, rj = 1.2e-3, 0.8e-3
+    d = ri + rj - 5e-6
+    u = np.array([0.6, 0.0, 0.8])
+    pos = np.array([[0.01, 0.01, 0.01, ri], [0.01 + d * u[0], 0.01, 0.01 + d * u[2], rj]])
+    cp = contact_property(EI, EJ, NUI, NUJ, REST, FRIC, 0.0)
+    vel = np.array([[0.02, -0.03, 0.0], [-0.01, 0.02, 0.01]])
+    s = SceneData.from_positions(pos, 2500.0, cp, walls=(), dt=1e-5, gravity=(0.0, 0.0, 0.0), vel=vel)
+    ref = xdem_ref.RefSim(s)
+    ref.ev[:] = [[0.0, 3.0, -2.0, 1.0], [0.0, -1.0, 2.0, 0.5]]
+    ref.grid()
+    ref.contacts()
+    Fi, Fj = ref.force
+    assert np.linalg.norm(np.cross(Fi, u)) > 1e-3 * np.linalg.norm(Fi)   # a tangential part
+    assert np.allclose(ref.moment[0], np.cross(ri * u, Fi), rtol=1e-12, atol=1e-15)
+    assert np.allclose(ref.moment[1], np.cross(-rj * u, Fj), rtol=1e-12, atol=1e-15)
+
+
+def test_wall_and_mesh_body_moments_are_minus_lever_arm_cross_force():
+    """A sphere sliding on a wall and on a triangle: the sphere's moment is (r u) x F with u
+    the unit vector from the centre to the contact point, and the body's moment is
+    -(contact point - body position) x F (impl:1040-1050, 1820-1830)."""
+    r, delta = 1e-3, 3e-6
+    cp = contact_property(EI, EJ, NUI, NUJ, REST, FRIC, 0.0)
+    # wall: the floor z = 0 of a 0.1 m square, body position at its centre
+    floor = make_plane((0, 0, 0), (0.1, 0, 0), (0.1, 0.1, 0), (0, 0.1, 0), 0, mass=3.0)
+    pos = np.array([[0.03, 0.07, r - delta, r]])
+    s = SceneData.from_positions(pos, 2500.0, cp, walls=[floor], dt=1e-5, gravity=(0.0, 0.0, 0.0),
+                                 vel=np.array([[0.05, -0.02, 0.0]]))
+    ref = xdem_ref.RefSim(s)
+    ref.grid()
+    ref.contacts()
+    F = ref.force[0]
+    cpt = pos[0, :3] + r * np.array([0.0, 0.0, -1.0])
+    xb = floor[1][1:4]
+    assert abs(F[0]) > 1e-6 * abs(F[2])
+    assert np.allclose(ref.moment[0], np.cross(cpt - pos[0, :3], F), rtol=1e-12, atol=1e-15)
+    assert np.allclose(ref.wrench[0, 3:], -np.cross(cpt - xb, F), rtol=1e-12, atol=1e-15)
+    assert np.allclose(ref.wrench[0, :3], -F, rtol=1e-12, atol=1e-15)
+    # triangle face, body position away from the contact
+    tris = [[(-0.05, -0.05, 0.0), (0.1, -0.05, 0.0), (0.01, 0.1, 0.0)]]
+    body = np.zeros(15); body[0] = 5.0; body[1:4] = (0.02, -0.01, 0.005); body[7] = 1.0
+    sp = (0.01, 0.01, r - delta, r)
+    F, M, n, _, bfm = _mesh_collide(tris, sp, vel=(0.04, 0.03, 0.0), body=body)
+    assert n == 1 and np.hypot(F[0], F[1]) > 1e-6 * abs(F[2])
+    cpt = np.array([sp[0], sp[1], 0.0])
+    assert np.allclose(M, np.cross(cpt - np.array(sp[:3]), F), rtol=1e-12, atol=1e-15)
+    assert np.allclose(bfm[:3], -F, rtol=1e-12, atol=1e-15)
+    assert np.allclose(bfm[3:], -np.cross(cpt - body[1:4], F), rtol=1e-12, atol=1e-15)
+
+
+# ------------------------------------------------------------------ free rotation
+@pytest.mark.parametrize("axis", [(0.0, 0.0, 1.0), (0.48, -0.6, 0.64)])
+def test_free_spin_of_a_sphere(axis):
+    """A sphere spinning freely (isotropic inertia, no force, no contact) against the exact
+    rotation e(t) = (cos(w t / 2), sin(w t / 2) n), omega = w n constant. The position update
+    normalises e every step (impl:135-147), so |e| = 1 to rounding; the rates e_dot are not
+    projected back, so e . e_dot drifts by O(dt) and the scheme converges at first order:
+    halving dt halves the errors of e and of omega = 2 L(e) e_dot (checked over a 2 rad turn)."""
+    w = 40.0
+    n = np.asarray(axis) / np.linalg.norm(axis)
+
+    def run(dt, steps):
+        pos = np.array([[0.0, 0.0, 0.0, 1e-3]])
+        cp = contact_property(EI, EJ, NUI, NUJ, REST, FRIC, 0.0)
+        s = SceneData.from_positions(pos, 2500.0, cp, walls=(), dt=dt, gravity=(0.0, 0.0, 0.0))
+        ref = xdem_ref.RefSim(s)
+        ref.ev[0] = np.r_[0.0, 0.5 * w * n]   # e = (1, 0, 0, 0): omega = 2 (e_dot_1, e_dot_2, e_dot_3)
+        ref.step(steps)
+        e = ref.ep[0].copy()
+        ed = ref.ev[0].copy()
+        om = np.zeros(3)
+        L.ref_to_angular_velocity(ptr(e), ptr(ed), ptr(om))
+        exact = np.r_[math.cos(0.5 * w * dt * steps), math.sin(0.5 * w * dt * steps) * n]
+        return e, np.abs(e - exact).max(), np.linalg.norm(om - w * n) / w
+
+    e1, err1, om1 = run(1e-4, 500)
+    e2, err2, om2 = run(5e-5, 1000)
+    for e in (e1, e2):
+        assert abs(np.linalg.norm(e) - 1.0) <= 1e-14
+    assert err1 < 5e-3 and om1 < 1e-2
+    assert 1.8 < err1 / err2 < 2.2, (err1, err2)
+    assert 1.8 < om1 / om2 < 2.2, (om1, om2)

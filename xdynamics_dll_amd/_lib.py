@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(BUILD_DIR, "libxdem.so")
 HOST_LIB_PATH = os.path.join(BUILD_DIR, "libxdem_host.so")
 
 XDEM_OK = 0
+STATUS_INVALID = -1
+STATUS_NOT_READY = -3
 STATUS = {0: "XDEM_OK", -1: "XDEM_E_INVALID", -2: "XDEM_E_HIP", -3: "XDEM_E_NOT_READY",
           -4: "XDEM_E_DEVICE_ERROR", -5: "XDEM_E_UNSUPPORTED"}
 
@@ -25,6 +27,9 @@ ERR_NONFINITE = 0x8
 ERR_MESH_LIST = 0x10
 ERR_NBR_OVERFLOW = 0x20
 ERR_GRID_SYNC = 0x40
+ERR_BUILD_INCOMPLETE = 0x80
+ERR_WORKSPACE = 0x100
+ERR_MESH_NEAR = 0x200
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -132,6 +137,9 @@ LAUNCHER_SIGNATURES = {
     "xdem_set_body_force_buffers": [P] * 6,
     "xdem_get_body_force_buffers": [C.POINTER(P)] * 6,
     "xdem_get_error_word": [C.POINTER(U), I],
+    "xdem_workspace_size": [U, C.POINTER(C.c_ulonglong)],
+    "xdem_reserve_workspace": [U],
+    "xdem_set_workspace": [P, C.c_ulonglong, U],
     "xdem_vv_update_position": [P] * 6 + [U],
     "xdem_vv_update_velocity": [P] * 9 + [U],
     "xdem_cu_calculateHashAndIndex": [P, P, P, U, U],

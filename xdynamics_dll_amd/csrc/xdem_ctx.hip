@@ -20,6 +20,8 @@ struct SceneHost {
     std::vector<double> pos, vel, mass, inertia, dbi;
     std::vector<xdem_plane_info> planes;
     std::vector<xdem_contact_property> plane_cp;
+    std::vector<double> wall_local;   // [12 * nplane] corners in the wall's body frame, fixed at
+                                      // add_scene (xPlaneObject::define's local_point)
     uint32_t p_begin = 0;
     uint32_t wall_begin = 0;
     SceneDev dev;
@@ -46,6 +48,29 @@ __global__ __launch_bounds__(256) void k_wrench_reduce(const double* wrench, uin
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) out[sc] = ((s[0] + s[1]) + s[2]) + s[3];
+}
+
+// Rotation matrix of Euler parameters e (xPointMass::setupTransformationMatrix,
+// source/xdynamics_object/xPointMass.cpp:296-301): o = A v, or A^T v when transpose is set.
+void euler_rotate(const double* e, const double v[3], bool transpose, double o[3])
+{
+    const double a00 = 2 * (e[0] * e[0] + e[1] * e[1] - 0.5), a01 = 2 * (e[1] * e[2] - e[0] * e[3]), a02 = 2 * (e[1] * e[3] + e[0] * e[2]);
+    const double a10 = 2 * (e[1] * e[2] + e[0] * e[3]), a11 = 2 * (e[0] * e[0] + e[2] * e[2] - 0.5), a12 = 2 * (e[2] * e[3] - e[0] * e[1]);
+    const double a20 = 2 * (e[1] * e[3] - e[0] * e[2]), a21 = 2 * (e[2] * e[3] + e[0] * e[1]), a22 = 2 * (e[0] * e[0] + e[3] * e[3] - 0.5);
+    if (!transpose) {
+        o[0] = a00 * v[0] + a01 * v[1] + a02 * v[2]; o[1] = a10 * v[0] + a11 * v[1] + a12 * v[2]; o[2] = a20 * v[0] + a21 * v[1] + a22 * v[2];
+    } else {
+        o[0] = a00 * v[0] + a10 * v[1] + a20 * v[2]; o[1] = a01 * v[0] + a11 * v[1] + a21 * v[2]; o[2] = a02 * v[0] + a12 * v[1] + a22 * v[2];
+    }
+}
+
+// A wall or mesh body moved by at most d since the last pose: the lists of the last build
+// stay complete while the summed motion stays below half the smallest skin; beyond, the
+// next step rebuilds them.
+__global__ void k_body_moved(ListCtl* L, double d, uint32_t* rebuild_next)
+{
+    L->body_path += d;
+    if (!(L->body_path - L->body_path_build < L->half_skin_min * (double)L->scale)) *rebuild_next = 1u;
 }
 
 uint32_t next_pow2(uint32_t v)
@@ -257,10 +282,12 @@ struct xdem_ctx {
         return hipSuccess;
     }
 
-    // A list rebuild on the next step (between steps: a wall moved).
-    hipError_t force_rebuild()
+    // Between steps: a wall or mesh body moved by at most d (its largest point displacement);
+    // the next step rebuilds the lists once the summed motion since the last build reaches
+    // half the smallest skin.
+    hipError_t body_moved(double d)
     {
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, rebuild + (rbp ^ 1), 1u, (size_t)1);
+        hipLaunchKernelGGL(k_body_moved, dim3(1), dim3(1), 0, stream, lctl, d, rebuild + (rbp ^ 1));
         return hipGetLastError();
     }
 
@@ -383,6 +410,17 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
     h.planes.assign(s->planes, s->planes + s->nplane);
     h.plane_cp.assign(s->plane_cp, s->plane_cp + s->nplane);
     h.dbi.assign(s->plane_dbi, s->plane_dbi + 15 * (size_t)s->nplane);
+    // local corners = A(e)^T (corner - body position) at the pose given with the scene
+    // (xPlaneObject::define, source/xdynamics_object/xPlaneObject.cpp:45-48)
+    h.wall_local.assign(12 * (size_t)s->nplane, 0.0);
+    for (unsigned k = 0; k < s->nplane; k++) {
+        const double* d = s->plane_dbi + 15 * k;
+        const xdem_double3 cs[4] = {s->planes[k].xw, s->planes[k].w2, s->planes[k].w3, s->planes[k].w4};
+        for (int q = 0; q < 4; q++) {
+            const double rel[3] = {cs[q].x - d[XDEM_PX], cs[q].y - d[XDEM_PY], cs[q].z - d[XDEM_PZ]};
+            euler_rotate(d + XDEM_E0, rel, true, &h.wall_local[12 * k + 3 * q]);
+        }
+    }
     h.p_begin = c->np;
     h.wall_begin = (uint32_t)c->walls.size();
     c->np += s->np;
@@ -613,6 +651,8 @@ int xdem_ctx_commit(xdem_ctx* c)
         L.adaptive = c->cfg.skin == 0.0 ? 1u : 0u;
         L.scale = L.adaptive ? 0.125f : 1.0f;
         L.min_scale = L.adaptive ? 0.0625f : 1.0f;
+        L.half_skin_min = 1e300;
+        for (const SceneHost& h : c->scenes) L.half_skin_min = std::min(L.half_skin_min, 0.5 * h.dev.skin);
         CTX_CHECK(hipStreamSynchronize(s));
         CTX_CHECK(hipMemcpy(c->lctl, &L, sizeof(L), hipMemcpyHostToDevice));
     }
@@ -889,26 +929,12 @@ int xdem_ctx_set_body_pose(xdem_ctx* c, unsigned int scene, unsigned int wall, c
     SceneHost& h = c->scenes[scene];
     if (wall >= h.s.nplane) return XDEM_E_INVALID;
     WallDev& W = c->walls[h.wall_begin + wall];
-    // xParticlePlaneContact::update: corners = pose position + A(e) * local corner, where the
-    // local corners were fixed at definition relative to the initial pose.
-    const double* old = W.dbi;
-    auto rot = [](const double* e, const double v[3], bool transpose, double o[3]) {
-        const double a00 = 2 * (e[0] * e[0] + e[1] * e[1] - 0.5), a01 = 2 * (e[1] * e[2] - e[0] * e[3]), a02 = 2 * (e[1] * e[3] + e[0] * e[2]);
-        const double a10 = 2 * (e[1] * e[2] + e[0] * e[3]), a11 = 2 * (e[0] * e[0] + e[2] * e[2] - 0.5), a12 = 2 * (e[2] * e[3] - e[0] * e[1]);
-        const double a20 = 2 * (e[1] * e[3] - e[0] * e[2]), a21 = 2 * (e[2] * e[3] + e[0] * e[1]), a22 = 2 * (e[0] * e[0] + e[3] * e[3] - 0.5);
-        if (!transpose) {
-            o[0] = a00 * v[0] + a01 * v[1] + a02 * v[2]; o[1] = a10 * v[0] + a11 * v[1] + a12 * v[2]; o[2] = a20 * v[0] + a21 * v[1] + a22 * v[2];
-        } else {
-            o[0] = a00 * v[0] + a10 * v[1] + a20 * v[2]; o[1] = a01 * v[0] + a11 * v[1] + a21 * v[2]; o[2] = a02 * v[0] + a12 * v[1] + a22 * v[2];
-        }
-    };
-    const xdem_double3 corners[4] = {W.info.xw, W.info.w2, W.info.w3, W.info.w4};
+    // xParticlePlaneContact::update (xParticlePlaneContact.cpp:96-110): corner k = pose
+    // position + A(e) LocalPoint(k), with the local corners fixed at add_scene
     double g[4][3];
     for (int q = 0; q < 4; q++) {
-        const double rel[3] = {corners[q].x - old[XDEM_PX], corners[q].y - old[XDEM_PY], corners[q].z - old[XDEM_PZ]};
-        double loc[3], gl[3];
-        rot(old + XDEM_E0, rel, true, loc);
-        rot(dbi + XDEM_E0, loc, false, gl);
+        double gl[3];
+        euler_rotate(dbi + XDEM_E0, &h.wall_local[12 * wall + 3 * q], false, gl);
         for (int a = 0; a < 3; a++) g[q][a] = dbi[XDEM_PX + a] + gl[a];
     }
     xdem_plane_info pi = W.info;
@@ -922,6 +948,16 @@ int xdem_ctx_set_body_pose(xdem_ctx* c, unsigned int scene, unsigned int wall, c
     pi.pb = {pb[0], pb[1], pb[2]};
     pi.l1 = std::sqrt(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]);
     pi.l2 = std::sqrt(pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2]);
+    // the largest displacement of the wall's points is that of a corner (rigid motion)
+    double dmax = 0.0;
+    {
+        const xdem_double3 oc[4] = {W.info.xw, W.info.w2, W.info.w3, W.info.w4};
+        for (int q = 0; q < 4; q++) {
+            const double dx = g[q][0] - oc[q].x, dy = g[q][1] - oc[q].y, dz = g[q][2] - oc[q].z;
+            dmax = std::max(dmax, std::sqrt(dx * dx + dy * dy + dz * dz));
+        }
+        dmax = dmax * (1.0 + 1e-9) + 1e-15;   // rounded up
+    }
     const double u1[3] = {pa[0] / pi.l1, pa[1] / pi.l1, pa[2] / pi.l1};
     const double u2[3] = {pb[0] / pi.l2, pb[1] / pi.l2, pb[2] / pi.l2};
     pi.u1 = {u1[0], u1[1], u1[2]};
@@ -932,7 +968,7 @@ int xdem_ctx_set_body_pose(xdem_ctx* c, unsigned int scene, unsigned int wall, c
     if (c->committed) {
         CTX_CHECK(hipSetDevice(c->cfg.device));
         CTX_CHECK(hipMemcpyAsync(c->d_walls + h.wall_begin + wall, &W, sizeof(WallDev), hipMemcpyHostToDevice, c->stream));
-        CTX_CHECK(c->force_rebuild());   // the wall-reach flags of the lists are for the old pose
+        CTX_CHECK(c->body_moved(dmax));   // the lists' wall-reach flags hold while it moved less than the skin
         CTX_CHECK(hipStreamSynchronize(c->stream));
     }
     return XDEM_OK;

@@ -111,6 +111,7 @@ __device__ __forceinline__ void list_ctl_update(const StepArgs& A, bool rebuild)
         st_co(reinterpret_cast<uint32_t*>(&L.scale), __float_as_uint(sc));
     }
     L.last_build = L.step;
+    L.body_path_build = L.body_path;   // the lists of this build see the bodies where they are
     *A.n_rebuild += 1ull;
 }
 
@@ -125,9 +126,11 @@ __device__ __forceinline__ void list_ctl_update(const StepArgs& A, bool rebuild)
 // other contexts may hold the device). Every wave waits for its stores to complete (they are
 // written through) before its unit is counted. k_touch, the next kernel of the step, sets the
 // cursors, counters and flags back to zero. The wait is bounded (kSyncSpins tries of ~1 us,
-// or the context's rebuild_wait_limit): beyond it XDEM_ERR_GRID_SYNC and the failed word are
-// set and the block leaves the kernel, so it always ends; every index a later phase reads
-// from an earlier one is bounds-checked, so a partial build reads and writes in bounds.
+// or the context's rebuild_wait_limit): beyond it the block sets XDEM_ERR_GRID_SYNC (a
+// warning) and leaves the kernel, so it always ends. The build still completes: the block
+// that finishes a phase's last unit has released the phase before it waits, so it passes and
+// claims the next phase's units, and every unit of every phase is claimed by a block that
+// stays. Every index a later phase reads from an earlier one is bounds-checked all the same.
 constexpr uint32_t kSyncSpins = 1u << 20;
 constexpr uint32_t kUnit = 4 * kBlock;
 
@@ -175,10 +178,7 @@ __device__ __forceinline__ bool run_phase(const StepArgs& A, uint32_t k, uint32_
             if (ok || ++tries >= limit) break;
             __builtin_amdgcn_s_sleep(2);
         }
-        if (!ok) {
-            atomicOr(A.err, XDEM_ERR_GRID_SYNC);
-            st_co(A.grid_bar + kSegStride * kRebuildFailed, 1u);
-        }
+        if (!ok) atomicOr(A.err, XDEM_ERR_GRID_SYNC);
         passed = ok ? 1u : 0u;
     }
     __syncthreads();

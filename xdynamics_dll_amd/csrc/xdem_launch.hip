@@ -18,33 +18,70 @@ Runtime& runtime()
 
 static std::mutex g_mutex;
 
-hipError_t runtime_reserve(size_t n_sort, size_t n_bf)
+namespace {
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Workspace layout for n elements: error word, 1024 reduction partials, 4 sort arrays,
+// sort histograms, 6 body-force arrays (each part 256-byte aligned).
+struct WsLayout {
+    size_t err, red, sort, scratch, bf, total;
+};
+
+WsLayout ws_layout(uint32_t n)
 {
-    Runtime& rt = runtime();
-    hipError_t e = hipSuccess;
-    if (!rt.err) {
-        if ((e = hipMalloc(&rt.err, sizeof(uint32_t))) != hipSuccess) return e;
-        if ((e = hipMemset(rt.err, 0, sizeof(uint32_t))) != hipSuccess) return e;
-        if ((e = hipMalloc(&rt.red_partial, sizeof(double) * 1024)) != hipSuccess) return e;
-    }
-    if (n_sort > rt.sort_cap) {
-        if (rt.sort_buf) (void)hipFree(rt.sort_buf);
-        if (rt.sort_scratch) (void)hipFree(rt.sort_scratch);
-        rt.sort_buf = nullptr;
-        rt.sort_scratch = nullptr;
-        if ((e = hipMalloc(&rt.sort_buf, sizeof(uint32_t) * 4 * n_sort)) != hipSuccess) return e;
-        rt.sort_scratch_bytes = radix_sort_scratch_bytes((uint32_t)n_sort);
-        if ((e = hipMalloc(&rt.sort_scratch, rt.sort_scratch_bytes)) != hipSuccess) return e;
-        rt.sort_cap = n_sort;
-    }
-    if (n_bf > rt.own_bf_n) {
-        if (rt.own_bf) (void)hipFree(rt.own_bf);
-        rt.own_bf = nullptr;
-        if ((e = hipMalloc(&rt.own_bf, sizeof(double) * 6 * n_bf)) != hipSuccess) return e;
-        rt.own_bf_n = n_bf;
-    }
+    WsLayout L;
+    size_t o = 0;
+    L.err = o; o += 256;
+    L.red = o; o += align256(sizeof(double) * 1024);
+    L.sort = o; o += align256(sizeof(uint32_t) * 4 * (size_t)n);
+    L.scratch = o; o += align256(radix_sort_scratch_bytes(n));
+    L.bf = o; o += align256(sizeof(double) * 6 * (size_t)n);
+    L.total = o;
+    return L;
+}
+
+// Points the runtime at workspace `base` for n elements and clears its error word.
+hipError_t ws_attach(Runtime& rt, char* base, size_t bytes, uint32_t n, bool owned)
+{
+    if (rt.ws_owned && rt.ws && rt.ws != base) (void)hipFree(rt.ws);   // replaced: nothing to report
+    const WsLayout L = ws_layout(n);
+    rt.ws = base;
+    rt.ws_bytes = bytes;
+    rt.ws_n = n;
+    rt.ws_owned = owned;
+    rt.err = reinterpret_cast<uint32_t*>(base + L.err);
+    rt.red_partial = reinterpret_cast<double*>(base + L.red);
+    rt.sort_buf = reinterpret_cast<uint32_t*>(base + L.sort);
+    rt.sort_scratch = base + L.scratch;
+    rt.own_bf = reinterpret_cast<double*>(base + L.bf);
+    hipError_t e = hipMemsetAsync(rt.err, 0, sizeof(uint32_t), rt.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(rt.stream);
     return e;
 }
+
+// A library-owned workspace for n elements (an init-time allocation).
+hipError_t ws_reserve(Runtime& rt, uint32_t n)
+{
+    if (rt.ws && (rt.ws_n >= n || !rt.ws_owned)) return hipSuccess;   // enough, or the caller's
+    const size_t bytes = ws_layout(n).total;
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return e;
+    return ws_attach(rt, static_cast<char*>(p), bytes, n, true);
+}
+
+// Status of a launch that needs n elements of workspace.
+int ws_require(Runtime& rt, uint32_t n)
+{
+    if (rt.ws && rt.ws_n >= n) return XDEM_OK;
+    rt.host_err |= XDEM_ERR_WORKSPACE;
+    return rt.ws ? XDEM_E_INVALID : XDEM_E_NOT_READY;
+}
+
+}  // namespace
+
+size_t workspace_bytes(uint32_t n) { return ws_layout(n).total; }
 
 namespace {
 
@@ -486,6 +523,11 @@ int xdem_setDEMSymbolicParameter(const xdem_dem_parameters* h)
     std::lock_guard<std::mutex> lk(g_mutex);
     runtime().prm = *h;
     runtime().prm_set = true;
+    // the launchers' workspace for the particles and the mesh proxy spheres the caller
+    // counts in nsphere, allocated here, at set-up (a caller-owned workspace is kept)
+    const unsigned long long n = (unsigned long long)h->np + h->nsphere;
+    if (n > 0xffffffffull) return XDEM_E_INVALID;
+    XDEM_HIP_CHECK(ws_reserve(runtime(), (uint32_t)n));
     return XDEM_OK;
 }
 
@@ -515,7 +557,7 @@ int xdem_get_body_force_buffers(double** fx, double** fy, double** fz, double** 
     Runtime& rt = runtime();
     for (int k = 0; k < 6; k++) {
         if (!o[k]) return XDEM_E_INVALID;
-        *o[k] = rt.bf[k] ? rt.bf[k] : (rt.own_bf ? rt.own_bf + k * rt.own_bf_n : nullptr);
+        *o[k] = rt.bf[k] ? rt.bf[k] : (rt.own_bf ? rt.own_bf + (size_t)k * rt.ws_n : nullptr);
     }
     return XDEM_OK;
 }
@@ -523,11 +565,53 @@ int xdem_get_body_force_buffers(double** fx, double** fy, double** fz, double** 
 int xdem_get_error_word(unsigned int* out, int clear)
 {
     if (!out) return XDEM_E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mutex);
     Runtime& rt = runtime();
-    if (!rt.err) { *out = 0; return XDEM_OK; }
-    XDEM_HIP_CHECK(hipMemcpyAsync(out, rt.err, sizeof(unsigned), hipMemcpyDeviceToHost, rt.stream));
-    XDEM_HIP_CHECK(hipStreamSynchronize(rt.stream));
-    if (clear) XDEM_HIP_CHECK(hipMemsetAsync(rt.err, 0, sizeof(unsigned), rt.stream));
+    unsigned dev_word = 0;
+    if (rt.err) {
+        XDEM_HIP_CHECK(hipMemcpyAsync(&dev_word, rt.err, sizeof(unsigned), hipMemcpyDeviceToHost, rt.stream));
+        XDEM_HIP_CHECK(hipStreamSynchronize(rt.stream));
+        if (clear) {
+            XDEM_HIP_CHECK(hipMemsetAsync(rt.err, 0, sizeof(unsigned), rt.stream));
+            XDEM_HIP_CHECK(hipStreamSynchronize(rt.stream));
+        }
+    }
+    *out = dev_word | rt.host_err;
+    if (clear) rt.host_err = 0;
+    return XDEM_OK;
+}
+
+int xdem_workspace_size(unsigned int n, unsigned long long* bytes)
+{
+    if (!bytes) return XDEM_E_INVALID;
+    *bytes = (unsigned long long)workspace_bytes(n);
+    return XDEM_OK;
+}
+
+int xdem_reserve_workspace(unsigned int n)
+{
+    std::lock_guard<std::mutex> lk(g_mutex);
+    XDEM_HIP_CHECK(ws_reserve(runtime(), n));
+    return XDEM_OK;
+}
+
+int xdem_set_workspace(void* d_ptr, unsigned long long bytes, unsigned int n)
+{
+    if (!d_ptr) {
+        // detach: the library no longer uses the caller's buffer (the next init call reserves
+        // a workspace of its own)
+        std::lock_guard<std::mutex> lk(g_mutex);
+        Runtime& rt = runtime();
+        if (rt.ws_owned && rt.ws) (void)hipFree(rt.ws);   // nothing to report on release
+        rt.ws = nullptr; rt.ws_bytes = 0; rt.ws_n = 0; rt.ws_owned = false;
+        rt.err = nullptr; rt.red_partial = nullptr; rt.sort_buf = nullptr; rt.sort_scratch = nullptr;
+        rt.own_bf = nullptr;
+        return XDEM_OK;
+    }
+    if (reinterpret_cast<uintptr_t>(d_ptr) & 255u) return XDEM_E_INVALID;
+    if (bytes < (unsigned long long)workspace_bytes(n)) return XDEM_E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mutex);
+    XDEM_HIP_CHECK(ws_attach(runtime(), static_cast<char*>(d_ptr), (size_t)bytes, n, false));
     return XDEM_OK;
 }
 
@@ -581,15 +665,16 @@ int xdem_cu_reorderDataAndFindCellStart(unsigned int* hash, unsigned int* index,
     if (ncell == 0) return XDEM_E_INVALID;
     std::lock_guard<std::mutex> lk(g_mutex);
     Runtime& rt = runtime();
-    XDEM_HIP_CHECK(runtime_reserve(np ? np : 1, 0));
+    const int rc = ws_require(rt, np);
+    if (rc) return rc;
     hipStream_t s = rt.stream;
     if (np > 0) {
         int bits = 1;
         while (bits < 32 && (1ull << bits) < (unsigned long long)ncell) bits++;
         uint32_t* kout = rt.sort_buf;
-        uint32_t* vout = rt.sort_buf + rt.sort_cap;
-        uint32_t* ktmp = rt.sort_buf + 2 * rt.sort_cap;
-        uint32_t* vtmp = rt.sort_buf + 3 * rt.sort_cap;
+        uint32_t* vout = rt.sort_buf + rt.ws_n;
+        uint32_t* ktmp = rt.sort_buf + 2 * (size_t)rt.ws_n;
+        uint32_t* vtmp = rt.sort_buf + 3 * (size_t)rt.ws_n;
         XDEM_HIP_CHECK(radix_sort_pairs(hash, index, kout, vout, ktmp, vtmp, np, bits, rt.sort_scratch, s));
         XDEM_HIP_CHECK(hipMemcpyAsync(hash, kout, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, s));
         XDEM_HIP_CHECK(hipMemcpyAsync(index, vout, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, s));
@@ -615,7 +700,11 @@ int xdem_cu_calculate_p2p(double* pos, double* ep, double* vel, double* ev, doub
     if (!pos || !ep || !vel || !ev || !force || !moment || !mass || !tmax || !rres || !pair_count || !pair_id || !tsd ||
         !sorted_index || !cstart || !cend || !cp)
         return XDEM_E_INVALID;
-    XDEM_HIP_CHECK(runtime_reserve(1, 0));
+    {
+        std::lock_guard<std::mutex> lk(g_mutex);
+        const int rc = ws_require(runtime(), 0);
+        if (rc) return rc;
+    }
     const xdem_quirks q = runtime().quirks;
     const dim3 g(nblocks(np)), b(kThreads);
     hipStream_t s = runtime().stream;
@@ -645,8 +734,9 @@ static int body_force_ptrs(unsigned np, double** bf)
         for (int k = 0; k < 6; k++) bf[k] = rt.bf[k];
         return XDEM_OK;
     }
-    if (runtime_reserve(1, np) != hipSuccess) return XDEM_E_HIP;
-    for (int k = 0; k < 6; k++) bf[k] = rt.own_bf + (size_t)k * rt.own_bf_n;
+    const int rc = ws_require(rt, np);
+    if (rc) return rc;
+    for (int k = 0; k < 6; k++) bf[k] = rt.own_bf + (size_t)k * rt.ws_n;
     return XDEM_OK;
 }
 
@@ -661,7 +751,8 @@ int xdem_cu_plane_contact_force(xdem_plane_info* plan, double* dbi, xdem_contact
         !pair_count || !pair_id || !tsd)
         return XDEM_E_INVALID;
     std::lock_guard<std::mutex> lk(g_mutex);
-    XDEM_HIP_CHECK(runtime_reserve(1, 0));
+    int rc0 = ws_require(runtime(), 0);
+    if (rc0) return rc0;
     double* bf[6];
     int rc = body_force_ptrs(np, bf);
     if (rc) return rc;
@@ -692,7 +783,8 @@ int xdem_cu_particle_polygonObject_collision(xdem_triangle_info* dpi, double* db
         !pair_id || !tsd || !sidx || !cstart || !cend || !cp || ntriangle == 0)
         return XDEM_E_INVALID;
     std::lock_guard<std::mutex> lk(g_mutex);
-    XDEM_HIP_CHECK(runtime_reserve(1, 0));
+    int rc0 = ws_require(runtime(), 0);
+    if (rc0) return rc0;
     double* bf[6];
     int rc = body_force_ptrs(np, bf);
     if (rc) return rc;
@@ -732,8 +824,9 @@ int xdem_reduction(double* in, unsigned int np, double* out)
     if (!out || (!in && np)) return XDEM_E_INVALID;
     if (np == 0) { *out = 0.0; return XDEM_OK; }
     std::lock_guard<std::mutex> lk(g_mutex);
-    XDEM_HIP_CHECK(runtime_reserve(1, 0));
     Runtime& rt = runtime();
+    const int rc = ws_require(rt, 0);
+    if (rc) return rc;
     const unsigned nb = std::min(1023u, (np + 255) / 256);
     hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(256), 0, rt.stream, in, np, rt.red_partial);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, rt.stream, rt.red_partial, nb, rt.red_partial + 1023);

@@ -18,21 +18,27 @@ struct Runtime {
     xdem_quirks quirks = {0, 0, 1, 1};
     hipStream_t stream = nullptr;
     double* bf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    // scratch owned by the library (grown on demand; launchers may allocate, as the
-    // reference's thrust calls do)
+    // Workspace of the launchers (device error word, reduction partials, sort buffers and
+    // histograms, body-force buffers used when none are registered) for launches of up to
+    // ws_n elements: reserved by the library at xdem_setDEMSymbolicParameter or
+    // xdem_reserve_workspace (init calls), or owned by the caller (xdem_set_workspace). No
+    // launcher allocates: one that needs more than the workspace holds returns an error.
+    char* ws = nullptr;
+    size_t ws_bytes = 0;
+    uint32_t ws_n = 0;
+    bool ws_owned = false;            // allocated by the library (freed when replaced)
     uint32_t* err = nullptr;          // device error word
+    double* red_partial = nullptr;    // reduction partials (1024)
+    uint32_t* sort_buf = nullptr;     // 4 arrays of ws_n entries
     void* sort_scratch = nullptr;
-    size_t sort_scratch_bytes = 0;
-    uint32_t* sort_buf = nullptr;     // 4 arrays of sort_cap entries
-    size_t sort_cap = 0;
-    double* red_partial = nullptr;    // reduction partials
-    double* own_bf = nullptr;         // body-force buffers used when none are registered
-    size_t own_bf_n = 0;
+    double* own_bf = nullptr;         // 6 arrays of ws_n entries
+    unsigned host_err = 0;            // XDEM_ERR_WORKSPACE, reported with the device word
 };
 
 Runtime& runtime();
-// Makes sure the runtime scratch holds at least n sort entries and nbf body-force entries.
-hipError_t runtime_reserve(size_t n_sort, size_t n_bf);
+
+// Bytes of the launcher workspace for launches of up to n elements.
+size_t workspace_bytes(uint32_t n);
 
 #define XDEM_HIP_CHECK(expr)                         \
     do {                                             \

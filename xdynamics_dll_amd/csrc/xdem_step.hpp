@@ -35,14 +35,13 @@ constexpr uint32_t kBarFlags = 64;            // release flags of k_rebuild's ph
 constexpr uint32_t kRebuildPhases = 6;        // k_rebuild: count, reduce, parts, apply, scatter, order
 constexpr uint32_t kClaimGroups = 8;          // unit ranges per phase, each with a cursor and a counter
 // k_rebuild's slots (each on its own 128-byte line): per phase and range a cursor and a done
-// counter, per phase a count of completed ranges, the release flags, and a word set when a
-// phase gave up waiting (k_touch then leaves the lists of the last build as they are)
+// counter, per phase a count of completed ranges, and the release flags (each holds the
+// number of the last completed phase + 1; kRebuildPhases once the build is complete)
 __host__ __device__ constexpr uint32_t bar_cursor(uint32_t k, uint32_t c) { return k * kClaimGroups + c; }
 __host__ __device__ constexpr uint32_t bar_done(uint32_t k, uint32_t c) { return (kRebuildPhases + k) * kClaimGroups + c; }
 __host__ __device__ constexpr uint32_t bar_groups(uint32_t k) { return 2 * kRebuildPhases * kClaimGroups + k; }
 __host__ __device__ constexpr uint32_t bar_flag(uint32_t f) { return 2 * kRebuildPhases * kClaimGroups + kRebuildPhases + f; }
-constexpr uint32_t kRebuildFailed = bar_flag(kBarFlags);
-constexpr uint32_t kBarSlots = kRebuildFailed + 1;
+constexpr uint32_t kBarSlots = bar_flag(kBarFlags);
 // Entries per segment for nsorted sorted slots launched in blocks of 256 (the most a
 // segment can receive: 64 spheres from each of its waves).
 inline uint32_t queue_segment_cap(uint32_t nsorted)
@@ -56,10 +55,15 @@ constexpr int kMaxWallsPerScene = 16;
 // on a list build with `adaptive` set, the skin scale doubles when the lists lasted fewer
 // than 16 steps and halves when they lasted more than 64 (within [min_scale, 1]). The scale
 // changes only at the start of a build, before anything uses it, so each list generation
-// has one skin throughout.
+// has one skin throughout. body_path is the summed displacement bound of every pose update of
+// a wall or mesh body (each update adds the largest distance any point of the body moved),
+// body_path_build its value at the last build: the bodies have moved at most their difference
+// since, and a pose update that makes it reach half the smallest skin asks for a build.
 struct ListCtl {
     float scale, min_scale;
     uint32_t step, last_build, adaptive, pad_;
+    double body_path, body_path_build;
+    double half_skin_min;     // half the smallest scene skin at scale 1
 };
 
 struct SceneDev {
@@ -519,10 +523,19 @@ template <bool MULTI>
 __global__ __launch_bounds__(256) void k_touch(StepArgs A)
 {
     const bool built = *A.rebuild_cur != 0u;
-    const bool failed = built && A.grid_bar[kSegStride * kRebuildFailed] != 0u;
-    // a build whose phases did not complete is tried again on the next step
-    if (failed && A.rebuild_next && blockIdx.x == 0 && threadIdx.x == 0) *A.rebuild_next = 1u;
-    if (built && !failed) {
+    // A build always completes: a rebuild block that stops waiting (XDEM_ERR_GRID_SYNC, a
+    // warning) has done every unit it claimed, and the block that completes a phase's last
+    // unit passes its own wait and goes on, so every later phase still runs to the end. The
+    // flags are checked anyway: without the last phase the bucket table may be inconsistent,
+    // so then no list is read at all; every sphere loses kTouch and the lite pass completes
+    // the step without contacts, and the error word says so.
+    if (built && A.grid_bar[kSegStride * bar_flag(0)] < kRebuildPhases) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(A.err, XDEM_ERR_GRID_SYNC | XDEM_ERR_BUILD_INCOMPLETE);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < A.np; i += gridDim.x * blockDim.x)
+            A.nbr_count[i] &= ~kTouch;
+        return;
+    }
+    if (built) {
         // virtual blocks of 256 sorted slots, block vb on the XCD of physical block vb mod
         // gridDim (gridDim is a multiple of 8)
         const uint32_t nvb = (A.nsorted + blockDim.x - 1) / blockDim.x;
@@ -559,7 +572,9 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
         if (threadIdx.x == 0) dm_sh = d;
     }
     __syncthreads();
-    const float dm = dm_sh;
+    // moving walls and meshes: their displacement since the build bounds the wall part of
+    // the gap the same way (taking the larger of the two keeps both bounds)
+    const float dm = fmaxf(dm_sh, __double2float_ru(A.lctl->body_path - A.lctl->body_path_build));
     for (; base < cnt; base += nchunk * blockDim.x) {
         const uint32_t e = base + threadIdx.x;
         bool touch = false;
@@ -818,8 +833,8 @@ __device__ __forceinline__ void lite_sphere(const StepArgs& A, const SceneDev& S
     if (err) atomicOr(A.err, err);
 }
 
-// k_rebuild's phase cursors, counters, release flags and failed word start at zero: cleared
-// by block 0 of the lite pass (after k_touch has read the failed word, before the next step's
+// k_rebuild's phase cursors, counters and release flags start at zero: cleared by block 0 of
+// the lite pass (after k_touch has read the release flags, before the next step's
 // k_rebuild).
 __device__ __forceinline__ void reset_rebuild_slots(const StepArgs& A)
 {
