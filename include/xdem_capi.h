@@ -94,6 +94,16 @@ int xdem_cu_particle_polygonObject_collision(xdem_triangle_info* dpi, double* db
                                              unsigned int* pair_id, double* tsd, double* dsph,
                                              unsigned int* sidx, unsigned int* cstart, unsigned int* cend,
                                              xdem_contact_property* cp, unsigned int np, unsigned int ntriangle);
+/* No counterpart in the reference. For callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation): the same sort and cell
+ * bounds as xdem_cu_reorderDataAndFindCellStart, but its sort buffers live in a file-scope
+ * device array of the library (16.25 MiB per GPU, held until the process ends), so it needs
+ * no workspace and calls no allocating routine. At most 1,048,576 elements per call (larger np:
+ * XDEM_E_INVALID, nothing queued). Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams or threads) race. */
+int xdem_cu_reorderDataAndFindCellStart_noalloc(unsigned int* hash, unsigned int* index, unsigned int* cstart,
+                                                unsigned int* cend, unsigned int* sorted_index, unsigned int np,
+                                                unsigned int ncell);
 /* cu_decide_rolling_friction_moment (decl:336-343) */
 int xdem_cu_decide_rolling_friction_moment(double* tmax, double* rres, double* inertia, double* ep, double* ev,
                                            double* moment, unsigned int np);

@@ -145,6 +145,7 @@ LAUNCHER_SIGNATURES = {
     "xdem_cu_calculateHashAndIndex": [P, P, P, U, U],
     "xdem_cu_calculateHashAndIndexForPolygonSphere": [P, P, U, U, P],
     "xdem_cu_reorderDataAndFindCellStart": [P] * 5 + [U, U],
+    "xdem_cu_reorderDataAndFindCellStart_noalloc": [P] * 5 + [U, U],
     "xdem_cu_calculate_p2p": [P] * 16 + [U],
     "xdem_cu_plane_contact_force": [P] * 15 + [U],
     "xdem_cu_particle_polygonObject_collision": [P] * 19 + [U, U],
