@@ -34,45 +34,63 @@ def step_bytes(cbar: float) -> float:
     return 436.0 + 40.0 * cbar
 
 
-def contact_kernel_bytes(cbar: float, fused: bool) -> float:
-    """Algorithmic bytes per sphere of the contact pass.
-    Fused (k_lite_move + k_lite_turn for the spheres outside this step's full pass, k_collide
-    for the others: contacts, rolling, velocity update and the next step's drift): own state
-    read pos 32, vel 24, acc 24, ep 32, ev 32, ea 32, mass 8, inertia 24 (= 208); state write
-    pos 32, vel 24, acc 24, ep 32, ev 32, ea 32 (= 176); list flags 2*4, path length 2*4;
-    sphere and wall history: counts 2*4, entries 2*20*cbar.
-    Split (k_collide_lite + k_collide, contacts only): read pos 32, vel 24, ep 32, ev 32,
-    mass 8 (= 128); write force, moment, rolling torque sum and magnitude 24+24+24+8 (= 80);
-    history 16 + 40*cbar."""
-    return (408.0 if fused else 224.0) + 40.0 * cbar
-
-
 # Algorithmic bytes per sphere of k_lite_move (the translation of every sphere outside the
 # step's full pass): the translational part of the per-sphere state traffic, read pos 32,
 # vel 24, acc 24, mass 8 and write pos 32, vel 24, acc 24.
 LITE_MOVE_BYTES = 168.0
 
 
-def roofline(tm, n, cbar, fuse, info, traffic):
-    """Roofline object of the dominant kernel. Fused mode: k_lite_move, timed with HIP events
-    recorded on the context's stream right before and after it; its units are the spheres
-    outside the full pass (n - contact_spheres of the last step). Split mode: the contact
-    pass (both streams, fork to join)."""
+def pairs_kernel_bytes(spheres: float, pairs: float) -> float:
+    """Algorithmic bytes of one k_pairs launch: per full-pass sphere its own pos 32, vel 24,
+    mass 8, ep 32, ev 32, list flags 4, pass mask 8, old count 4, new count 4 and the ten pair
+    sums 80 (= 228); per listed pair that passes, its list entry 4, the old history slot read
+    (id 4, tsd 16) and the new one written (id 4, tsd 16) (= 44). A neighbour's state is some
+    other sphere's own state: it is counted once, as that sphere's (re-reads hit the caches
+    or are waste)."""
+    return 228.0 * spheres + 44.0 * pairs
+
+
+def collide_kernel_bytes(spheres: float, wall_contacts: float) -> float:
+    """Algorithmic bytes of one fused k_collide launch: per full-pass sphere its state read
+    (pos 32, vel 24, acc 24, ep 32, ev 32, ea 32, mass 8, inertia 24 = 208), the pair sums of
+    k_pairs 80, the list flags 4 and path length 4, the state written (pos 32, vel 24, acc 24,
+    ep 32, ev 32, ea 32 = 176) and the wall-slot count read and written 8 (= 480); per wall
+    contact its slot read and written (id 4 + tsd 16, twice = 40)."""
+    return 480.0 * spheres + 40.0 * wall_contacts
+
+
+def kernel_rooflines(tm, n, info, n_pp, n_wall, fuse):
+    """Per-step time and achieved HBM bandwidth of the step's kernels from the event-timed
+    steps: k_lite_move (spheres outside the full pass), k_pairs and k_collide (the full-pass
+    spheres, contact_spheres of the last step)."""
     steps = max(tm["steps"], 1)
-    if fuse:
-        k_ms = tm["lite_move_ms"] / steps
-        units = n - info["contact_spheres"]
-        achieved = LITE_MOVE_BYTES * units / (k_ms * 1e-3) / 1e9
-        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                "kernel": "k_lite_move (translation of the spheres outside the full contact pass)",
-                "kernel_ms": k_ms, "bytes_per_unit": LITE_MOVE_BYTES, "units": units}
-    k_ms = tm["contact_ms"] / steps
-    achieved = contact_kernel_bytes(cbar, False) * n / (k_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-            "kernel": "contact pass: k_collide_lite, k_collide on a second stream (contacts)",
-            "kernel_ms": k_ms, "bytes_per_unit": contact_kernel_bytes(cbar, False), "units": n}
+    full = info["contact_spheres"]
+    out = {}
+    if fuse and tm["lite_move_ms"] > 0:
+        ms = tm["lite_move_ms"] / steps
+        b = LITE_MOVE_BYTES * (n - full)
+        out["k_lite_move"] = {"ms": ms, "bytes": b, "units": n - full, "bytes_per_unit": LITE_MOVE_BYTES}
+    if tm.get("pair_pass_ms", 0) > 0:
+        ms = tm["pair_pass_ms"] / steps
+        b = pairs_kernel_bytes(full, n_pp)
+        out["k_pairs"] = {"ms": ms, "bytes": b, "units": full, "bytes_per_unit": b / max(full, 1)}
+    if fuse and tm.get("collide_ms", 0) > 0:
+        ms = tm["collide_ms"] / steps
+        b = collide_kernel_bytes(full, n_wall)
+        out["k_collide"] = {"ms": ms, "bytes": b, "units": full, "bytes_per_unit": b / max(full, 1)}
+    for k in out.values():
+        k["gbps"] = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+        k["frac"] = k["gbps"] / HBM_PEAK_GBPS
+    return out
+
+
+def dominant_roofline(kr, traffic=None):
+    """The roofline object of the kernel with the longest time per step."""
+    name = max(kr, key=lambda k: kr[k]["ms"])
+    k = kr[name]
+    return {"bound": "hbm", "achieved": k["gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": k["frac"],
+            "traffic": traffic.get(name) if traffic else None, "kernel": name, "kernel_ms": k["ms"],
+            "bytes_per_unit": k["bytes_per_unit"], "units": k["units"]}
 
 
 def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0, skin=0.0, graph=0):
@@ -160,23 +178,55 @@ def force_error(scene, capacity, nsteps=40):
     return out
 
 
-def cpu_baseline(scene, capacity, budget_s=12.0):
+def host_cpu():
+    """The host's CPU model and the cores this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, usable, os.cpu_count() or 1
+
+
+def cpu_baseline(scene, capacity, budget_s=12.0, state=None, mesh=None, label=None):
+    """The oracle (FP64 C restatement, OpenMP over spheres) stepping the same scene on the host:
+    first on every core this process may use, then on one thread. state (field -> array) and
+    mesh ((triangles, pose, ContactProperty)) set it to a given step of the GPU run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import xdem_ref
 
-    threads = min(16, os.cpu_count() or 1)
-    xdem_ref.set_threads(threads)
-    ref = xdem_ref.RefSim(scene, capacity=capacity)
-    t0 = time.perf_counter()
-    ref.step(1)
-    first = time.perf_counter() - t0
-    nsteps = max(1, min(20, int(budget_s / max(first, 1e-3))))
-    t0 = time.perf_counter()
-    ref.step(nsteps)
-    t = time.perf_counter() - t0
-    return {"value": scene.np * nsteps / t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{nsteps} steps of the same 1,000,000-sphere scene after 1 untimed step, "
-                      f"FP64 C restatement, OpenMP over spheres"}
+    model, usable, total = host_cpu()
+
+    def timed(threads, budget):
+        xdem_ref.set_threads(threads)
+        ref = xdem_ref.RefSim(scene, capacity=capacity)
+        if mesh is not None:
+            ref.add_mesh(*mesh)
+        for f, v in (state or {}).items():
+            ref.write(f, v)
+        t0 = time.perf_counter()
+        ref.step(1)
+        first = time.perf_counter() - t0
+        nsteps = max(1, min(20, int(budget / max(first, 1e-3))))
+        t0 = time.perf_counter()
+        ref.step(nsteps)
+        return scene.np * nsteps / (time.perf_counter() - t0), nsteps
+
+    v_all, n_all = timed(usable, budget_s)
+    v_one, n_one = timed(1, 0.5 * budget_s)
+    what = label or f"the same {scene.np:,}-sphere scene"
+    return {"value": v_all, "unit": "particle-steps/s", "cores": usable, "kind": "port",
+            "sample": f"{n_all} steps ({usable} threads) and {n_one} steps (1 thread) of {what} after 1 untimed "
+                      "step each, FP64 C restatement, OpenMP over spheres",
+            "value_1_thread": v_one, "cpu_model": model, "host_cpus": total}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_lite_move_dam_break_1m.json")
@@ -247,7 +297,8 @@ def batch_main(args):
                                "dt = min(1e-5 s, 0.2 Rayleigh time) per scene",
                    "np_total": n, "nscene": args.nscene, "nbucket": mixed["info"]["nbucket"],
                    "parallelism": "single GPU, all scenes in one context"},
-        "roofline": roofline(tm, n, cbar, fuse, mixed["info"], None),
+        "roofline": dominant_roofline(kernel_rooflines(tm, n, mixed["info"], mixed["n_pp"], mixed["n_wall"], fuse)),
+        "kernels": kernel_rooflines(tm, n, mixed["info"], mixed["n_pp"], mixed["n_wall"], fuse),
         "mean_contacts_per_sphere": cbar,
         "contact_pass_ms_per_step": tm["contact_ms"] / max(tm["steps"], 1),
         "rebuild_kernel_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
@@ -279,6 +330,151 @@ def batch_main(args):
     print(json.dumps(out), flush=True)
 
 
+def wheel_main(args):
+    """The wheel on a granular bed (seed 1003): 500,000 spheres r ~ U[2, 3] mm inserted loose
+    into a 1.0 x 0.4 m bin and settled under gravity (untimed), then a grousered wheel (4,096
+    triangles, R = 0.15 m, width 0.1 m, 40 kg) set on the bed with its lowest point 1 mm in it
+    and driven at 2 rad/s about y by the host HHT integrator (alpha = -0.1) under a 200 N load,
+    sphere-triangle friction 0.6, dt = 1e-5 s, mixed precision. Timed: the coupled steps (DEM
+    step with the wheel's wrench reduced on the device, the HHT step on the host, the new pose
+    back), run by the native loop xdem_cosim_wheel_steps."""
+    import torch
+    from xdynamics_dll_amd import DemContext, SceneData, contact_property, wheel_mesh
+    from xdynamics_dll_amd.mbd import HhtWheel, cosim_wheel_steps
+
+    t_build = time.perf_counter()
+    scene = SceneData.build(3, 1003, np_override=args.np)
+    tris = wheel_mesh()
+    n = scene.np
+    L, W = scene.planes[2].xw.x, scene.planes[4].xw.y
+    xc, yc = 0.5 * L, 0.5 * W
+    R = float(np.sqrt((tris.reshape(-1, 3)[:, [0, 2]] ** 2).sum(1)).max())   # rim or grouser tip
+    park = np.zeros(15)
+    park[0], park[7] = 40.0, 1.0
+    park[1:4] = (xc, yc, 1.0)          # above the bin while the bed settles
+    cp_wheel = contact_property(2e11, 5e7, 0.3, 0.25, 0.4, 0.6, 0.05)
+    ctx = DemContext(scene, precision="mixed", p2p_capacity=args.capacity, fuse_velocity=True, step_graph=50,
+                     meshes=[(0, tris, park, cp_wheel)])
+    t_build = time.perf_counter() - t_build
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.settle:      # progress every 5,000 steps
+        k = min(5000, args.settle - done)
+        ctx.step(k)
+        ctx.synchronize()
+        done += k
+        print(f"[wheel] settled {done} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    t_settle = time.perf_counter() - t0
+    pos = ctx.read("pos")
+    vel = ctx.read("vel")
+    under = (np.abs(pos[:, 0] - xc) < 0.08) & (np.abs(pos[:, 1] - yc) < 0.06)
+    top = float((pos[under, 2] + pos[under, 3]).max())
+    bed = {"top_m": top, "max_speed_after_settle": float(np.linalg.norm(vel, axis=1).max()),
+           "mean_speed_after_settle": float(np.linalg.norm(vel, axis=1).mean())}
+    omega = 2.0
+    start = park.copy()
+    start[3] = top + R - 1e-3
+    start[13] = 0.5 * omega            # e_dot of a turn about y at omega from e = (1, 0, 0, 0)
+    ctx.set_mesh_pose(0, start)
+    inertia = (40.0 * (3 * 0.15 ** 2 + 0.1 ** 2) / 12.0, 0.5 * 40.0 * 0.15 ** 2, 40.0 * (3 * 0.15 ** 2 + 0.1 ** 2) / 12.0)
+    hht = HhtWheel(40.0, inertia, start, omega, scene.dt, alpha=-0.1, gravity=scene.gravity, load=(0.0, 0.0, -200.0))
+    warm = cosim_wheel_steps(ctx, hht, 0, args.warmup)
+    ctx.synchronize()
+    info0 = ctx.info()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    track = cosim_wheel_steps(ctx, hht, 0, args.steps)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    info = ctx.info()
+    n_pp, n_wall = ctx.contact_count()
+    n_mesh = int(ctx.history("mesh")[0].sum())
+    pose = hht.state()[0]
+    state = {f: ctx.read(f) for f in ("pos", "vel", "acc", "ep", "ev", "ea")}
+    # phase breakdown: 200 further steps launched one by one with HIP events (wheel held at
+    # its last pose)
+    ctx.set_timing(True)
+    ctx.step(200)
+    ctx.synchronize()
+    ctx.set_timing(False)
+    tm = ctx.timing()
+    err = ctx.error_word()
+    ctx.close()
+    value = n * args.steps / t
+    kr = kernel_rooflines(tm, n, info, n_pp, n_wall, True)
+    track = np.concatenate([warm, track])
+    out = {
+        "metric": "particle-steps/s, wheel on a granular bed (DEM + host HHT coupling)",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-contact/fp64-state", "data": "synthetic",
+        "config": {"workload": f"wheel on a granular bed: {n:,} spheres r~U[2,3] mm settled {args.settle} steps in a "
+                               f"{L:.2f}x{W:.2f} m bin, 4,096-triangle grousered wheel R=0.15 m, 40 kg, "
+                               "2 rad/s about y, 200 N load, HHT alpha=-0.1, mu=0.6, dt=1e-5 s, mixed",
+                   "np": n, "ntriangle": len(tris), "parallelism": "single GPU + host HHT"},
+        "roofline": dominant_roofline(kr),
+        "kernels": kr,
+        "mean_contacts_per_sphere": (n_pp + n_wall + n_mesh) / n,
+        "sphere_contacts": n_pp, "wall_contacts": n_wall, "mesh_contacts": n_mesh,
+        "contact_spheres": info["contact_spheres"], "mesh_spheres": info["mesh_spheres"],
+        "contact_pass_ms_per_step": tm["contact_ms"] / max(tm["steps"], 1),
+        "mesh_pass_ms_per_step": tm["mesh_pass_ms"] / max(tm["steps"], 1),
+        "rebuild_kernel_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
+        "touch_ms_per_step": tm["touch_ms"] / max(tm["steps"], 1),
+        "list_builds_timed": info["list_builds"] - info0["list_builds"],
+        "bed": bed,
+        "wheel": {"x_travel_m": float(track[-1, 0] - start[1]), "sinkage_m": float(start[3] - track[-1, 1]),
+                  "mean_Fz_N_last_1000": float(track[-1000:, 5].mean()),
+                  "mean_Fx_N_last_1000": float(track[-1000:, 4].mean()),
+                  "max_newton_iterations": int(track[:, 7].max())},
+        "error_word": err, "scene_build_s": t_build, "settle_s": t_settle,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, args.capacity, state=state, mesh=(tris, pose, cp_wheel),
+                                           label=f"the {n:,}-sphere bed with the wheel at the end of the timed run")
+    print(json.dumps(out), flush=True)
+
+
+def lattice_main(args):
+    """The lattice box (seed 1001): 10,000 spheres r = 1 mm on a 2.2 mm cubic lattice with
+    +-0.05 mm jitter, at rest in a 0.1 x 0.1 x 0.3 m five-wall box, E = 1e7 Pa, e = 0.3,
+    mu = 0.3, mu_r = 0.01, gravity -z, dt = 1e-5 s, FP64. Timed: the steps after warmup (the
+    lattice falls and collapses onto the floor)."""
+    import torch
+    from xdynamics_dll_amd import SceneData
+
+    scene = SceneData.build(1, 1001, np_override=args.np)
+    n = scene.np
+    fuse = not args.split
+    r = run_gpu(scene, "fp64", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin,
+                args.graph)
+    value = n * args.steps / r["seconds"]
+    tm = r["timing"]
+    kr = kernel_rooflines(tm, n, r["info"], r["n_pp"], r["n_wall"], fuse)
+    out = {
+        "metric": "particle-steps/s, lattice box, FP64",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["seconds"] / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"lattice box: {n:,} spheres r=1 mm, 2.2 mm lattice +-0.05 mm jitter, "
+                               "0.1x0.1x0.3 m 5-wall box, dt=1e-5 s, FP64", "np": n, "seed": 1001,
+                   "parallelism": "single GPU"},
+        "roofline": dominant_roofline(kr),
+        "kernels": kr,
+        "mean_contacts_per_sphere": (r["n_pp"] + r["n_wall"]) / n,
+        "contact_spheres": r["info"]["contact_spheres"],
+        "contact_pass_ms_per_step": tm["contact_ms"] / max(tm["steps"], 1),
+        "rebuild_kernel_ms_per_step": tm["grid_ms"] / max(tm["steps"], 1),
+        "touch_ms_per_step": tm["touch_ms"] / max(tm["steps"], 1),
+        "list_builds_timed": r["info"]["timed_list_builds"],
+        "error_word": r["err"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, args.capacity)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,8 +492,11 @@ def main():
                     help="neighbour-list skin in m (0: adaptive, <0: rebuild every step)")
     ap.add_argument("--buckets-per-sphere", type=int, default=0,
                     help="bucket budget of the grid per sphere (0 = library default)")
-    ap.add_argument("--workload", choices=["dam_break", "batch"], default="dam_break",
-                    help="dam_break: the 1,000,000-sphere column (default); batch: the batched scene sweep")
+    ap.add_argument("--workload", choices=["dam_break", "batch", "wheel", "lattice"], default="dam_break",
+                    help="dam_break: the 1,000,000-sphere column (default); batch: the batched scene sweep; "
+                         "wheel: the wheel on a 500,000-sphere bed with HHT coupling; lattice: the 10,000-sphere "
+                         "lattice box in FP64")
+    ap.add_argument("--settle", type=int, default=20000, help="untimed settling steps of the wheel's bed")
     ap.add_argument("--nscene", type=int, default=256, help="scenes of the batch workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
@@ -317,6 +516,12 @@ def main():
     if args.workload == "batch":
         batch_main(args)
         return
+    if args.workload == "wheel":
+        wheel_main(args)
+        return
+    if args.workload == "lattice":
+        lattice_main(args)
+        return
     t_build = time.perf_counter()
     scene = SceneData.build(2, SEED, np_override=args.np)
     t_build = time.perf_counter() - t_build
@@ -328,6 +533,7 @@ def main():
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
     step_gbps = step_bytes(cbar) * n * args.steps / mixed["seconds"] / 1e9
+    kr = kernel_rooflines(tm, n, mixed["info"], mixed["n_pp"], mixed["n_wall"], fuse)
 
     out = {
         "metric": "particle-steps/s @1M spheres, mixed-FP32 vs FP64 GPU (<=1e-4 force err), GB/s",
@@ -346,7 +552,8 @@ def main():
                                "5 walls, mixed FP32 contact arithmetic (FP64 positions/history/sums)",
                    "np": n, "seed": SEED, "p2p_capacity": args.capacity, "parallelism": "single GPU",
                    "nbucket": mixed["info"]["nbucket"]},
-        "roofline": roofline(tm, n, cbar, fuse, mixed["info"], measured_traffic(n, fuse)),
+        "roofline": dominant_roofline(kr, {"k_lite_move": measured_traffic(n, fuse)}),
+        "kernels": kr,
         "step_gbps": step_gbps,
         "step_roofline_frac": step_gbps / HBM_PEAK_GBPS,
         "mean_contacts_per_sphere": cbar,

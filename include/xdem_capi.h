@@ -172,6 +172,22 @@ typedef struct xdem_ctx_config {
 /* History kinds for xdem_ctx_read_history. */
 #define XDEM_HISTORY_P2P   0
 #define XDEM_HISTORY_PLANE 1
+#define XDEM_HISTORY_MESH  2  /* 5 slots; ids are triangle indices of the scene's mesh */
+
+/* A rigid triangle-mesh body (reference: xMeshObject + xParticleMeshObjectContact::define,
+ * source/xdynamics_object/xParticleMeshObjectContact.cpp:77-167). Vertices are given in the
+ * body frame; the pose places them (global = position + A(e) local). The proxy sphere of each
+ * triangle (xUtilityFunctions::FitSphereToTriangle with factor 0.8, fitted to the global
+ * triangle at this pose and kept in the body frame) fixes the reference's visiting order and
+ * enlarges the scene's cell to twice the largest proxy radius
+ * (xDiscreteElementMethodSimulation.cpp:91-100). A body with mass > 0 is dynamic: its
+ * contact wrench is reduced every step for a multibody solver. */
+typedef struct xdem_mesh {
+    unsigned int ntriangle;
+    const double* vertices;      /* [9*ntriangle] P, Q, R of every triangle, body frame */
+    double dbi[15];              /* pose at definition, MM..ED3 */
+    xdem_contact_property cp;    /* sphere-mesh material pair */
+} xdem_mesh;
 
 typedef struct xdem_ctx_info {
     unsigned int nscene, np, nsorted, nplane_total;
@@ -183,6 +199,8 @@ typedef struct xdem_ctx_info {
     unsigned long long list_builds;  /* steps that rebuilt the grid and the neighbour lists */
     unsigned int list_spheres;       /* spheres with listed neighbours or a wall near (last build) */
     unsigned int contact_spheres;    /* spheres of the last step's full contact pass */
+    unsigned int mesh_spheres;       /* spheres with triangles listed (last build) */
+    unsigned int ntriangle;          /* triangles of all mesh bodies */
 } xdem_ctx_info;
 
 int xdem_ctx_create(const xdem_ctx_config* cfg, xdem_ctx** out);
@@ -210,10 +228,16 @@ int xdem_ctx_write(xdem_ctx* ctx, int field, unsigned int scene, const double* h
  * step to step: fetch the pointer again after every step. */
 int xdem_ctx_device_ptr(xdem_ctx* ctx, int field, void** out);
 /* History of one scene in the reference layout: count[np], ids[np*cap], tsd[2*np*cap]
- * (cap = p2p_capacity for XDEM_HISTORY_P2P, 3 for XDEM_HISTORY_PLANE). Sphere ids are
- * scene-local particle indices. */
+ * (cap = p2p_capacity for XDEM_HISTORY_P2P, 3 for XDEM_HISTORY_PLANE, 5 for XDEM_HISTORY_MESH).
+ * Sphere ids are scene-local particle indices. */
 int xdem_ctx_read_history(xdem_ctx* ctx, int kind, unsigned int scene, unsigned int* count, unsigned int* ids,
                           double* tsd);
+/* Writes one scene's history in the same layout (the inverse of xdem_ctx_read_history; the
+ * restart of xContactManager::set_from_part_result, source/xdynamics_object/xContactManager.cpp
+ * via xDynamicSimulator.cpp:468-487). A count above the slots, or a sphere or triangle id
+ * outside the scene, refuses the whole history with XDEM_E_INVALID and writes nothing. */
+int xdem_ctx_write_history(xdem_ctx* ctx, int kind, unsigned int scene, const unsigned int* count,
+                           const unsigned int* ids, const double* tsd);
 /* Device error word (XDEM_ERR_* bits) accumulated since the last clear. */
 int xdem_ctx_error_word(xdem_ctx* ctx, unsigned int* out, int clear);
 /* Body wrench of every wall of a scene from the last step: out[6*k .. 6*k+5] =
@@ -223,7 +247,38 @@ int xdem_ctx_body_wrench(xdem_ctx* ctx, unsigned int scene, double* out);
 /* New pose of wall k of a scene, 15 doubles in the MM..ED3 layout; the wall geometry is
  * moved with it (xParticlePlaneContact::update, xParticlePlaneContact.cpp:96-131). */
 int xdem_ctx_set_body_pose(xdem_ctx* ctx, unsigned int scene, unsigned int wall, const double dbi[15]);
+/* Adds the mesh body of a scene (at most one per scene; all of a scene's triangle contacts
+ * share the reference's 5 history slots per sphere). Between add_scene and commit. */
+int xdem_ctx_add_mesh(xdem_ctx* ctx, unsigned int scene, const xdem_mesh* mesh, unsigned int* mesh_index);
+/* New pose of a mesh body (15 doubles, MM..ED3): its triangles and proxy centres are placed
+ * from their body-frame coordinates (xParticleMeshObjectContact::update, impl:2544-2601). The
+ * neighbour lists stay valid until the summed motion of the bodies since the last build
+ * reaches half the skin. */
+int xdem_ctx_set_mesh_pose(xdem_ctx* ctx, unsigned int mesh, const double dbi[15]);
+/* Wrench (Fx, Fy, Fz, Mx, My, Mz) the spheres exerted on a dynamic mesh body in the last step
+ * (-F and -(contact point - body position) x F summed over its contacts; the reference's six
+ * reductions per body, xParticleMeshObjectContact.cpp:209-219). Zeros for a fixed mesh. */
+int xdem_ctx_mesh_wrench(xdem_ctx* ctx, unsigned int mesh, double out[6]);
 int xdem_ctx_info_get(xdem_ctx* ctx, xdem_ctx_info* out);
+/* Sizes of one scene (valid after add_scene). */
+typedef struct xdem_ctx_scene_info {
+    unsigned int np, nplane, p2p_capacity, has_mesh;
+    unsigned int first_particle;   /* its first particle in the context's concatenated arrays */
+} xdem_ctx_scene_info;
+int xdem_ctx_scene_info_get(xdem_ctx* ctx, unsigned int scene, xdem_ctx_scene_info* out);
+/* Part file of one scene: the step's result file of xResultManager::export_step_data_to_file
+ * (source/xdynamics_manager/xResultManager.cpp:828-925, version 2 layout) -- ct, np, np,
+ * mass[np], pos[4np], vel[3np], acc[3np], ep[4np], ev[4np], ea[4np], then the sphere, wall
+ * (scenes with walls) and triangle (scenes with a mesh) histories as count[np], ids[np cap],
+ * tsd[2 np cap]. load restores the state from such a file and returns its ct (the restart of
+ * xDynamicsSimulator::set_from_part_result, xDynamicSimulator.cpp:392-492, reading the mass
+ * block that the reference's reader skips); it keeps the contact histories as they are, as
+ * the reference's restart does (xContactManager.cpp:316-322), unless flags holds
+ * XDEM_PART_HISTORY, which restores them too. A file whose size does not fit the scene is
+ * refused with XDEM_E_INVALID before anything is written. */
+#define XDEM_PART_HISTORY 1
+int xdem_ctx_save_part(xdem_ctx* ctx, unsigned int scene, const char* path, double ct);
+int xdem_ctx_load_part(xdem_ctx* ctx, unsigned int scene, const char* path, int flags, double* ct);
 /* Number of stored sphere-sphere and sphere-wall contacts of all scenes at the last step. */
 int xdem_ctx_contact_count(xdem_ctx* ctx, unsigned long long* n_p2p, unsigned long long* n_wall);
 /* hipStream_t the context works on. */
@@ -239,9 +294,11 @@ int xdem_ctx_graph_launch(xdem_ctx* ctx, unsigned int nlaunch);
  * (the step's counters; bucket grid, sorted copies and neighbour lists on a rebuild) ms,
  * out[2] = number of steps timed, out[3] = rolling + velocity kernel ms (0 when fused
  * into the contact passes), out[4] = per-step pair test (k_touch) ms, out[5] = k_lite_move
- * ms (fused mode; 0 otherwise). */
+ * ms (fused mode; 0 otherwise), out[6] = k_collide ms, out[7] = the triangle pass before it
+ * (k_mesh_pass; 0 without meshes) ms, out[8] = the sphere-pair pass (k_pairs) ms, all three on
+ * the second stream. */
 int xdem_ctx_set_timing(xdem_ctx* ctx, int on);
-int xdem_ctx_timing(xdem_ctx* ctx, double out[6]);
+int xdem_ctx_timing(xdem_ctx* ctx, double out[9]);
 
 #ifdef __cplusplus
 }

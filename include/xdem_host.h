@@ -46,6 +46,9 @@ typedef struct xdem_scene {
 /* Workload ids for xdem_scene_build. */
 #define XDEM_SCENE_BOX_LATTICE   1  /* 10,000 spheres r=1 mm on a 2.2 mm jittered lattice, 0.1x0.1x0.3 m box */
 #define XDEM_SCENE_DAM_BREAK     2  /* 1,000,000 spheres r~U[1,2] mm, 0.3x0.3x0.6 m column, 1.2x0.3x0.7 m box */
+#define XDEM_SCENE_WHEEL_BED     3  /* 500,000 spheres r~U[2,3] mm over a 1.0x0.4 m bin floor, loose (solid fraction
+                                     ~0.3, twice the 0.15 m bed height), to settle before the wheel runs; the
+                                     wheel is a mesh body (xdem_wheel_mesh) */
 #define XDEM_SCENE_BATCH_MEMBER  4  /* one box-settling scene of the 256-scene sweep (index = variant);
                                      dt = min(1e-5 s, 0.2 x Rayleigh time of its smallest sphere) */
 #define XDEM_SCENE_PILE          5  /* 20,000 spheres r~U[1,1.5] mm in a 30 mm cylinder region over a floor */
@@ -56,6 +59,16 @@ typedef struct xdem_scene {
 int xdem_scene_build(int workload, unsigned long long seed, unsigned int np_override,
                      unsigned int variant, xdem_scene* out);
 void xdem_scene_free(xdem_scene* s);
+
+/* Triangles of a grousered wheel in its body frame (axis along y, centred at the origin): a
+ * rim of n_theta x n_y quads, two side rings (annuli of depth grouser_height, n_ring quads
+ * each), and n_grouser bars of the given height and thickness across the rim (top, two
+ * sides of n_y quads, two end caps). 2 n_theta n_y + 4 n_ring + n_grouser (6 n_y + 4)
+ * triangles: 4,096 for (170, 8, 12 grousers, 188). With vertices NULL, only the count is
+ * returned; else *ntriangle holds the capacity on entry. */
+int xdem_wheel_mesh(double radius, double width, unsigned int n_theta, unsigned int n_y, unsigned int n_grouser,
+                    double grouser_height, double grouser_thickness, unsigned int n_ring, double* vertices,
+                    unsigned int* ntriangle);
 
 /* Reference wall set-up: four corners p0..p3 (p0 -> p1 is u1, p0 -> p3 is u2, the
  * normal is u1 x u2), identity orientation, zero velocity. */

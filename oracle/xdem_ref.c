@@ -938,6 +938,26 @@ void ref_update_mesh_object_data(const double* dbi, const double* vList, double*
     }
 }
 
+/* ------------------------ mesh definition (xParticleMeshObjectContact.cpp:104-127) */
+/* Proxy sphere of every triangle: the triangle placed at the body pose (p = pos + A(e) v),
+ * FitSphereToTriangle(p, q, r, 0.8), its centre kept in the body frame (A(e)^T (c - pos)). */
+void ref_mesh_define(const double* dbi, const double* vList, unsigned int ntriangle, double* dlocal,
+                     double* radius)
+{
+    v3 bp = mk3(dbi[REF_PX], dbi[REF_PY], dbi[REF_PZ]);
+    v4 e = {dbi[REF_E0], dbi[REF_E1], dbi[REF_E2], dbi[REF_E3]};
+    for (unsigned int i = 0; i < ntriangle; i++) {
+        const double* s = vList + 9 * i;
+        double P[3], Q[3], R[3], c[4];
+        st3(P, add3(bp, to_global(ld3(s + 0), e)));
+        st3(Q, add3(bp, to_global(ld3(s + 3), e)));
+        st3(R, add3(bp, to_global(ld3(s + 6), e)));
+        ref_fit_sphere_to_triangle(P, Q, R, 0.8, c);
+        st3(dlocal + 3 * i, to_local(sub3(mk3(c[0], c[1], c[2]), bp), e));
+        radius[i] = c[3];
+    }
+}
+
 /* ------------------------------------------- wall pose (xParticlePlaneContact.cpp:96-110) */
 /* Local corners of a wall at its pose (xPlaneObject::define, xPlaneObject.cpp:45-48). */
 void ref_plane_local_points(const ref_plane_info* pl, const double* dbi, double out[12])

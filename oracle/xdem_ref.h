@@ -154,6 +154,11 @@ void ref_update_mesh_object_data(const double* dbi, const double* vList, double*
                                  const double* dlocal, ref_triangle_info* dpi,
                                  unsigned int ntriangle);
 
+/* xParticleMeshObjectContact::define (xParticleMeshObjectContact.cpp:104-127): proxy spheres
+ * fitted to the triangles at pose dbi, centres returned in the body frame. */
+void ref_mesh_define(const double* dbi, const double* vList, unsigned int ntriangle, double* dlocal,
+                     double* radius);
+
 /* xPlaneObject::define's local corners A(e)^T (corner - position) (xPlaneObject.cpp:45-48),
  * and xParticlePlaneContact::update (xParticlePlaneContact.cpp:96-110): the wall at pose dbi. */
 void ref_plane_local_points(const ref_plane_info* pl, const double* dbi, double out[12]);

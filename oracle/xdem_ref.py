@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
             "ref_set_pair_magnitude_output": ([P], None),
             "ref_plane_local_points": ([P, P, P], None),
             "ref_plane_update": ([P, P, U, P], None),
+            "ref_mesh_define": ([P, P, U, P, P], None),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)
@@ -81,6 +82,11 @@ def quirks_struct(q: dict | None = None) -> RefQuirks:
 
 def _p(a):
     return a.ctypes.data
+
+
+class RefTriangle(C.Structure):
+    _fields_ = [("id", C.c_int), ("tid", C.c_uint), ("P", C.c_double * 3), ("Q", C.c_double * 3),
+                ("R", C.c_double * 3)]
 
 
 class RefSim:
@@ -128,7 +134,50 @@ class RefSim:
         self.cend = np.zeros(self.ncell, dtype=np.uint32)
         self.bf = [np.zeros(n) for _ in range(6)]
         self.wrench = np.zeros((len(self.planes), 6))
-        self.overflow = {"p2p": 0, "plane": 0}
+        self.overflow = {"p2p": 0, "plane": 0, "mesh": 0}
+        self.mesh = None
+
+    def add_mesh(self, vertices, dbi, cp) -> None:
+        """One rigid mesh body (xParticleMeshObjectContact::define,
+        xParticleMeshObjectContact.cpp:77-167): vertices (T, 3, 3) in the body frame, pose dbi
+        (15 doubles), material cp. Proxy spheres are fitted at this pose and kept in the body
+        frame; they are hashed with the spheres (entries np .. np + T - 1) and the cell grows to
+        twice the largest proxy radius (xDiscreteElementMethodSimulation.cpp:91-100). The
+        triangle history has the reference's 5 slots per sphere."""
+        v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float64).reshape(-1, 9))
+        T = v.shape[0]
+        d = np.ascontiguousarray(dbi, dtype=np.float64).copy()
+        dlocal = np.zeros((T, 3))
+        rad = np.zeros(T)
+        self.L.ref_mesh_define(_p(d), _p(v), T, _p(dlocal), _p(rad))
+        tri = (RefTriangle * T)()
+        for t in range(T):
+            tri[t].id = t
+            tri[t].tid = t
+        sphere = np.zeros((T, 4))
+        sphere[:, 3] = rad
+        self.mesh = dict(T=T, vlist=v, dlocal=dlocal, dbi=d, cp=cp, tri=tri, sphere=sphere, wrench=np.zeros(6))
+        self.prm.cell_size = max(self.prm.cell_size, 2.0 * float(rad.max()))
+        n = self.np
+        self.hash = np.zeros(n + T, dtype=np.uint32)
+        self.index = np.zeros(n + T, dtype=np.uint32)
+        self.sorted_index = np.zeros(n + T, dtype=np.uint32)
+        self.ms_count = np.zeros(n, dtype=np.uint32)
+        self.ms_id = np.zeros((n, 5), dtype=np.uint32)
+        self.ms_tsd = np.zeros((n, 5, 2))
+        self.set_mesh_pose(d)
+
+    def set_mesh_pose(self, dbi) -> None:
+        """Places the mesh at pose dbi (xParticleMeshObjectContact::update, impl:2544-2601)."""
+        m = self.mesh
+        m["dbi"] = np.ascontiguousarray(dbi, dtype=np.float64).copy()
+        self.L.ref_update_mesh_object_data(_p(m["dbi"]), _p(m["vlist"]), _p(m["sphere"]), _p(m["dlocal"]),
+                                           C.addressof(m["tri"]), m["T"])
+
+    def triangles(self) -> np.ndarray:
+        """Global triangles (T, 3, 3) at the current pose."""
+        m = self.mesh
+        return np.frombuffer(bytes(m["tri"]), dtype=np.float64).reshape(m["T"], 10)[:, 1:].reshape(m["T"], 3, 3)
 
     def set_plane_pose(self, k: int, dbi) -> None:
         """Moves wall k to pose dbi (15 doubles) from its local corners at the first pose
@@ -155,8 +204,14 @@ class RefSim:
 
     def grid(self):
         self.L.ref_calculate_hash_and_index(C.byref(self.prm), _p(self.hash), _p(self.index), _p(self.pos), 0, self.np)
+        n = self.np
+        if self.mesh is not None:
+            # the proxy spheres follow the particles in the hash (impl:290-315, sid = np)
+            self.L.ref_calculate_hash_and_index(C.byref(self.prm), _p(self.hash), _p(self.index),
+                                                _p(self.mesh["sphere"]), self.np, self.mesh["T"])
+            n += self.mesh["T"]
         self.L.ref_reorder_data_and_find_cell_start(_p(self.hash), _p(self.index), _p(self.cstart), _p(self.cend),
-                                                    _p(self.sorted_index), self.np, self.ncell)
+                                                    _p(self.sorted_index), n, self.ncell)
 
     def contacts(self, pair_magnitude: np.ndarray | None = None):
         """All contacts; pair_magnitude (np,) receives sum |F_pair| per sphere when given."""
@@ -176,6 +231,15 @@ class RefSim:
                 _p(self.pl_tsd), self.np, 3)
             for c in range(6):
                 self.wrench[k, c] = L.ref_reduction(_p(self.bf[c]), self.np)
+        if self.mesh is not None:
+            m = self.mesh
+            self.overflow["mesh"] += L.ref_particle_polygon_collision(
+                C.byref(self.prm), q, C.addressof(m["tri"]), _p(m["dbi"]), *(_p(b) for b in self.bf), _p(self.pos),
+                _p(self.ep), _p(self.vel), _p(self.ev), _p(self.force), _p(self.moment), _p(self.mass), _p(self.tmax),
+                _p(self.rres), _p(self.ms_count), _p(self.ms_id), _p(self.ms_tsd), _p(self.sorted_index),
+                _p(self.cstart), _p(self.cend), C.byref(m["cp"]), self.np, m["T"], 5)
+            for c in range(6):
+                m["wrench"][c] = L.ref_reduction(_p(self.bf[c]), self.np)
 
     def rolling(self):
         self.L.ref_decide_rolling_friction_moment(C.byref(self.prm), C.byref(self.q), _p(self.tmax), _p(self.rres),

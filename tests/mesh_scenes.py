@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 
 import xdem_ref
-from xdynamics_dll_amd import SceneData, _lib, contact_property
+from xdynamics_dll_amd import SceneData, _lib, contact_property, wheel_mesh
 
 
 def terrain(nx=10, h=2.0e-3, amp=6.0e-4, seed=3):
@@ -85,19 +85,29 @@ def closest_distance(tris_g, p):
     return min(best, seg(a, b), seg(b, c), seg(c, a))
 
 
-def mesh_scene(n=400, seed=5, coh=0.0):
-    """Spheres r ~ U[0.8, 1.2] mm over the terrain: 60% overlap it by up to 10% of r, 15% sit
-    within 0.2 um outside it (the JKR separation band when cohesive), the rest just out of
-    reach. Returns (SceneData, local triangles, proxies, dbi, contact property)."""
+def mesh_scene(n=400, seed=5, coh=0.0, max_overlap=0.1, nx=10, separated=False, dt=1e-5):
+    """Spheres r ~ U[0.8, 1.2] mm over an nx-by-nx terrain: 60% overlap it by up to max_overlap
+    of r, 15% sit within 0.2 um outside it (the JKR separation band when cohesive), the rest just
+    out of reach. With separated, a sphere overlaps the spheres placed before it by at most
+    max_overlap of the smaller radius (a scene to follow through many steps; without it the
+    spheres may overlap each other deeply, which only a one-step test can use). Returns
+    (SceneData, local triangles, proxies, dbi, contact property). The reference's CPU Hertz law
+    damps with 2 beta sqrt(m kn) dv (xContact.cpp:244), about 1.6 N s/m for these spheres: an
+    explicit step of 1e-5 s is unstable for it (c dt / m > 2), so a scene followed through many
+    steps takes dt = 1e-6."""
     rng = np.random.default_rng(seed)
-    tris = terrain()
+    tris = terrain(nx=nx)
     prox = fit_proxies(tris)
     dbi = body_pose()
     A = rot(dbi[7:11])
     tris_g = tris @ A.T + dbi[1:4]
     lo, hi = tris_g.reshape(-1, 3).min(0), tris_g.reshape(-1, 3).max(0)
     pos = []
+    tries = 0
     while len(pos) < n:
+        tries += 1
+        if tries > 100 * n:
+            raise RuntimeError(f"placed {len(pos)} of {n} spheres")
         r = rng.uniform(0.8e-3, 1.2e-3)
         nrm = A[:, 2]
         # start above the terrain and move along the body normal until the distance to the
@@ -106,7 +116,7 @@ def mesh_scene(n=400, seed=5, coh=0.0):
                       hi[2]]) + 3e-3 * nrm
         u = rng.random()
         if u < 0.6:
-            target = r * rng.uniform(0.9, 1.0)           # overlapping
+            target = r * rng.uniform(1.0 - max_overlap, 1.0)   # overlapping
         elif u < 0.75:
             target = r + rng.uniform(0.0, 2e-7)          # inside the JKR separation band
         else:
@@ -116,11 +126,17 @@ def mesh_scene(n=400, seed=5, coh=0.0):
             if abs(d - target) < 1e-9:
                 break
             p = p + (target - d) * nrm
-        if abs(closest_distance(tris_g, p) - target) < 1e-7:
-            pos.append([*p, r])
+        if abs(closest_distance(tris_g, p) - target) >= 1e-7:
+            continue
+        if separated and pos:
+            q = np.asarray(pos)
+            d = np.linalg.norm(q[:, :3] - p, axis=1)
+            if (d < q[:, 3] + r - max_overlap * np.minimum(q[:, 3], r)).any():
+                continue
+        pos.append([*p, r])
     pos = np.array(pos)
     cp = contact_property(4e7, 2e8, 0.28, 0.3, 0.45, 0.55, 0.03, coh)
-    scene = SceneData.from_positions(pos, 2500.0, cp, walls=(), dt=1e-5,
+    scene = SceneData.from_positions(pos, 2500.0, cp, walls=(), dt=dt,
                                      vel=rng.normal(0, 0.03, size=(n, 3)))
     return scene, tris, prox, dbi, cp
 
@@ -131,3 +147,24 @@ def triangle_infos(n):
         arr[t].id = t
         arr[t].tid = 0
     return arr
+
+
+def wheel_over_bed(n, seed=1003, sink=1.5e-3):
+    """The grousered wheel (4,096 triangles, R = 0.15 m, width 0.1 m, 40 kg) over the loose bed
+    generator of n spheres, its lowest point `sink` below the highest sphere top under it."""
+    scene = SceneData.build(3, seed, np_override=n)
+    tris = wheel_mesh()
+    assert len(tris) == 4096
+    p = scene.pos
+    L = scene.planes[2].xw.x
+    W = scene.planes[4].xw.y
+    xc, yc = 0.5 * L, 0.5 * W
+    under = (np.abs(p[:, 0] - xc) < 0.06) & (np.abs(p[:, 1] - yc) < 0.05)
+    top = (p[under, 2] + p[under, 3]).max()
+    low = -tris.reshape(-1, 3)[:, 2].min()   # the lowest point below the axis (rim or grouser)
+    dbi = np.zeros(15)
+    dbi[0] = 40.0
+    dbi[1:4] = (xc, yc, top + low - sink)
+    dbi[7] = 1.0
+    cp = contact_property(2e11, 5e7, 0.3, 0.25, 0.4, 0.6, 0.05)
+    return scene, tris, dbi, cp

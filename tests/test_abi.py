@@ -9,7 +9,7 @@ import pytest
 from conftest import ROOT
 import xdynamics_dll_amd as xd
 
-HEADERS = ["xdem_capi.h", "xdem_host.h"]
+HEADERS = ["xdem_capi.h", "xdem_host.h", "xdem_mbd.h"]
 
 
 def declared_functions(header):
@@ -34,7 +34,9 @@ def test_gpu_library_exports_every_declared_symbol(header):
 
 def test_host_library_exports_host_header():
     lib = ctypes.CDLL(xd._lib.HOST_LIB_PATH)
-    missing = [n for n in declared_functions("xdem_host.h") if not hasattr(lib, n)]
+    names = declared_functions("xdem_host.h") + [n for n in declared_functions("xdem_mbd.h")
+                                                 if n not in xd._lib.COSIM_SIGNATURES]
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 
@@ -42,6 +44,9 @@ def test_bindings_cover_the_header():
     declared = set(declared_functions("xdem_capi.h"))
     bound = set(xd._lib.LAUNCHER_SIGNATURES) | set(xd._lib.CTX_SIGNATURES)
     assert declared == bound, (declared ^ bound)
+    mbd = set(declared_functions("xdem_mbd.h"))
+    mbd_bound = {n for n in xd._lib.HOST_SIGNATURES if n.startswith("xdem_hht")} | set(xd._lib.COSIM_SIGNATURES)
+    assert mbd == mbd_bound, (mbd ^ mbd_bound)
 
 
 def test_struct_layouts_match_reference_sizes():

@@ -21,20 +21,35 @@ def test_more_than_one_gpu_exits_with_a_message():
     assert r.stdout == ""
 
 
-def test_roofline_of_k_lite_move_is_bytes_over_event_time():
-    tm = {"steps": 200, "lite_move_ms": 200 * 0.025, "contact_ms": 200 * 0.06}
+def test_kernel_rooflines_are_bytes_over_event_time():
+    tm = {"steps": 200, "lite_move_ms": 200 * 0.025, "contact_ms": 200 * 0.09, "pair_pass_ms": 200 * 0.04,
+          "collide_ms": 200 * 0.03}
     info = {"contact_spheres": 1000}
-    r = bench.roofline(tm, 1_000_000, 0.004, True, info, 1.5e8)
+    kr = bench.kernel_rooflines(tm, 1_000_000, info, 6000, 200, True)
     units = 1_000_000 - 1000
-    assert r["kernel_ms"] == pytest.approx(0.025)
-    assert r["units"] == units
-    assert r["achieved"] == pytest.approx(168.0 * units / 25e-6 / 1e9)
-    assert r["frac"] == pytest.approx(r["achieved"] / 8000.0)
-    assert r["traffic"] == 1.5e8 and r["bound"] == "hbm" and r["unit"] == "GB/s"
-    # split mode: the contact pass over all spheres
-    s = bench.roofline(tm, 1_000_000, 0.004, False, info, None)
-    assert s["kernel_ms"] == pytest.approx(0.06)
-    assert s["achieved"] == pytest.approx(bench.contact_kernel_bytes(0.004, False) * 1e6 / 60e-6 / 1e9)
+    assert kr["k_lite_move"]["ms"] == pytest.approx(0.025)
+    assert kr["k_lite_move"]["units"] == units
+    assert kr["k_lite_move"]["gbps"] == pytest.approx(168.0 * units / 25e-6 / 1e9)
+    # k_pairs: 228 B per full-pass sphere and 44 B per passing pair; k_collide: 480 B per
+    # sphere and 40 B per wall contact
+    assert kr["k_pairs"]["bytes"] == 228.0 * 1000 + 44.0 * 6000
+    assert kr["k_pairs"]["gbps"] == pytest.approx((228.0 * 1000 + 44.0 * 6000) / 40e-6 / 1e9)
+    assert kr["k_collide"]["bytes"] == 480.0 * 1000 + 40.0 * 200
+    for k in kr.values():
+        assert k["frac"] == pytest.approx(k["gbps"] / 8000.0)
+    r = bench.dominant_roofline(kr, {"k_lite_move": 1.5e8})
+    assert r["kernel"] == "k_pairs" and r["kernel_ms"] == pytest.approx(0.04)
+    assert r["traffic"] is None and r["bound"] == "hbm" and r["unit"] == "GB/s"
+    tm["lite_move_ms"] = 200 * 0.05
+    r = bench.dominant_roofline(bench.kernel_rooflines(tm, 1_000_000, info, 6000, 200, True), {"k_lite_move": 1.5e8})
+    assert r["kernel"] == "k_lite_move" and r["traffic"] == 1.5e8
+    # split mode: no fused kernels
+    assert set(bench.kernel_rooflines(tm, 1_000_000, info, 6000, 200, False)) == {"k_pairs"}
+
+
+def test_host_cpu_reports_model_and_cores():
+    model, usable, total = bench.host_cpu()
+    assert isinstance(model, str) and 1 <= usable <= total
 
 
 def test_committed_traffic_summary_matches_its_workload_only():

@@ -173,7 +173,10 @@ def test_moving_wall_through_1000_poses_matches_oracle(fuse):
         if k % 100 == 0:
             print(f"[moving floor] fuse={fuse} step {k}", flush=True)
             assert_state_equal(gpu, ref, f"step {k}")
-            assert np.array_equal(gpu.wrench()[0], ref.wrench[0]), k
+            # the wrench is a sum over spheres whose order (unspecified in the reference's
+            # thrust::reduce) differs from the oracle's left-to-right sum
+            w, r = gpu.wrench()[0], ref.wrench[0]
+            assert np.allclose(w, r, rtol=1e-11, atol=1e-12 * np.abs(r).max()), (k, w, r)
     assert ref.pl_count.sum() > 0
     assert gpu.error_word() == 0
     gpu.close()

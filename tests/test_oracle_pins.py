@@ -760,3 +760,48 @@ def test_free_spin_of_a_sphere(axis):
     assert err1 < 5e-3 and om1 < 1e-2
     assert 1.8 < err1 / err2 < 2.2, (err1, err2)
     assert 1.8 < om1 / om2 < 2.2, (om1, om2)
+
+
+def test_library_and_oracle_fit_the_same_proxy_spheres():
+    """The step layer fits its proxy spheres with the library's FitSphereToTriangle and the
+    oracle with its own restatement (xUtilityFunction.cpp:379-415): they must agree bit for bit,
+    as the largest proxy radius sets the reference's cell (and so the visiting order)."""
+    rng = np.random.default_rng(17)
+    H = _lib.host()
+    for _ in range(2000):
+        P, Q, R = (np.ascontiguousarray(rng.normal(size=3) * 10.0 ** rng.uniform(-4, 0)) for _ in range(3))
+        a, b = np.zeros(4), np.zeros(4)
+        H.xdem_fit_sphere_to_triangle(P.ctypes.data, Q.ctypes.data, R.ctypes.data, 0.8, a.ctypes.data)
+        L.ref_fit_sphere_to_triangle(ptr(P), ptr(Q), ptr(R), 0.8, ptr(b))
+        assert np.array_equal(a, b) or (np.isnan(a).all() and np.isnan(b).all())
+
+
+def test_wheel_mesh_geometry():
+    """The grousered wheel: 4,096 triangles; rim vertices at R = 0.15 m, grouser tops at
+    R + 15 mm, side rings inside R; the width spans y in [-0.05, 0.05]."""
+    from xdynamics_dll_amd import wheel_mesh
+
+    t = wheel_mesh()
+    assert t.shape == (4096, 3, 3)
+    v = t.reshape(-1, 3)
+    r = np.hypot(v[:, 0], v[:, 2])
+    assert abs(r.max() - 0.165) < 1e-12 and abs(r.min() - 0.135) < 1e-12
+    assert abs(v[:, 1].min() + 0.05) < 1e-15 and abs(v[:, 1].max() - 0.05) < 1e-15
+    rim = t[: 2 * 170 * 8].reshape(-1, 3)
+    assert np.allclose(np.hypot(rim[:, 0], rim[:, 2]), 0.15, rtol=0, atol=1e-15)
+
+
+def test_oracle_mesh_body_balances_its_spheres():
+    """The oracle's step with a mesh body (RefSim.add_mesh): the body's wrench force is minus
+    the sum of the triangle forces on the spheres (the sphere forces with and without the mesh)."""
+    from mesh_scenes import mesh_scene
+
+    scene, tris, _, dbi, cp = mesh_scene(n=200)
+    with_mesh = xdem_ref.RefSim(scene)
+    with_mesh.add_mesh(tris, dbi, cp)
+    bare = xdem_ref.RefSim(scene)
+    with_mesh.collide()
+    bare.collide()
+    f = (with_mesh.force - bare.force).sum(0)
+    assert np.abs(f).max() > 0 and with_mesh.ms_count.sum() > 0
+    assert np.allclose(with_mesh.mesh["wrench"][:3], -f, rtol=1e-10, atol=1e-12 * np.abs(f).max())

@@ -7,8 +7,8 @@ the library is missing.
 """
 from . import _lib
 from ._lib import XdemError
-from .scene import PARITY_QUIRKS, SceneData, box_walls, contact_property, make_plane
+from .scene import PARITY_QUIRKS, SceneData, box_walls, contact_property, make_plane, wheel_mesh
 from .simulation import DemContext, xIntegratorVV
 
 __all__ = ["_lib", "XdemError", "PARITY_QUIRKS", "SceneData", "box_walls", "contact_property", "make_plane",
-           "DemContext", "xIntegratorVV"]
+           "wheel_mesh", "DemContext", "xIntegratorVV"]

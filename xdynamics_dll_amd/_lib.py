@@ -39,9 +39,12 @@ FIELD = {"pos": (0, 4), "vel": (1, 3), "acc": (2, 3), "ep": (3, 4), "ev": (4, 4)
          "rres": (11, 1)}
 HISTORY_P2P = 0
 HISTORY_PLANE = 1
+HISTORY_MESH = 2
+PART_HISTORY = 1
 
 SCENE_BOX_LATTICE = 1
 SCENE_DAM_BREAK = 2
+SCENE_WHEEL_BED = 3
 SCENE_BATCH_MEMBER = 4
 SCENE_PILE = 5
 
@@ -105,6 +108,19 @@ class Scene(C.Structure):
                 ("density", C.c_double), ("min_radius", C.c_double), ("max_radius", C.c_double)]
 
 
+class Mesh(C.Structure):
+    """xdem_mesh: a rigid triangle-mesh body (vertices in the body frame)."""
+    _fields_ = [("ntriangle", C.c_uint), ("vertices", C.POINTER(C.c_double)), ("dbi", C.c_double * 15),
+                ("cp", ContactProperty)]
+
+
+class HhtWheelConfig(C.Structure):
+    """xdem_hht_wheel_config (include/xdem_mbd.h)."""
+    _fields_ = [("mass", C.c_double), ("inertia", C.c_double * 3), ("dbi", C.c_double * 15),
+                ("gravity", C.c_double * 3), ("load", C.c_double * 3), ("omega", C.c_double), ("dt", C.c_double),
+                ("alpha", C.c_double), ("tolerance", C.c_double), ("max_iterations", C.c_int)]
+
+
 class CtxConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("p2p_capacity", C.c_uint), ("quirks", Quirks),
                 ("block_size", C.c_uint), ("grid_dim_max", C.c_uint), ("fuse_velocity", C.c_int),
@@ -116,7 +132,13 @@ class CtxInfo(C.Structure):
     _fields_ = [("nscene", C.c_uint), ("np", C.c_uint), ("nsorted", C.c_uint), ("nplane_total", C.c_uint),
                 ("nbucket", C.c_uint), ("key_bits", C.c_uint), ("p2p_capacity", C.c_uint),
                 ("device_bytes", C.c_ulonglong), ("steps_done", C.c_ulonglong),
-                ("list_builds", C.c_ulonglong), ("list_spheres", C.c_uint), ("contact_spheres", C.c_uint)]
+                ("list_builds", C.c_ulonglong), ("list_spheres", C.c_uint), ("contact_spheres", C.c_uint),
+                ("mesh_spheres", C.c_uint), ("ntriangle", C.c_uint)]
+
+
+class CtxSceneInfo(C.Structure):
+    _fields_ = [("np", C.c_uint), ("nplane", C.c_uint), ("p2p_capacity", C.c_uint), ("has_mesh", C.c_uint),
+                ("first_particle", C.c_uint)]
 
 
 assert C.sizeof(DemParameters) == 144
@@ -167,9 +189,16 @@ CTX_SIGNATURES = {
     "xdem_ctx_write": [P, I, U, P],
     "xdem_ctx_device_ptr": [P, I, C.POINTER(P)],
     "xdem_ctx_read_history": [P, I, U, P, P, P],
+    "xdem_ctx_write_history": [P, I, U, P, P, P],
+    "xdem_ctx_scene_info_get": [P, U, C.POINTER(CtxSceneInfo)],
+    "xdem_ctx_save_part": [P, U, C.c_char_p, D],
+    "xdem_ctx_load_part": [P, U, C.c_char_p, I, C.POINTER(D)],
     "xdem_ctx_error_word": [P, C.POINTER(U), I],
     "xdem_ctx_body_wrench": [P, U, P],
     "xdem_ctx_set_body_pose": [P, U, U, P],
+    "xdem_ctx_add_mesh": [P, U, C.c_void_p, C.POINTER(U)],
+    "xdem_ctx_set_mesh_pose": [P, U, P],
+    "xdem_ctx_mesh_wrench": [P, U, P],
     "xdem_ctx_info_get": [P, C.POINTER(CtxInfo)],
     "xdem_ctx_contact_count": [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)],
     "xdem_ctx_stream": [P, C.POINTER(P)],
@@ -177,6 +206,11 @@ CTX_SIGNATURES = {
     "xdem_ctx_graph_launch": [P, U],
     "xdem_ctx_set_timing": [P, I],
     "xdem_ctx_timing": [P, C.POINTER(D)],
+}
+
+# include/xdem_mbd.h entries that live in libxdem.so (they drive a context)
+COSIM_SIGNATURES = {
+    "xdem_cosim_wheel_steps": [P, P, U, U, P],
 }
 
 HOST_SIGNATURES = {
@@ -189,6 +223,11 @@ HOST_SIGNATURES = {
     "xdem_fit_sphere_to_triangle": ([P, P, P, D, P], None),
     "xdem_rng_next": ([C.POINTER(C.c_ulonglong)], C.c_ulonglong),
     "xdem_rng_uniform": ([C.POINTER(C.c_ulonglong)], C.c_double),
+    "xdem_wheel_mesh": ([D, D, U, U, U, D, D, U, P, C.POINTER(U)], C.c_int),
+    "xdem_hht_wheel_create": ([C.POINTER(HhtWheelConfig), C.POINTER(P)], C.c_int),
+    "xdem_hht_wheel_destroy": ([P], C.c_int),
+    "xdem_hht_wheel_step": ([P, P, P, C.POINTER(I)], C.c_int),
+    "xdem_hht_wheel_state": ([P, P, P, P, C.POINTER(D)], C.c_int),
 }
 
 _lib = None
@@ -226,6 +265,7 @@ def lib() -> C.CDLL:
         L = C.CDLL(LIB_PATH)
         _bind(L, LAUNCHER_SIGNATURES)
         _bind(L, CTX_SIGNATURES)
+        _bind(L, COSIM_SIGNATURES)
         for name, (argtypes, restype) in HOST_SIGNATURES.items():
             fn = getattr(L, name)
             fn.argtypes = argtypes

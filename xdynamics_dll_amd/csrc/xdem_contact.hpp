@@ -101,6 +101,50 @@ __device__ __forceinline__ void sphere_contact(const ContactConst& cc, double dt
     acc.M = acc.M + cvt3<double>(cross(dc, fnt));
 }
 
+// One contact of sphere i with sphere j evaluated on its own (the pair pass): the parts of
+// sphere_contact that do not depend on the earlier pairs of the sphere. out = Fn (3), Ft (3),
+// the lever arm dc (3), in T and stored as double, and the rolling-resistance term; returns
+// whether Ft was set (|e| != 0). sphere_pair_sum then adds the pairs in visiting order with
+// the same operations as sphere_contact, carrying Ft over where it was not set (quirk Q4).
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ bool sphere_pair_eval(const ContactConst& cc, double dt, V4<double> ipos, V3<double> ivel,
+                                                 V3<double> iom, double im, V4<double> jpos, V3<double> jvel,
+                                                 V3<double> jom, double jm, const PairGeom& g, double& sd0,
+                                                 double& sd1, double out[10])
+{
+    const double ir = ipos.w, jr = jpos.w;
+    const PairConst<T> c = pair_const<T, MODEL>(cc, ir, jr, im, jm);
+    const V3<double> unit = g.rp / g.dist;
+    const V3<double> ip3 = mk3(ipos.x, ipos.y, ipos.z);
+    const V3<double> cpt = ip3 + ir * unit;
+    const V3<double> dcpr = cpt - ip3;
+    const V3<double> dcpr_j = cpt - mk3(jpos.x, jpos.y, jpos.z);
+    const V3<double> rv = (jvel + cross(jom, dcpr_j)) - (ivel + cross(iom, dcpr));
+    V3<T> Fn, Ft = mk3(T(0), T(0), T(0));
+    const bool set = force_law<T, MODEL, GPU_LAW>(cc, c, g.cdist, dt, sd0, sd1, rv, unit, Ft, Fn);
+    const V3<T> dc = cvt3<T>(dcpr);
+    out[0] = (double)Fn.x; out[1] = (double)Fn.y; out[2] = (double)Fn.z;
+    out[3] = (double)Ft.x; out[4] = (double)Ft.y; out[5] = (double)Ft.z;
+    out[6] = (double)dc.x; out[7] = (double)dc.y; out[8] = (double)dc.z;
+    out[9] = (double)(c.eq_r * T(cc.rfric) * length(Fn));
+    return set;
+}
+
+// Adds one evaluated contact to the sphere's sums: sphere_contact's accumulation, in its order.
+template <class T>
+__device__ __forceinline__ void sphere_pair_sum(const double v[10], bool set, bool stale_ft, V3<T>& Ft, Accum& acc)
+{
+    if (!stale_ft) Ft = mk3(T(0), T(0), T(0));
+    if (set) Ft = mk3(T(v[3]), T(v[4]), T(v[5]));
+    const V3<T> Fn = mk3(T(v[0]), T(v[1]), T(v[2]));
+    const V3<T> dc = mk3(T(v[6]), T(v[7]), T(v[8]));
+    const V3<T> fnt = Fn + Ft;
+    acc.T = acc.T + cvt3<double>(cross(dc, fnt));
+    acc.R += v[9];
+    acc.F = acc.F + cvt3<double>(fnt);
+    acc.M = acc.M + cvt3<double>(cross(dc, fnt));
+}
+
 // Cohesion-only attraction of a separated pair (impl:708-718), FP64 for every precision.
 __device__ __forceinline__ void sphere_cohesion(const ContactConst& cc, double ir, double jr, const PairGeom& g, Accum& acc)
 {

@@ -24,30 +24,107 @@ struct SceneHost {
                                       // add_scene (xPlaneObject::define's local_point)
     uint32_t p_begin = 0;
     uint32_t wall_begin = 0;
+    double rmax = 0.0;                // largest sphere radius
     SceneDev dev;
 };
+
+struct MeshHost {
+    uint32_t scene = 0;
+    std::vector<double> vlist;        // [9 T] vertices, body frame
+    std::vector<double> plocal;       // [3 T] proxy sphere centres, body frame
+    std::vector<double> prad;         // [T] proxy radii
+    std::vector<uint32_t> cell_start; // body-frame grid (local cell indices, local triangle order)
+    std::vector<uint32_t> cell_tris;
+    double rmax_local = 0.0;          // largest distance of a vertex from the body origin
+    MeshDev dev;
+};
+
+// A dynamic body's wrench: the spheres of [begin, end) whose flags hold `mask` wrote their
+// contribution to its per-sphere slot this step (kTouch for a wall: the full pass writes every
+// dynamic wall's slot; kMeshNear for a mesh: the triangle pass writes its slot).
+struct WrenchSlot {
+    uint32_t begin, end, mask, pad;
+};
+
+// Triangles and proxy centres of mesh m at its pose (impl:2544-2601; the proxies of the first
+// mesh are the reference's proxy spheres 0..T-1, tid = index).
+__global__ __launch_bounds__(256) void k_mesh_pose(const MeshDev* meshes, uint32_t m, const double* vlist,
+                                                  const double* plocal, const double* prad,
+                                                  xdem_triangle_info* tri, double* prox)
+{
+    const MeshDev& M = meshes[m];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M.ntri) return;
+    const size_t g = M.tri_begin + t;
+    const V3<double> bp = mk3(M.dbi[XDEM_PX], M.dbi[XDEM_PY], M.dbi[XDEM_PZ]);
+    const V4<double> e = {M.dbi[XDEM_E0], M.dbi[XDEM_E1], M.dbi[XDEM_E2], M.dbi[XDEM_E3]};
+    const double* v = vlist + 9 * g;
+    const V3<double> P = bp + rot_global(ld3(v + 0), e);
+    const V3<double> Q = bp + rot_global(ld3(v + 3), e);
+    const V3<double> R = bp + rot_global(ld3(v + 6), e);
+    const V3<double> c = bp + rot_global(ld3(plocal + 3 * g), e);
+    xdem_triangle_info ti;
+    ti.id = (int)t;
+    ti.tid = t;
+    ti.P = {P.x, P.y, P.z};
+    ti.Q = {Q.x, Q.y, Q.z};
+    ti.R = {R.x, R.y, R.z};
+    tri[g] = ti;
+    prox[4 * g + 0] = c.x; prox[4 * g + 1] = c.y; prox[4 * g + 2] = c.z; prox[4 * g + 3] = prad[g];
+}
+
+// Wrench of every dynamic body, in a fixed order (the same bits on every run): block (b, s)
+// sums the flagged spheres of chunk b of slot s's range, each thread a strided subset, then
+// the waves and the block in a fixed order; k_wrench_final adds the kWrenchBlocks partials of
+// each slot the same way.
+__global__ __launch_bounds__(256) void k_wrench_partial(const double* wrench, const uint32_t* flags, uint32_t np,
+                                                       const WrenchSlot* slots, double* partial)
+{
+    __shared__ double sh[6][4];
+    const uint32_t slot = blockIdx.y, b = blockIdx.x;
+    const WrenchSlot W = slots[slot];
+    const uint32_t n = W.end - W.begin;
+    const uint32_t chunk = (n + kWrenchBlocks - 1) / kWrenchBlocks;
+    const uint32_t lo = W.begin + min(n, b * chunk), hi = W.begin + min(n, (b + 1) * chunk);
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += 256)
+        if (flags[i] & W.mask) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) v[c] += wrench[((size_t)slot * 6 + c) * np + i];
+        }
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const double x = wave_sum(v[c]);
+        if ((threadIdx.x & 63) == 0) sh[c][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6)
+        partial[((size_t)slot * kWrenchBlocks + b) * 6 + threadIdx.x] =
+            ((sh[threadIdx.x][0] + sh[threadIdx.x][1]) + sh[threadIdx.x][2]) + sh[threadIdx.x][3];
+}
+
+__global__ __launch_bounds__(256) void k_wrench_final(const double* partial, double* out)
+{
+    __shared__ double sh[6][4];
+    const uint32_t slot = blockIdx.x;
+    double v[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) v[c] = partial[((size_t)slot * kWrenchBlocks + threadIdx.x) * 6 + c];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const double x = wave_sum(v[c]);
+        if ((threadIdx.x & 63) == 0) sh[c][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6)
+        out[(size_t)slot * 6 + threadIdx.x] =
+            ((sh[threadIdx.x][0] + sh[threadIdx.x][1]) + sh[threadIdx.x][2]) + sh[threadIdx.x][3];
+}
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
-}
-
-// Sum of wrench[slot][c][p_begin .. p_end) for every (slot, c) = one block each, fixed order.
-__global__ __launch_bounds__(256) void k_wrench_reduce(const double* wrench, uint32_t np, const uint32_t* range,
-                                                      double* out)
-{
-    __shared__ double s[4];
-    const uint32_t sc = blockIdx.x;       // slot * 6 + c
-    const uint32_t slot = sc / 6;
-    const double* src = wrench + (size_t)sc * np;
-    const uint32_t b = range[2 * slot], e = range[2 * slot + 1];
-    double v = 0.0;
-    for (uint32_t i = b + threadIdx.x; i < e; i += 256) v += src[i];
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) out[sc] = ((s[0] + s[1]) + s[2]) + s[3];
 }
 
 // Rotation matrix of Euler parameters e (xPointMass::setupTransformationMatrix,
@@ -86,7 +163,8 @@ struct xdem_ctx {
     xdem_ctx_config cfg;
     std::vector<SceneHost> scenes;
     std::vector<WallDev> walls;
-    std::vector<uint32_t> wslot_range;  // [2*slot] sphere range of each dynamic wall's scene
+    std::vector<MeshHost> meshes;
+    std::vector<WrenchSlot> wslots;     // per dynamic body (wall or mesh): sphere range and flag
     std::vector<int> wall_slot;         // global wall -> slot or -1
     bool committed = false;
     bool multi = false;
@@ -142,7 +220,18 @@ struct xdem_ctx {
     double2* pl_tsd = nullptr;
     double* wrench_pp = nullptr;
     double* wrench_out = nullptr;
-    uint32_t* d_wslot_range = nullptr;
+    double* wrench_partial = nullptr;
+    WrenchSlot* d_wslots = nullptr;
+    // meshes (all scenes' triangles concatenated)
+    MeshDev* d_meshes = nullptr;
+    double *d_vlist = nullptr, *d_plocal = nullptr, *d_prad = nullptr, *d_prox = nullptr, *msum = nullptr;
+    double* ppsum = nullptr;
+    xdem_triangle_info* d_tri = nullptr;
+    uint32_t *d_mcell_start = nullptr, *d_mcell_tris = nullptr;
+    uint32_t *tri_count = nullptr, *tri_list = nullptr, *mesh_work = nullptr, *mesh_count = nullptr;
+    uint32_t *ms_count = nullptr, *ms_id = nullptr;
+    double2* ms_tsd = nullptr;
+    uint32_t ntri = 0;
     uint32_t* err = nullptr;
     unsigned long long* counters = nullptr;
     std::vector<void*> allocs;
@@ -160,7 +249,8 @@ struct xdem_ctx {
 
     bool timing = false;
     std::vector<hipEvent_t> events;  // kEventsPerStep per timed step
-    double t_contact = 0.0, t_grid = 0.0, t_finish = 0.0, t_pairs = 0.0, t_lite_move = 0.0;
+    double t_contact = 0.0, t_grid = 0.0, t_finish = 0.0, t_pairs = 0.0, t_lite_move = 0.0, t_collide = 0.0;
+    double t_pairpass = 0.0, t_meshpass = 0.0;
     unsigned t_steps = 0;
     unsigned long long steps_done = 0;
 
@@ -201,6 +291,13 @@ struct xdem_ctx {
         A.pp_count_out = pp_count[cur ^ 1]; A.pp_id_out = pp_id[cur ^ 1]; A.pp_tsd_out = pp_tsd[cur ^ 1];
         A.pl_count = pl_count; A.pl_id = pl_id; A.pl_tsd = pl_tsd;
         A.wrench = wrench_pp; A.err = err; A.counters = counters;
+        A.meshes = d_meshes; A.tri_glob = d_tri; A.prox_glob = d_prox;
+        A.mcell_start = d_mcell_start; A.mcell_tris = d_mcell_tris;
+        A.tri_count = tri_count; A.tri_list = tri_list; A.msum = msum;
+        A.ms_count = ms_count; A.ms_id = ms_id; A.ms_tsd = ms_tsd;
+        A.mesh_work = mesh_work; A.mesh_count = mesh_count;
+        A.ppsum = ppsum;
+        A.has_mesh = meshes.empty() ? 0 : 1;
         A.np = np; A.nsorted = np; A.cap = cap; A.nwslot = nwslot; A.nbucket = nbucket;
         A.stale_ft = cfg.quirks.stale_ft; A.flat_inertia = cfg.quirks.flat_inertia;
         A.keep_velocity = cfg.quirks.velocity_not_updated; A.tmax_persist = tmax_persist;
@@ -232,7 +329,12 @@ struct xdem_ctx {
         const bool gpu_law = cfg.quirks.gpu_force_law != 0;
         if ((e = hipEventRecord(fork_ev, stream)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(stream2, fork_ev, 0)) != hipSuccess) return e;
+        if (!meshes.empty() && (e = collide_fn(A, multi, fuse, model, gpu_law, 2, stream2)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[9], stream2)) != hipSuccess) return e;
+        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 3, stream2)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[7], stream2)) != hipSuccess) return e;
         if ((e = collide_fn(A, multi, fuse, model, gpu_law, 1, stream2)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[8], stream2)) != hipSuccess) return e;
         if ((e = hipEventRecord(join_ev, stream2)) != hipSuccess) return e;
         if (ev && fuse) {
             // timed steps: k_lite_move bracketed by its own events
@@ -249,7 +351,9 @@ struct xdem_ctx {
         if ((e = hipStreamWaitEvent(stream, join_ev, 0)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[3], stream)) != hipSuccess) return e;
         if (nwslot) {
-            hipLaunchKernelGGL(k_wrench_reduce, dim3(nwslot * 6), dim3(256), 0, stream, wrench_pp, np, d_wslot_range,
+            hipLaunchKernelGGL(k_wrench_partial, dim3(kWrenchBlocks, nwslot), dim3(256), 0, stream, wrench_pp,
+                               nbr_count, np, d_wslots, wrench_partial);
+            hipLaunchKernelGGL(k_wrench_final, dim3(nwslot), dim3(kWrenchBlocks), 0, stream, wrench_partial,
                                wrench_out);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
@@ -326,7 +430,7 @@ struct xdem_ctx {
     }
 };
 
-constexpr unsigned kEventsPerStep = 7;
+constexpr unsigned kEventsPerStep = 10;
 
 static int hip_status(hipError_t e) { return e == hipSuccess ? XDEM_OK : XDEM_E_HIP; }
 
@@ -495,6 +599,8 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
     D.wall_begin = h.wall_begin;
     D.nwall = s->nplane;
     D.has_pp = 1;
+    D.mesh = -1;
+    h.rmax = rmax;
     if (s->cp_pp.rfric == 0.0) c->tmax_persist = 1;
     for (unsigned k = 0; k < s->nplane; k++) {
         WallDev W;
@@ -506,8 +612,7 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
         W.wslot = -1;
         if (W.dbi[XDEM_MM] > 0.0) {  // a wall with mass is a dynamic body: reduce its wrench
             W.wslot = (int32_t)c->nwslot++;
-            c->wslot_range.push_back(D.p_begin);
-            c->wslot_range.push_back(D.p_end);
+            c->wslots.push_back({D.p_begin, D.p_end, kTouch, 0u});
         }
         if (W.info.id == 0) D.wall_id0 = 1;
         if (W.wslot >= 0) D.wall_dynamic = 1;
@@ -585,13 +690,50 @@ int xdem_ctx_commit(xdem_ctx* c)
         CTX_CHECK(c->alloc(&c->pp_id[b], sizeof(uint32_t) * n * c->cap));
         CTX_CHECK(c->alloc(&c->pp_tsd[b], sizeof(double2) * n * c->cap));
     }
+    CTX_CHECK(c->alloc(&c->ppsum, sizeof(double) * 10 * n));
     CTX_CHECK(c->alloc(&c->pl_count, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->pl_id, sizeof(uint32_t) * n * XDEM_MAX_P2PL_COUNT));
     CTX_CHECK(c->alloc(&c->pl_tsd, sizeof(double2) * n * XDEM_MAX_P2PL_COUNT));
     if (c->nwslot) {
         CTX_CHECK(c->alloc(&c->wrench_pp, sizeof(double) * 6 * n * c->nwslot));
         CTX_CHECK(c->alloc(&c->wrench_out, sizeof(double) * 6 * c->nwslot));
-        CTX_CHECK(c->alloc(&c->d_wslot_range, sizeof(uint32_t) * 2 * c->nwslot));
+        CTX_CHECK(c->alloc(&c->wrench_partial, sizeof(double) * 6 * kWrenchBlocks * c->nwslot));
+        CTX_CHECK(c->alloc(&c->d_wslots, sizeof(WrenchSlot) * c->nwslot));
+    }
+    std::vector<MeshDev> mdev;
+    std::vector<double> all_vlist, all_plocal, all_prad;
+    std::vector<uint32_t> all_cell_start, all_cell_tris;
+    if (!c->meshes.empty()) {
+        // mesh bodies: body-frame data, the grids' cell tables concatenated (each mesh's cell
+        // starts offset by its first entry), per-sphere triangle lists, sums and history
+        for (MeshHost& m : c->meshes) {
+            m.dev.cell_base = (uint32_t)all_cell_start.size();
+            const uint32_t off = (uint32_t)all_cell_tris.size();
+            for (size_t k = 0; k + 1 < m.cell_start.size(); k++) all_cell_start.push_back(off + m.cell_start[k]);
+            all_cell_tris.insert(all_cell_tris.end(), m.cell_tris.begin(), m.cell_tris.end());
+            all_vlist.insert(all_vlist.end(), m.vlist.begin(), m.vlist.end());
+            all_plocal.insert(all_plocal.end(), m.plocal.begin(), m.plocal.end());
+            all_prad.insert(all_prad.end(), m.prad.begin(), m.prad.end());
+            mdev.push_back(m.dev);
+        }
+        all_cell_start.push_back((uint32_t)all_cell_tris.size());
+        const size_t T = c->ntri;
+        CTX_CHECK(c->alloc(&c->d_meshes, sizeof(MeshDev) * mdev.size()));
+        CTX_CHECK(c->alloc(&c->d_vlist, sizeof(double) * 9 * T));
+        CTX_CHECK(c->alloc(&c->d_plocal, sizeof(double) * 3 * T));
+        CTX_CHECK(c->alloc(&c->d_prad, sizeof(double) * T));
+        CTX_CHECK(c->alloc(&c->d_prox, sizeof(double) * 4 * T));
+        CTX_CHECK(c->alloc(&c->d_tri, sizeof(xdem_triangle_info) * T));
+        CTX_CHECK(c->alloc(&c->d_mcell_start, sizeof(uint32_t) * all_cell_start.size()));
+        CTX_CHECK(c->alloc(&c->d_mcell_tris, sizeof(uint32_t) * std::max<size_t>(1, all_cell_tris.size())));
+        CTX_CHECK(c->alloc(&c->tri_count, sizeof(uint32_t) * n));
+        CTX_CHECK(c->alloc(&c->tri_list, sizeof(uint32_t) * n * kTriCap));
+        CTX_CHECK(c->alloc(&c->msum, sizeof(double) * 10 * n));
+        CTX_CHECK(c->alloc(&c->ms_count, sizeof(uint32_t) * n));
+        CTX_CHECK(c->alloc(&c->ms_id, sizeof(uint32_t) * n * XDEM_MAX_P2MS_COUNT));
+        CTX_CHECK(c->alloc(&c->ms_tsd, sizeof(double2) * n * XDEM_MAX_P2MS_COUNT));
+        CTX_CHECK(c->alloc(&c->mesh_work, sizeof(uint32_t) * n));
+        CTX_CHECK(c->alloc(&c->mesh_count, sizeof(uint32_t)));
     }
     CTX_CHECK(c->alloc(&c->err, sizeof(uint32_t)));
     CTX_CHECK(c->alloc(&c->counters, sizeof(unsigned long long) * 4));
@@ -626,8 +768,30 @@ int xdem_ctx_commit(xdem_ctx* c)
     if (!c->walls.empty())
         CTX_CHECK(hipMemcpyAsync(c->d_walls, c->walls.data(), sizeof(WallDev) * c->walls.size(), hipMemcpyHostToDevice, s));
     if (c->nwslot)
-        CTX_CHECK(hipMemcpyAsync(c->d_wslot_range, c->wslot_range.data(), sizeof(uint32_t) * 2 * c->nwslot,
+        CTX_CHECK(hipMemcpyAsync(c->d_wslots, c->wslots.data(), sizeof(WrenchSlot) * c->nwslot, hipMemcpyHostToDevice,
+                                 s));
+    if (!c->meshes.empty()) {
+        const size_t T = c->ntri;
+        CTX_CHECK(hipMemcpyAsync(c->d_meshes, mdev.data(), sizeof(MeshDev) * mdev.size(), hipMemcpyHostToDevice, s));
+        CTX_CHECK(hipMemcpyAsync(c->d_vlist, all_vlist.data(), sizeof(double) * 9 * T, hipMemcpyHostToDevice, s));
+        CTX_CHECK(hipMemcpyAsync(c->d_plocal, all_plocal.data(), sizeof(double) * 3 * T, hipMemcpyHostToDevice, s));
+        CTX_CHECK(hipMemcpyAsync(c->d_prad, all_prad.data(), sizeof(double) * T, hipMemcpyHostToDevice, s));
+        CTX_CHECK(hipMemcpyAsync(c->d_mcell_start, all_cell_start.data(), sizeof(uint32_t) * all_cell_start.size(),
                                  hipMemcpyHostToDevice, s));
+        if (!all_cell_tris.empty())
+            CTX_CHECK(hipMemcpyAsync(c->d_mcell_tris, all_cell_tris.data(), sizeof(uint32_t) * all_cell_tris.size(),
+                                     hipMemcpyHostToDevice, s));
+        CTX_CHECK(hipMemsetAsync(c->tri_count, 0, sizeof(uint32_t) * n, s));
+        CTX_CHECK(hipMemsetAsync(c->ms_count, 0, sizeof(uint32_t) * n, s));
+        CTX_CHECK(hipMemsetAsync(c->ms_id, 0, sizeof(uint32_t) * n * XDEM_MAX_P2MS_COUNT, s));
+        CTX_CHECK(hipMemsetAsync(c->ms_tsd, 0, sizeof(double2) * n * XDEM_MAX_P2MS_COUNT, s));
+        CTX_CHECK(hipMemsetAsync(c->mesh_count, 0, sizeof(uint32_t), s));
+        for (uint32_t m = 0; m < c->meshes.size(); m++) {
+            hipLaunchKernelGGL(k_mesh_pose, dim3((c->meshes[m].dev.ntri + 255) / 256), dim3(256), 0, s, c->d_meshes, m,
+                               c->d_vlist, c->d_plocal, c->d_prad, c->d_tri, c->d_prox);
+            CTX_CHECK(hipGetLastError());
+        }
+    }
     CTX_CHECK(hipMemsetAsync(c->ev, 0, sizeof(double) * 4 * n, s));
     CTX_CHECK(hipMemsetAsync(c->ea, 0, sizeof(double) * 4 * n, s));
     CTX_CHECK(hipMemsetAsync(c->force, 0, sizeof(double) * 3 * n, s));
@@ -745,6 +909,9 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
         c->t_finish = 0.0;
         c->t_pairs = 0.0;
         c->t_lite_move = 0.0;
+        c->t_collide = 0.0;
+        c->t_pairpass = 0.0;
+        c->t_meshpass = 0.0;
         c->t_steps = std::min(nsteps, kmax);
         for (unsigned k = 0; k < c->t_steps; k++) {
             const hipEvent_t* ev = &c->events[kEventsPerStep * k];
@@ -754,6 +921,13 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
             CTX_CHECK(hipEventElapsedTime(&b, ev[2], ev[3]));
             CTX_CHECK(hipEventElapsedTime(&f, ev[3], ev[4]));
             CTX_CHECK(hipEventElapsedTime(&m, ev[5], ev[6]));
+            float kc = 0.f, kp = 0.f, km = 0.f;
+            CTX_CHECK(hipEventElapsedTime(&kc, ev[7], ev[8]));
+            CTX_CHECK(hipEventElapsedTime(&km, ev[2], ev[9]));
+            CTX_CHECK(hipEventElapsedTime(&kp, ev[9], ev[7]));
+            c->t_collide += kc;
+            c->t_meshpass += km;
+            c->t_pairpass += kp;
             c->t_grid += a;
             c->t_pairs += p;
             c->t_contact += b;
@@ -875,6 +1049,9 @@ int xdem_ctx_read_history(xdem_ctx* c, int kind, unsigned int scene, unsigned in
         cap = c->cap; dc = c->pp_count[c->cur]; di = c->pp_id[c->cur]; dt = c->pp_tsd[c->cur];
     } else if (kind == XDEM_HISTORY_PLANE) {
         cap = XDEM_MAX_P2PL_COUNT; dc = c->pl_count; di = c->pl_id; dt = c->pl_tsd;
+    } else if (kind == XDEM_HISTORY_MESH) {
+        if (h.dev.mesh < 0) return XDEM_E_INVALID;
+        cap = XDEM_MAX_P2MS_COUNT; dc = c->ms_count; di = c->ms_id; dt = c->ms_tsd;
     } else {
         return XDEM_E_INVALID;
     }
@@ -885,6 +1062,63 @@ int xdem_ctx_read_history(xdem_ctx* c, int kind, unsigned int scene, unsigned in
     if (kind == XDEM_HISTORY_P2P)
         for (size_t i = 0; i < m; i++)
             for (uint32_t s = 0; s < count[i] && s < cap; s++) ids[i * cap + s] -= (uint32_t)b;
+    return XDEM_OK;
+}
+
+int xdem_ctx_write_history(xdem_ctx* c, int kind, unsigned int scene, const unsigned int* count,
+                           const unsigned int* ids, const double* tsd)
+{
+    if (!c || !count || !ids || !tsd) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    const SceneHost& h = c->scenes[scene];
+    const size_t b = h.p_begin, m = h.s.np;
+    uint32_t cap, id_limit;
+    uint32_t *dc, *di;
+    double2* dt;
+    if (kind == XDEM_HISTORY_P2P) {
+        cap = c->cap; dc = c->pp_count[c->cur]; di = c->pp_id[c->cur]; dt = c->pp_tsd[c->cur];
+        id_limit = (uint32_t)m;
+    } else if (kind == XDEM_HISTORY_PLANE) {
+        cap = XDEM_MAX_P2PL_COUNT; dc = c->pl_count; di = c->pl_id; dt = c->pl_tsd;
+        id_limit = UINT32_MAX;
+    } else if (kind == XDEM_HISTORY_MESH) {
+        if (h.dev.mesh < 0) return XDEM_E_INVALID;
+        cap = XDEM_MAX_P2MS_COUNT; dc = c->ms_count; di = c->ms_id; dt = c->ms_tsd;
+        id_limit = c->meshes[h.dev.mesh].dev.ntri;
+    } else {
+        return XDEM_E_INVALID;
+    }
+    // a count past the slots or an id outside the scene would index past its arrays in the
+    // contact kernels: refuse the whole history
+    std::vector<uint32_t> idv(ids, ids + m * cap);
+    for (size_t i = 0; i < m; i++) {
+        if (count[i] > cap) return XDEM_E_INVALID;
+        for (uint32_t s = 0; s < count[i]; s++) {
+            uint32_t& v = idv[i * cap + s];
+            if (id_limit != UINT32_MAX && v >= id_limit) return XDEM_E_INVALID;
+            if (kind == XDEM_HISTORY_P2P) v += (uint32_t)b;
+        }
+    }
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    CTX_CHECK(hipMemcpyAsync(dc + b, count, sizeof(uint32_t) * m, hipMemcpyHostToDevice, c->stream));
+    CTX_CHECK(hipMemcpyAsync(di + b * cap, idv.data(), sizeof(uint32_t) * m * cap, hipMemcpyHostToDevice, c->stream));
+    CTX_CHECK(hipMemcpyAsync(dt + b * cap, tsd, sizeof(double2) * m * cap, hipMemcpyHostToDevice, c->stream));
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    return XDEM_OK;
+}
+
+int xdem_ctx_scene_info_get(xdem_ctx* c, unsigned int scene, xdem_ctx_scene_info* out)
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    const SceneHost& h = c->scenes[scene];
+    std::memset(out, 0, sizeof(*out));
+    out->np = h.s.np;
+    out->nplane = h.s.nplane;
+    out->p2p_capacity = c->cap;
+    out->has_mesh = h.dev.mesh >= 0 ? 1u : 0u;
+    out->first_particle = h.p_begin;
     return XDEM_OK;
 }
 
@@ -974,6 +1208,168 @@ int xdem_ctx_set_body_pose(xdem_ctx* c, unsigned int scene, unsigned int wall, c
     return XDEM_OK;
 }
 
+int xdem_ctx_add_mesh(xdem_ctx* c, unsigned int scene, const xdem_mesh* m, unsigned int* idx)
+{
+    if (!c || !m || !m->vertices || m->ntriangle == 0) return XDEM_E_INVALID;
+    if (c->committed) return XDEM_E_NOT_READY;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    SceneHost& sh = c->scenes[scene];
+    if (sh.dev.mesh >= 0) return XDEM_E_UNSUPPORTED;   // one mesh body per scene
+    if ((unsigned long long)c->ntri + m->ntriangle >= (1ull << kIdBits)) return XDEM_E_INVALID;
+    const uint32_t T = m->ntriangle;
+    MeshHost h;
+    h.scene = scene;
+    h.vlist.assign(m->vertices, m->vertices + 9 * (size_t)T);
+    for (double v : h.vlist)
+        if (!std::isfinite(v)) return XDEM_E_INVALID;
+    const double* d = m->dbi;
+    // proxy spheres: fitted to the global triangles at this pose, kept in the body frame
+    // (xParticleMeshObjectContact.cpp:104-127)
+    h.plocal.resize(3 * (size_t)T);
+    h.prad.resize(T);
+    double maxprox = 0.0;
+    for (uint32_t t = 0; t < T; t++) {
+        double g[3][3];
+        for (int q = 0; q < 3; q++) {
+            double gl[3];
+            euler_rotate(d + XDEM_E0, &h.vlist[9 * (size_t)t + 3 * q], false, gl);
+            for (int a = 0; a < 3; a++) g[q][a] = d[XDEM_PX + a] + gl[a];
+        }
+        double sph[4];
+        xdem_fit_sphere_to_triangle(g[0], g[1], g[2], 0.8, sph);
+        const double rel[3] = {sph[0] - d[XDEM_PX], sph[1] - d[XDEM_PY], sph[2] - d[XDEM_PZ]};
+        euler_rotate(d + XDEM_E0, rel, true, &h.plocal[3 * (size_t)t]);
+        h.prad[t] = sph[3];
+        maxprox = std::max(maxprox, sph[3]);
+        for (int q = 0; q < 3; q++) {
+            const double* v = &h.vlist[9 * (size_t)t + 3 * q];
+            h.rmax_local = std::max(h.rmax_local, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+        }
+    }
+    MeshDev& D = h.dev;
+    std::memset(&D, 0, sizeof(D));
+    std::memcpy(D.dbi, d, sizeof(double) * 15);
+    D.cc = make_contact_const(m->cp, sh.rmax);
+    D.tri_begin = c->ntri;
+    D.ntri = T;
+    D.wslot = -1;
+    // body-frame grid: every triangle in each cell its bounding box, grown by the largest
+    // reach of a list (radius, cohesion reach and skin at scale 1, with a rounding margin),
+    // overlaps; cells of twice that reach or the mean edge length, at most 2^22 cells
+    const double Rq = (sh.rmax + D.cc.coh_reach + sh.dev.skin) * (1.0 + 1e-6) + 1e-9;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, edges = 0.0;
+    for (uint32_t t = 0; t < T; t++)
+        for (int q = 0; q < 3; q++) {
+            const double* v = &h.vlist[9 * (size_t)t + 3 * q];
+            const double* w = &h.vlist[9 * (size_t)t + 3 * ((q + 1) % 3)];
+            edges += std::sqrt((v[0] - w[0]) * (v[0] - w[0]) + (v[1] - w[1]) * (v[1] - w[1]) + (v[2] - w[2]) * (v[2] - w[2]));
+            for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], v[a]); hi[a] = std::max(hi[a], v[a]); }
+        }
+    for (int a = 0; a < 3; a++) { lo[a] -= Rq; hi[a] += Rq; }
+    double hc = std::max(2.0 * Rq, edges / (3.0 * T));
+    uint32_t dims[3];
+    for (;;) {
+        unsigned long long cells = 1;
+        for (int a = 0; a < 3; a++) {
+            dims[a] = (uint32_t)std::max(1.0, std::ceil((hi[a] - lo[a]) / hc));
+            cells *= dims[a];
+        }
+        if (cells <= (1ull << 22)) break;
+        hc *= 1.25;
+    }
+    const size_t ncell = (size_t)dims[0] * dims[1] * dims[2];
+    auto cell_range = [&](uint32_t t, int a, uint32_t& c0, uint32_t& c1) {
+        double mn = 1e300, mx = -1e300;
+        for (int q = 0; q < 3; q++) {
+            mn = std::min(mn, h.vlist[9 * (size_t)t + 3 * q + a]);
+            mx = std::max(mx, h.vlist[9 * (size_t)t + 3 * q + a]);
+        }
+        const double f0 = std::floor((mn - Rq - lo[a]) / hc), f1 = std::floor((mx + Rq - lo[a]) / hc);
+        c0 = (uint32_t)std::max(0.0, std::min(f0, (double)dims[a] - 1.0));
+        c1 = (uint32_t)std::max(0.0, std::min(f1, (double)dims[a] - 1.0));
+    };
+    h.cell_start.assign(ncell + 1, 0u);
+    for (int pass = 0; pass < 2; pass++) {
+        std::vector<uint32_t> fill;
+        if (pass == 1) {
+            for (size_t k = 0; k < ncell; k++) h.cell_start[k + 1] += h.cell_start[k];
+            h.cell_tris.resize(h.cell_start[ncell]);
+            fill.assign(h.cell_start.begin(), h.cell_start.end() - 1);
+        }
+        for (uint32_t t = 0; t < T; t++) {
+            uint32_t r[3][2];
+            for (int a = 0; a < 3; a++) cell_range(t, a, r[a][0], r[a][1]);
+            for (uint32_t z = r[2][0]; z <= r[2][1]; z++)
+                for (uint32_t y = r[1][0]; y <= r[1][1]; y++)
+                    for (uint32_t x = r[0][0]; x <= r[0][1]; x++) {
+                        const size_t cell = ((size_t)z * dims[1] + y) * dims[0] + x;
+                        if (pass == 0) h.cell_start[cell + 1]++;
+                        else h.cell_tris[fill[cell]++] = t;
+                    }
+        }
+    }
+    D.lo_x = lo[0]; D.lo_y = lo[1]; D.lo_z = lo[2]; D.h = hc;
+    D.dim_x = dims[0]; D.dim_y = dims[1]; D.dim_z = dims[2];
+    if (d[XDEM_MM] > 0.0) {   // a dynamic body: its wrench is reduced every step
+        D.wslot = (int32_t)c->nwslot++;
+        c->wslots.push_back({sh.dev.p_begin, sh.dev.p_end, kMeshNear, 0u});
+    }
+    // the reference's cell is twice the largest of the sphere and proxy radii
+    sh.dev.ref_cs = std::max(sh.dev.ref_cs, 2.0 * maxprox);
+    sh.dev.mesh = (int32_t)c->meshes.size();
+    c->ntri += T;
+    if (idx) *idx = (unsigned)c->meshes.size();
+    c->meshes.push_back(std::move(h));
+    return XDEM_OK;
+}
+
+int xdem_ctx_set_mesh_pose(xdem_ctx* c, unsigned int mesh, const double dbi[15])
+{
+    if (!c || !dbi) return XDEM_E_INVALID;
+    if (mesh >= c->meshes.size()) return XDEM_E_INVALID;
+    for (int q = 0; q < 15; q++)
+        if (!std::isfinite(dbi[q])) return XDEM_E_INVALID;
+    MeshHost& m = c->meshes[mesh];
+    // largest displacement of a point of the body: |dP| + |A(e') - A(e)|_F r_max, rounded up
+    double A0[9], A1[9];
+    for (int k = 0; k < 3; k++) {
+        const double ek[3] = {k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+        double c0[3], c1[3];
+        euler_rotate(m.dev.dbi + XDEM_E0, ek, false, c0);
+        euler_rotate(dbi + XDEM_E0, ek, false, c1);
+        for (int a = 0; a < 3; a++) { A0[3 * a + k] = c0[a]; A1[3 * a + k] = c1[a]; }
+    }
+    double fro = 0.0, dp = 0.0;
+    for (int k = 0; k < 9; k++) fro += (A1[k] - A0[k]) * (A1[k] - A0[k]);
+    for (int a = 0; a < 3; a++) dp += (dbi[XDEM_PX + a] - m.dev.dbi[XDEM_PX + a]) * (dbi[XDEM_PX + a] - m.dev.dbi[XDEM_PX + a]);
+    const double dmax = (std::sqrt(dp) + std::sqrt(fro) * m.rmax_local) * (1.0 + 1e-9) + 1e-15;
+    std::memcpy(m.dev.dbi, dbi, sizeof(double) * 15);
+    if (c->committed) {
+        CTX_CHECK(hipSetDevice(c->cfg.device));
+        CTX_CHECK(hipMemcpyAsync(c->d_meshes + mesh, &m.dev, sizeof(MeshDev), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_mesh_pose, dim3((m.dev.ntri + 255) / 256), dim3(256), 0, c->stream, c->d_meshes, mesh,
+                           c->d_vlist, c->d_plocal, c->d_prad, c->d_tri, c->d_prox);
+        CTX_CHECK(hipGetLastError());
+        CTX_CHECK(c->body_moved(dmax));
+        CTX_CHECK(hipStreamSynchronize(c->stream));   // the host's pose copy may change after return
+    }
+    return XDEM_OK;
+}
+
+int xdem_ctx_mesh_wrench(xdem_ctx* c, unsigned int mesh, double* out)
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    if (mesh >= c->meshes.size()) return XDEM_E_INVALID;
+    const int slot = c->meshes[mesh].dev.wslot;
+    for (int q = 0; q < 6; q++) out[q] = 0.0;
+    if (slot < 0) return XDEM_OK;
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    CTX_CHECK(hipMemcpyAsync(out, c->wrench_out + 6 * (size_t)slot, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    return XDEM_OK;
+}
+
 int xdem_ctx_info_get(xdem_ctx* c, xdem_ctx_info* out)
 {
     if (!c || !out) return XDEM_E_INVALID;
@@ -986,6 +1382,7 @@ int xdem_ctx_info_get(xdem_ctx* c, xdem_ctx_info* out)
     out->key_bits = c->key_bits;
     out->p2p_capacity = c->cap;
     out->device_bytes = c->bytes;
+    out->ntriangle = c->ntri;
     out->steps_done = c->steps_done;
     if (c->committed) {
         CTX_CHECK(hipSetDevice(c->cfg.device));
@@ -1000,6 +1397,11 @@ int xdem_ctx_info_get(xdem_ctx* c, xdem_ctx_info* out)
         for (uint32_t k = 0; k < kQueueSegs; k++) {
             out->list_spheres += q1[k * kSegStride];
             out->contact_spheres += q2[k * kSegStride];
+        }
+        if (c->mesh_count) {
+            CTX_CHECK(hipMemcpyAsync(&out->mesh_spheres, c->mesh_count, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                     c->stream));
+            CTX_CHECK(hipStreamSynchronize(c->stream));
         }
     }
     return XDEM_OK;
@@ -1067,9 +1469,12 @@ int xdem_ctx_set_timing(xdem_ctx* c, int on)
     return XDEM_OK;
 }
 
-int xdem_ctx_timing(xdem_ctx* c, double out[6])
+int xdem_ctx_timing(xdem_ctx* c, double out[9])
 {
     if (!c || !out) return XDEM_E_INVALID;
+    out[6] = c->t_collide;
+    out[7] = c->t_meshpass;
+    out[8] = c->t_pairpass;
     out[5] = c->t_lite_move;
     out[0] = c->t_contact;
     out[1] = c->t_grid;

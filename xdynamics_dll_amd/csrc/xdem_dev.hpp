@@ -191,8 +191,9 @@ __device__ __forceinline__ double cohesion_limit_depth(const ContactConst& cc, d
 // normal the tangential slip is a difference of nearly equal velocities, so FP32 there
 // would miss the 1e-4 force bound by orders of magnitude. With T = float (the mixed path)
 // the tangential spring-damper magnitude, its Coulomb cap and the output vectors are FP32.
+// Returns true when Ft was set (|e| != 0).
 template <class T, int MODEL, bool GPU_LAW>
-__device__ __forceinline__ void force_law(const ContactConst& cc, const PairConst<T>& c, double cdist, double dt,
+__device__ __forceinline__ bool force_law(const ContactConst& cc, const PairConst<T>& c, double cdist, double dt,
                                           double& ds, double& dots, V3<double> dv, V3<double> unit, V3<T>& Ft,
                                           V3<T>& Fn)
 {
@@ -229,7 +230,9 @@ __device__ __forceinline__ void force_law(const ContactConst& cc, const PairCons
         }
         const T ft2 = c.mu * length(Fn);
         Ft = fmin(ft1, ft2) * cvt3<T>(sh);
+        return true;
     }
+    return false;
 }
 
 template <class T, int MODEL>

@@ -397,6 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(StepArgs A, uint32_t ntile)
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) A.seg_count[threadIdx.x * kSegStride] = 0u;   // candidate queue
+    if (blockIdx.x == 0 && threadIdx.x == 0 && A.has_mesh) *A.mesh_count = 0u;                      // mesh queue
     const uint32_t sp_units = (A.np + kUnit - 1) / kUnit;
     if (!run_phase(A, 0, sp_units, [&](uint32_t u) {
             if (u == 0 && threadIdx.x == 0) list_ctl_update(A, true);   // before the order phase reads the skin

@@ -179,6 +179,73 @@ double xdem_rng_uniform(unsigned long long* state)
     return (double)(xdem_rng_next(state) >> 11) * (1.0 / 9007199254740992.0);
 }
 
+int xdem_wheel_mesh(double radius, double width, unsigned int n_theta, unsigned int n_y, unsigned int n_grouser,
+                    double grouser_height, double grouser_thickness, unsigned int n_ring, double* vertices,
+                    unsigned int* ntriangle)
+{
+    if (!ntriangle || !(radius > 0.0) || !(width > 0.0) || n_theta < 3 || n_y < 1 || n_ring < 3) return XDEM_E_INVALID;
+    const unsigned T = 2 * n_theta * n_y + 4 * n_ring + n_grouser * (6 * n_y + 4);
+    if (!vertices) {
+        *ntriangle = T;
+        return XDEM_OK;
+    }
+    if (*ntriangle < T) return XDEM_E_INVALID;
+    const double pi = 3.14159265358979323846;
+    unsigned k = 0;
+    auto tri = [&](V3 a, V3 b, V3 c) {
+        const V3 v[3] = {a, b, c};
+        for (int q = 0; q < 3; q++) {
+            vertices[9 * k + 3 * q + 0] = v[q].x;
+            vertices[9 * k + 3 * q + 1] = v[q].y;
+            vertices[9 * k + 3 * q + 2] = v[q].z;
+        }
+        k++;
+    };
+    // a point at angle phi (about y, from +x towards +z), radius rr, axial position y
+    auto at = [](double rr, double phi, double y) { return V3{rr * std::cos(phi), y, rr * std::sin(phi)}; };
+    const double y0 = -0.5 * width, dy = width / n_y;
+    // the rim: n_theta x n_y quads, outward normals
+    for (unsigned i = 0; i < n_theta; i++) {
+        const double p0 = 2 * pi * i / n_theta, p1 = 2 * pi * (i + 1) / n_theta;
+        for (unsigned j = 0; j < n_y; j++) {
+            const double ya = y0 + j * dy, yb = y0 + (j + 1) * dy;
+            tri(at(radius, p0, ya), at(radius, p0, yb), at(radius, p1, yb));
+            tri(at(radius, p0, ya), at(radius, p1, yb), at(radius, p1, ya));
+        }
+    }
+    // side rings: annuli from radius - grouser_height to radius at y = -width/2 and +width/2
+    const double ri = radius - grouser_height;
+    for (int side = 0; side < 2; side++) {
+        const double y = side ? 0.5 * width : -0.5 * width;
+        for (unsigned i = 0; i < n_ring; i++) {
+            const double p0 = 2 * pi * i / n_ring, p1 = 2 * pi * (i + 1) / n_ring;
+            tri(at(ri, p0, y), at(radius, p0, y), at(radius, p1, y));
+            tri(at(ri, p0, y), at(radius, p1, y), at(ri, p1, y));
+        }
+    }
+    // grousers: bars of height grouser_height and thickness grouser_thickness on the rim
+    for (unsigned g = 0; g < n_grouser; g++) {
+        const double pc = 2 * pi * (g + 0.5) / n_grouser, half = 0.5 * grouser_thickness / radius;
+        const double pa = pc - half, pb = pc + half, ro = radius + grouser_height;
+        for (unsigned j = 0; j < n_y; j++) {
+            const double ya = y0 + j * dy, yb = y0 + (j + 1) * dy;
+            tri(at(ro, pa, ya), at(ro, pa, yb), at(ro, pb, yb));          // top
+            tri(at(ro, pa, ya), at(ro, pb, yb), at(ro, pb, ya));
+            tri(at(radius, pa, ya), at(radius, pa, yb), at(ro, pa, yb));  // leading side
+            tri(at(radius, pa, ya), at(ro, pa, yb), at(ro, pa, ya));
+            tri(at(radius, pb, ya), at(ro, pb, yb), at(radius, pb, yb));  // trailing side
+            tri(at(radius, pb, ya), at(ro, pb, ya), at(ro, pb, yb));
+        }
+        for (int side = 0; side < 2; side++) {                            // end caps
+            const double y = side ? 0.5 * width : -0.5 * width;
+            tri(at(radius, pa, y), at(ro, pa, y), at(ro, pb, y));
+            tri(at(radius, pa, y), at(ro, pb, y), at(radius, pb, y));
+        }
+    }
+    *ntriangle = k;
+    return XDEM_OK;
+}
+
 int xdem_make_plane(const double p0[3], const double p1[3], const double p2[3], const double p3[3],
                     unsigned int id, double mass, xdem_plane_info* out, double dbi[15])
 {
@@ -356,6 +423,42 @@ int xdem_scene_build(int workload, unsigned long long seed, unsigned int np_over
         set_walls(s, pl, dbi, wall);
         s->dt = 5e-6;
         s->nsteps = 2000;
+        return XDEM_OK;
+    }
+    if (workload == XDEM_SCENE_WHEEL_BED) {
+        // spheres r ~ U[2, 3] mm for a 1.0 x 0.4 x 0.15 m bed (500,000 by default), inserted
+        // without overlap into a column of twice the bed's height (solid fraction ~0.3) over
+        // the bin floor, to settle under gravity; a smaller count shrinks the bin's footprint
+        unsigned n = np_override ? np_override : 500000u;
+        const double scale = std::sqrt((double)n / 5.0e5);
+        std::vector<double> radii(n);
+        double vol = 0.0;
+        for (unsigned i = 0; i < n; i++) {
+            radii[i] = 2.0e-3 + 1.0e-3 * xdem_rng_uniform(&rng);
+            vol += 4.0 / 3.0 * pi * radii[i] * radii[i] * radii[i];
+        }
+        const double L = 1.0 * scale, W = 0.4 * scale;
+        const double H = vol / (0.30 * L * W);
+        std::sort(radii.begin(), radii.end(), [](double a, double b) { return a > b; });
+        V3 lo = {0, 0, 0}, hi = {L, W, H};
+        Rsa rsa(lo, hi, 3.0e-3);
+        for (unsigned i = 0; i < n; i++) {
+            double r = radii[i];
+            bool ok = false;
+            for (int a = 0; a < 100000 && !ok; a++) {
+                V3 c = {r + (L - 2 * r) * xdem_rng_uniform(&rng), r + (W - 2 * r) * xdem_rng_uniform(&rng),
+                        r + (H - 2 * r) * xdem_rng_uniform(&rng)};
+                if (rsa.fits(c, r)) { rsa.add(c, r); ok = true; }
+            }
+            if (!ok) return XDEM_E_INVALID;
+        }
+        spatial_order(rsa.p, 6.0e-3, lo);
+        fill_common(s, rsa.p, 2500.0);
+        xdem_contact_property_make(5e7, 5e7, 0.25, 0.25, 0.4, 0.5, 0.05, 0.0, 0.0, &s->cp_pp);
+        box_walls(L, W, std::max(0.35, 1.2 * H), pl, dbi);
+        set_walls(s, pl, dbi, s->cp_pp);
+        s->dt = 1e-5;
+        s->nsteps = 5000;
         return XDEM_OK;
     }
     if (workload == XDEM_SCENE_BATCH_MEMBER) {

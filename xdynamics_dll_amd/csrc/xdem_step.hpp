@@ -22,10 +22,12 @@ constexpr uint32_t kWallNear = 0x40000000u;   // a wall within reach + skin (bui
 constexpr uint32_t kTouch = 0x20000000u;      // this step: a pair passes or a wall is in reach (k_touch)
 constexpr uint32_t kSpinStable = 0x10000000u; // both ev and ep buffers hold the sphere's current values
 constexpr uint32_t kAccStable = 0x08000000u;  // acc holds the sphere's contact-free acceleration
-constexpr uint32_t kNoContacts = 0x04000000u; // both p2p count buffers and the wall count hold 0, the
-                                              // dynamic-wall wrench slots hold zeros
+constexpr uint32_t kNoContacts = 0x04000000u; // both p2p count buffers, the wall count and the mesh
+                                              // count hold 0
+constexpr uint32_t kMeshNear = 0x02000000u;   // triangles of the scene's mesh within reach + skin (build);
+                                              // such a sphere goes to the full pass every step
 constexpr uint32_t kLiteStable = kSpinStable | kAccStable | kNoContacts;   // set by the lite pass
-constexpr uint32_t kCountMask = 0x03ffffffu;
+constexpr uint32_t kCountMask = 0x01ffffffu;
 // The full-pass queue is split in kQueueSegs segments, each with its own counter on its own
 // 128-byte line: the wave of sorted slots [64 w, 64 w + 64) appends to segment w mod
 // kQueueSegs, so the appends of different waves rarely hit the same counter.
@@ -50,6 +52,8 @@ inline uint32_t queue_segment_cap(uint32_t nsorted)
     return (nwaves + kQueueSegs - 1u) / kQueueSegs * 64u;
 }
 constexpr int kMaxWallsPerScene = 16;
+constexpr uint32_t kTriCap = 16;              // triangles listed per sphere (mesh contacts)
+constexpr uint32_t kWrenchBlocks = 256;       // partial sums per body of the wrench reduction
 
 // Neighbour-list skin control (device memory, one per context). Every step counts `step`;
 // on a list build with `adaptive` set, the skin scale doubles when the lists lasted fewer
@@ -81,6 +85,23 @@ struct SceneDev {
                               // the lists use skin * ListCtl::scale
     double ref_cs;            // the reference's cell size (visiting order of the contacts);
                               // grid.cs, the bucket size, is larger by the skin and reach
+    int32_t mesh;             // the scene's mesh body (index into StepArgs::meshes), -1 for none
+    uint32_t pad2_;
+};
+
+// A rigid triangle mesh body of a scene. Triangles keep their body-frame vertices; the
+// global ones (and the proxy sphere centres of the reference's cell binning) are recomputed
+// from them at every pose. A static grid in the body frame (cells of side h over the mesh's
+// local bounding box) lists every triangle within reach of any point of a cell: a sphere
+// centre's cell holds every triangle its list can need.
+struct MeshDev {
+    double dbi[16];           // pose MM..ED3 (15 used)
+    ContactConst cc;          // sphere-mesh material pair
+    uint32_t tri_begin, ntri; // triangles in the context's triangle arrays
+    int32_t wslot;            // per-sphere wrench slot of a dynamic mesh, -1 for a fixed one
+    uint32_t cell_base;       // first cell of the body-frame grid in the context's cell table
+    double lo_x, lo_y, lo_z, h;
+    uint32_t dim_x, dim_y, dim_z, pad_;
 };
 
 struct WallDev {
@@ -133,7 +154,21 @@ struct StepArgs {
     const uint32_t* pp_count_in; const uint32_t* pp_id_in; const double2* pp_tsd_in;
     uint32_t* pp_count_out; uint32_t* pp_id_out; double2* pp_tsd_out;
     uint32_t* pl_count; uint32_t* pl_id; double2* pl_tsd;
-    double* wrench;          // [nwslot][6][np] per-sphere contributions to dynamic walls
+    double* wrench;          // [nwslot][6][np] per-sphere contributions to dynamic walls and meshes
+    // meshes
+    const MeshDev* meshes;
+    const xdem_triangle_info* tri_glob;      // global triangles at the current pose
+    const double* prox_glob;                 // [4 T] global proxy sphere centres (+ radius)
+    const uint32_t* mcell_start;             // body-frame grid: triangles of cell c are
+    const uint32_t* mcell_tris;              // mcell_tris[mcell_start[c] .. mcell_start[c + 1])
+    uint32_t* tri_count;                     // per sphere: listed triangles (build)
+    uint32_t* tri_list;                      // their indices in the mesh, kTriCap per sphere
+    double* msum;                            // per sphere: triangle sums F, M, T (3 each), R
+    double* ppsum;                           // per sphere of the full pass: sphere-contact sums F, M, T, R
+    uint32_t* ms_count; uint32_t* ms_id; double2* ms_tsd;   // triangle history, 5 slots (in place)
+    uint32_t* mesh_work;                     // spheres with listed triangles (build), mesh_count of them
+    uint32_t* mesh_count;
+    int has_mesh;
     uint32_t* err;
     unsigned long long* counters;  // [0] stored sphere contacts, [1] stored wall contacts
     uint32_t* seg_count;           // spheres per candidate queue segment (kSegStride apart)
@@ -258,6 +293,42 @@ __device__ __forceinline__ uint32_t pairs_slot(const StepArgs& A, uint32_t j, ui
     const bool full = n != 0u || wall_near;
     if (n) A.gap[i] = fminf(A.gap[i], pgap);
     return n | (full ? kCandidate : 0u) | (wall_near ? kWallNear : 0u);
+}
+
+// Triangle search (steps that rebuild the lists): every triangle of the scene's mesh whose
+// closest point (the reference's closestPtPointTriangle, FP64) is within r + reach + skin of
+// sphere i, from the body-frame grid cell of the sphere's centre (the grid lists every
+// triangle within r_max + reach + the largest skin of any point of a cell). A triangle the
+// reference's predicate can accept before either the sphere or the body has moved half the
+// skin is within that distance now. Writes the sphere's triangle list and returns its length.
+__device__ __forceinline__ uint32_t mesh_slot(const StepArgs& A, const SceneDev& S, uint32_t i)
+{
+    const MeshDev& M = A.meshes[S.mesh];
+    const V4<double> p = ld4(A.pos + 4 * (size_t)i);
+    const V4<double> e = {M.dbi[XDEM_E0], M.dbi[XDEM_E1], M.dbi[XDEM_E2], M.dbi[XDEM_E3]};
+    const V3<double> loc = rot_local(mk3(p.x - M.dbi[XDEM_PX], p.y - M.dbi[XDEM_PY], p.z - M.dbi[XDEM_PZ]), e);
+    const double fx = floor((loc.x - M.lo_x) / M.h), fy = floor((loc.y - M.lo_y) / M.h),
+                 fz = floor((loc.z - M.lo_z) / M.h);
+    uint32_t n = 0;
+    if (fx >= 0.0 && fx < (double)M.dim_x && fy >= 0.0 && fy < (double)M.dim_y && fz >= 0.0 && fz < (double)M.dim_z) {
+        const uint32_t c = M.cell_base + ((uint32_t)fz * M.dim_y + (uint32_t)fy) * M.dim_x + (uint32_t)fx;
+        const uint32_t b = A.mcell_start[c], en = A.mcell_start[c + 1];
+        const double lim = (p.w + M.cc.coh_reach + S.skin * (double)A.lctl->scale) * (1.0 + 1e-9) + 1e-12;
+        const V3<double> ip = mk3(p.x, p.y, p.z);
+        uint32_t* out = A.tri_list + (size_t)i * kTriCap;
+        for (uint32_t k = b; k < en; k++) {
+            const uint32_t t = A.mcell_tris[k];
+            int ct;
+            const V3<double> d = ip - closest_on_triangle(A.tri_glob[M.tri_begin + t], ip, ct);
+            if (!(dot(d, d) <= lim * lim)) continue;
+            if (n < kTriCap) out[n] = t;
+            n++;
+        }
+        if (n > kTriCap) atomicOr(A.err, XDEM_ERR_MESH_NEAR);
+    }
+    const uint32_t m = min(n, kTriCap);
+    A.tri_count[i] = m;
+    return m;
 }
 
 // Drift of the next step (impl:135-147; xIntegratorVV.cpp:66-100): Euler parameters and
@@ -445,11 +516,14 @@ __device__ __forceinline__ bool pair_passes(const ContactConst& cc, double ir, d
 template <bool MULTI>
 __device__ __forceinline__ bool touch_sphere(const StepArgs& A, uint32_t i, uint32_t fl, bool built, float dm)
 {
-    bool touch = false;
+    // a sphere with listed triangles goes to the full pass every step (the triangle pass
+    // evaluates its list exactly)
+    const bool mesh_near = (fl & kMeshNear) != 0u;
+    bool touch = mesh_near;
     // a pair can pass only once the two spheres have moved its gap together, a wall come
     // into reach only once the sphere has moved its gap
     const float di = built ? 0.0f : A.disp[i];
-    const bool may = !(A.gap[i] > (di + dm) * 1.0001f + 1e-9f);
+    const bool may = mesh_near || !(A.gap[i] > (di + dm) * 1.0001f + 1e-9f);
     const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
     const V4<double> p = ld4(A.pos + 4 * i);
     if (may && (fl & kWallNear))
@@ -542,9 +616,14 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
         for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
             const uint32_t j = xcd_block(vb, nvb) * blockDim.x + threadIdx.x;
             const uint32_t i = j < A.nsorted ? A.svals[j] : kInvalid;
-            bool full = false, touch = false;
+            bool full = false, touch = false, mesh = false;
             if (i < A.np) {
-                const uint32_t fl = pairs_slot<MULTI>(A, j, i);
+                uint32_t fl = pairs_slot<MULTI>(A, j, i);
+                if (A.has_mesh) {
+                    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+                    if (S.mesh >= 0 && mesh_slot(A, S, i)) fl |= kMeshNear | kCandidate;
+                    mesh = (fl & kMeshNear) != 0u;
+                }
                 full = (fl & kCandidate) != 0u;
                 if (full) touch = touch_sphere<MULTI>(A, i, fl, true, 0.0f);
                 A.nbr_count[i] = touch ? (fl | kTouch) : fl;
@@ -552,6 +631,7 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
             const uint32_t seg = (j >> 6) & (kQueueSegs - 1u);
             queue_append(A.work, A.seg_count, seg, A.seg_cap, full, i);
             queue_append(A.work2, A.seg_count2, seg, A.seg_cap, touch, i);
+            if (A.has_mesh) queue_append(A.mesh_work, A.mesh_count, 0u, 0u, mesh, i);
         }
         return;
     }
@@ -651,6 +731,7 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
     // the pairs)
     OwnState o = load_own<false>(A, S, i);
     const uint32_t flags = A.nbr_count[i];
+    n_pp = 0;
     WallSlots ws;
     const uint32_t count0 = S.nwall ? A.pl_count[i] : 0u;
     ws.init(count0, A.pl_id + XDEM_MAX_P2PL_COUNT * i,
@@ -658,76 +739,14 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
     const V3<double> iom = omega_of(o.ep, o.ev);
     uint32_t err = 0;
 
-    // ---- sphere-sphere (impl:613-733)
+    // ---- sphere-sphere (impl:613-733): the sums of k_pairs, which ran before this kernel
     Accum pp = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
     if (S.has_pp) {
-        const uint32_t cap = A.cap;
-        const uint32_t hb = i * cap;
-        uint32_t old_count = A.pp_count_in[i];
-        if (old_count > cap) old_count = cap;
-        // the history ids in 16-byte loads when the slots allow (cap a multiple of 4); the
-        // first four are loaded once for all pairs
-        const bool vec = (cap & 3u) == 0u;
-        const uint4* hid4 = reinterpret_cast<const uint4*>(A.pp_id_in + hb);
-        const uint4 h0 = (vec && old_count) ? hid4[0] : make_uint4(0u, 0u, 0u, 0u);
-        V3<T> Ft = mk3(T(0), T(0), T(0));
-        uint32_t n_new = 0, n_seen = 0;
-        const uint32_t nlist_all = flags & kCountMask;
-        const uint32_t lc = A.list_cap;
-        if (nlist_all > lc) err |= XDEM_ERR_NBR_OVERFLOW;
-        const uint32_t nlist = min(nlist_all, lc);
-        const uint32_t* lst = A.nbr + (size_t)i * lc;
-        // the listed pairs that pass (k_touch's mask), in list order, which k_touch has made
-        // the reference's visiting order
-        unsigned long long m = nlist ? A.pass_mask[i] : 0ull;
-        // each pair's list entry is read one pair ahead, and all of the neighbour's state in
-        // one go, so a pair costs one memory round trip
-        uint32_t k_next = m ? lst[__ffsll((long long)m) - 1] : 0u;
-        while (m) {
-            m &= m - 1ull;
-            const uint32_t k = k_next;
-            if (m) k_next = lst[__ffsll((long long)m) - 1];
-            const V4<double> jpos = ld4(A.pos + 4 * (size_t)k);
-            const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
-            const V4<double> jep = ld4(A.ep + 4 * (size_t)k);
-            const V4<double> jev = ld4(A.ev + 4 * (size_t)k);
-            const double jmass = A.mass[k];
-            // the first slot of the previous step holding k
-            uint32_t slot = kInvalid;
-            if (vec) {
-                for (uint32_t c = 0; c < old_count && slot == kInvalid; c += 4u) {
-                    const uint4 q = c ? hid4[c / 4u] : h0;
-                    if (q.x == k) slot = c;
-                    else if (c + 1u < old_count && q.y == k) slot = c + 1u;
-                    else if (c + 2u < old_count && q.z == k) slot = c + 2u;
-                    else if (c + 3u < old_count && q.w == k) slot = c + 3u;
-                }
-            } else {
-                for (uint32_t s = 0; s < old_count; s++)
-                    if (A.pp_id_in[hb + s] == k) { slot = s; break; }
-            }
-            const double2 h = slot != kInvalid ? A.pp_tsd_in[hb + slot] : make_double2(0.0, 0.0);
-            const PairGeom g = pair_geom(o.pos, jpos);
-            double sd0 = 0.0, sd1 = 0.0;
-            if (g.cdist > 0) {
-                sd0 = h.x; sd1 = h.y;
-                const V3<double> jom = omega_of(jep, jev);
-                if (!A.stale_ft) Ft = mk3(T(0), T(0), T(0));
-                sphere_contact<T, MODEL, GPU_LAW>(S.cc, S.dt, o.pos, o.vel, iom, o.mass, jpos, jvel, jom,
-                                                  jmass, g, sd0, sd1, Ft, pp);
-            } else {
-                sphere_cohesion(S.cc, o.pos.w, jpos.w, g, pp);
-            }
-            n_seen++;
-            if (n_new < cap) {
-                A.pp_id_out[hb + n_new] = k;
-                A.pp_tsd_out[hb + n_new] = make_double2(sd0, sd1);
-                n_new++;
-            }
-        }
-        A.pp_count_out[i] = n_new;
-        if (n_seen > cap) err |= XDEM_ERR_P2P_OVERFLOW;
-        n_pp = n_new;
+        const double* ps = A.ppsum + (size_t)i * 10;
+        pp.F = mk3(ps[0], ps[1], ps[2]);
+        pp.M = mk3(ps[3], ps[4], ps[5]);
+        pp.T = mk3(ps[6], ps[7], ps[8]);
+        pp.R = ps[9];
     }
 
     // ---- walls, in id order (impl:950-1080 once per wall)
@@ -761,9 +780,303 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
         n_wall = ws.count;
     }
 
+    // ---- the scene's mesh: this step's triangle sums (k_mesh_pass), added after the walls, as
+    // the reference's mesh launch adds its sums to force, moment, tmax and rres (impl:2100-2108)
+    if (A.has_mesh && S.mesh >= 0) {
+        if (flags & kMeshNear) {
+            const double* ms = A.msum + (size_t)i * 10;
+            tot.F = tot.F + mk3(ms[0], ms[1], ms[2]);
+            tot.M = tot.M + mk3(ms[3], ms[4], ms[5]);
+            tot.T = tot.T + mk3(ms[6], ms[7], ms[8]);
+            tot.R += ms[9];
+        } else {
+            tot.F = tot.F + mk3(0.0, 0.0, 0.0);
+            tot.M = tot.M + mk3(0.0, 0.0, 0.0);
+            tot.T = tot.T + mk3(0.0, 0.0, 0.0);
+            tot.R += 0.0;
+            A.ms_count[i] = 0u;   // no triangle within reach: the reference's mesh pass stores none
+        }
+    }
+
     if (FUSE) load_own_fused(A, S, i, o);
     sphere_tail<FUSE>(A, S, i, o, tot, err, dmx);
     if (err) atomicOr(A.err, err);
+}
+
+// ----------------------------------------------------------------- triangle pass
+// Triangle contacts of sphere i with its scene's mesh (impl:1930-2109 for one sphere): the
+// listed triangles are taken in the reference's visiting order (cells of the proxy sphere
+// centres, z, y, x offsets, then triangle index: the key (code << 27) | t as for spheres);
+// contacts (and cohesive near-contacts) are sorted into faces, edges and vertices, an edge or
+// vertex point equal within 1e-9 to the previous one of its kind is dropped (the previous
+// points start at the origin, as in the reference), and faces apply, else edges, else
+// vertices. Writes the sphere's triangle sums (k_collide adds them after its walls), its
+// body-wrench contribution and its triangle history (5 slots shared by all meshes, in place:
+// the old slots are read into registers first, as the reference's kernel does).
+template <class T, bool MULTI, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ void mesh_sphere(const StepArgs& A, uint32_t i)
+{
+    constexpr uint32_t CAP = XDEM_MAX_P2MS_COUNT;
+    const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+    const MeshDev& M = A.meshes[S.mesh];
+    const V4<double> p4 = ld4(A.pos + 4 * (size_t)i);
+    const V3<double> ip = mk3(p4.x, p4.y, p4.z);
+    const double ir = p4.w;
+    const V3<double> ivel = ld3(A.vel + 3 * (size_t)i);
+    const V3<double> iom = omega_of(ld4(A.ep + 4 * (size_t)i), ld4(A.ev + 4 * (size_t)i));
+    const double im = A.mass[i];
+    uint32_t pid[CAP];
+    double ps0[CAP], ps1[CAP];
+#pragma unroll
+    for (uint32_t q = 0; q < CAP; q++) {
+        pid[q] = A.ms_id[CAP * (size_t)i + q];
+        const double2 h = A.ms_tsd[CAP * (size_t)i + q];
+        ps0[q] = h.x;
+        ps1[q] = h.y;
+    }
+    double dbi[15];
+#pragma unroll
+    for (int q = 0; q < 15; q++) dbi[q] = M.dbi[q];
+    const ContactConst& cc = M.cc;
+    const double coh_s = cc.coh ? cohesion_limit_depth(cc, ir) : 0.0;
+    // the listed triangles that touch (or are inside the cohesion depth), by visiting key
+    int ix, iy, iz;
+    ref_cell(S, p4, ix, iy, iz);
+    const uint32_t n = min(A.tri_count[i], kTriCap);
+    const uint32_t* lst = A.tri_list + (size_t)i * kTriCap;
+    uint32_t key[kTriCap];
+    uint32_t nk = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t t = lst[k];
+        int ct;
+        const V3<double> cpt = closest_on_triangle(A.tri_glob[M.tri_begin + t], ip, ct);
+        const double cdist = ir - length(ip - cpt);
+        if (!(cdist > 0 || fabs(cdist) < fabs(coh_s))) continue;
+        const double* pc = A.prox_glob + 4 * (size_t)(M.tri_begin + t);
+        const uint32_t code = visit_code(S, ix, iy, iz, V4<double>{pc[0], pc[1], pc[2], 0.0});
+        if (code == kNotVisited) continue;
+        uint32_t x = (code << kIdBits) | t, u = nk;
+        while (u > 0u && key[u - 1u] > x) {   // insertion sort
+            key[u] = key[u - 1u];
+            u--;
+        }
+        key[u] = x;
+        nk++;
+    }
+    // the reference's selection in visiting order: which of the sorted entries are faces,
+    // edges and vertices
+    uint32_t faces = 0, edges = 0, verts = 0;
+    V3<double> prev_line = mk3(0.0, 0.0, 0.0), prev_point = mk3(0.0, 0.0, 0.0);
+    for (uint32_t k = 0; k < nk; k++) {
+        const uint32_t t = key[k] & kIdMask;
+        int ct;
+        const V3<double> cpt = closest_on_triangle(A.tri_glob[M.tri_begin + t], ip, ct);
+        const double cdist = ir - length(ip - cpt);
+        if (cdist > 0) {
+            if (ct == 1) {
+                if (same_point(prev_line, cpt)) continue;
+                prev_line = cpt;
+            } else if (ct == 0) {
+                if (same_point(prev_point, cpt)) continue;
+                prev_point = cpt;
+            }
+        }
+        if (ct == 2) faces |= 1u << k;
+        else if (ct == 1) edges |= 1u << k;
+        else if (ct == 0) verts |= 1u << k;
+    }
+    const uint32_t apply = faces ? faces : (edges ? edges : verts);
+    Accum acc = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
+    double bf[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t n_new = 0, n_seen = 0;
+    for (uint32_t m = apply; m; m &= m - 1u) {
+        const uint32_t k = (uint32_t)__ffs((int)m) - 1u;
+        const uint32_t t = key[k] & kIdMask;
+        int ct;
+        const TriHit h = {t, closest_on_triangle(A.tri_glob[M.tri_begin + t], ip, ct)};
+        double sd0, sd1;
+        if (triangle_contact<T, MODEL, GPU_LAW>(cc, dbi, S.dt, h, ip, ir, im, ivel, iom, pid, ps0, ps1, sd0, sd1, acc,
+                                                bf)) {
+            n_seen++;
+            if (n_new < CAP) {
+                A.ms_id[CAP * (size_t)i + n_new] = t;
+                A.ms_tsd[CAP * (size_t)i + n_new] = make_double2(sd0, sd1);
+                n_new++;
+            }
+        }
+    }
+    A.ms_count[i] = n_new;
+    double* ms = A.msum + (size_t)i * 10;
+    st3(ms, acc.F);
+    st3(ms + 3, acc.M);
+    st3(ms + 6, acc.T);
+    ms[9] = acc.R;
+    if (M.wslot >= 0) {
+        double* wr = A.wrench + (size_t)M.wslot * 6 * A.np;
+#pragma unroll
+        for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = bf[c];
+    }
+    if (n_seen > CAP) atomicOr(A.err, XDEM_ERR_MESH_OVERFLOW);
+}
+
+// The triangle pass over the spheres with listed triangles (the mesh queue of the last build);
+// it runs before k_collide, which adds the sums.
+template <class T, bool MULTI, int MODEL, bool GPU_LAW>
+__global__ __launch_bounds__(256) void k_mesh_pass(StepArgs A)
+{
+    const uint32_t cnt = *A.mesh_count;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < cnt; e += gridDim.x * blockDim.x) {
+        const uint32_t i = A.mesh_work[e];
+        if (i < A.np) mesh_sphere<T, MULTI, MODEL, GPU_LAW>(A, i);
+    }
+}
+
+// ------------------------------------------------------------------------- pair pass
+// The sphere contacts of the full pass, kPairLanes lanes per sphere: lane g evaluates the
+// passing pairs of rank g, g + kPairLanes, ... (rank = place among the sphere's passing list
+// entries, i.e. the reference's visiting order), writes their history slots (slot = rank, up
+// to the capacity) and stages its results in LDS; then the sphere's first lane adds them up in
+// rank order with sphere_contact's operations (pairs beyond the LDS stage are evaluated by
+// that lane itself, in order), so the sums are bit-identical to one thread walking the pairs.
+// The sums go to ppsum for k_collide. Cohesion-only pairs stage their attraction.
+constexpr uint32_t kPairLanes = 8;
+constexpr uint32_t kPairStage = 16;                       // pair results per sphere in LDS
+constexpr uint32_t kPairSpheres = 256 / kPairLanes;       // spheres per block and round
+constexpr uint32_t kPairSet = 0, kPairStale = 1, kPairCoh = 2;   // kinds of a staged pair
+
+// Sphere i's pair of list entry t: its neighbour's state, history and force; returns the kind.
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ uint32_t pair_of(const StepArgs& A, const SceneDev& S, uint32_t i, const OwnState& o,
+                                            V3<double> iom, uint32_t k, uint32_t rank, uint32_t old_count,
+                                            double v[10])
+{
+    const uint32_t cap = A.cap, hb = i * cap;
+    const V4<double> jpos = ld4(A.pos + 4 * (size_t)k);
+    const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
+    const V4<double> jep = ld4(A.ep + 4 * (size_t)k);
+    const V4<double> jev = ld4(A.ev + 4 * (size_t)k);
+    const double jmass = A.mass[k];
+    // the first slot of the previous step holding k
+    uint32_t slot = kInvalid;
+    if ((cap & 3u) == 0u) {
+        const uint4* hid4 = reinterpret_cast<const uint4*>(A.pp_id_in + hb);
+        for (uint32_t c = 0; c < old_count && slot == kInvalid; c += 4u) {
+            const uint4 q = hid4[c / 4u];
+            if (q.x == k) slot = c;
+            else if (c + 1u < old_count && q.y == k) slot = c + 1u;
+            else if (c + 2u < old_count && q.z == k) slot = c + 2u;
+            else if (c + 3u < old_count && q.w == k) slot = c + 3u;
+        }
+    } else {
+        for (uint32_t s2 = 0; s2 < old_count; s2++)
+            if (A.pp_id_in[hb + s2] == k) { slot = s2; break; }
+    }
+    const double2 h = slot != kInvalid ? A.pp_tsd_in[hb + slot] : make_double2(0.0, 0.0);
+    const PairGeom g = pair_geom(o.pos, jpos);
+    double sd0 = 0.0, sd1 = 0.0;
+    uint32_t kind;
+    if (g.cdist > 0) {
+        sd0 = h.x; sd1 = h.y;
+        const bool set = sphere_pair_eval<T, MODEL, GPU_LAW>(S.cc, S.dt, o.pos, o.vel, iom, o.mass, jpos, jvel,
+                                                             omega_of(jep, jev), jmass, g, sd0, sd1, v);
+        kind = set ? kPairSet : kPairStale;
+    } else {
+        // cohesion only: the attraction cf * unit that sphere_cohesion subtracts
+        const double ir = o.pos.w, jr = jpos.w;
+        const double Req = jr ? (ir * jr) / (ir + jr) : ir;
+        const double cf = cohesion_force<double>(S.cc, Req, S.cc.jk * Req);
+        const V3<double> u = g.rp / g.dist;
+        const V3<double> a = cf * u;
+        v[0] = a.x; v[1] = a.y; v[2] = a.z;
+        kind = kPairCoh;
+    }
+    if (rank < cap) {
+        A.pp_id_out[hb + rank] = k;
+        A.pp_tsd_out[hb + rank] = make_double2(sd0, sd1);
+    }
+    return kind;
+}
+
+template <class T, bool MULTI, int MODEL, bool GPU_LAW>
+__global__ __launch_bounds__(256) void k_pairs(StepArgs A)
+{
+    __shared__ double stage[kPairSpheres][kPairStage][10];
+    __shared__ uint32_t kinds[kPairSpheres][kPairStage];
+    const uint32_t sg = blockIdx.x % kQueueSegs;
+    const uint32_t* q = A.work2 + (size_t)sg * A.seg_cap;
+    const uint32_t cnt = A.seg_count2[sg * kSegStride];
+    const uint32_t lane = threadIdx.x % kPairLanes, ls = threadIdx.x / kPairLanes;
+    const uint32_t stride = gridDim.x / kQueueSegs * kPairSpheres;
+    unsigned long long c0 = 0;
+    for (uint32_t base = blockIdx.x / kQueueSegs * kPairSpheres; base < cnt; base += stride) {
+        const uint32_t e = base + ls;
+        const uint32_t i = e < cnt ? q[e] : kInvalid;
+        const SceneDev* Sp = i != kInvalid ? &A.scenes[scene_index(A, i, MULTI)] : nullptr;
+        const bool work = Sp && Sp->has_pp;
+        unsigned long long m = 0;
+        uint32_t old_count = 0, nlist_all = 0;
+        OwnState o;
+        V3<double> iom = mk3(0.0, 0.0, 0.0);
+        if (work) {
+            o.pos = ld4(A.pos + 4 * (size_t)i);
+            o.vel = ld3(A.vel + 3 * (size_t)i);
+            o.mass = A.mass[i];
+            iom = omega_of(ld4(A.ep + 4 * (size_t)i), ld4(A.ev + 4 * (size_t)i));
+            nlist_all = A.nbr_count[i] & kCountMask;
+            m = min(nlist_all, A.list_cap) ? A.pass_mask[i] : 0ull;
+            old_count = min(A.pp_count_in[i], A.cap);
+        }
+        const uint32_t* lst = A.nbr + (size_t)(work ? i : 0u) * A.list_cap;
+        // this lane's pairs: ranks lane, lane + kPairLanes, ... (staged ones only)
+        {
+            uint32_t rank = 0;
+            for (unsigned long long mm = m; mm; mm &= mm - 1ull, rank++) {
+                if (rank >= kPairStage) break;
+                if (rank % kPairLanes != lane) continue;
+                const uint32_t t = (uint32_t)__ffsll((long long)mm) - 1u;
+                double v[10];
+                kinds[ls][rank] = pair_of<T, MODEL, GPU_LAW>(A, *Sp, i, o, iom, lst[t], rank, old_count, v);
+#pragma unroll
+                for (int c = 0; c < 10; c++) stage[ls][rank][c] = v[c];
+            }
+        }
+        __syncthreads();
+        if (work && lane == 0) {
+            Accum pp = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
+            V3<T> Ft = mk3(T(0), T(0), T(0));
+            uint32_t rank = 0;
+            for (unsigned long long mm = m; mm; mm &= mm - 1ull, rank++) {
+                double v[10];
+                uint32_t kind;
+                if (rank < kPairStage) {
+                    kind = kinds[ls][rank];
+#pragma unroll
+                    for (int c = 0; c < 10; c++) v[c] = stage[ls][rank][c];
+                } else {
+                    const uint32_t t = (uint32_t)__ffsll((long long)mm) - 1u;
+                    kind = pair_of<T, MODEL, GPU_LAW>(A, *Sp, i, o, iom, lst[t], rank, old_count, v);
+                }
+                if (kind == kPairCoh) pp.F = pp.F - mk3(v[0], v[1], v[2]);
+                else sphere_pair_sum<T>(v, kind == kPairSet, A.stale_ft != 0, Ft, pp);
+            }
+            const uint32_t n_new = min(rank, A.cap);
+            A.pp_count_out[i] = n_new;
+            uint32_t err = 0;
+            if (rank > A.cap) err |= XDEM_ERR_P2P_OVERFLOW;
+            if (nlist_all > A.list_cap) err |= XDEM_ERR_NBR_OVERFLOW;
+            if (err) atomicOr(A.err, err);
+            double* ps = A.ppsum + (size_t)i * 10;
+            st3(ps, pp.F);
+            st3(ps + 3, pp.M);
+            st3(ps + 6, pp.T);
+            ps[9] = pp.R;
+            c0 += n_new;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c0 += __shfl_xor(c0, off, 64);
+    if ((threadIdx.x & 63) == 0 && c0) atomicAdd(&A.counters[0], c0);
 }
 
 // Wave-wide sums of the stored-contact counts, one atomic per wave.
@@ -814,16 +1127,8 @@ __device__ __forceinline__ void lite_sphere(const StepArgs& A, const SceneDev& S
     tot.M = mk3(0.0, 0.0, 0.0);
     tot.T = in.o.t0 + mk3(0.0, 0.0, 0.0);
     tot.R = 0.0;
+    if (S.mesh >= 0) A.ms_count[i] = 0u;
     if (S.nwall) {
-        if (S.wall_dynamic)
-            for (uint32_t w = 0; w < S.nwall; w++) {
-                const int32_t slot = A.walls[S.wall_begin + w].wslot;
-                if (slot >= 0) {
-                    double* wr = A.wrench + (size_t)slot * 6 * A.np;
-#pragma unroll
-                    for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
-                }
-            }
         const uint32_t count = S.wall_id0 ? 0u : min(in.count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
         if (count != in.count0) A.pl_count[i] = count;
         n_wall += count;
@@ -885,6 +1190,7 @@ __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
         const bool acc_kept = flags & kAccStable;
         const uint32_t count0 = (S.nwall && !settled) ? A.pl_count[i] : 0u;
         const uint32_t pp_in = (S.has_pp && !settled) ? A.pp_count_in[i] : 0u;
+        const uint32_t msc = (S.mesh >= 0 && !settled) ? A.ms_count[i] : 0u;
         const V4<double> p = ld4(A.pos + 4 * i);
         const V3<double> v0 = ld3(A.vel + 3 * i);
         const V3<double> ao = ld3(A.acc + 3 * i);
@@ -892,26 +1198,18 @@ __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
         if (flags & skip) return;
         uint32_t nf = flags;
         if (!settled) {
-            // contact bookkeeping of a sphere without contacts: empty p2p history, the walls'
-            // slot-count rule, zero dynamic-wall wrench; once all of it holds zeros (the p2p
-            // count in both buffers) nothing changes until the sphere is touched again
+            // contact bookkeeping of a sphere without contacts: empty p2p and triangle history,
+            // the walls' slot-count rule; once all of it holds zeros (the p2p count in both
+            // buffers) nothing changes until the sphere is touched again
             if (S.has_pp) A.pp_count_out[i] = 0u;
+            if (msc) A.ms_count[i] = 0u;
             uint32_t count = 0u;
             if (S.nwall) {
-                if (S.wall_dynamic)
-                    for (uint32_t w = 0; w < S.nwall; w++) {
-                        const int32_t slot = A.walls[S.wall_begin + w].wslot;
-                        if (slot >= 0) {
-                            double* wr = A.wrench + (size_t)slot * 6 * A.np;
-#pragma unroll
-                            for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
-                        }
-                    }
                 count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
                 if (count != count0) A.pl_count[i] = count;
                 n_wall += count;
             }
-            if (pp_in == 0u && count == 0u) nf |= kNoContacts;
+            if (pp_in == 0u && count == 0u && msc == 0u) nf |= kNoContacts;
         }
         V3<double> a = ao;
         if (!acc_kept) {
@@ -1062,7 +1360,7 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
 }
 
 // Contact kernels of one precision (xdem_step_f64.hip / xdem_step_mixed.hip); pass 0 is the
-// lite pass, pass 1 the full pass. The other kernels are FP64 only.
+// lite pass, pass 1 the full pass, pass 2 the triangle pass. The other kernels are FP64 only.
 using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int model, bool gpu_law, int pass,
                                      hipStream_t);
 // which: 0 = k_lite_move then k_lite_turn, 1 = k_lite_move only, 2 = k_lite_turn only

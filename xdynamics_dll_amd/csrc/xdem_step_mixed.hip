@@ -12,7 +12,8 @@ constexpr unsigned kBlock = 256;
 inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
 // pass 0: the lite pass over every sphere outside this step's full pass (two kernels in
-// fused mode); pass 1: the full pass over the spheres k_touch queued (kQueueSegs blocks per
+// fused mode); pass 2: the triangle pass (meshes) and pass 3: the sphere pairs, both before
+// pass 1, the rest of the full pass over the spheres k_touch queued (kQueueSegs blocks per
 // chunk of each queue segment, at most kFullBlocks in all, one per CU: the kernel runs one
 // wave per SIMD; the queue length is read on the device). The two touch disjoint spheres
 // and may run concurrently.
@@ -29,6 +30,12 @@ void collide_passes(const StepArgs& A, int pass, hipStream_t s)
     const dim3 g = grid_for(A.np), b(kBlock);
     if (pass == 1) {
         hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, MODEL, GPU_LAW>), full_grid(A.np), b, 0, s, A);
+    } else if (pass == 3) {
+        // the pair pass: kPairSpheres queued spheres per block and round
+        hipLaunchKernelGGL((k_pairs<float, MULTI, MODEL, GPU_LAW>), full_grid(A.np), b, 0, s, A);
+    } else if (pass == 2) {
+        // the triangle pass (grid-strided over the mesh queue, whose length is on the device)
+        hipLaunchKernelGGL((k_mesh_pass<float, MULTI, MODEL, GPU_LAW>), dim3(kFullBlocks), b, 0, s, A);
     } else if (FUSE) {
         launch_lite_fused(A, MULTI, s);
     } else {

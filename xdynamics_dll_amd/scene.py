@@ -175,3 +175,16 @@ class SceneData:
         p.gravity = _lib.double3(*self.gravity)
         p.world_origin = _lib.double3(*self.world_origin)
         return p
+
+
+def wheel_mesh(radius: float = 0.15, width: float = 0.1, n_theta: int = 170, n_y: int = 8, n_grouser: int = 12,
+               grouser_height: float = 0.015, grouser_thickness: float = 0.006, n_ring: int = 188) -> np.ndarray:
+    """Triangles (T, 3, 3) of a grousered wheel in its body frame, axis along y (xdem_wheel_mesh):
+    4,096 triangles with the defaults."""
+    n = C.c_uint()
+    H = _lib.host()
+    args = (radius, width, n_theta, n_y, n_grouser, grouser_height, grouser_thickness, n_ring)
+    _lib.check(H.xdem_wheel_mesh(*args, None, C.byref(n)), "xdem_wheel_mesh")
+    v = np.zeros(9 * n.value)
+    _lib.check(H.xdem_wheel_mesh(*args, v.ctypes.data, C.byref(n)), "xdem_wheel_mesh")
+    return v.reshape(n.value, 3, 3)

@@ -8,6 +8,7 @@ xIngegratorVV.h:13-14) on top of it. All arithmetic runs in the gfx950 library.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -19,7 +20,7 @@ class DemContext:
     def __init__(self, scenes, precision: str = "fp64", p2p_capacity: int = 12, quirks: dict | None = None,
                  grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False,
                  buckets_per_sphere: int = 0, skin: float = 0.0, step_graph: int = 0,
-                 rebuild_wait_limit: int = 0):
+                 rebuild_wait_limit: int = 0, meshes=()):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -45,6 +46,18 @@ class DemContext:
             cs, keep = s.to_c()
             _lib.check(self.L.xdem_ctx_add_scene(self._h, C.byref(cs), None), "xdem_ctx_add_scene")
             del keep
+        # mesh bodies: (scene index, vertices (T, 3, 3) in the body frame, pose (15), ContactProperty)
+        self.meshes = []
+        for scene, verts, dbi, cp in meshes:
+            v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).reshape(-1, 9))
+            m = _lib.Mesh()
+            m.ntriangle = v.shape[0]
+            m.vertices = v.ctypes.data_as(C.POINTER(C.c_double))
+            m.dbi[:] = [float(x) for x in dbi]
+            m.cp = cp
+            idx = C.c_uint()
+            _lib.check(self.L.xdem_ctx_add_mesh(self._h, scene, C.byref(m), C.byref(idx)), "xdem_ctx_add_mesh")
+            self.meshes.append((scene, v.shape[0]))
         _lib.check(self.L.xdem_ctx_commit(self._h), "xdem_ctx_commit")
 
     # ------------------------------------------------------------------ stepping
@@ -88,14 +101,36 @@ class DemContext:
 
     def history(self, kind: str = "p2p", scene: int = 0):
         n = self.scenes[scene].np
-        k = _lib.HISTORY_P2P if kind == "p2p" else _lib.HISTORY_PLANE
-        cap = self.capacity if kind == "p2p" else 3
+        k = {"p2p": _lib.HISTORY_P2P, "plane": _lib.HISTORY_PLANE, "mesh": _lib.HISTORY_MESH}[kind]
+        cap = {"p2p": self.capacity, "plane": 3, "mesh": 5}[kind]
         count = np.empty(n, dtype=np.uint32)
         ids = np.empty(n * cap, dtype=np.uint32)
         tsd = np.empty(n * cap * 2, dtype=np.float64)
         _lib.check(self.L.xdem_ctx_read_history(self._h, k, scene, count.ctypes.data, ids.ctypes.data,
                                                 tsd.ctypes.data), "xdem_ctx_read_history")
         return count, ids.reshape(n, cap), tsd.reshape(n, cap, 2)
+
+    def write_history(self, kind: str, count, ids, tsd, scene: int = 0) -> None:
+        """Inverse of history(): count (np,), ids (np, cap), tsd (np, cap, 2)."""
+        k = {"p2p": _lib.HISTORY_P2P, "plane": _lib.HISTORY_PLANE, "mesh": _lib.HISTORY_MESH}[kind]
+        c = np.ascontiguousarray(count, dtype=np.uint32)
+        i = np.ascontiguousarray(ids, dtype=np.uint32)
+        t = np.ascontiguousarray(tsd, dtype=np.float64)
+        _lib.check(self.L.xdem_ctx_write_history(self._h, k, scene, c.ctypes.data, i.ctypes.data, t.ctypes.data),
+                   "xdem_ctx_write_history")
+
+    def save_part(self, path: str, ct: float, scene: int = 0) -> None:
+        """Part file of one scene in the reference's step-result layout (xResultManager.cpp:828-925)."""
+        _lib.check(self.L.xdem_ctx_save_part(self._h, scene, os.fsencode(path), ct), "xdem_ctx_save_part")
+
+    def load_part(self, path: str, scene: int = 0, history: bool = False) -> float:
+        """Restores the state (and with history=True the contact histories) from a part file;
+        returns its time (xDynamicSimulator.cpp:392-492)."""
+        ct = C.c_double()
+        flags = _lib.PART_HISTORY if history else 0
+        _lib.check(self.L.xdem_ctx_load_part(self._h, scene, os.fsencode(path), flags, C.byref(ct)),
+                   "xdem_ctx_load_part")
+        return ct.value
 
     def error_word(self, clear: bool = True) -> int:
         w = C.c_uint()
@@ -112,6 +147,15 @@ class DemContext:
         a = np.ascontiguousarray(dbi, dtype=np.float64)
         _lib.check(self.L.xdem_ctx_set_body_pose(self._h, scene, wall, a.ctypes.data), "xdem_ctx_set_body_pose")
 
+    def set_mesh_pose(self, mesh: int, dbi) -> None:
+        a = np.ascontiguousarray(dbi, dtype=np.float64)
+        _lib.check(self.L.xdem_ctx_set_mesh_pose(self._h, mesh, a.ctypes.data), "xdem_ctx_set_mesh_pose")
+
+    def mesh_wrench(self, mesh: int = 0) -> np.ndarray:
+        out = np.zeros(6)
+        _lib.check(self.L.xdem_ctx_mesh_wrench(self._h, mesh, out.ctypes.data), "xdem_ctx_mesh_wrench")
+        return out
+
     def contact_count(self):
         a, b = C.c_ulonglong(), C.c_ulonglong()
         _lib.check(self.L.xdem_ctx_contact_count(self._h, C.byref(a), C.byref(b)), "xdem_ctx_contact_count")
@@ -126,10 +170,11 @@ class DemContext:
         _lib.check(self.L.xdem_ctx_set_timing(self._h, 1 if on else 0), "xdem_ctx_set_timing")
 
     def timing(self):
-        out = (C.c_double * 6)()
+        out = (C.c_double * 9)()
         _lib.check(self.L.xdem_ctx_timing(self._h, out), "xdem_ctx_timing")
         return {"contact_ms": out[0], "grid_ms": out[1], "steps": int(out[2]), "finish_ms": out[3],
-                "touch_ms": out[4], "lite_move_ms": out[5]}
+                "touch_ms": out[4], "lite_move_ms": out[5], "collide_ms": out[6], "mesh_pass_ms": out[7],
+                "pair_pass_ms": out[8]}
 
     def close(self) -> None:
         if self._h:
@@ -181,6 +226,11 @@ class xIntegratorVV:
 
     def num_particles(self) -> int:
         return sum(s.np for s in self._scenes)
+
+    def set_from_part_result(self, path: str) -> float:
+        """xDynamicsSimulator::set_from_part_result: the state from a part file, contact
+        histories kept as they are (the reference's contact-manager restart is empty)."""
+        return self.ctx.load_part(path)
 
     def set_dem_data(self, cpos, pos, vel, acc, ep, ev, ea) -> int:
         for name, a in (("pos", pos), ("vel", vel), ("acc", acc), ("ep", ep), ("ev", ev), ("ea", ea)):
