@@ -93,12 +93,13 @@ def dominant_roofline(kr, traffic=None):
             "bytes_per_unit": k["bytes_per_unit"], "units": k["units"]}
 
 
-def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0, skin=0.0, graph=0):
+def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0, skin=0.0, graph=0,
+            full_blocks=0):
     import torch
     from xdynamics_dll_amd import DemContext
 
     ctx = DemContext(scene, precision=precision, p2p_capacity=capacity, fuse_velocity=fuse,
-                     buckets_per_sphere=buckets_per_sphere, skin=skin)
+                     buckets_per_sphere=buckets_per_sphere, skin=skin, full_blocks=full_blocks)
     ctx.step(warmup)
     ctx.synchronize()
     torch.cuda.synchronize()
@@ -267,23 +268,93 @@ def copy_probe(mib):
                       "ms": dt * 1e3, "gbps": 2 * n * 4 / dt / 1e9}), flush=True)
 
 
+def batch_force_error(scenes, capacity, nsteps=40):
+    """Per-scene force error of the batch at one state: an FP64 context of all scenes is
+    advanced nsteps, its state copied into a fresh mixed context, a fresh FP64 context and one
+    oracle per scene, and each runs one contact pass. Per sphere the error is |F - F_oracle| /
+    (sum of the sphere's |F_pair| + |F_oracle|); per scene its maximum over the spheres with a
+    force. Pair sets are compared exactly."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import xdem_ref
+    from xdynamics_dll_amd import DemContext
+
+    model, usable, _ = host_cpu()
+    xdem_ref.set_threads(usable)
+    fields = ("pos", "vel", "acc", "ep", "ev", "ea")
+    a = DemContext(scenes, precision="fp64", p2p_capacity=capacity)
+    a.step(nsteps)
+    states = [{f: a.read(f, k) for f in fields} for k in range(len(scenes))]
+    a.close()
+    b = DemContext(scenes, precision="mixed", p2p_capacity=capacity)
+    c = DemContext(scenes, precision="fp64", p2p_capacity=capacity)
+    for k in range(len(scenes)):
+        for f in fields:
+            b.write(f, states[k][f], k)
+            c.write(f, states[k][f], k)
+    b.collide()
+    c.collide()
+    mixed_err, fp64_err, pairs_exact, contacts = [], [], True, 0
+    for k, sc in enumerate(scenes):
+        r = xdem_ref.RefSim(sc, capacity=capacity)
+        for f in fields:
+            r.write(f, states[k][f])
+        mag = np.zeros(sc.np)
+        r.collide(pair_magnitude=mag)
+        norm = mag + np.linalg.norm(r.force, axis=1)
+        has = norm > 0
+        for ctx, out in ((b, mixed_err), (c, fp64_err)):
+            f = ctx.read("force", k)
+            out.append(float((np.linalg.norm(f - r.force, axis=1)[has] / norm[has]).max()) if has.any() else 0.0)
+            cnt, ids, _ = ctx.history("p2p", k)
+            if not np.array_equal(cnt, r.pp_count):
+                pairs_exact = False
+            else:
+                for i in np.nonzero(cnt)[0]:
+                    if not np.array_equal(np.sort(ids[i, : cnt[i]]), np.sort(r.pp_id[i, : r.pp_count[i]])):
+                        pairs_exact = False
+                        break
+        contacts += int(r.pp_count.sum())
+    errs = [b.error_word(), c.error_word()]
+    b.close()
+    c.close()
+    me, fe = np.array(mixed_err), np.array(fp64_err)
+    return {"mixed_force_err_max": float(me.max()), "mixed_force_err_median": float(np.median(me)),
+            "worst_scene": int(me.argmax()), "fp64_force_err_max": float(fe.max()), "pairs_exact": pairs_exact,
+            "contacts_checked": contacts, "state_step": nsteps, "error_words": errs,
+            "mixed_force_err_per_scene": [float(x) for x in me]}
+
+
 def batch_main(args):
     """The 256-scene batched sweep (seed 1004): independent box-settling scenes of
     floor(exp(U[ln 2000, ln 64000])) spheres each, r ~ U[0.5, 2] mm, per-scene random
     friction, restitution and modulus, advanced together in one context (one launch per
-    kernel for all scenes). Mixed precision, fused velocity update."""
+    kernel for all scenes). Mixed precision, fused velocity update. The full-pass blocks and
+    the bucket budget come from the autotuner (xdem_autotune) unless given."""
+    import tempfile
+
     import torch
-    from xdynamics_dll_amd import SceneData
+    from xdynamics_dll_amd import SceneData, autotune
 
     t_build = time.perf_counter()
     scenes = [SceneData.build(4, 1004, variant=k) for k in range(args.nscene)]
     t_build = time.perf_counter() - t_build
     n = sum(s.np for s in scenes)
     fuse = not args.split
-    mixed = run_gpu(scenes, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin, args.graph)
+    tuned = None
+    if not args.full_blocks and not args.buckets_per_sphere:
+        t0 = time.perf_counter()
+        cache = os.path.join(tempfile.gettempdir(), "xdem_autotune_bench.txt")
+        tuned = autotune(scenes, precision="mixed", fuse_velocity=fuse, p2p_capacity=args.capacity, warmup=20,
+                         steps=50, cache_path=cache)
+        tuned["seconds"] = time.perf_counter() - t0
+        args.full_blocks, args.buckets_per_sphere = tuned["full_blocks"], tuned["buckets_per_sphere"]
+        print(f"[batch] autotuned: {tuned['full_blocks']} full-pass blocks, {tuned['buckets_per_sphere']} buckets "
+              f"per sphere ({tuned['ms_per_step']:.4f} ms/step, {len(tuned['trials'])} trials)", file=sys.stderr,
+              flush=True)
+    mixed = run_gpu(scenes, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin, args.graph, args.full_blocks)
     value = n * args.steps / mixed["seconds"]
     f64 = run_gpu(scenes, "fp64", args.steps, args.warmup, args.capacity, fuse,
-                  args.buckets_per_sphere, args.skin, args.graph)
+                  args.buckets_per_sphere, args.skin, args.graph, args.full_blocks)
     f64_value = n * args.steps / f64["seconds"]
     tm = mixed["timing"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
@@ -308,25 +379,35 @@ def batch_main(args):
         "contact_spheres": mixed["info"]["contact_spheres"],
         "error_word": mixed["err"], "scene_build_s": t_build, "fp64_value": f64_value,
         "mixed_over_fp64": value / f64_value,
+        "autotune": tuned, "full_blocks": args.full_blocks, "buckets_per_sphere": args.buckets_per_sphere,
     }
+    if not args.no_fp64:
+        out["force_check"] = batch_force_error(scenes, args.capacity)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import xdem_ref
 
-        threads = min(16, os.cpu_count() or 1)
-        xdem_ref.set_threads(threads)
-        done, t = 0, 0.0
-        for sc in scenes:   # whole scenes, 5 steps each, until about 10 s of CPU work
-            ref = xdem_ref.RefSim(sc, capacity=args.capacity)
-            t0 = time.perf_counter()
-            ref.step(5)
-            t += time.perf_counter() - t0
-            done += sc.np * 5
-            if t > 10.0:
-                break
-        out["cpu_baseline"] = {"value": done / t, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-                               "sample": f"the first scenes of the batch, 5 steps each ({done} particle-steps), "
-                                         "FP64 C restatement, OpenMP over spheres"}
+        model, usable, total = host_cpu()
+
+        def sweep(threads, budget):
+            xdem_ref.set_threads(threads)
+            done, t = 0, 0.0
+            for sc in scenes:   # whole scenes, 5 steps each, until the budget of CPU time
+                ref = xdem_ref.RefSim(sc, capacity=args.capacity)
+                t0 = time.perf_counter()
+                ref.step(5)
+                t += time.perf_counter() - t0
+                done += sc.np * 5
+                if t > budget:
+                    break
+            return done / t, done
+
+        v_all, d_all = sweep(usable, 10.0)
+        v_one, d_one = sweep(1, 5.0)
+        out["cpu_baseline"] = {"value": v_all, "unit": "particle-steps/s", "cores": usable, "kind": "port",
+                               "sample": f"the first scenes of the batch, 5 steps each ({d_all} particle-steps on "
+                                         f"{usable} threads, {d_one} on 1), FP64 C restatement, OpenMP over spheres",
+                               "value_1_thread": v_one, "cpu_model": model, "host_cpus": total}
     print(json.dumps(out), flush=True)
 
 
@@ -354,7 +435,7 @@ def wheel_main(args):
     park[1:4] = (xc, yc, 1.0)          # above the bin while the bed settles
     cp_wheel = contact_property(2e11, 5e7, 0.3, 0.25, 0.4, 0.6, 0.05)
     ctx = DemContext(scene, precision="mixed", p2p_capacity=args.capacity, fuse_velocity=True, step_graph=50,
-                     meshes=[(0, tris, park, cp_wheel)])
+                     meshes=[(0, tris, park, cp_wheel)], full_blocks=args.full_blocks)
     t_build = time.perf_counter() - t_build
     t0 = time.perf_counter()
     done = 0
@@ -448,7 +529,7 @@ def lattice_main(args):
     n = scene.np
     fuse = not args.split
     r = run_gpu(scene, "fp64", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin,
-                args.graph)
+                args.graph, args.full_blocks)
     value = n * args.steps / r["seconds"]
     tm = r["timing"]
     kr = kernel_rooflines(tm, n, r["info"], r["n_pp"], r["n_wall"], fuse)
@@ -497,6 +578,8 @@ def main():
                          "wheel: the wheel on a 500,000-sphere bed with HHT coupling; lattice: the 10,000-sphere "
                          "lattice box in FP64")
     ap.add_argument("--settle", type=int, default=20000, help="untimed settling steps of the wheel's bed")
+    ap.add_argument("--full-blocks", type=int, default=0,
+                    help="blocks of the full-pass kernels (0 = library default)")
     ap.add_argument("--nscene", type=int, default=256, help="scenes of the batch workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
@@ -528,7 +611,7 @@ def main():
     n = scene.np
 
     fuse = not args.split
-    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin, args.graph)
+    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin, args.graph, args.full_blocks)
     value = n * args.steps / mixed["seconds"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
@@ -569,7 +652,7 @@ def main():
     }
     if not args.no_fp64:
         f64 = run_gpu(scene, "fp64", args.steps, args.warmup, args.capacity,
-                      fuse, args.buckets_per_sphere, args.skin, args.graph)
+                      fuse, args.buckets_per_sphere, args.skin, args.graph, args.full_blocks)
         f64_value = n * args.steps / f64["seconds"]
         out["fp64_value"] = f64_value
         out["mixed_over_fp64"] = value / f64_value

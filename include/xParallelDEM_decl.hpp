@@ -21,6 +21,7 @@ typedef xdem_dem_parameters device_dem_parameters;       // decl:128-150
 typedef xdem_contact_property device_contact_property;   // decl:230-243
 typedef xdem_plane_info device_plane_info;               // decl:161-174
 typedef xdem_triangle_info device_triangle_info;         // decl:152-159
+typedef xdem_cylinder_info device_cylinder_info;         // decl:220-228
 
 // First non-zero status returned by a forwarded call since the last clear.
 inline int& xdem_shim_status()
@@ -96,6 +97,16 @@ inline void cu_plane_contact_force(device_plane_info* plan, double* dbi, device_
 {
     xdem_shim_latch(xdem_cu_plane_contact_force(plan, dbi, cp, pos, ep, vel, omega, force, moment, mass, tmax,
                                                 rres, pair_count, pair_id, tsd, np));
+}
+
+// decl:326-333
+inline void cu_cylinder_contact_force(device_cylinder_info* cyl, double* bi, device_contact_property* cp, double* pos,
+                                      double* ep, double* vel, double* omega, double* force, double* moment,
+                                      double* mass, double* tmax, double* rres, unsigned int* pair_count,
+                                      unsigned int* pair_id, double* tsd, unsigned int np)
+{
+    xdem_shim_latch(xdem_cu_cylinder_contact_force(cyl, bi, cp, pos, ep, vel, omega, force, moment, mass, tmax, rres,
+                                                   pair_count, pair_id, tsd, np));
 }
 
 // decl:317-324

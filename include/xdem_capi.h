@@ -87,6 +87,15 @@ int xdem_cu_plane_contact_force(xdem_plane_info* plan, double* dbi, xdem_contact
                                 double* ep, double* vel, double* omega, double* force, double* moment,
                                 double* mass, double* tmax, double* rres, unsigned int* pair_count,
                                 unsigned int* pair_id, double* tsd, unsigned int np);
+/* cu_cylinder_contact_force (decl:326-333); cyl, dbi (15 doubles) and cp are device
+ * pointers, 3 cylinder slots per particle (ids: the cylinder's id for its side wall, lid ring
+ * and rim, id + 1000 for a lid seen from inside); writes the registered body-force buffers.
+ * Slots past the third are not written (XDEM_ERR_CYL_OVERFLOW; the reference writes them into
+ * the next particle's). */
+int xdem_cu_cylinder_contact_force(xdem_cylinder_info* cyl, double* dbi, xdem_contact_property* cp, double* pos,
+                                   double* ep, double* vel, double* omega, double* force, double* moment,
+                                   double* mass, double* tmax, double* rres, unsigned int* pair_count,
+                                   unsigned int* pair_id, double* tsd, unsigned int np);
 /* cu_particle_polygonObject_collision (decl:317-324); 5 triangle slots per particle. */
 int xdem_cu_particle_polygonObject_collision(xdem_triangle_info* dpi, double* dbi, double* pos, double* ep,
                                              double* vel, double* omega, double* force, double* moment,
@@ -153,6 +162,11 @@ typedef struct xdem_ctx_config {
                                     about a second); the blocks that stay complete the build,
                                     so results do not change. Small values exist to test that
                                     path. */
+    unsigned int full_blocks;    /* blocks of the full-pass kernels (pair, triangle and contact
+                                    passes), rounded down to a multiple of 64 (0 = 1024). They
+                                    stride over the queued spheres, so results do not depend on
+                                    it; a few blocks per CU keep several waves per SIMD to hide
+                                    the neighbour gathers. */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */
@@ -299,6 +313,32 @@ int xdem_ctx_graph_launch(xdem_ctx* ctx, unsigned int nlaunch);
  * the second stream. */
 int xdem_ctx_set_timing(xdem_ctx* ctx, int on);
 int xdem_ctx_timing(xdem_ctx* ctx, double out[9]);
+
+/* ---------------------------------------------------------------- launch-shape tuner
+ * No counterpart in the reference (its launch shapes are constants, decl:17 and
+ * xDiscreteElementMethodSimulation.cpp:100). Times candidate launch shapes of the step layer
+ * on the caller's scenes: for each candidate a context is built from `base` with that shape,
+ * `warmup` steps run untimed and `steps` are timed; coordinate descent over the full-pass
+ * blocks (256, 512, 1024, 2048) at 4 buckets per sphere, then the bucket budget (2, 4, 8).
+ * Results do not depend on the shapes, so only speed is compared; a candidate whose run sets
+ * an error bit other than XDEM_ERR_GRID_SYNC is discarded. The winner is appended to
+ * `cache_path` (a text file, one line per scene signature; NULL: no cache) and read back from
+ * it on later calls with the same signature (no trials then, from_cache = 1). Use the result
+ * as xdem_ctx_config.full_blocks / buckets_per_sphere. Allocates and frees contexts: call it
+ * at set-up. */
+#define XDEM_AUTOTUNE_MAX_TRIALS 16
+typedef struct xdem_autotune_result {
+    unsigned int full_blocks, buckets_per_sphere;
+    double ms_per_step;             /* of the winner */
+    int from_cache;
+    unsigned int ntrial;
+    unsigned int trial_full_blocks[XDEM_AUTOTUNE_MAX_TRIALS];
+    unsigned int trial_buckets_per_sphere[XDEM_AUTOTUNE_MAX_TRIALS];
+    double trial_ms_per_step[XDEM_AUTOTUNE_MAX_TRIALS];
+    char key[256];                  /* the scene signature */
+} xdem_autotune_result;
+int xdem_autotune(const xdem_scene* scenes, unsigned int nscene, const xdem_ctx_config* base, unsigned int warmup,
+                  unsigned int steps, const char* cache_path, xdem_autotune_result* out);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,40 @@ void xdem_contact_property_make(double Ei, double Ej, double nui, double nuj, do
 void xdem_fit_sphere_to_triangle(const double P[3], const double Q[3], const double R[3],
                                  double ft, double out[4]);
 
+/* ---------------------------------------------------------------- reference generators
+ * The reference's particle generators (xParticleManager.cpp:646-880) with its data structs
+ * (include/xTypes.h:244-261), its layouts and its draws from frand() = rand() / RAND_MAX. rand()
+ * is the C library's of the reference's platform (Windows): x <- 214013 x + 2531011 (mod 2^32),
+ * result bits 16..30, RAND_MAX 32767, initial state 1; here it runs on the caller's state.
+ * With pos == NULL each returns the particle count in *np; otherwise *np is the capacity of pos
+ * ([4 * np] x, y, z, r) and receives the count. Sizes under two sphere diameters per counted
+ * axis (which wrap the reference's unsigned counts) are refused. */
+typedef struct xdem_cube_particle_data { double dx, dy, dz, lx, ly, lz, minr, maxr; } xdem_cube_particle_data;
+typedef struct xdem_plane_particle_data {
+    double dx, dy, dz, drx, dry, drz, lx, ly, lz, minr, maxr;
+} xdem_plane_particle_data;
+typedef struct xdem_circle_particle_data {
+    double diameter, sx, sy, sz, dx, dy, dz, minr, maxr;
+} xdem_circle_particle_data;
+unsigned int xdem_crt_rand(unsigned int* state);
+double xdem_crt_frand(unsigned int* state);
+/* CreateCubeParticle (xParticleManager.cpp:646-712): a lattice of (int)(d / 2 maxr) - 1 per axis,
+ * spacing 2 maxr + space, each coordinate jittered by maxr * space * frand() (z drawn first),
+ * then radii minr + (maxr - minr) frand() when they differ. */
+int xdem_create_cube_particles(const xdem_cube_particle_data* d, unsigned int* rand_state, double* pos,
+                               unsigned int* np);
+/* CreatePlaneParticle (xParticleManager.cpp:714-792): a layer of nx x nz spheres in the plane of
+ * pu, qu through (lx, ly, lz), repeated ceil(dy) times at the same place (dy counts layers).
+ * reference_quirks = 1 also reproduces its copy loop with equal radii: every sphere written to
+ * pos[0] (the last stays) and the others zero. */
+int xdem_create_plane_particles(const xdem_plane_particle_data* d, unsigned int* rand_state, int reference_quirks,
+                                double* pos, unsigned int* np);
+/* CreateCircleParticle (xParticleManager.cpp:799-880): rings i = 1..nr of radius i (2 maxr + space)
+ * about (sx, sy, sz) in the plane of pu, qu, floor(2 pi / dth) spheres each; radii drawn from the
+ * caller's state (the reference seeds with the clock). */
+int xdem_create_circle_particles(const xdem_circle_particle_data* d, unsigned int* rand_state, double* pos,
+                                 unsigned int* np);
+
 /* Deterministic 64-bit generator used by the scene builders (SplitMix64). */
 unsigned long long xdem_rng_next(unsigned long long* state);
 double xdem_rng_uniform(unsigned long long* state);

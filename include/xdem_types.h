@@ -9,6 +9,7 @@
  *   xdem_contact_property   <-> device_contact_property include/xdynamics_parallel/xParallelDEM_decl.cuh:230-243
  *   xdem_plane_info         <-> device_plane_info       include/xdynamics_parallel/xParallelDEM_decl.cuh:161-174
  *   xdem_triangle_info      <-> device_triangle_info    include/xdynamics_parallel/xParallelDEM_decl.cuh:152-159
+ *   xdem_cylinder_info      <-> device_cylinder_info    include/xdynamics_parallel/xParallelDEM_decl.cuh:220-228
  *   body pose double[15]    <-> MM..ED3 enum layout     include/xdynamics_parallel/xParallelDEM_decl.cuh:18-20
  *
  * Array arguments elsewhere in the ABI keep the reference's AoS layouts:
@@ -83,6 +84,19 @@ typedef struct xdem_triangle_info {
     xdem_double3 P, Q, R;
 } xdem_triangle_info;
 
+/* Cylinder body (reference: device_cylinder_info, decl:220-228): a tube of zero wall thickness
+ * about the body-frame segment pbase -> ptop, placed by the body pose. len_rr = (length, radius
+ * of the side wall, radius of the inner test and the rim), which xParticleCylinderContact::define
+ * fills from (cylinder_length, cylinder_bottom_radius, cylinder_top_radius). empty_part: 0 both
+ * lids closed, 1 top lid open, 2 bottom lid open (xCylinderObject::empty_part). */
+typedef struct xdem_cylinder_info {
+    unsigned int id;
+    unsigned int empty_part;
+    double thickness;
+    xdem_double3 len_rr;
+    xdem_double3 pbase, ptop;
+} xdem_cylinder_info;
+
 /* Indices into the 15-double body pose (reference enum at decl:20). */
 enum {
     XDEM_MM = 0, XDEM_PX, XDEM_PY, XDEM_PZ, XDEM_VX, XDEM_VY, XDEM_VZ,
@@ -103,6 +117,7 @@ typedef struct xdem_quirks {
 #define XDEM_MAX_P2P_COUNT 12
 #define XDEM_MAX_P2PL_COUNT 3
 #define XDEM_MAX_P2MS_COUNT 5
+#define XDEM_MAX_P2CY_COUNT 3
 
 /* Device error word bits, readable after a step (the reference only printf's). */
 #define XDEM_ERR_P2P_OVERFLOW   0x1u  /* more sphere contacts than history slots */
@@ -122,6 +137,8 @@ typedef struct xdem_quirks {
                                          workspace reserved for it; the launch did not run */
 #define XDEM_ERR_MESH_NEAR      0x200u /* (step layer) more triangles within reach of a sphere than its
                                          triangle list holds (the contacts beyond it are missed) */
+#define XDEM_ERR_CYL_OVERFLOW   0x400u /* more cylinder contacts than cylinder slots (the reference writes
+                                         past them) */
 
 /* Status codes returned by every C entry point. */
 #define XDEM_OK              0
@@ -142,6 +159,7 @@ static_assert(offsetof(xdem_dem_parameters, dt) == 64, "device_dem_parameters la
 static_assert(sizeof(xdem_contact_property) == 104, "device_contact_property layout");
 static_assert(sizeof(xdem_plane_info) == 240, "device_plane_info layout");
 static_assert(sizeof(xdem_triangle_info) == 80, "device_triangle_info layout");
+static_assert(sizeof(xdem_cylinder_info) == 88, "device_cylinder_info layout");
 #endif
 
 #endif /* XDEM_TYPES_H */

@@ -681,6 +681,226 @@ unsigned int ref_plane_contact_force(const ref_dem_parameters* p, const ref_quir
     return overflow;
 }
 
+/* ----------------------------------------------------------- cylinders (impl:1111-1460) */
+typedef struct { v3 pos, base, top, ab; v4 e; } cyl_axis;
+
+static cyl_axis cylinder_axis(const ref_cylinder_info* cy, const double* bi)
+{
+    cyl_axis a;
+    a.pos = mk3(bi[REF_PX], bi[REF_PY], bi[REF_PZ]);
+    v4 e = {bi[REF_E0], bi[REF_E1], bi[REF_E2], bi[REF_E3]};
+    a.e = e;
+    a.base = add3(a.pos, to_global(ld3(cy->pbase), e));   /* impl:1202 */
+    a.top = add3(a.pos, to_global(ld3(cy->ptop), e));     /* impl:1204 */
+    a.ab = sub3(a.top, a.base);
+    return a;
+}
+
+/* particle_cylinder_contact_detection, impl:1189-1288 */
+static double cylinder_detection(const ref_cylinder_info* cy, cyl_axis a, v3 p, double r, v3* u, int* inner)
+{
+    double t = dot3(sub3(p, a.base), a.ab) / dot3(a.ab, a.ab);
+    v3 cp = add3(a.base, scl3(t, a.ab));
+    double dist = len3(sub3(p, cp));
+    if (t >= 0 && t <= 1) {                                /* impl:1210-1229 */
+        if (dist == 0) { *inner = 1; return 0; }
+        *u = div3(sub3(cp, p), dist);
+        if (dist < cy->len_rr[2]) {
+            *inner = 1;
+            *u = neg3(*u);
+            return dist + r - cy->len_rr[1];
+        }
+        return cy->len_rr[1] + r - dist;
+    }
+    double thick = cy->thickness;                          /* impl:1233-1241 */
+    int one = thick ? 1 : 0;
+    if (dist < cy->len_rr[2] + thick && dist > one * cy->len_rr[2]) {
+        v3 oc = sub3(a.pos, cp);
+        double ocl = len3(oc);
+        *u = div3(oc, ocl);
+        return cy->len_rr[0] * 0.5 + r - ocl;
+    }
+    v3 at = to_local(sub3(p, a.top), a.e);                 /* impl:1245-1252 */
+    v3 rim = a.top;
+    if (fabs(at.y) > cy->len_rr[0]) {
+        at = to_local(sub3(p, a.base), a.e);
+        rim = a.base;
+    }
+    double rr = cy->len_rr[2];
+    double phi = atan(at.x / at.z);                        /* impl:1253-1270 */
+    double lx;
+    if (phi < 0 && at.z < 0) lx = rr * sin(-phi);
+    else if (phi > 0 && at.x < 0 && at.z < 0) lx = rr * sin(-phi);
+    else lx = rr * sin(phi);
+    double lz = rr * cos(phi);
+    if (at.z < 0 && lz > 0) lz = -lz;
+    else if (at.z > 0 && lz < 0) lz = -lz;
+    v3 c = add3(rim, to_global(mk3(lx, 0.0, lz), a.e));
+    v3 dv = sub3(c, p);
+    double d = len3(dv);
+    *u = div3(dv, d);
+    if (d < r) return r - d;
+    return -1.0;
+}
+
+/* particle_cylinder_inner_base_or_top_contact_detection, impl:1290-1325 */
+static double cylinder_inner_lid(const ref_cylinder_info* cy, cyl_axis a, v3 p, double r, v3* u, v3* cpt)
+{
+    double t = dot3(sub3(p, a.base), a.ab) / dot3(a.ab, a.ab);
+    v3 cp = add3(a.base, scl3(t, a.ab));
+    double dist = len3(sub3(p, cp));
+    if (cy->empty_part == 1 && t > 0.6) return 0.0;
+    if (cy->empty_part == 2 && t < 0.4) return 0.0;
+    double gab = -1.0;
+    if (dist < cy->len_rr[2]) {
+        v3 oc = sub3(a.pos, cp);
+        double ocl = len3(oc);
+        *u = div3(neg3(oc), ocl);
+        *cpt = add3(p, scl3(r, *u));
+        gab = r + ocl - cy->len_rr[0] * 0.5;
+    }
+    return gab;
+}
+
+double ref_cylinder_detection(const ref_cylinder_info* cy, const double* dbi, const double p[3], double r,
+                              double u_io[3], int* inner_out)
+{
+    v3 u = ld3(u_io);
+    int inner = 0;
+    double d = cylinder_detection(cy, cylinder_axis(cy, dbi), ld3(p), r, &u, &inner);
+    st3(u_io, u);
+    if (inner_out) *inner_out = inner;
+    return d;
+}
+
+double ref_cylinder_inner_lid(const ref_cylinder_info* cy, const double* dbi, const double p[3], double r,
+                              double u_io[3], double cpt_io[3])
+{
+    v3 u = ld3(u_io), c = ld3(cpt_io);
+    double d = cylinder_inner_lid(cy, cylinder_axis(cy, dbi), ld3(p), r, &u, &c);
+    st3(u_io, u);
+    st3(cpt_io, c);
+    return d;
+}
+
+#define REF_CYL_SLOTS 3
+
+/* one contact (particle_cylinder_contact_force, impl:1111-1187) or its cohesive attraction
+ * (impl:1419-1441, 1444-1458), stored in the slot after *n_new */
+static void cylinder_pair(const ref_dem_parameters* p, const ref_quirks* q, const ref_contact_property* cp,
+                          const double* bi, v3 ip3, double r, double m, v3 ivel, v3 iom, v3 cpt, v3 unit,
+                          double cdist, double coh_s, unsigned int pid, const unsigned int* p_id,
+                          const double* p_sd, unsigned int* pair_id, double* tsd, unsigned int sid,
+                          unsigned int* n_new, int* overflow, v3* sumF, v3* sumM, v3* tma, double* res, double* bf)
+{
+    v3 bpos = mk3(bi[REF_PX], bi[REF_PY], bi[REF_PZ]);
+    double sd0 = 0.0, sd1 = 0.0;
+    int stored = 0;
+    if (cdist > 0) {
+        v3 po2cp = sub3(cpt, bpos);
+        v4 be = {bi[REF_E0], bi[REF_E1], bi[REF_E2], bi[REF_E3]};
+        v4 bd = {bi[REF_ED0], bi[REF_ED1], bi[REF_ED2], bi[REF_ED3]};
+        v3 om = ang_vel(be, bd);
+        for (unsigned int s = 0; s < REF_CYL_SLOTS; s++)
+            if (p_id[s] == pid) { sd0 = p_sd[2 * s]; sd1 = p_sd[2 * s + 1]; break; }
+        v3 rc = sub3(cpt, ip3);
+        v3 dv = sub3(add3(mk3(bi[REF_VX], bi[REF_VY], bi[REF_VZ]), cross3(om, po2cp)), add3(ivel, cross3(iom, rc)));
+        fconst c = get_constant(p->contact_model, r, 0.0, m, 0.0, cp);
+        v3 Fn = mk3(0, 0, 0), Ft = mk3(0, 0, 0);
+        force_law(p->contact_model, q->gpu_force_law, p->dt, c, r, 0, cp, cdist, &sd0, &sd1, dv, unit, &Ft, &Fn);
+        v3 mf = add3(Fn, Ft);
+        *tma = add3(*tma, cross3(rc, mf));       /* calculate_previous_rolling_resistance, impl:476-483 */
+        *res += r * cp->rfric * len3(Fn);
+        *sumF = add3(*sumF, mf);
+        *sumM = add3(*sumM, cross3(rc, mf));
+        bf[0] += -mf.x; bf[1] += -mf.y; bf[2] += -mf.z;
+        v3 bm = neg3(cross3(po2cp, mf));
+        bf[3] += bm.x; bf[4] += bm.y; bf[5] += bm.z;
+        stored = 1;
+    } else if (cdist <= 0 && fabs(cdist) < fabs(coh_s)) {
+        double f = jkr_separation_force(r, 0, cp->coh);
+        double cf = ref_cohesion_force(r, 0, cp, f);
+        v3 mf = scl3(-cf, unit);
+        *sumF = add3(*sumF, mf);
+        bf[0] += -mf.x; bf[1] += -mf.y; bf[2] += -mf.z;
+        v3 bm = neg3(cross3(sub3(cpt, bpos), mf));
+        bf[3] += bm.x; bf[4] += bm.y; bf[5] += bm.z;
+        stored = 1;
+    }
+    if (!stored) return;
+    if (*n_new < REF_CYL_SLOTS) {
+        tsd[2 * (sid + *n_new)] = sd0;
+        tsd[2 * (sid + *n_new) + 1] = sd1;
+        pair_id[sid + *n_new] = pid;
+        (*n_new)++;
+    } else {
+        *overflow = 1;
+    }
+}
+
+unsigned int ref_cylinder_contact_force(const ref_dem_parameters* p, const ref_quirks* q,
+                                        const ref_cylinder_info* cy, const double* dbi,
+                                        const ref_contact_property* cp,
+                                        double* fx, double* fy, double* fz,
+                                        double* mx, double* my, double* mz,
+                                        const double* pos, const double* ep, const double* vel, const double* ev,
+                                        double* force, double* moment, const double* mass,
+                                        double* tmax, double* rres,
+                                        unsigned int* pair_count, unsigned int* pair_id, double* tsd,
+                                        unsigned int np)
+{
+    unsigned int overflow = 0;
+    cyl_axis a = cylinder_axis(cy, dbi);
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+:overflow)
+    for (long ii = 0; ii < (long)np; ii++) {
+        unsigned int id = (unsigned int)ii;
+        unsigned int sid = id * REF_CYL_SLOTS;
+        unsigned int p_id[REF_CYL_SLOTS];
+        double p_sd[2 * REF_CYL_SLOTS];
+        for (unsigned int s = 0; s < REF_CYL_SLOTS; s++) {       /* impl:1337-1343 */
+            p_id[s] = pair_id[sid + s];
+            p_sd[2 * s] = tsd[2 * (sid + s)];
+            p_sd[2 * s + 1] = tsd[2 * (sid + s) + 1];
+        }
+        double bf[6] = {0, 0, 0, 0, 0, 0};
+        unsigned int old_count = pair_count[id];
+        unsigned int k = cy->id;
+        double m = mass[id];
+        v4 ipos = ld4(pos + 4 * id);
+        v3 ip3 = mk3(ipos.x, ipos.y, ipos.z);
+        double r = ipos.w;
+        v3 ivel = ld3(vel + 3 * id);
+        v3 iom = ang_vel(ld4(ep + 4 * id), ld4(ev + 4 * id));
+        v3 sumF = mk3(0, 0, 0), sumM = mk3(0, 0, 0), tma = mk3(0, 0, 0);
+        double res = 0.0;
+        unsigned int n_new = k ? old_count : 0;               /* impl:1361: a later cylinder appends */
+        if (n_new > REF_CYL_SLOTS) n_new = REF_CYL_SLOTS;
+        int inner = 0, ovf = 0;
+        v3 unit = mk3(0, 0, 0);
+        double cdist = cylinder_detection(cy, a, ip3, r, &unit, &inner);
+        double coh_s = 0;
+        if (cp->coh) coh_s = ref_cohesion_limit_depth(r, 0, cp);
+        v3 cpt = add3(ip3, scl3(r, unit));
+        cylinder_pair(p, q, cp, dbi, ip3, r, m, ivel, iom, cpt, unit, cdist, coh_s, k, p_id, p_sd, pair_id, tsd, sid,
+                      &n_new, &ovf, &sumF, &sumM, &tma, &res, bf);
+        if (inner) {                                          /* impl:1418-1441 */
+            cdist = cylinder_inner_lid(cy, a, ip3, r, &unit, &cpt);
+            cylinder_pair(p, q, cp, dbi, ip3, r, m, ivel, iom, cpt, unit, cdist, coh_s, k + 1000u, p_id, p_sd, pair_id,
+                          tsd, sid, &n_new, &ovf, &sumF, &sumM, &tma, &res, bf);
+        }
+        overflow += (unsigned int)ovf;
+        if (fx) { fx[id] = bf[0]; fy[id] = bf[1]; fz[id] = bf[2]; mx[id] = bf[3]; my[id] = bf[4]; mz[id] = bf[5]; }
+        double* F = force + 3 * id;
+        double* M = moment + 3 * id;
+        F[0] += sumF.x; F[1] += sumF.y; F[2] += sumF.z;
+        M[0] += sumM.x; M[1] += sumM.y; M[2] += sumM.z;
+        pair_count[id] = n_new;
+        tmax[3 * id + 0] += tma.x; tmax[3 * id + 1] += tma.y; tmax[3 * id + 2] += tma.z;
+        rres[id] += res;
+    }
+    return overflow;
+}
+
 /* ---------------------------------------------------------- triangles (impl:1655-1746) */
 static int closest_pt_triangle(const ref_triangle_info* t, v3 p, v3* out)
 {

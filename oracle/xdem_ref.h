@@ -63,6 +63,14 @@ typedef struct ref_triangle_info {
 } ref_triangle_info;
 
 /* Quirk switches (the survey's quirk register; defaults give the parity profile). */
+/* device_cylinder_info (decl:220-228): len_rr = (length, side-wall radius, inner/rim radius). */
+typedef struct ref_cylinder_info {
+    unsigned int id, empty_part;
+    double thickness;
+    double len_rr[3];
+    double pbase[3], ptop[3];
+} ref_cylinder_info;
+
 typedef struct ref_quirks {
     int gpu_force_law;        /* Q2/Q3: 0 = CPU formula (xContact.cpp:238-264), 1 = kernel text (impl:432-452) */
     int velocity_not_updated; /* Q1: 1 = kernel text stores the old velocity (impl:207-221); 0 = CPU Verlet */
@@ -170,6 +178,24 @@ double ref_reduction(const double* in, unsigned int n);
 /* Single-function restatements exposed for unit tests. */
 int ref_closest_pt_point_triangle(const ref_triangle_info* t, const double p[3], double out[3]);
 double ref_plane_detection(const ref_plane_info* pe, const double xp[3], double r, double u_out[3]);
+/* Cylinder regions (impl:1189-1325): the outer/inner wall, lid ring and rim test, and the lid
+ * seen from inside; inner_out receives the inner flag, cpt_io the contact point (in/out). */
+double ref_cylinder_detection(const ref_cylinder_info* cy, const double* dbi, const double p[3], double r,
+                              double u_io[3], int* inner_out);
+double ref_cylinder_inner_lid(const ref_cylinder_info* cy, const double* dbi, const double p[3], double r,
+                              double u_io[3], double cpt_io[3]);
+/* cylinder_contact_force_kernel (impl:1327-1460) over np spheres, 3 slots per sphere;
+ * returns the number of spheres with more entries than slots (entries past them dropped). */
+unsigned int ref_cylinder_contact_force(const ref_dem_parameters* p, const ref_quirks* q,
+                                        const ref_cylinder_info* cy, const double* dbi,
+                                        const ref_contact_property* cp,
+                                        double* fx, double* fy, double* fz,
+                                        double* mx, double* my, double* mz,
+                                        const double* pos, const double* ep, const double* vel, const double* ev,
+                                        double* force, double* moment, const double* mass,
+                                        double* tmax, double* rres,
+                                        unsigned int* pair_count, unsigned int* pair_id, double* tsd,
+                                        unsigned int np);
 void ref_calculate_uceom(const double J[3], const double ep[4], const double ev[4],
                          const double n_prime[3], double out[4]);
 void ref_to_angular_velocity(const double e[4], const double d[4], double out[3]);

@@ -61,6 +61,9 @@ def lib() -> C.CDLL:
             "ref_plane_local_points": ([P, P, P], None),
             "ref_plane_update": ([P, P, U, P], None),
             "ref_mesh_define": ([P, P, U, P, P], None),
+            "ref_cylinder_detection": ([P, P, P, D, P, P], D),
+            "ref_cylinder_inner_lid": ([P, P, P, D, P, P], D),
+            "ref_cylinder_contact_force": ([P] * 23 + [U], U),
         }
         for name, (a, r) in sig.items():
             fn = getattr(L, name)

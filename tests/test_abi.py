@@ -55,6 +55,8 @@ def test_struct_layouts_match_reference_sizes():
     assert ctypes.sizeof(xd._lib.ContactProperty) == 104
     assert ctypes.sizeof(xd._lib.PlaneInfo) == 240
     assert ctypes.sizeof(xd._lib.TriangleInfo) == 80
+    assert ctypes.sizeof(xd._lib.CylinderInfo) == 88       # device_cylinder_info
+    assert xd._lib.CylinderInfo.len_rr.offset == 16 and xd._lib.CylinderInfo.ptop.offset == 64
     assert xd._lib.DemParameters.dt.offset == 64
     assert xd._lib.DemParameters.grid_size.offset == 52
     assert xd._lib.PlaneInfo.l1.offset == 8

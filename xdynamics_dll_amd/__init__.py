@@ -7,8 +7,10 @@ the library is missing.
 """
 from . import _lib
 from ._lib import XdemError
-from .scene import PARITY_QUIRKS, SceneData, box_walls, contact_property, make_plane, wheel_mesh
-from .simulation import DemContext, xIntegratorVV
+from .scene import (PARITY_QUIRKS, SceneData, box_walls, contact_property, generate_circle, generate_cube,
+                    generate_plane, make_cylinder, make_plane, wheel_mesh)
+from .simulation import DemContext, autotune, xIntegratorVV
 
-__all__ = ["_lib", "XdemError", "PARITY_QUIRKS", "SceneData", "box_walls", "contact_property", "make_plane",
-           "wheel_mesh", "DemContext", "xIntegratorVV"]
+__all__ = ["_lib", "XdemError", "PARITY_QUIRKS", "SceneData", "box_walls", "contact_property", "generate_circle",
+           "generate_cube", "generate_plane", "make_cylinder", "make_plane", "wheel_mesh", "DemContext", "autotune",
+           "xIntegratorVV"]

@@ -30,6 +30,7 @@ ERR_GRID_SYNC = 0x40
 ERR_BUILD_INCOMPLETE = 0x80
 ERR_WORKSPACE = 0x100
 ERR_MESH_NEAR = 0x200
+ERR_CYL_OVERFLOW = 0x400
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -93,6 +94,24 @@ class TriangleInfo(C.Structure):
     _fields_ = [("id", C.c_int), ("tid", C.c_uint), ("P", double3), ("Q", double3), ("R", double3)]
 
 
+class CylinderInfo(C.Structure):
+    """xdem_cylinder_info (device_cylinder_info, decl:220-228)."""
+    _fields_ = [("id", C.c_uint), ("empty_part", C.c_uint), ("thickness", C.c_double), ("len_rr", double3),
+                ("pbase", double3), ("ptop", double3)]
+
+
+class CubeParticleData(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("dx", "dy", "dz", "lx", "ly", "lz", "minr", "maxr")]
+
+
+class PlaneParticleData(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("dx", "dy", "dz", "drx", "dry", "drz", "lx", "ly", "lz", "minr", "maxr")]
+
+
+class CircleParticleData(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("diameter", "sx", "sy", "sz", "dx", "dy", "dz", "minr", "maxr")]
+
+
 class Quirks(C.Structure):
     _fields_ = [("gpu_force_law", C.c_int), ("velocity_not_updated", C.c_int), ("stale_ft", C.c_int),
                 ("flat_inertia", C.c_int)]
@@ -125,7 +144,7 @@ class CtxConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("p2p_capacity", C.c_uint), ("quirks", Quirks),
                 ("block_size", C.c_uint), ("grid_dim_max", C.c_uint), ("fuse_velocity", C.c_int),
                 ("device", C.c_int), ("buckets_per_sphere", C.c_uint), ("skin", C.c_double),
-                ("step_graph", C.c_int), ("rebuild_wait_limit", C.c_uint)]
+                ("step_graph", C.c_int), ("rebuild_wait_limit", C.c_uint), ("full_blocks", C.c_uint)]
 
 
 class CtxInfo(C.Structure):
@@ -141,10 +160,18 @@ class CtxSceneInfo(C.Structure):
                 ("first_particle", C.c_uint)]
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [("full_blocks", C.c_uint), ("buckets_per_sphere", C.c_uint), ("ms_per_step", C.c_double),
+                ("from_cache", C.c_int), ("ntrial", C.c_uint), ("trial_full_blocks", C.c_uint * 16),
+                ("trial_buckets_per_sphere", C.c_uint * 16), ("trial_ms_per_step", C.c_double * 16),
+                ("key", C.c_char * 256)]
+
+
 assert C.sizeof(DemParameters) == 144
 assert C.sizeof(ContactProperty) == 104
 assert C.sizeof(PlaneInfo) == 240
 assert C.sizeof(TriangleInfo) == 80
+assert C.sizeof(CylinderInfo) == 88
 
 P = C.c_void_p
 U = C.c_uint
@@ -171,6 +198,7 @@ LAUNCHER_SIGNATURES = {
     "xdem_cu_calculate_p2p": [P] * 16 + [U],
     "xdem_cu_plane_contact_force": [P] * 15 + [U],
     "xdem_cu_particle_polygonObject_collision": [P] * 19 + [U, U],
+    "xdem_cu_cylinder_contact_force": [P] * 15 + [U],
     "xdem_cu_decide_rolling_friction_moment": [P] * 6 + [U],
     "xdem_reduction": [P, U, C.POINTER(D)],
     "xdem_cu_update_meshObjectData": [P] * 5 + [U],
@@ -193,6 +221,7 @@ CTX_SIGNATURES = {
     "xdem_ctx_scene_info_get": [P, U, C.POINTER(CtxSceneInfo)],
     "xdem_ctx_save_part": [P, U, C.c_char_p, D],
     "xdem_ctx_load_part": [P, U, C.c_char_p, I, C.POINTER(D)],
+    "xdem_autotune": [C.POINTER(Scene), U, C.POINTER(CtxConfig), U, U, C.c_char_p, C.POINTER(AutotuneResult)],
     "xdem_ctx_error_word": [P, C.POINTER(U), I],
     "xdem_ctx_body_wrench": [P, U, P],
     "xdem_ctx_set_body_pose": [P, U, U, P],
@@ -224,6 +253,11 @@ HOST_SIGNATURES = {
     "xdem_rng_next": ([C.POINTER(C.c_ulonglong)], C.c_ulonglong),
     "xdem_rng_uniform": ([C.POINTER(C.c_ulonglong)], C.c_double),
     "xdem_wheel_mesh": ([D, D, U, U, U, D, D, U, P, C.POINTER(U)], C.c_int),
+    "xdem_crt_rand": ([C.POINTER(U)], U),
+    "xdem_crt_frand": ([C.POINTER(U)], D),
+    "xdem_create_cube_particles": ([C.POINTER(CubeParticleData), C.POINTER(U), P, C.POINTER(U)], C.c_int),
+    "xdem_create_plane_particles": ([C.POINTER(PlaneParticleData), C.POINTER(U), I, P, C.POINTER(U)], C.c_int),
+    "xdem_create_circle_particles": ([C.POINTER(CircleParticleData), C.POINTER(U), P, C.POINTER(U)], C.c_int),
     "xdem_hht_wheel_create": ([C.POINTER(HhtWheelConfig), C.POINTER(P)], C.c_int),
     "xdem_hht_wheel_destroy": ([P], C.c_int),
     "xdem_hht_wheel_step": ([P, P, P, C.POINTER(I)], C.c_int),

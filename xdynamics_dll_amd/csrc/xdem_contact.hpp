@@ -353,6 +353,206 @@ __device__ __forceinline__ bool triangle_contact(const ContactConst& cc, const d
     return true;
 }
 
+// ---------------------------------------------------------------------- cylinders
+// Ends of the cylinder's axis at the body pose: pos + A(e) pbase and pos + A(e) ptop.
+struct CylAxis {
+    V3<double> pos, base, top, ab;
+    V4<double> e;
+};
+
+__device__ __forceinline__ CylAxis cylinder_axis(const xdem_cylinder_info& cy, const double* bi)
+{
+    CylAxis a;
+    a.pos = mk3(bi[XDEM_PX], bi[XDEM_PY], bi[XDEM_PZ]);
+    a.e = {bi[XDEM_E0], bi[XDEM_E1], bi[XDEM_E2], bi[XDEM_E3]};
+    a.base = a.pos + rot_global(mk3(cy.pbase.x, cy.pbase.y, cy.pbase.z), a.e);
+    a.top = a.pos + rot_global(mk3(cy.ptop.x, cy.ptop.y, cy.ptop.z), a.e);
+    a.ab = a.top - a.base;
+    return a;
+}
+
+// Overlap of a sphere (centre p, radius r) with the cylinder (impl:1189-1288); u is the unit
+// vector along which the force acts, inner is set for a sphere inside the tube (or on its
+// axis). The regions, in the reference's order:
+//  - beside the tube (axial parameter t in [0, 1]): the side wall at len_rr.y, from outside
+//    (dist > len_rr.z) or from inside (u reversed); a sphere on the axis returns 0;
+//  - beyond an end, within len_rr.z (+ thickness; > len_rr.z when there is a thickness) of
+//    the axis: the lid plane, at half the length from the body position, u from the axis
+//    point towards the body position;
+//  - otherwise the rim circle of radius len_rr.z at the nearer end (the top unless the local
+//    y offset from it exceeds the length), its point in the local xz plane found by the
+//    atan / sin / cos construction of the kernel text: overlap r - dist when positive, -1
+//    otherwise.
+__device__ __forceinline__ double cylinder_detection(const xdem_cylinder_info& cy, const CylAxis& a, V3<double> p,
+                                                     double r, V3<double>& u, bool& inner)
+{
+    const double t = dot(p - a.base, a.ab) / dot(a.ab, a.ab);
+    const V3<double> cp = a.base + t * a.ab;
+    const double dist = length(p - cp);
+    if (t >= 0 && t <= 1) {
+        if (dist == 0) {
+            inner = true;
+            return 0;
+        }
+        u = (cp - p) / dist;
+        if (dist < cy.len_rr.z) {
+            inner = true;
+            u = -u;
+            return dist + r - cy.len_rr.y;
+        }
+        return cy.len_rr.y + r - dist;
+    }
+    const double thick = cy.thickness;
+    const double one = thick ? 1.0 : 0.0;
+    if (dist < cy.len_rr.z + thick && dist > one * cy.len_rr.z) {
+        const V3<double> oc = a.pos - cp;
+        const double ocl = length(oc);
+        u = oc / ocl;
+        return cy.len_rr.x * 0.5 + r - ocl;
+    }
+    V3<double> at = rot_local(p - a.top, a.e);
+    V3<double> rim = a.top;
+    if (fabs(at.y) > cy.len_rr.x) {
+        at = rot_local(p - a.base, a.e);
+        rim = a.base;
+    }
+    const double rr = cy.len_rr.z;
+    const double phi = atan(at.x / at.z);
+    double lx;
+    if (phi < 0 && at.z < 0) lx = rr * sin(-phi);
+    else if (phi > 0 && at.x < 0 && at.z < 0) lx = rr * sin(-phi);
+    else lx = rr * sin(phi);
+    double lz = rr * cos(phi);
+    if (at.z < 0 && lz > 0) lz = -lz;
+    else if (at.z > 0 && lz < 0) lz = -lz;
+    const V3<double> c = rim + rot_global(mk3(lx, 0.0, lz), a.e);
+    const V3<double> dv = c - p;
+    const double d = length(dv);
+    u = dv / d;
+    if (d < r) return r - d;
+    return -1.0;
+}
+
+// The closed lids seen from inside the tube (impl:1290-1325): an open lid (empty_part 1: top
+// for t > 0.6, 2: bottom for t < 0.4) gives 0 with u and cpt untouched; within len_rr.z of
+// the axis, u points from the body position along the axis towards the sphere, the contact
+// point is p + r u and the overlap r + |axial offset| - length / 2; -1 otherwise.
+__device__ __forceinline__ double cylinder_inner_lid(const xdem_cylinder_info& cy, const CylAxis& a, V3<double> p,
+                                                     double r, V3<double>& u, V3<double>& cpt)
+{
+    const double t = dot(p - a.base, a.ab) / dot(a.ab, a.ab);
+    const V3<double> cp = a.base + t * a.ab;
+    const double dist = length(p - cp);
+    if (cy.empty_part == 1 && t > 0.6) return 0.0;
+    if (cy.empty_part == 2 && t < 0.4) return 0.0;
+    double gab = -1.0;
+    if (dist < cy.len_rr.z) {
+        const V3<double> oc = a.pos - cp;
+        const double ocl = length(oc);
+        u = (-oc) / ocl;
+        cpt = p + r * u;
+        gab = r + ocl - cy.len_rr.x * 0.5;
+    }
+    return gab;
+}
+
+// A sphere's cylinder history slots: the previous step's entries (looked up over all slots,
+// as the kernel does) and the entries of this step appended after `base`.
+struct CylSlots {
+    uint32_t old_id[XDEM_MAX_P2CY_COUNT];
+    double old_s0[XDEM_MAX_P2CY_COUNT], old_s1[XDEM_MAX_P2CY_COUNT];
+    uint32_t id[XDEM_MAX_P2CY_COUNT];
+    double s0[XDEM_MAX_P2CY_COUNT], s1[XDEM_MAX_P2CY_COUNT];
+    uint32_t count;      // entries valid after the call
+    bool overflow;
+};
+
+// One contact of the sphere with the cylinder body, or its cohesive attraction: force,
+// moment, rolling sums, body wrench and a history entry with id pid (impl:1111-1187 and the
+// cohesion branches of impl:1391-1460).
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ void cylinder_pair(const ContactConst& cc, const double* bi, double dt, V3<double> ip3,
+                                              double r, double m, V3<double> ivel, V3<double> iom, V3<double> cpt,
+                                              V3<double> unit, double cdist, double coh_s, uint32_t pid,
+                                              CylSlots& cs, Accum& acc, double bf[6])
+{
+    const V3<double> bpos = mk3(bi[XDEM_PX], bi[XDEM_PY], bi[XDEM_PZ]);
+    double sd0 = 0.0, sd1 = 0.0;
+    bool stored = false;
+    if (cdist > 0) {
+        const V3<double> po2cp = cpt - bpos;
+        V4<double> be = {bi[XDEM_E0], bi[XDEM_E1], bi[XDEM_E2], bi[XDEM_E3]};
+        V4<double> bd = {bi[XDEM_ED0], bi[XDEM_ED1], bi[XDEM_ED2], bi[XDEM_ED3]};
+        const V3<double> om = omega_of(be, bd);
+        for (uint32_t s = 0; s < XDEM_MAX_P2CY_COUNT; s++)
+            if (cs.old_id[s] == pid) { sd0 = cs.old_s0[s]; sd1 = cs.old_s1[s]; break; }
+        const V3<double> rc = cpt - ip3;
+        const V3<double> dv = mk3(bi[XDEM_VX], bi[XDEM_VY], bi[XDEM_VZ]) + cross(om, po2cp) - (ivel + cross(iom, rc));
+        const PairConst<T> c = pair_const<T, MODEL>(cc, r, 0.0, m, 0.0);
+        V3<T> Fn, Ft = mk3(T(0), T(0), T(0));
+        force_law<T, MODEL, GPU_LAW>(cc, c, cdist, dt, sd0, sd1, dv, unit, Ft, Fn);
+        const V3<double> mf = cvt3<double>(Fn + Ft);
+        acc.T = acc.T + cross(rc, mf);
+        acc.R += (double)(c.eq_r * T(cc.rfric) * length(Fn));
+        acc.F = acc.F + mf;
+        acc.M = acc.M + cross(rc, mf);
+        bf[0] += -mf.x; bf[1] += -mf.y; bf[2] += -mf.z;
+        const V3<double> bm = -cross(po2cp, mf);
+        bf[3] += bm.x; bf[4] += bm.y; bf[5] += bm.z;
+        stored = true;
+    } else if (cdist <= 0 && fabs(cdist) < fabs(coh_s)) {
+        const double f = cc.jk * r;
+        const double cf = cohesion_force<double>(cc, r, f);
+        const V3<double> mf = (-cf) * unit;
+        acc.F = acc.F + mf;
+        bf[0] += -mf.x; bf[1] += -mf.y; bf[2] += -mf.z;
+        const V3<double> bm = -cross(cpt - bpos, mf);
+        bf[3] += bm.x; bf[4] += bm.y; bf[5] += bm.z;
+        stored = true;
+    }
+    if (!stored) return;
+    if (cs.count < XDEM_MAX_P2CY_COUNT) {
+        for (uint32_t s = 0; s < XDEM_MAX_P2CY_COUNT; s++)
+            if (s == cs.count) { cs.id[s] = pid; cs.s0[s] = sd0; cs.s1[s] = sd1; }
+        cs.count++;
+    } else {
+        cs.overflow = true;
+    }
+}
+
+// The sphere against one cylinder body (cylinder_contact_force_kernel, impl:1327-1460): the
+// first cylinder (id 0) starts the slots afresh, a later one appends after the entries
+// already there. cs.id/s0/s1 must hold the previous entries (copied from cs.old_*).
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ void cylinder_contact(const ContactConst& cc, const xdem_cylinder_info& cy, const double* bi,
+                                                 double dt, V4<double> ipos, V3<double> ivel, V3<double> iom, double m,
+                                                 CylSlots& cs, Accum& acc, double bf[6])
+{
+    const V3<double> ip3 = mk3(ipos.x, ipos.y, ipos.z);
+    const double r = ipos.w;
+    const uint32_t k = cy.id;
+    if (!k) cs.count = 0;
+    else if (cs.count > XDEM_MAX_P2CY_COUNT) cs.count = XDEM_MAX_P2CY_COUNT;
+    const CylAxis a = cylinder_axis(cy, bi);
+    bool inner = false;
+    V3<double> unit = mk3(0.0, 0.0, 0.0);
+    double cdist = cylinder_detection(cy, a, ip3, r, unit, inner);
+    double coh_s = 0;
+    if (cc.coh) coh_s = cohesion_limit_depth(cc, r);
+    V3<double> cpt = ip3 + r * unit;
+    Accum part = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
+    cylinder_pair<T, MODEL, GPU_LAW>(cc, bi, dt, ip3, r, m, ivel, iom, cpt, unit, cdist, coh_s, k, cs, part, bf);
+    if (inner) {
+        cdist = cylinder_inner_lid(cy, a, ip3, r, unit, cpt);
+        cylinder_pair<T, MODEL, GPU_LAW>(cc, bi, dt, ip3, r, m, ivel, iom, cpt, unit, cdist, coh_s, k + 1000u, cs,
+                                         part, bf);
+    }
+    acc.F = acc.F + part.F;
+    acc.M = acc.M + part.M;
+    acc.T = acc.T + part.T;
+    acc.R += part.R;
+}
+
 // ---------------------------------------------------------------- rolling resistance
 // impl:2420-2454 on the accumulated sums; returns true when tmax/rres are consumed
 // (reset), false when rres == 0 and they are left as they are.

@@ -302,6 +302,7 @@ struct xdem_ctx {
         A.stale_ft = cfg.quirks.stale_ft; A.flat_inertia = cfg.quirks.flat_inertia;
         A.keep_velocity = cfg.quirks.velocity_not_updated; A.tmax_persist = tmax_persist;
         A.zero_force = 1;
+        A.full_blocks = cfg.full_blocks ? std::max(cfg.full_blocks, kQueueSegs) : 1024u;
         return A;
     }
 

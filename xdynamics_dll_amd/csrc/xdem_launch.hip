@@ -332,6 +332,57 @@ __global__ __launch_bounds__(kThreads) void k_plane(LaunchPrm L, const xdem_plan
     rres[id] += acc.R;
 }
 
+// ---------------------------------------------------------------- cylinders (impl:1327-1460)
+template <int MODEL, bool GPU_LAW>
+__global__ __launch_bounds__(kThreads) void k_cylinder(LaunchPrm L, const xdem_cylinder_info* cyl, const double* dbi,
+                                                       const xdem_contact_property* cp, double* fx, double* fy,
+                                                       double* fz, double* mx, double* my, double* mz,
+                                                       const double* pos, const double* ep, const double* vel,
+                                                       const double* ev, double* force, double* moment,
+                                                       const double* mass, double* tmax, double* rres,
+                                                       unsigned* pair_count, unsigned* pair_id, double* tsd,
+                                                       unsigned np, unsigned* err)
+{
+    constexpr unsigned CAP = XDEM_MAX_P2CY_COUNT;
+    const unsigned id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= np) return;
+    const ContactConst cc = make_contact_const(*cp, 0.0);
+    const xdem_cylinder_info cy = *cyl;
+    double b[15];
+#pragma unroll
+    for (int q = 0; q < 15; q++) b[q] = dbi[q];
+    const unsigned sid = id * CAP;
+    CylSlots cs;
+#pragma unroll
+    for (unsigned s = 0; s < CAP; s++) {
+        cs.old_id[s] = cs.id[s] = pair_id[sid + s];
+        cs.old_s0[s] = cs.s0[s] = tsd[2 * (sid + s)];
+        cs.old_s1[s] = cs.s1[s] = tsd[2 * (sid + s) + 1];
+    }
+    cs.count = pair_count[id];
+    cs.overflow = false;
+    const V4<double> ipos = ld4(pos + 4 * id);
+    const V3<double> ivel = ld3(vel + 3 * id);
+    const V3<double> iom = omega_of(ld4(ep + 4 * id), ld4(ev + 4 * id));
+    Accum acc = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
+    double bf[6] = {0, 0, 0, 0, 0, 0};
+    cylinder_contact<double, MODEL, GPU_LAW>(cc, cy, b, L.dt, ipos, ivel, iom, mass[id], cs, acc, bf);
+    if (cs.overflow) atomicOr(err, XDEM_ERR_CYL_OVERFLOW);
+#pragma unroll
+    for (unsigned s = 0; s < CAP; s++) {
+        pair_id[sid + s] = cs.id[s];
+        tsd[2 * (sid + s)] = cs.s0[s];
+        tsd[2 * (sid + s) + 1] = cs.s1[s];
+    }
+    pair_count[id] = cs.count;
+    fx[id] = bf[0]; fy[id] = bf[1]; fz[id] = bf[2];
+    mx[id] = bf[3]; my[id] = bf[4]; mz[id] = bf[5];
+    force[3 * id + 0] += acc.F.x; force[3 * id + 1] += acc.F.y; force[3 * id + 2] += acc.F.z;
+    moment[3 * id + 0] += acc.M.x; moment[3 * id + 1] += acc.M.y; moment[3 * id + 2] += acc.M.z;
+    tmax[3 * id + 0] += acc.T.x; tmax[3 * id + 1] += acc.T.y; tmax[3 * id + 2] += acc.T.z;
+    rres[id] += acc.R;
+}
+
 // -------------------------------------------------------------- triangles (impl:1930-2109)
 constexpr unsigned kTriList = 20;
 
@@ -766,6 +817,36 @@ int xdem_cu_plane_contact_force(xdem_plane_info* plan, double* dbi, xdem_contact
     if (L.model == 1) { if (gl) XDEM_PL(1, true); else XDEM_PL(1, false); }
     else { if (gl) XDEM_PL(0, true); else XDEM_PL(0, false); }
 #undef XDEM_PL
+    XDEM_LAUNCH_CHECK();
+    return XDEM_OK;
+}
+
+int xdem_cu_cylinder_contact_force(xdem_cylinder_info* cyl, double* dbi, xdem_contact_property* cp, double* pos,
+                                   double* ep, double* vel, double* ev, double* force, double* moment, double* mass,
+                                   double* tmax, double* rres, unsigned int* pair_count, unsigned int* pair_id,
+                                   double* tsd, unsigned int np)
+{
+    XDEM_REQUIRE_PARAMS();
+    if (np == 0) return XDEM_OK;
+    if (!cyl || !dbi || !cp || !pos || !ep || !vel || !ev || !force || !moment || !mass || !tmax || !rres ||
+        !pair_count || !pair_id || !tsd)
+        return XDEM_E_INVALID;
+    std::lock_guard<std::mutex> lk(g_mutex);
+    int rc0 = ws_require(runtime(), 0);
+    if (rc0) return rc0;
+    double* bf[6];
+    int rc = body_force_ptrs(np, bf);
+    if (rc) return rc;
+    const bool gl = runtime().quirks.gpu_force_law != 0;
+    const dim3 g(nblocks(np)), b(kThreads);
+    hipStream_t s = runtime().stream;
+    unsigned* err = runtime().err;
+#define XDEM_CY(M, G)                                                                                                  \
+    hipLaunchKernelGGL((k_cylinder<M, G>), g, b, 0, s, L, cyl, dbi, cp, bf[0], bf[1], bf[2], bf[3], bf[4], bf[5], pos, \
+                       ep, vel, ev, force, moment, mass, tmax, rres, pair_count, pair_id, tsd, np, err)
+    if (L.model == 1) { if (gl) XDEM_CY(1, true); else XDEM_CY(1, false); }
+    else { if (gl) XDEM_CY(0, true); else XDEM_CY(0, false); }
+#undef XDEM_CY
     XDEM_LAUNCH_CHECK();
     return XDEM_OK;
 }

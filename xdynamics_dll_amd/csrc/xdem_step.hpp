@@ -181,6 +181,7 @@ struct StepArgs {
     uint32_t np, nsorted, cap;
     uint32_t nwslot, nbucket;
     uint32_t list_cap;                       // neighbour-list entries per sorted slot
+    uint32_t full_blocks;                    // grid of the full-pass kernels (host side: launch shape)
     int stale_ft, flat_inertia, keep_velocity, tmax_persist;
     int zero_force;  // k_finish leaves force/moment zero (the reference's state after a step)
 };

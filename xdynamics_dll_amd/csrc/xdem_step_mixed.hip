@@ -13,15 +13,14 @@ inline dim3 grid_for(unsigned n) { return dim3((n + kBlock - 1) / kBlock); }
 
 // pass 0: the lite pass over every sphere outside this step's full pass (two kernels in
 // fused mode); pass 2: the triangle pass (meshes) and pass 3: the sphere pairs, both before
-// pass 1, the rest of the full pass over the spheres k_touch queued (kQueueSegs blocks per
-// chunk of each queue segment, at most kFullBlocks in all, one per CU: the kernel runs one
-// wave per SIMD; the queue length is read on the device). The two touch disjoint spheres
-// and may run concurrently.
-constexpr unsigned kFullBlocks = 256;
-inline dim3 full_grid(unsigned np)
+// pass 1, the rest of the full pass over the spheres k_touch queued. The full-pass kernels
+// stride over the queue segments (the queue lengths are on the device) with A.full_blocks
+// blocks, a multiple of kQueueSegs: a few blocks per CU keep several waves per SIMD for the
+// neighbour gathers. The two passes touch disjoint spheres and may run concurrently.
+inline dim3 full_grid(const StepArgs& A)
 {
-    const unsigned chunks = std::max(1u, std::min(grid_for(np).x, kFullBlocks) / kQueueSegs);
-    return dim3(chunks * kQueueSegs);
+    const unsigned want = std::min(A.full_blocks, std::max(grid_for(A.np).x, kQueueSegs));
+    return dim3(std::max(1u, want / kQueueSegs) * kQueueSegs);
 }
 
 template <bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
@@ -29,13 +28,13 @@ void collide_passes(const StepArgs& A, int pass, hipStream_t s)
 {
     const dim3 g = grid_for(A.np), b(kBlock);
     if (pass == 1) {
-        hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, MODEL, GPU_LAW>), full_grid(A.np), b, 0, s, A);
+        hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, MODEL, GPU_LAW>), full_grid(A), b, 0, s, A);
     } else if (pass == 3) {
         // the pair pass: kPairSpheres queued spheres per block and round
-        hipLaunchKernelGGL((k_pairs<float, MULTI, MODEL, GPU_LAW>), full_grid(A.np), b, 0, s, A);
+        hipLaunchKernelGGL((k_pairs<float, MULTI, MODEL, GPU_LAW>), full_grid(A), b, 0, s, A);
     } else if (pass == 2) {
         // the triangle pass (grid-strided over the mesh queue, whose length is on the device)
-        hipLaunchKernelGGL((k_mesh_pass<float, MULTI, MODEL, GPU_LAW>), dim3(kFullBlocks), b, 0, s, A);
+        hipLaunchKernelGGL((k_mesh_pass<float, MULTI, MODEL, GPU_LAW>), full_grid(A), b, 0, s, A);
     } else if (FUSE) {
         launch_lite_fused(A, MULTI, s);
     } else {

@@ -177,6 +177,59 @@ class SceneData:
         return p
 
 
+def make_cylinder(p0, p1, r_top: float, r_bottom: float, length: float | None = None, thickness: float = 0.0,
+                  empty: int = 0, cyl_id: int = 0, mass: float = 0.0):
+    """Cylinder body from its end points (xCylinderObject::SetupDataFromStructure,
+    xCylinderObject.cpp:77-109, then xParticleCylinderContact::define, xParticleCylinderContact.cpp:68-80):
+    the body position is the midpoint, the pose's Euler parameters the identity, the ends kept
+    relative to the position, len_rr = (length, r_bottom, r_top). The rim region of the contact
+    takes the body's local y axis as the cylinder axis, so a cylinder meant to have rims is
+    given along y. Returns (CylinderInfo, dbi)."""
+    a = np.asarray(p0, dtype=np.float64)
+    b = np.asarray(p1, dtype=np.float64)
+    pos = 0.5 * (a + b)
+    info = _lib.CylinderInfo()
+    info.id = cyl_id
+    info.empty_part = empty
+    info.thickness = thickness
+    info.len_rr = _lib.double3(float(np.linalg.norm(b - a)) if length is None else length, r_bottom, r_top)
+    info.pbase = _lib.double3(*(a - pos))
+    info.ptop = _lib.double3(*(b - pos))
+    dbi = np.zeros(15)
+    dbi[0] = mass
+    dbi[1:4] = pos
+    dbi[7] = 1.0
+    return info, dbi
+
+
+def _generate(fn, data, *extra, seed: int = 1):
+    H = _lib.host()
+    n = C.c_uint()
+    state = C.c_uint(seed)
+    _lib.check(getattr(H, fn)(C.byref(data), C.byref(state), *extra, None, C.byref(n)), fn)
+    pos = np.zeros((n.value, 4))
+    _lib.check(getattr(H, fn)(C.byref(data), C.byref(state), *extra, pos.ctypes.data, C.byref(n)), fn)
+    return pos[: n.value], state.value
+
+
+def generate_cube(dx, dy, dz, lx, ly, lz, minr, maxr, seed: int = 1):
+    """CreateCubeParticle's spheres (xParticleManager.cpp:646-712), (n, 4), and the generator
+    state after the draws (seed: the C library's rand() state, 1 = never seeded)."""
+    return _generate("xdem_create_cube_particles", _lib.CubeParticleData(dx, dy, dz, lx, ly, lz, minr, maxr), seed=seed)
+
+
+def generate_plane(dx, layers, dz, direction, corner, minr, maxr, seed: int = 1, reference_quirks: bool = False):
+    """CreatePlaneParticle's spheres (xParticleManager.cpp:714-792)."""
+    d = _lib.PlaneParticleData(dx, layers, dz, *direction, *corner, minr, maxr)
+    return _generate("xdem_create_plane_particles", d, 1 if reference_quirks else 0, seed=seed)
+
+
+def generate_circle(diameter, center, direction, minr, maxr, seed: int = 1):
+    """CreateCircleParticle's spheres (xParticleManager.cpp:799-880)."""
+    d = _lib.CircleParticleData(diameter, *center, *direction, minr, maxr)
+    return _generate("xdem_create_circle_particles", d, seed=seed)
+
+
 def wheel_mesh(radius: float = 0.15, width: float = 0.1, n_theta: int = 170, n_y: int = 8, n_grouser: int = 12,
                grouser_height: float = 0.015, grouser_thickness: float = 0.006, n_ring: int = 188) -> np.ndarray:
     """Triangles (T, 3, 3) of a grousered wheel in its body frame, axis along y (xdem_wheel_mesh):

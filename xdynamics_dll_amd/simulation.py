@@ -20,7 +20,7 @@ class DemContext:
     def __init__(self, scenes, precision: str = "fp64", p2p_capacity: int = 12, quirks: dict | None = None,
                  grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False,
                  buckets_per_sphere: int = 0, skin: float = 0.0, step_graph: int = 0,
-                 rebuild_wait_limit: int = 0, meshes=()):
+                 rebuild_wait_limit: int = 0, meshes=(), full_blocks: int = 0):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -38,6 +38,7 @@ class DemContext:
         cfg.skin = skin   # 0: adaptive (1.25-20% of the smallest radius); < 0: lists rebuilt every step
         cfg.step_graph = step_graph   # > 0: step() replays HIP graphs of that many steps; 0: eager launches
         cfg.rebuild_wait_limit = rebuild_wait_limit   # 0: ~1 s before a rebuild phase gives up
+        cfg.full_blocks = full_blocks   # 0: 1024 blocks for the full-pass kernels
         self.precision = precision
         self.capacity = p2p_capacity
         self._h = C.c_void_p()
@@ -186,6 +187,33 @@ class DemContext:
             self.close()
         except Exception:
             pass
+
+
+def autotune(scenes, precision: str = "mixed", fuse_velocity: bool = True, p2p_capacity: int = 12,
+             warmup: int = 20, steps: int = 40, cache_path: str | None = None, step_graph: int = 0) -> dict:
+    """Launch shapes of the step layer for these scenes (xdem_autotune): full-pass blocks and
+    buckets per sphere, chosen by timing short runs, cached per scene signature in cache_path.
+    Returns the choice, whether it came from the cache, and every trial's time."""
+    if isinstance(scenes, SceneData):
+        scenes = [scenes]
+    L = _lib.lib()
+    cfg = _lib.CtxConfig()
+    cfg.precision = {"fp64": _lib.PRECISION_FP64, "mixed": _lib.PRECISION_MIXED}[precision]
+    cfg.p2p_capacity = p2p_capacity
+    q = dict(PARITY_QUIRKS)
+    cfg.quirks = _lib.Quirks(q["gpu_force_law"], q["velocity_not_updated"], q["stale_ft"], q["flat_inertia"])
+    cfg.fuse_velocity = 1 if fuse_velocity else 0
+    cfg.step_graph = step_graph
+    keeps = [sc.to_c() for sc in scenes]
+    arr = (_lib.Scene * len(scenes))(*(k[0] for k in keeps))
+    res = _lib.AutotuneResult()
+    path = None if cache_path is None else os.fsencode(cache_path)
+    _lib.check(L.xdem_autotune(arr, len(scenes), C.byref(cfg), warmup, steps, path, C.byref(res)), "xdem_autotune")
+    del keeps
+    return {"full_blocks": res.full_blocks, "buckets_per_sphere": res.buckets_per_sphere,
+            "ms_per_step": res.ms_per_step, "from_cache": bool(res.from_cache), "key": res.key.decode(),
+            "trials": [{"full_blocks": res.trial_full_blocks[k], "buckets_per_sphere": res.trial_buckets_per_sphere[k],
+                        "ms_per_step": res.trial_ms_per_step[k]} for k in range(res.ntrial)]}
 
 
 class xIntegratorVV:
