@@ -291,6 +291,8 @@ int xdem_ctx_scene_info_get(xdem_ctx* ctx, unsigned int scene, xdem_ctx_scene_in
  * XDEM_PART_HISTORY, which restores them too. A file whose size does not fit the scene is
  * refused with XDEM_E_INVALID before anything is written. */
 #define XDEM_PART_HISTORY 1
+#define XDEM_PART_POSITIONS 2   /* only the positions (xParticleManager::SetCurrentParticlesFromPartResult,
+                                   xParticleManager.cpp:563-644); not with XDEM_PART_HISTORY */
 int xdem_ctx_save_part(xdem_ctx* ctx, unsigned int scene, const char* path, double ct);
 int xdem_ctx_load_part(xdem_ctx* ctx, unsigned int scene, const char* path, int flags, double* ct);
 /* Number of stored sphere-sphere and sphere-wall contacts of all scenes at the last step. */

@@ -94,6 +94,26 @@ def test_restart_without_histories_keeps_them(tmp_path):
     b.close()
 
 
+def test_positions_only_restart(tmp_path):
+    """SetCurrentParticlesFromPartResult: the positions from the file, everything else kept."""
+    scene, meta, new = make("lattice")
+    a = new()
+    a.step(20)
+    path = str(tmp_path / "part.bin")
+    a.save_part(path, 0.5)
+    b = new()
+    b.step(3)
+    before = {f: b.read(f) for f in STATE}
+    assert b.load_part(path, positions_only=True) == 0.5
+    assert np.array_equal(b.read("pos"), a.read("pos"))
+    for f in STATE[1:]:
+        assert np.array_equal(b.read(f), before[f]), f
+    with pytest.raises(XdemError):
+        b.load_part(path, history=True, positions_only=True)
+    a.close()
+    b.close()
+
+
 def test_foreign_file_and_overfull_history_are_refused(tmp_path):
     scene, meta, new = make("lattice")
     a = new()

@@ -42,6 +42,7 @@ HISTORY_P2P = 0
 HISTORY_PLANE = 1
 HISTORY_MESH = 2
 PART_HISTORY = 1
+PART_POSITIONS = 2
 
 SCENE_BOX_LATTICE = 1
 SCENE_DAM_BREAK = 2

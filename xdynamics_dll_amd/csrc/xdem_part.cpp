@@ -97,7 +97,8 @@ int xdem_ctx_save_part(xdem_ctx* ctx, unsigned int scene, const char* path, doub
 
 int xdem_ctx_load_part(xdem_ctx* ctx, unsigned int scene, const char* path, int flags, double* ct)
 {
-    if (!ctx || !path || (flags & ~XDEM_PART_HISTORY)) return XDEM_E_INVALID;
+    const int known = XDEM_PART_HISTORY | XDEM_PART_POSITIONS;
+    if (!ctx || !path || (flags & ~known) || flags == known) return XDEM_E_INVALID;
     Sections s;
     int rc = sections(ctx, scene, s);
     if (rc) return rc;
@@ -135,9 +136,13 @@ int xdem_ctx_load_part(xdem_ctx* ctx, unsigned int scene, const char* path, int 
     }
     std::fclose(f);
     if (!ok) return XDEM_E_INVALID;
-    // the state (set_dem_data) and the contact histories (xContactManager::set_from_part_result)
-    for (int k = 0; k < 7; k++)
+    // the state (set_dem_data) and the contact histories (xContactManager::set_from_part_result);
+    // XDEM_PART_POSITIONS: the positions only (xParticleManager::SetCurrentParticlesFromPartResult,
+    // xParticleManager.cpp:563-644, which copies pos and nothing else for single spheres)
+    for (int k = 0; k < 7; k++) {
+        if ((flags & XDEM_PART_POSITIONS) && kFields[k] != XDEM_FIELD_POS) continue;
         if ((rc = xdem_ctx_write(ctx, kFields[k], scene, fields[k].data()))) return rc;
+    }
     for (int h = 0; h < 3 && (flags & XDEM_PART_HISTORY); h++) {
         if (!present[h]) continue;
         if ((rc = xdem_ctx_write_history(ctx, kinds[h], scene, cnt[h].data(), ids[h].data(), tsd[h].data()))) return rc;

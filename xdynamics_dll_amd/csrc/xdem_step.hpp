@@ -28,9 +28,10 @@ constexpr uint32_t kMeshNear = 0x02000000u;   // triangles of the scene's mesh w
                                               // such a sphere goes to the full pass every step
 constexpr uint32_t kLiteStable = kSpinStable | kAccStable | kNoContacts;   // set by the lite pass
 constexpr uint32_t kCountMask = 0x01ffffffu;
-// The full-pass queue is split in kQueueSegs segments, each with its own counter on its own
-// 128-byte line: the wave of sorted slots [64 w, 64 w + 64) appends to segment w mod
-// kQueueSegs, so the appends of different waves rarely hit the same counter.
+// The candidate and full-pass queues are split in kQueueSegs segments, each with its own
+// counter on its own 128-byte line: segment s holds the spheres of the s-th of 64 contiguous
+// ranges of sorted slots (the wave of slots [64 w, 64 w + 64) of nw waves appends to segment
+// 64 w / nw), so a segment is one region of the scene (see queue_block for the XCDs).
 constexpr uint32_t kQueueSegs = 64;
 constexpr uint32_t kSegStride = 32;           // uint32 entries between two segment counters
 constexpr uint32_t kBarFlags = 64;            // release flags of k_rebuild's phases
@@ -185,6 +186,22 @@ struct StepArgs {
     int stale_ft, flat_inertia, keep_velocity, tmax_persist;
     int zero_force;  // k_finish leaves force/moment zero (the reference's state after a step)
 };
+
+// Queue segment and chunk of block b of a kernel that walks a queue with nblocks blocks (a
+// multiple of kQueueSegs): segment b mod 64, chunk b / 64 of it. Work-groups are dealt
+// round-robin to the 8 XCDs, so XCD x walks segments x, x + 8, ..., x + 56: eight slices of
+// the scene, each spatially coherent (a segment is a contiguous range of sorted slots), which
+// keeps its L2 working set to an eighth of the state while a contact region that covers few
+// segments (the base of a collapsing column) still spreads over all XCDs. (Giving each XCD
+// eight adjacent segments was 6% faster on a settled bed and 2% slower on the dam break.)
+struct QueueBlock {
+    uint32_t seg, chunk, nchunk;
+};
+
+__device__ __forceinline__ QueueBlock queue_block(uint32_t b, uint32_t nblocks)
+{
+    return {b % kQueueSegs, b / kQueueSegs, nblocks / kQueueSegs};
+}
 
 __device__ __forceinline__ uint32_t scene_index(const StepArgs& A, uint32_t i, bool multi)
 {
@@ -629,17 +646,18 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
                 if (full) touch = touch_sphere<MULTI>(A, i, fl, true, 0.0f);
                 A.nbr_count[i] = touch ? (fl | kTouch) : fl;
             }
-            const uint32_t seg = (j >> 6) & (kQueueSegs - 1u);
+            const uint32_t seg = (j >> 6) * kQueueSegs / (nvb * (256u / 64u));
             queue_append(A.work, A.seg_count, seg, A.seg_cap, full, i);
             queue_append(A.work2, A.seg_count2, seg, A.seg_cap, touch, i);
             if (A.has_mesh) queue_append(A.mesh_work, A.mesh_count, 0u, 0u, mesh, i);
         }
         return;
     }
-    const uint32_t seg = blockIdx.x % kQueueSegs;
-    const uint32_t nchunk = gridDim.x / kQueueSegs;
+    const QueueBlock qb = queue_block(blockIdx.x, gridDim.x);
+    const uint32_t seg = qb.seg;
+    const uint32_t nchunk = qb.nchunk;
     const uint32_t* wq = A.work + (size_t)seg * A.seg_cap;
-    uint32_t base = (blockIdx.x / kQueueSegs) * blockDim.x;
+    uint32_t base = qb.chunk * blockDim.x;
     // the first entry is read together with the segment's count
     uint32_t i_next = base + threadIdx.x < A.seg_cap ? wq[base + threadIdx.x] : 0u;
     const uint32_t cnt = A.seg_count[seg * kSegStride];
@@ -719,12 +737,140 @@ __device__ __forceinline__ void sphere_tail(const StepArgs& A, const SceneDev& S
     }
 }
 
+// ------------------------------------------------------------------------- pair pass
+// The sphere contacts of the full pass, one thread per queued sphere: its passing pairs in
+// rank order (rank = place among the sphere's passing list entries, i.e. the reference's
+// visiting order), each looked up in the previous step's slots, evaluated
+// (sphere_pair_eval), written to its new slot (= rank, up to the capacity) and added with
+// sphere_contact's operations, so the sums are those of the reference's walk over the pairs.
+// The first kPairIdRegs slot ids of the previous step are read once into registers, and the
+// next pair's list entry is loaded while the current pair is evaluated; the neighbour's state
+// and its history entry are then one round trip. The sums go to ppsum for k_collide.
+// Cohesion-only pairs add their attraction.
+constexpr uint32_t kPairIdRegs = 16;
+constexpr uint32_t kPairSet = 0, kPairStale = 1, kPairCoh = 2;   // kinds of an evaluated pair
+
+// The first slot of the previous step holding sphere k (kInvalid: none).
+__device__ __forceinline__ uint32_t old_slot(const StepArgs& A, uint32_t hb, const uint32_t (&ids)[kPairIdRegs],
+                                             uint32_t old_count, uint32_t k)
+{
+    uint32_t slot = kInvalid;
+#pragma unroll
+    for (int s = (int)kPairIdRegs - 1; s >= 0; s--)   // downwards: the lowest match stays
+        if ((uint32_t)s < old_count && ids[s] == k) slot = (uint32_t)s;
+    for (uint32_t s = kPairIdRegs; s < old_count && slot == kInvalid; s++)
+        if (A.pp_id_in[hb + s] == k) slot = s;
+    return slot;
+}
+
+// Sphere i's pair with neighbour k: the neighbour's state, the history entry of `slot`, the
+// force law; writes the new slot `rank` and returns the kind.
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ uint32_t pair_of(const StepArgs& A, const SceneDev& S, uint32_t i, const OwnState& o,
+                                            V3<double> iom, uint32_t k, uint32_t rank, uint32_t slot, double v[10])
+{
+    const uint32_t hb = i * A.cap;
+    const V4<double> jpos = ld4(A.pos + 4 * (size_t)k);
+    const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
+    const V4<double> jep = ld4(A.ep + 4 * (size_t)k);
+    const V4<double> jev = ld4(A.ev + 4 * (size_t)k);
+    const double jmass = A.mass[k];
+    const double2 h = slot != kInvalid ? A.pp_tsd_in[hb + slot] : make_double2(0.0, 0.0);
+    const PairGeom g = pair_geom(o.pos, jpos);
+    double sd0 = 0.0, sd1 = 0.0;
+    uint32_t kind;
+    if (g.cdist > 0) {
+        sd0 = h.x; sd1 = h.y;
+        const bool set = sphere_pair_eval<T, MODEL, GPU_LAW>(S.cc, S.dt, o.pos, o.vel, iom, o.mass, jpos, jvel,
+                                                             omega_of(jep, jev), jmass, g, sd0, sd1, v);
+        kind = set ? kPairSet : kPairStale;
+    } else {
+        // cohesion only: the attraction cf * unit that sphere_cohesion subtracts
+        const double ir = o.pos.w, jr = jpos.w;
+        const double Req = jr ? (ir * jr) / (ir + jr) : ir;
+        const double cf = cohesion_force<double>(S.cc, Req, S.cc.jk * Req);
+        const V3<double> u = g.rp / g.dist;
+        const V3<double> a = cf * u;
+        v[0] = a.x; v[1] = a.y; v[2] = a.z;
+        kind = kPairCoh;
+    }
+    if (rank < A.cap) {
+        A.pp_id_out[hb + rank] = k;
+        A.pp_tsd_out[hb + rank] = make_double2(sd0, sd1);
+    }
+    return kind;
+}
+
+// The full-pass spheres of this step (all queue segments): one counter per lane of a wave.
+__device__ __forceinline__ uint32_t full_queue_total(const StepArgs& A)
+{
+    static_assert(kQueueSegs == 64, "one segment counter per lane");
+    uint32_t c = A.seg_count2[(threadIdx.x & 63u) * kSegStride];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    return c;
+}
+
+// Below this many full-pass spheres k_collide walks the pairs itself and k_pairs returns at
+// once: the spheres then fit in one wave per SIMD of k_collide, and the pair sums skip a
+// launch and two dependent memory round trips (ppsum, own state) on the step's critical path.
+// Above it k_pairs (three waves per SIMD) hides the neighbour gathers of dense packs.
+constexpr uint32_t kMergePairsBelow = 65536;
+
+// Sphere i's passing pairs in rank order (the k_pairs walk): the pair sums, the new slots and
+// count, the overflow bits. Returns the stored count.
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ uint32_t pairs_sphere(const StepArgs& A, const SceneDev& S, uint32_t i, const OwnState& o,
+                                                 V3<double> iom, uint32_t flags, Accum& pp)
+{
+    const uint32_t nlist_all = flags & kCountMask;
+    const unsigned long long m = min(nlist_all, A.list_cap) ? A.pass_mask[i] : 0ull;
+    const uint32_t old_count = min(A.pp_count_in[i], A.cap);
+    const uint32_t hb = i * A.cap;
+    // the previous step's slot ids (at most the capacity: the last sphere's slots end the array)
+    uint32_t ids[kPairIdRegs];
+    const uint32_t nid = min(A.cap, kPairIdRegs);
+    if ((A.cap & 3u) == 0u) {
+        const uint4* p4 = reinterpret_cast<const uint4*>(A.pp_id_in + hb);
+#pragma unroll
+        for (uint32_t c = 0; c < kPairIdRegs / 4u; c++) {
+            const uint4 w = 4u * c < nid ? p4[c] : make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
+            ids[4 * c] = w.x; ids[4 * c + 1] = w.y; ids[4 * c + 2] = w.z; ids[4 * c + 3] = w.w;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t s = 0; s < kPairIdRegs; s++) ids[s] = s < nid ? A.pp_id_in[hb + s] : kInvalid;
+    }
+    const uint32_t* lst = A.nbr + (size_t)i * A.list_cap;
+    V3<T> Ft = mk3(T(0), T(0), T(0));
+    uint32_t rank = 0;
+    unsigned long long mm = m;
+    uint32_t k_next = mm ? lst[(uint32_t)__ffsll((long long)mm) - 1u] : 0u;
+    for (; mm; rank++) {
+        const uint32_t k = k_next;
+        mm &= mm - 1ull;
+        if (mm) k_next = lst[(uint32_t)__ffsll((long long)mm) - 1u];   // in flight during this pair
+        double v[10];
+        const uint32_t kind =
+            pair_of<T, MODEL, GPU_LAW>(A, S, i, o, iom, k, rank, old_slot(A, hb, ids, old_count, k), v);
+        if (kind == kPairCoh) pp.F = pp.F - mk3(v[0], v[1], v[2]);
+        else sphere_pair_sum<T>(v, kind == kPairSet, A.stale_ft != 0, Ft, pp);
+    }
+    const uint32_t n_new = min(rank, A.cap);
+    A.pp_count_out[i] = n_new;
+    uint32_t err = 0;
+    if (rank > A.cap) err |= XDEM_ERR_P2P_OVERFLOW;
+    if (nlist_all > A.list_cap) err |= XDEM_ERR_NBR_OVERFLOW;
+    if (err) atomicOr(A.err, err);
+    return n_new;
+}
+
 // Full contact pass of sphere i: the near neighbours the pair search listed, in their order, under
 // the exact FP64 predicate; every wall of the scene in id order; then sphere_tail. The
 // sphere's own state and outputs are read and written by id.
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
-__device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, uint32_t& n_pp, uint32_t& n_wall,
-                                               float& dmx)
+__device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, bool merged, uint32_t& n_pp,
+                                               uint32_t& n_wall, float& dmx)
 {
     const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
     // the sphere's own state, list flags and wall slot count, issued together (the fields
@@ -740,14 +886,19 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, ui
     const V3<double> iom = omega_of(o.ep, o.ev);
     uint32_t err = 0;
 
-    // ---- sphere-sphere (impl:613-733): the sums of k_pairs, which ran before this kernel
+    // ---- sphere-sphere (impl:613-733): walked here when the full pass is small, else the
+    // sums of k_pairs, which ran before this kernel
     Accum pp = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
     if (S.has_pp) {
-        const double* ps = A.ppsum + (size_t)i * 10;
-        pp.F = mk3(ps[0], ps[1], ps[2]);
-        pp.M = mk3(ps[3], ps[4], ps[5]);
-        pp.T = mk3(ps[6], ps[7], ps[8]);
-        pp.R = ps[9];
+        if (merged) {
+            n_pp = pairs_sphere<T, MODEL, GPU_LAW>(A, S, i, o, iom, flags, pp);
+        } else {
+            const double* ps = A.ppsum + (size_t)i * 10;
+            pp.F = mk3(ps[0], ps[1], ps[2]);
+            pp.M = mk3(ps[3], ps[4], ps[5]);
+            pp.T = mk3(ps[6], ps[7], ps[8]);
+            pp.R = ps[9];
+        }
     }
 
     // ---- walls, in id order (impl:950-1080 once per wall)
@@ -932,148 +1083,31 @@ __global__ __launch_bounds__(256) void k_mesh_pass(StepArgs A)
     }
 }
 
-// ------------------------------------------------------------------------- pair pass
-// The sphere contacts of the full pass, kPairLanes lanes per sphere: lane g evaluates the
-// passing pairs of rank g, g + kPairLanes, ... (rank = place among the sphere's passing list
-// entries, i.e. the reference's visiting order), writes their history slots (slot = rank, up
-// to the capacity) and stages its results in LDS; then the sphere's first lane adds them up in
-// rank order with sphere_contact's operations (pairs beyond the LDS stage are evaluated by
-// that lane itself, in order), so the sums are bit-identical to one thread walking the pairs.
-// The sums go to ppsum for k_collide. Cohesion-only pairs stage their attraction.
-constexpr uint32_t kPairLanes = 8;
-constexpr uint32_t kPairStage = 16;                       // pair results per sphere in LDS
-constexpr uint32_t kPairSpheres = 256 / kPairLanes;       // spheres per block and round
-constexpr uint32_t kPairSet = 0, kPairStale = 1, kPairCoh = 2;   // kinds of a staged pair
-
-// Sphere i's pair of list entry t: its neighbour's state, history and force; returns the kind.
-template <class T, int MODEL, bool GPU_LAW>
-__device__ __forceinline__ uint32_t pair_of(const StepArgs& A, const SceneDev& S, uint32_t i, const OwnState& o,
-                                            V3<double> iom, uint32_t k, uint32_t rank, uint32_t old_count,
-                                            double v[10])
-{
-    const uint32_t cap = A.cap, hb = i * cap;
-    const V4<double> jpos = ld4(A.pos + 4 * (size_t)k);
-    const V3<double> jvel = ld3(A.vel + 3 * (size_t)k);
-    const V4<double> jep = ld4(A.ep + 4 * (size_t)k);
-    const V4<double> jev = ld4(A.ev + 4 * (size_t)k);
-    const double jmass = A.mass[k];
-    // the first slot of the previous step holding k
-    uint32_t slot = kInvalid;
-    if ((cap & 3u) == 0u) {
-        const uint4* hid4 = reinterpret_cast<const uint4*>(A.pp_id_in + hb);
-        for (uint32_t c = 0; c < old_count && slot == kInvalid; c += 4u) {
-            const uint4 q = hid4[c / 4u];
-            if (q.x == k) slot = c;
-            else if (c + 1u < old_count && q.y == k) slot = c + 1u;
-            else if (c + 2u < old_count && q.z == k) slot = c + 2u;
-            else if (c + 3u < old_count && q.w == k) slot = c + 3u;
-        }
-    } else {
-        for (uint32_t s2 = 0; s2 < old_count; s2++)
-            if (A.pp_id_in[hb + s2] == k) { slot = s2; break; }
-    }
-    const double2 h = slot != kInvalid ? A.pp_tsd_in[hb + slot] : make_double2(0.0, 0.0);
-    const PairGeom g = pair_geom(o.pos, jpos);
-    double sd0 = 0.0, sd1 = 0.0;
-    uint32_t kind;
-    if (g.cdist > 0) {
-        sd0 = h.x; sd1 = h.y;
-        const bool set = sphere_pair_eval<T, MODEL, GPU_LAW>(S.cc, S.dt, o.pos, o.vel, iom, o.mass, jpos, jvel,
-                                                             omega_of(jep, jev), jmass, g, sd0, sd1, v);
-        kind = set ? kPairSet : kPairStale;
-    } else {
-        // cohesion only: the attraction cf * unit that sphere_cohesion subtracts
-        const double ir = o.pos.w, jr = jpos.w;
-        const double Req = jr ? (ir * jr) / (ir + jr) : ir;
-        const double cf = cohesion_force<double>(S.cc, Req, S.cc.jk * Req);
-        const V3<double> u = g.rp / g.dist;
-        const V3<double> a = cf * u;
-        v[0] = a.x; v[1] = a.y; v[2] = a.z;
-        kind = kPairCoh;
-    }
-    if (rank < cap) {
-        A.pp_id_out[hb + rank] = k;
-        A.pp_tsd_out[hb + rank] = make_double2(sd0, sd1);
-    }
-    return kind;
-}
-
 template <class T, bool MULTI, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 {
-    __shared__ double stage[kPairSpheres][kPairStage][10];
-    __shared__ uint32_t kinds[kPairSpheres][kPairStage];
-    const uint32_t sg = blockIdx.x % kQueueSegs;
-    const uint32_t* q = A.work2 + (size_t)sg * A.seg_cap;
-    const uint32_t cnt = A.seg_count2[sg * kSegStride];
-    const uint32_t lane = threadIdx.x % kPairLanes, ls = threadIdx.x / kPairLanes;
-    const uint32_t stride = gridDim.x / kQueueSegs * kPairSpheres;
+    if (full_queue_total(A) < kMergePairsBelow) return;   // k_collide walks the pairs
+    const QueueBlock qb = queue_block(blockIdx.x, gridDim.x);
+    const uint32_t stride = qb.nchunk * blockDim.x;
+    const uint32_t* q = A.work2 + (size_t)qb.seg * A.seg_cap;
+    const uint32_t cnt = A.seg_count2[qb.seg * kSegStride];
     unsigned long long c0 = 0;
-    for (uint32_t base = blockIdx.x / kQueueSegs * kPairSpheres; base < cnt; base += stride) {
-        const uint32_t e = base + ls;
-        const uint32_t i = e < cnt ? q[e] : kInvalid;
-        const SceneDev* Sp = i != kInvalid ? &A.scenes[scene_index(A, i, MULTI)] : nullptr;
-        const bool work = Sp && Sp->has_pp;
-        unsigned long long m = 0;
-        uint32_t old_count = 0, nlist_all = 0;
+    for (uint32_t e = qb.chunk * blockDim.x + threadIdx.x; e < cnt; e += stride) {
+        const uint32_t i = q[e];
+        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
+        if (!S.has_pp) continue;
         OwnState o;
-        V3<double> iom = mk3(0.0, 0.0, 0.0);
-        if (work) {
-            o.pos = ld4(A.pos + 4 * (size_t)i);
-            o.vel = ld3(A.vel + 3 * (size_t)i);
-            o.mass = A.mass[i];
-            iom = omega_of(ld4(A.ep + 4 * (size_t)i), ld4(A.ev + 4 * (size_t)i));
-            nlist_all = A.nbr_count[i] & kCountMask;
-            m = min(nlist_all, A.list_cap) ? A.pass_mask[i] : 0ull;
-            old_count = min(A.pp_count_in[i], A.cap);
-        }
-        const uint32_t* lst = A.nbr + (size_t)(work ? i : 0u) * A.list_cap;
-        // this lane's pairs: ranks lane, lane + kPairLanes, ... (staged ones only)
-        {
-            uint32_t rank = 0;
-            for (unsigned long long mm = m; mm; mm &= mm - 1ull, rank++) {
-                if (rank >= kPairStage) break;
-                if (rank % kPairLanes != lane) continue;
-                const uint32_t t = (uint32_t)__ffsll((long long)mm) - 1u;
-                double v[10];
-                kinds[ls][rank] = pair_of<T, MODEL, GPU_LAW>(A, *Sp, i, o, iom, lst[t], rank, old_count, v);
-#pragma unroll
-                for (int c = 0; c < 10; c++) stage[ls][rank][c] = v[c];
-            }
-        }
-        __syncthreads();
-        if (work && lane == 0) {
-            Accum pp = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
-            V3<T> Ft = mk3(T(0), T(0), T(0));
-            uint32_t rank = 0;
-            for (unsigned long long mm = m; mm; mm &= mm - 1ull, rank++) {
-                double v[10];
-                uint32_t kind;
-                if (rank < kPairStage) {
-                    kind = kinds[ls][rank];
-#pragma unroll
-                    for (int c = 0; c < 10; c++) v[c] = stage[ls][rank][c];
-                } else {
-                    const uint32_t t = (uint32_t)__ffsll((long long)mm) - 1u;
-                    kind = pair_of<T, MODEL, GPU_LAW>(A, *Sp, i, o, iom, lst[t], rank, old_count, v);
-                }
-                if (kind == kPairCoh) pp.F = pp.F - mk3(v[0], v[1], v[2]);
-                else sphere_pair_sum<T>(v, kind == kPairSet, A.stale_ft != 0, Ft, pp);
-            }
-            const uint32_t n_new = min(rank, A.cap);
-            A.pp_count_out[i] = n_new;
-            uint32_t err = 0;
-            if (rank > A.cap) err |= XDEM_ERR_P2P_OVERFLOW;
-            if (nlist_all > A.list_cap) err |= XDEM_ERR_NBR_OVERFLOW;
-            if (err) atomicOr(A.err, err);
-            double* ps = A.ppsum + (size_t)i * 10;
-            st3(ps, pp.F);
-            st3(ps + 3, pp.M);
-            st3(ps + 6, pp.T);
-            ps[9] = pp.R;
-            c0 += n_new;
-        }
-        __syncthreads();
+        o.pos = ld4(A.pos + 4 * (size_t)i);
+        o.vel = ld3(A.vel + 3 * (size_t)i);
+        o.mass = A.mass[i];
+        const V3<double> iom = omega_of(ld4(A.ep + 4 * (size_t)i), ld4(A.ev + 4 * (size_t)i));
+        Accum pp = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
+        c0 += pairs_sphere<T, MODEL, GPU_LAW>(A, S, i, o, iom, A.nbr_count[i], pp);
+        double* ps = A.ppsum + (size_t)i * 10;
+        st3(ps, pp.F);
+        st3(ps + 3, pp.M);
+        st3(ps + 6, pp.T);
+        ps[9] = pp.R;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) c0 += __shfl_xor(c0, off, 64);
@@ -1290,17 +1324,18 @@ __global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
 template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_collide(StepArgs A)
 {
-    const uint32_t sg = blockIdx.x % kQueueSegs;
-    const uint32_t stride = gridDim.x / kQueueSegs * blockDim.x;
-    const uint32_t* q = A.work2 + (size_t)sg * A.seg_cap;
-    uint32_t e = blockIdx.x / kQueueSegs * blockDim.x + threadIdx.x;
+    const QueueBlock qb = queue_block(blockIdx.x, gridDim.x);
+    const uint32_t stride = qb.nchunk * blockDim.x;
+    const uint32_t* q = A.work2 + (size_t)qb.seg * A.seg_cap;
+    uint32_t e = qb.chunk * blockDim.x + threadIdx.x;
     uint32_t i = e < A.seg_cap ? q[e] : 0u;
-    const uint32_t cnt = A.seg_count2[sg * kSegStride];
+    const uint32_t cnt = A.seg_count2[qb.seg * kSegStride];
     unsigned long long c0 = 0, c1 = 0;
     float dmx = 0.0f;
+    const bool merged = full_queue_total(A) < kMergePairsBelow;
     while (e < cnt) {
         uint32_t n_pp = 0, n_wall = 0;
-        collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, i, n_pp, n_wall, dmx);
+        collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, i, merged, n_pp, n_wall, dmx);
         c0 += n_pp;
         c1 += n_wall;
         e += stride;

@@ -30,7 +30,7 @@ void collide_passes(const StepArgs& A, int pass, hipStream_t s)
     if (pass == 1) {
         hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW>), full_grid(A), b, 0, s, A);
     } else if (pass == 3) {
-        // the pair pass: kPairSpheres queued spheres per block and round
+        // the pair pass: one thread per queued sphere
         hipLaunchKernelGGL((k_pairs<double, MULTI, MODEL, GPU_LAW>), full_grid(A), b, 0, s, A);
     } else if (pass == 2) {
         // the triangle pass (grid-strided over the mesh queue, whose length is on the device)

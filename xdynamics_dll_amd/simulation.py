@@ -124,11 +124,12 @@ class DemContext:
         """Part file of one scene in the reference's step-result layout (xResultManager.cpp:828-925)."""
         _lib.check(self.L.xdem_ctx_save_part(self._h, scene, os.fsencode(path), ct), "xdem_ctx_save_part")
 
-    def load_part(self, path: str, scene: int = 0, history: bool = False) -> float:
+    def load_part(self, path: str, scene: int = 0, history: bool = False, positions_only: bool = False) -> float:
         """Restores the state (and with history=True the contact histories) from a part file;
-        returns its time (xDynamicSimulator.cpp:392-492)."""
+        returns its time (xDynamicSimulator.cpp:392-492). positions_only: the positions alone
+        (xParticleManager::SetCurrentParticlesFromPartResult)."""
         ct = C.c_double()
-        flags = _lib.PART_HISTORY if history else 0
+        flags = (_lib.PART_HISTORY if history else 0) | (_lib.PART_POSITIONS if positions_only else 0)
         _lib.check(self.L.xdem_ctx_load_part(self._h, scene, os.fsencode(path), flags, C.byref(ct)),
                    "xdem_ctx_load_part")
         return ct.value
