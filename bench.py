@@ -34,6 +34,13 @@ def step_bytes(cbar: float) -> float:
     return 436.0 + 40.0 * cbar
 
 
+def merged_collide_kernel_bytes(spheres: float, wall_contacts: float, pairs: float) -> float:
+    """k_collide walking the pairs too: its own bytes without the pair sums' read (480 - 80),
+    plus the pair walk's pass mask 8 and old and new counts 4 + 4 per sphere and 44 per
+    passing pair (the own state is read once)."""
+    return (480.0 - 80.0 + 16.0) * spheres + 40.0 * wall_contacts + 44.0 * pairs
+
+
 # Algorithmic bytes per sphere of k_lite_move (the translation of every sphere outside the
 # step's full pass): the translational part of the per-sphere state traffic, read pos 32,
 # vel 24, acc 24, mass 8 and write pos 32, vel 24, acc 24.
@@ -59,25 +66,33 @@ def collide_kernel_bytes(spheres: float, wall_contacts: float) -> float:
     return 480.0 * spheres + 40.0 * wall_contacts
 
 
+# The library walks the pairs inside k_collide (k_pairs returns at once) when the step's full
+# pass holds fewer spheres than this.
+MERGE_PAIRS_BELOW = 65536
+
+
 def kernel_rooflines(tm, n, info, n_pp, n_wall, fuse):
     """Per-step time and achieved HBM bandwidth of the step's kernels from the event-timed
     steps: k_lite_move (spheres outside the full pass), k_pairs and k_collide (the full-pass
-    spheres, contact_spheres of the last step)."""
+    spheres, contact_spheres of the last step). On a small full pass k_collide also walks the
+    pairs (their bytes count for it) and k_pairs only reads the queue counters."""
     steps = max(tm["steps"], 1)
     full = info["contact_spheres"]
+    merged = full < MERGE_PAIRS_BELOW
     out = {}
     if fuse and tm["lite_move_ms"] > 0:
         ms = tm["lite_move_ms"] / steps
         b = LITE_MOVE_BYTES * (n - full)
         out["k_lite_move"] = {"ms": ms, "bytes": b, "units": n - full, "bytes_per_unit": LITE_MOVE_BYTES}
-    if tm.get("pair_pass_ms", 0) > 0:
+    if tm.get("pair_pass_ms", 0) > 0 and not merged:
         ms = tm["pair_pass_ms"] / steps
         b = pairs_kernel_bytes(full, n_pp)
         out["k_pairs"] = {"ms": ms, "bytes": b, "units": full, "bytes_per_unit": b / max(full, 1)}
     if fuse and tm.get("collide_ms", 0) > 0:
         ms = tm["collide_ms"] / steps
-        b = collide_kernel_bytes(full, n_wall)
-        out["k_collide"] = {"ms": ms, "bytes": b, "units": full, "bytes_per_unit": b / max(full, 1)}
+        b = merged_collide_kernel_bytes(full, n_wall, n_pp) if merged else collide_kernel_bytes(full, n_wall)
+        out["k_collide"] = {"ms": ms, "bytes": b, "units": full, "bytes_per_unit": b / max(full, 1),
+                            "walks_pairs": merged}
     for k in out.values():
         k["gbps"] = k["bytes"] / (k["ms"] * 1e-3) / 1e9
         k["frac"] = k["gbps"] / HBM_PEAK_GBPS

@@ -187,6 +187,7 @@ typedef struct xdem_ctx_config {
 #define XDEM_HISTORY_P2P   0
 #define XDEM_HISTORY_PLANE 1
 #define XDEM_HISTORY_MESH  2  /* 5 slots; ids are triangle indices of the scene's mesh */
+#define XDEM_HISTORY_CYLINDER 3 /* 3 slots; ids are cylinder ids (id + 1000: a lid seen from inside) */
 
 /* A rigid triangle-mesh body (reference: xMeshObject + xParticleMeshObjectContact::define,
  * source/xdynamics_object/xParticleMeshObjectContact.cpp:77-167). Vertices are given in the
@@ -242,7 +243,8 @@ int xdem_ctx_write(xdem_ctx* ctx, int field, unsigned int scene, const double* h
  * step to step: fetch the pointer again after every step. */
 int xdem_ctx_device_ptr(xdem_ctx* ctx, int field, void** out);
 /* History of one scene in the reference layout: count[np], ids[np*cap], tsd[2*np*cap]
- * (cap = p2p_capacity for XDEM_HISTORY_P2P, 3 for XDEM_HISTORY_PLANE, 5 for XDEM_HISTORY_MESH).
+ * (cap = p2p_capacity for XDEM_HISTORY_P2P, 3 for XDEM_HISTORY_PLANE, 5 for XDEM_HISTORY_MESH,
+ * 3 for XDEM_HISTORY_CYLINDER).
  * Sphere ids are scene-local particle indices. */
 int xdem_ctx_read_history(xdem_ctx* ctx, int kind, unsigned int scene, unsigned int* count, unsigned int* ids,
                           double* tsd);
@@ -273,18 +275,36 @@ int xdem_ctx_set_mesh_pose(xdem_ctx* ctx, unsigned int mesh, const double dbi[15
  * (-F and -(contact point - body position) x F summed over its contacts; the reference's six
  * reductions per body, xParticleMeshObjectContact.cpp:209-219). Zeros for a fixed mesh. */
 int xdem_ctx_mesh_wrench(xdem_ctx* ctx, unsigned int mesh, double out[6]);
+/* Adds a cylinder body to a scene (xParticleCylinderContact::define and its contact launch,
+ * cu_cylinder_contact_force, decl:326-333 / impl:1327-1460): geometry as for
+ * xdem_cu_cylinder_contact_force, pose dbi (15 doubles, MM..ED3), material pair cp. Every step
+ * evaluates the scene's cylinders after its walls, in the order added, on XDEM_MAX_P2CY_COUNT
+ * slots per sphere (a cylinder with id 0 restarts them, a later one appends). A cylinder with
+ * mass > 0 is dynamic: its contact wrench is reduced every step. At most 16 per scene; between
+ * add_scene and commit. The neighbour lists treat a cylinder whose axis ends are not on the body
+ * frame's y axis, a length apart, as always in reach (every sphere of its scene takes the full
+ * contact pass). */
+int xdem_ctx_add_cylinder(xdem_ctx* ctx, unsigned int scene, const xdem_cylinder_info* cylinder,
+                          const double dbi[15], const xdem_contact_property* cp, unsigned int* cylinder_index);
+/* New pose of a cylinder body (15 doubles, MM..ED3; xParticleCylinderContact::update). A
+ * rotation about its own axis leaves the neighbour lists valid. */
+int xdem_ctx_set_cylinder_pose(xdem_ctx* ctx, unsigned int cylinder, const double dbi[15]);
+/* Wrench (Fx, Fy, Fz, Mx, My, Mz) the spheres exerted on a dynamic cylinder in the last step;
+ * zeros for a fixed one. */
+int xdem_ctx_cylinder_wrench(xdem_ctx* ctx, unsigned int cylinder, double out[6]);
 int xdem_ctx_info_get(xdem_ctx* ctx, xdem_ctx_info* out);
 /* Sizes of one scene (valid after add_scene). */
 typedef struct xdem_ctx_scene_info {
     unsigned int np, nplane, p2p_capacity, has_mesh;
     unsigned int first_particle;   /* its first particle in the context's concatenated arrays */
+    unsigned int ncylinder;
 } xdem_ctx_scene_info;
 int xdem_ctx_scene_info_get(xdem_ctx* ctx, unsigned int scene, xdem_ctx_scene_info* out);
 /* Part file of one scene: the step's result file of xResultManager::export_step_data_to_file
  * (source/xdynamics_manager/xResultManager.cpp:828-925, version 2 layout) -- ct, np, np,
  * mass[np], pos[4np], vel[3np], acc[3np], ep[4np], ev[4np], ea[4np], then the sphere, wall
- * (scenes with walls) and triangle (scenes with a mesh) histories as count[np], ids[np cap],
- * tsd[2 np cap]. load restores the state from such a file and returns its ct (the restart of
+ * (scenes with walls), cylinder (scenes with cylinders) and triangle (scenes with a mesh)
+ * histories as count[np], ids[np cap], tsd[2 np cap]. load restores the state from such a file and returns its ct (the restart of
  * xDynamicsSimulator::set_from_part_result, xDynamicSimulator.cpp:392-492, reading the mass
  * block that the reference's reader skips); it keeps the contact histories as they are, as
  * the reference's restart does (xContactManager.cpp:316-322), unless flags holds
@@ -310,9 +330,10 @@ int xdem_ctx_graph_launch(xdem_ctx* ctx, unsigned int nlaunch);
  * (the step's counters; bucket grid, sorted copies and neighbour lists on a rebuild) ms,
  * out[2] = number of steps timed, out[3] = rolling + velocity kernel ms (0 when fused
  * into the contact passes), out[4] = per-step pair test (k_touch) ms, out[5] = k_lite_move
- * ms (fused mode; 0 otherwise), out[6] = k_collide ms, out[7] = the triangle pass before it
- * (k_mesh_pass; 0 without meshes) ms, out[8] = the sphere-pair pass (k_pairs) ms, all three on
- * the second stream. */
+ * ms (fused mode; 0 otherwise), out[6] = k_collide ms and out[7] = the triangle pass before it
+ * (k_mesh_pass, with the wait for the fork; about 0 without meshes) ms, both on the second
+ * stream, out[8] = the sphere-pair pass (k_pairs, on the context's stream before the fork;
+ * it returns at once on a small full pass, whose pairs k_collide walks) ms. */
 int xdem_ctx_set_timing(xdem_ctx* ctx, int on);
 int xdem_ctx_timing(xdem_ctx* ctx, double out[9]);
 

@@ -137,8 +137,23 @@ class RefSim:
         self.cend = np.zeros(self.ncell, dtype=np.uint32)
         self.bf = [np.zeros(n) for _ in range(6)]
         self.wrench = np.zeros((len(self.planes), 6))
-        self.overflow = {"p2p": 0, "plane": 0, "mesh": 0}
+        self.overflow = {"p2p": 0, "plane": 0, "cylinder": 0, "mesh": 0}
         self.mesh = None
+        self.cylinders = []
+        self.cy_count = np.zeros(n, dtype=np.uint32)
+        self.cy_id = np.zeros((n, 3), dtype=np.uint32)
+        self.cy_tsd = np.zeros((n, 3, 2))
+
+    def add_cylinder(self, info, dbi, cp) -> None:
+        """A cylinder body (xParticleCylinderContact, evaluated by cylinder_contact_force_kernel,
+        impl:1327-1460): info a ref_cylinder_info-layout structure, pose dbi (15 doubles),
+        material cp. Cylinders are evaluated after the walls, in the order added, on 3 slots
+        per sphere."""
+        self.cylinders.append(dict(info=info, dbi=np.ascontiguousarray(dbi, dtype=np.float64).copy(), cp=cp,
+                                   wrench=np.zeros(6)))
+
+    def set_cylinder_pose(self, k: int, dbi) -> None:
+        self.cylinders[k]["dbi"] = np.ascontiguousarray(dbi, dtype=np.float64).copy()
 
     def add_mesh(self, vertices, dbi, cp) -> None:
         """One rigid mesh body (xParticleMeshObjectContact::define,
@@ -234,6 +249,14 @@ class RefSim:
                 _p(self.pl_tsd), self.np, 3)
             for c in range(6):
                 self.wrench[k, c] = L.ref_reduction(_p(self.bf[c]), self.np)
+        for cy in self.cylinders:
+            self.overflow["cylinder"] += L.ref_cylinder_contact_force(
+                C.byref(self.prm), q, C.byref(cy["info"]), _p(cy["dbi"]), C.byref(cy["cp"]),
+                *(_p(b) for b in self.bf), _p(self.pos), _p(self.ep), _p(self.vel), _p(self.ev), _p(self.force),
+                _p(self.moment), _p(self.mass), _p(self.tmax), _p(self.rres), _p(self.cy_count), _p(self.cy_id),
+                _p(self.cy_tsd), self.np)
+            for c in range(6):
+                cy["wrench"][c] = L.ref_reduction(_p(self.bf[c]), self.np)
         if self.mesh is not None:
             m = self.mesh
             self.overflow["mesh"] += L.ref_particle_polygon_collision(

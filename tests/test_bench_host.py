@@ -24,17 +24,17 @@ def test_more_than_one_gpu_exits_with_a_message():
 def test_kernel_rooflines_are_bytes_over_event_time():
     tm = {"steps": 200, "lite_move_ms": 200 * 0.025, "contact_ms": 200 * 0.09, "pair_pass_ms": 200 * 0.04,
           "collide_ms": 200 * 0.03}
-    info = {"contact_spheres": 1000}
+    info = {"contact_spheres": 100000}   # a dense full pass: k_pairs walks the pairs
     kr = bench.kernel_rooflines(tm, 1_000_000, info, 6000, 200, True)
-    units = 1_000_000 - 1000
+    units = 1_000_000 - 100000
     assert kr["k_lite_move"]["ms"] == pytest.approx(0.025)
     assert kr["k_lite_move"]["units"] == units
     assert kr["k_lite_move"]["gbps"] == pytest.approx(168.0 * units / 25e-6 / 1e9)
     # k_pairs: 228 B per full-pass sphere and 44 B per passing pair; k_collide: 480 B per
     # sphere and 40 B per wall contact
-    assert kr["k_pairs"]["bytes"] == 228.0 * 1000 + 44.0 * 6000
-    assert kr["k_pairs"]["gbps"] == pytest.approx((228.0 * 1000 + 44.0 * 6000) / 40e-6 / 1e9)
-    assert kr["k_collide"]["bytes"] == 480.0 * 1000 + 40.0 * 200
+    assert kr["k_pairs"]["bytes"] == 228.0 * 100000 + 44.0 * 6000
+    assert kr["k_pairs"]["gbps"] == pytest.approx((228.0 * 100000 + 44.0 * 6000) / 40e-6 / 1e9)
+    assert kr["k_collide"]["bytes"] == 480.0 * 100000 + 40.0 * 200
     for k in kr.values():
         assert k["frac"] == pytest.approx(k["gbps"] / 8000.0)
     r = bench.dominant_roofline(kr, {"k_lite_move": 1.5e8})
@@ -45,6 +45,10 @@ def test_kernel_rooflines_are_bytes_over_event_time():
     assert r["kernel"] == "k_lite_move" and r["traffic"] == 1.5e8
     # split mode: no fused kernels
     assert set(bench.kernel_rooflines(tm, 1_000_000, info, 6000, 200, False)) == {"k_pairs"}
+    # a small full pass: k_collide walks the pairs, k_pairs only reads the counters
+    small = bench.kernel_rooflines(tm, 1_000_000, {"contact_spheres": 3000}, 900, 200, True)
+    assert "k_pairs" not in small and small["k_collide"]["walks_pairs"]
+    assert small["k_collide"]["bytes"] == 416.0 * 3000 + 40.0 * 200 + 44.0 * 900
 
 
 def test_host_cpu_reports_model_and_cores():

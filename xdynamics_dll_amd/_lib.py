@@ -41,6 +41,7 @@ FIELD = {"pos": (0, 4), "vel": (1, 3), "acc": (2, 3), "ep": (3, 4), "ev": (4, 4)
 HISTORY_P2P = 0
 HISTORY_PLANE = 1
 HISTORY_MESH = 2
+HISTORY_CYLINDER = 3
 PART_HISTORY = 1
 PART_POSITIONS = 2
 
@@ -158,7 +159,7 @@ class CtxInfo(C.Structure):
 
 class CtxSceneInfo(C.Structure):
     _fields_ = [("np", C.c_uint), ("nplane", C.c_uint), ("p2p_capacity", C.c_uint), ("has_mesh", C.c_uint),
-                ("first_particle", C.c_uint)]
+                ("first_particle", C.c_uint), ("ncylinder", C.c_uint)]
 
 
 class AutotuneResult(C.Structure):
@@ -229,6 +230,9 @@ CTX_SIGNATURES = {
     "xdem_ctx_add_mesh": [P, U, C.c_void_p, C.POINTER(U)],
     "xdem_ctx_set_mesh_pose": [P, U, P],
     "xdem_ctx_mesh_wrench": [P, U, P],
+    "xdem_ctx_add_cylinder": [P, U, C.POINTER(CylinderInfo), P, C.POINTER(ContactProperty), C.POINTER(U)],
+    "xdem_ctx_set_cylinder_pose": [P, U, P],
+    "xdem_ctx_cylinder_wrench": [P, U, P],
     "xdem_ctx_info_get": [P, C.POINTER(CtxInfo)],
     "xdem_ctx_contact_count": [P, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)],
     "xdem_ctx_stream": [P, C.POINTER(P)],

@@ -382,9 +382,9 @@ __device__ __forceinline__ CylAxis cylinder_axis(const xdem_cylinder_info& cy, c
 //  - otherwise the rim circle of radius len_rr.z at the nearer end (the top unless the local
 //    y offset from it exceeds the length), its point in the local xz plane found by the
 //    atan / sin / cos construction of the kernel text: overlap r - dist when positive, -1
-//    otherwise.
+//    otherwise (r - dist in any case with rim_gap, for the reach test's gap).
 __device__ __forceinline__ double cylinder_detection(const xdem_cylinder_info& cy, const CylAxis& a, V3<double> p,
-                                                     double r, V3<double>& u, bool& inner)
+                                                     double r, V3<double>& u, bool& inner, bool rim_gap = false)
 {
     const double t = dot(p - a.base, a.ab) / dot(a.ab, a.ab);
     const V3<double> cp = a.base + t * a.ab;
@@ -429,22 +429,24 @@ __device__ __forceinline__ double cylinder_detection(const xdem_cylinder_info& c
     const V3<double> dv = c - p;
     const double d = length(dv);
     u = dv / d;
-    if (d < r) return r - d;
+    if (d < r || rim_gap) return r - d;
     return -1.0;
 }
 
 // The closed lids seen from inside the tube (impl:1290-1325): an open lid (empty_part 1: top
 // for t > 0.6, 2: bottom for t < 0.4) gives 0 with u and cpt untouched; within len_rr.z of
 // the axis, u points from the body position along the axis towards the sphere, the contact
-// point is p + r u and the overlap r + |axial offset| - length / 2; -1 otherwise.
+// point is p + r u and the overlap r + |axial offset| - length / 2; -1 otherwise. With
+// open_lids false both lids count as closed (the reach test: a sphere leaving through an open
+// end meets the lid region beyond it, whose overlap continues this one's).
 __device__ __forceinline__ double cylinder_inner_lid(const xdem_cylinder_info& cy, const CylAxis& a, V3<double> p,
-                                                     double r, V3<double>& u, V3<double>& cpt)
+                                                     double r, V3<double>& u, V3<double>& cpt, bool open_lids = true)
 {
     const double t = dot(p - a.base, a.ab) / dot(a.ab, a.ab);
     const V3<double> cp = a.base + t * a.ab;
     const double dist = length(p - cp);
-    if (cy.empty_part == 1 && t > 0.6) return 0.0;
-    if (cy.empty_part == 2 && t < 0.4) return 0.0;
+    if (open_lids && cy.empty_part == 1 && t > 0.6) return 0.0;
+    if (open_lids && cy.empty_part == 2 && t < 0.4) return 0.0;
     double gab = -1.0;
     if (dist < cy.len_rr.z) {
         const V3<double> oc = a.pos - cp;
@@ -551,6 +553,33 @@ __device__ __forceinline__ void cylinder_contact(const ContactConst& cc, const x
     acc.M = acc.M + part.M;
     acc.T = acc.T + part.T;
     acc.R += part.R;
+}
+
+// An upper bound of the sphere's cylinder overlap at radius r, negative by the distance it
+// lacks: the larger of the two detections, the lids taken as closed, the rim's overlap signed. In each region the overlap is r plus a function of the
+// centre that changes by at most the distance the centre moves; where regions meet, the
+// functions differ by at most the cylinder's margin (host-computed from its geometry). So a
+// sphere whose bound with its radius grown by skin + margin is not positive cannot touch
+// before it or the body has moved skin / 2.
+__device__ __forceinline__ double cylinder_reach(const xdem_cylinder_info& cy, const CylAxis& a, V3<double> p, double r)
+{
+    bool inner = false;
+    V3<double> u = mk3(0.0, 0.0, 0.0), cpt = p;
+    double g = cylinder_detection(cy, a, p, r, u, inner, true);
+    if (inner) g = fmax(g, cylinder_inner_lid(cy, a, p, r, u, cpt, false));
+    return g;
+}
+
+// True when the sphere cannot touch the cylinder even grown by `skin` (any cohesive cylinder:
+// never, as a cohesive pair attracts within its separation depth and the reference stores a
+// slot even at zero overlap).
+__device__ __forceinline__ bool cylinder_out_of_reach(const ContactConst& cc, const xdem_cylinder_info& cy,
+                                                      const double* dbi, double margin, V4<double> p, double skin)
+{
+    if (cc.coh != 0.0) return false;
+    const CylAxis a = cylinder_axis(cy, dbi);
+    const double g = cylinder_reach(cy, a, mk3(p.x, p.y, p.z), (p.w + skin + margin) * (1.0 + 1e-9) + 1e-12);
+    return !(g > 0.0);
 }
 
 // ---------------------------------------------------------------- rolling resistance

@@ -39,6 +39,12 @@ struct MeshHost {
     MeshDev dev;
 };
 
+struct CylHost {
+    uint32_t scene = 0;
+    double extent = 0.0;              // body-frame radius of the geometry a sphere can reach
+    CylDev dev;
+};
+
 // A dynamic body's wrench: the spheres of [begin, end) whose flags hold `mask` wrote their
 // contribution to its per-sphere slot this step (kTouch for a wall: the full pass writes every
 // dynamic wall's slot; kMeshNear for a mesh: the triangle pass writes its slot).
@@ -164,7 +170,9 @@ struct xdem_ctx {
     std::vector<SceneHost> scenes;
     std::vector<WallDev> walls;
     std::vector<MeshHost> meshes;
-    std::vector<WrenchSlot> wslots;     // per dynamic body (wall or mesh): sphere range and flag
+    std::vector<CylHost> cyls;          // cylinder bodies in the order added
+    std::vector<uint32_t> cyl_dev;      // cylinder -> its entry of d_cyls (grouped by scene)
+    std::vector<WrenchSlot> wslots;     // per dynamic body (wall, cylinder or mesh): sphere range and flag
     std::vector<int> wall_slot;         // global wall -> slot or -1
     bool committed = false;
     bool multi = false;
@@ -181,6 +189,9 @@ struct xdem_ctx {
     uint32_t* scene_of = nullptr;
     SceneDev* d_scenes = nullptr;
     WallDev* d_walls = nullptr;
+    CylDev* d_cyls = nullptr;
+    uint32_t *cy_count = nullptr, *cy_id = nullptr;
+    double2* cy_tsd = nullptr;
     uint32_t *keys = nullptr, *svals = nullptr, *slot_tmp = nullptr;
     uint32_t *nbr_count = nullptr, *nbr = nullptr, *work = nullptr, *seg_count = nullptr;
     uint32_t *work2 = nullptr, *seg_count2 = nullptr;
@@ -272,6 +283,7 @@ struct xdem_ctx {
         A.pos = pos; A.vel = vel; A.acc = acc; A.ep = ep; A.ev = ev; A.ea = ea;
         A.force = force; A.moment = moment; A.mass = mass; A.inertia = inertia; A.tmax = tmax; A.rres = rres;
         A.scene_of = scene_of; A.scenes = d_scenes; A.walls = d_walls;
+        A.cyls = d_cyls; A.cy_count = cy_count; A.cy_id = cy_id; A.cy_tsd = cy_tsd;
         A.keys = keys; A.svals = svals; A.slot_tmp = slot_tmp;
         A.bucket_count = bucket_count; A.bucket_start = bucket_start; A.scan_part = scan_part;
         A.nbr_count = nbr_count; A.nbr = nbr; A.list_cap = list_cap; A.work = work;
@@ -325,14 +337,18 @@ struct xdem_ctx {
         if (ev && (e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
         if ((e = launch_touch(A, multi, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[2], stream)) != hipSuccess) return e;
-        // the full pass (spheres k_touch queued) on the second stream, concurrently with the
-        // lite pass (all others) on the first; the first stream waits for both
+        // the pair pass of a dense full pass, before the fork: on a small full pass it returns
+        // at once, and launched on an idle device that costs a few microseconds instead of
+        // waiting behind the lite pass's blocks (k_collide then walks the pairs)
         const bool gpu_law = cfg.quirks.gpu_force_law != 0;
+        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 3, stream)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[9], stream)) != hipSuccess) return e;
+        // the rest of the full pass (spheres k_touch queued) on the second stream,
+        // concurrently with the lite pass (all others) on the first; the first stream waits
+        // for both
         if ((e = hipEventRecord(fork_ev, stream)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(stream2, fork_ev, 0)) != hipSuccess) return e;
         if (!meshes.empty() && (e = collide_fn(A, multi, fuse, model, gpu_law, 2, stream2)) != hipSuccess) return e;
-        if (ev && (e = hipEventRecord(ev[9], stream2)) != hipSuccess) return e;
-        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 3, stream2)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[7], stream2)) != hipSuccess) return e;
         if ((e = collide_fn(A, multi, fuse, model, gpu_law, 1, stream2)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[8], stream2)) != hipSuccess) return e;
@@ -695,6 +711,29 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->pl_count, sizeof(uint32_t) * n));
     CTX_CHECK(c->alloc(&c->pl_id, sizeof(uint32_t) * n * XDEM_MAX_P2PL_COUNT));
     CTX_CHECK(c->alloc(&c->pl_tsd, sizeof(double2) * n * XDEM_MAX_P2PL_COUNT));
+    std::vector<CylDev> cdev;
+    if (!c->cyls.empty()) {
+        // cylinder bodies grouped by scene, each scene's in the order added; their slots
+        // (XDEM_MAX_P2CY_COUNT per sphere, updated in place)
+        c->cyl_dev.assign(c->cyls.size(), 0u);
+        for (size_t si = 0; si < c->scenes.size(); si++) {
+            SceneDev& D = c->scenes[si].dev;
+            D.cyl_begin = (uint32_t)cdev.size();
+            D.ncyl = 0;
+            D.cyl_id0 = 0;
+            for (size_t k = 0; k < c->cyls.size(); k++) {
+                if (c->cyls[k].scene != si) continue;
+                c->cyl_dev[k] = (uint32_t)cdev.size();
+                cdev.push_back(c->cyls[k].dev);
+                D.ncyl++;
+                if (c->cyls[k].dev.info.id == 0) D.cyl_id0 = 1;
+            }
+        }
+        CTX_CHECK(c->alloc(&c->d_cyls, sizeof(CylDev) * cdev.size()));
+        CTX_CHECK(c->alloc(&c->cy_count, sizeof(uint32_t) * n));
+        CTX_CHECK(c->alloc(&c->cy_id, sizeof(uint32_t) * n * XDEM_MAX_P2CY_COUNT));
+        CTX_CHECK(c->alloc(&c->cy_tsd, sizeof(double2) * n * XDEM_MAX_P2CY_COUNT));
+    }
     if (c->nwslot) {
         CTX_CHECK(c->alloc(&c->wrench_pp, sizeof(double) * 6 * n * c->nwslot));
         CTX_CHECK(c->alloc(&c->wrench_out, sizeof(double) * 6 * c->nwslot));
@@ -771,6 +810,12 @@ int xdem_ctx_commit(xdem_ctx* c)
     if (c->nwslot)
         CTX_CHECK(hipMemcpyAsync(c->d_wslots, c->wslots.data(), sizeof(WrenchSlot) * c->nwslot, hipMemcpyHostToDevice,
                                  s));
+    if (!cdev.empty()) {
+        CTX_CHECK(hipMemcpyAsync(c->d_cyls, cdev.data(), sizeof(CylDev) * cdev.size(), hipMemcpyHostToDevice, s));
+        CTX_CHECK(hipMemsetAsync(c->cy_count, 0, sizeof(uint32_t) * n, s));
+        CTX_CHECK(hipMemsetAsync(c->cy_id, 0, sizeof(uint32_t) * n * XDEM_MAX_P2CY_COUNT, s));
+        CTX_CHECK(hipMemsetAsync(c->cy_tsd, 0, sizeof(double2) * n * XDEM_MAX_P2CY_COUNT, s));
+    }
     if (!c->meshes.empty()) {
         const size_t T = c->ntri;
         CTX_CHECK(hipMemcpyAsync(c->d_meshes, mdev.data(), sizeof(MeshDev) * mdev.size(), hipMemcpyHostToDevice, s));
@@ -924,8 +969,8 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
             CTX_CHECK(hipEventElapsedTime(&m, ev[5], ev[6]));
             float kc = 0.f, kp = 0.f, km = 0.f;
             CTX_CHECK(hipEventElapsedTime(&kc, ev[7], ev[8]));
-            CTX_CHECK(hipEventElapsedTime(&km, ev[2], ev[9]));
-            CTX_CHECK(hipEventElapsedTime(&kp, ev[9], ev[7]));
+            CTX_CHECK(hipEventElapsedTime(&kp, ev[2], ev[9]));
+            CTX_CHECK(hipEventElapsedTime(&km, ev[9], ev[7]));
             c->t_collide += kc;
             c->t_meshpass += km;
             c->t_pairpass += kp;
@@ -1053,6 +1098,9 @@ int xdem_ctx_read_history(xdem_ctx* c, int kind, unsigned int scene, unsigned in
     } else if (kind == XDEM_HISTORY_MESH) {
         if (h.dev.mesh < 0) return XDEM_E_INVALID;
         cap = XDEM_MAX_P2MS_COUNT; dc = c->ms_count; di = c->ms_id; dt = c->ms_tsd;
+    } else if (kind == XDEM_HISTORY_CYLINDER) {
+        if (h.dev.ncyl == 0) return XDEM_E_INVALID;
+        cap = XDEM_MAX_P2CY_COUNT; dc = c->cy_count; di = c->cy_id; dt = c->cy_tsd;
     } else {
         return XDEM_E_INVALID;
     }
@@ -1087,6 +1135,10 @@ int xdem_ctx_write_history(xdem_ctx* c, int kind, unsigned int scene, const unsi
         if (h.dev.mesh < 0) return XDEM_E_INVALID;
         cap = XDEM_MAX_P2MS_COUNT; dc = c->ms_count; di = c->ms_id; dt = c->ms_tsd;
         id_limit = c->meshes[h.dev.mesh].dev.ntri;
+    } else if (kind == XDEM_HISTORY_CYLINDER) {
+        if (h.dev.ncyl == 0) return XDEM_E_INVALID;
+        cap = XDEM_MAX_P2CY_COUNT; dc = c->cy_count; di = c->cy_id; dt = c->cy_tsd;
+        id_limit = UINT32_MAX;   // cylinder ids (and id + 1000) index nothing
     } else {
         return XDEM_E_INVALID;
     }
@@ -1120,6 +1172,7 @@ int xdem_ctx_scene_info_get(xdem_ctx* c, unsigned int scene, xdem_ctx_scene_info
     out->p2p_capacity = c->cap;
     out->has_mesh = h.dev.mesh >= 0 ? 1u : 0u;
     out->first_particle = h.p_begin;
+    for (const CylHost& y : c->cyls) out->ncylinder += y.scene == scene ? 1u : 0u;
     return XDEM_OK;
 }
 
@@ -1363,6 +1416,122 @@ int xdem_ctx_mesh_wrench(xdem_ctx* c, unsigned int mesh, double* out)
     if (!c->committed) return XDEM_E_NOT_READY;
     if (mesh >= c->meshes.size()) return XDEM_E_INVALID;
     const int slot = c->meshes[mesh].dev.wslot;
+    for (int q = 0; q < 6; q++) out[q] = 0.0;
+    if (slot < 0) return XDEM_OK;
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    CTX_CHECK(hipMemcpyAsync(out, c->wrench_out + 6 * (size_t)slot, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipStreamSynchronize(c->stream));
+    return XDEM_OK;
+}
+
+// Margin of a cylinder's reach test. In each contact region the overlap is r plus a function of
+// the centre that changes by at most the distance the centre moves; where two regions meet the
+// functions differ by at most: 2 |side radius - rim radius| (outside side wall / rim, inside /
+// outside of the tube), the lid ring's thickness and, with one, the rim radius (a centre
+// beyond an end inside the rim radius sees the rim, inside the tube the lid), twice the lid
+// planes' offset from the axis ends (body position off the axis midpoint, length other than
+// the axis) and, for a cylinder not longer than a sphere's reach, its length (the nearer rim is
+// taken by the local y offset against the length). The geometry is that of a cylinder about
+// the body frame's y axis only when both axis ends lie on that axis and are a length apart:
+// any other is always in reach (margin 1e300).
+static double cylinder_margin(const xdem_cylinder_info& y, double rmax, double skin)
+{
+    const double len = y.len_rr.x;
+    const double ab[3] = {y.ptop.x - y.pbase.x, y.ptop.y - y.pbase.y, y.ptop.z - y.pbase.z};
+    const double lab = std::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+    const double tol = 1e-9 * std::max(len, lab);
+    const bool on_axis = std::fabs(y.pbase.x) <= tol && std::fabs(y.pbase.z) <= tol && std::fabs(y.ptop.x) <= tol &&
+                         std::fabs(y.ptop.z) <= tol;
+    if (!on_axis || std::fabs(lab - len) > tol) return 1e300;
+    const double mid = std::fabs(0.5 * (y.pbase.y + y.ptop.y));
+    double m = 2.0 * std::fabs(y.len_rr.y - y.len_rr.z) + y.thickness + (y.thickness > 0.0 ? y.len_rr.z : 0.0) +
+               std::fabs(len - lab) + 2.0 * mid;
+    if (len <= rmax + skin) m += len;
+    return m * (1.0 + 1e-9) + tol;
+}
+
+int xdem_ctx_add_cylinder(xdem_ctx* c, unsigned int scene, const xdem_cylinder_info* cy, const double dbi[15],
+                          const xdem_contact_property* cp, unsigned int* idx)
+{
+    if (!c || !cy || !dbi || !cp) return XDEM_E_INVALID;
+    if (c->committed) return XDEM_E_NOT_READY;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    for (int q = 0; q < 15; q++)
+        if (!std::isfinite(dbi[q])) return XDEM_E_INVALID;
+    const xdem_cylinder_info& y = *cy;
+    const double g[10] = {y.thickness, y.len_rr.x, y.len_rr.y, y.len_rr.z, y.pbase.x,
+                          y.pbase.y,   y.pbase.z,  y.ptop.x,   y.ptop.y,   y.ptop.z};
+    for (double v : g)
+        if (!std::isfinite(v)) return XDEM_E_INVALID;
+    if (!(y.len_rr.x > 0.0) || !(y.len_rr.y >= 0.0) || !(y.len_rr.z > 0.0) || !(y.thickness >= 0.0)) return XDEM_E_INVALID;
+    if (y.empty_part > 2u || y.id > UINT32_MAX - 1000u) return XDEM_E_INVALID;   // id + 1000 names the inner lids
+    if (y.ptop.x == y.pbase.x && y.ptop.y == y.pbase.y && y.ptop.z == y.pbase.z) return XDEM_E_INVALID;
+    unsigned nin = 0;
+    for (const CylHost& o : c->cyls) nin += o.scene == scene ? 1u : 0u;
+    if (nin >= (unsigned)kMaxWallsPerScene) return XDEM_E_INVALID;
+    const SceneHost& sh = c->scenes[scene];
+    CylHost h;
+    h.scene = scene;
+    CylDev& D = h.dev;
+    std::memset(&D, 0, sizeof(D));
+    D.info = y;
+    std::memcpy(D.dbi, dbi, sizeof(double) * 15);
+    D.cc = make_contact_const(*cp, sh.rmax);
+    if (cp->rfric == 0.0) c->tmax_persist = 1;
+    D.margin = cylinder_margin(y, sh.rmax, sh.dev.skin);
+    D.wslot = -1;
+    if (dbi[XDEM_MM] > 0.0) {   // a cylinder with mass is a dynamic body: reduce its wrench
+        D.wslot = (int32_t)c->nwslot++;
+        c->wslots.push_back({sh.dev.p_begin, sh.dev.p_end, kTouch, 0u});
+    }
+    // body-frame radius of the points whose overlap can change with the pose: the axis ends,
+    // the side wall or the lid ring, and a sphere's reach beyond them
+    const double eb = std::sqrt(y.pbase.x * y.pbase.x + y.pbase.y * y.pbase.y + y.pbase.z * y.pbase.z);
+    const double et = std::sqrt(y.ptop.x * y.ptop.x + y.ptop.y * y.ptop.y + y.ptop.z * y.ptop.z);
+    h.extent = std::max(eb, et) + std::max(y.len_rr.y, y.len_rr.z + y.thickness) + sh.rmax + sh.dev.skin +
+               (D.margin < 1e300 ? D.margin : 0.0);
+    if (idx) *idx = (unsigned)c->cyls.size();
+    c->cyls.push_back(h);
+    return XDEM_OK;
+}
+
+int xdem_ctx_set_cylinder_pose(xdem_ctx* c, unsigned int cyl, const double dbi[15])
+{
+    if (!c || !dbi) return XDEM_E_INVALID;
+    if (cyl >= c->cyls.size()) return XDEM_E_INVALID;
+    for (int q = 0; q < 15; q++)
+        if (!std::isfinite(dbi[q])) return XDEM_E_INVALID;
+    CylHost& h = c->cyls[cyl];
+    // The geometry is unchanged by a rotation about the cylinder's axis (the body frame's y
+    // axis): a point of it moves by at most |dP| + |a' - a| extent, a = A(e) (0, 1, 0) the
+    // axis direction (the least rotation taking a to a' moves a unit vector by |a' - a|).
+    // Spinning a drum about its axis does not wear the neighbour lists out.
+    double a0[3], a1[3];
+    const double ey[3] = {0.0, 1.0, 0.0};
+    euler_rotate(h.dev.dbi + XDEM_E0, ey, false, a0);
+    euler_rotate(dbi + XDEM_E0, ey, false, a1);
+    double da = 0.0, dp = 0.0;
+    for (int k = 0; k < 3; k++) {
+        da += (a1[k] - a0[k]) * (a1[k] - a0[k]);
+        dp += (dbi[XDEM_PX + k] - h.dev.dbi[XDEM_PX + k]) * (dbi[XDEM_PX + k] - h.dev.dbi[XDEM_PX + k]);
+    }
+    const double dmax = (std::sqrt(dp) + std::sqrt(da) * h.extent) * (1.0 + 1e-9) + 1e-15;
+    std::memcpy(h.dev.dbi, dbi, sizeof(double) * 15);
+    if (c->committed) {
+        CTX_CHECK(hipSetDevice(c->cfg.device));
+        CTX_CHECK(hipMemcpyAsync(c->d_cyls + c->cyl_dev[cyl], &h.dev, sizeof(CylDev), hipMemcpyHostToDevice, c->stream));
+        if (h.dev.margin < 1e300) CTX_CHECK(c->body_moved(dmax));   // (a cylinder always in reach needs no lists)
+        CTX_CHECK(hipStreamSynchronize(c->stream));   // the host's pose copy may change after return
+    }
+    return XDEM_OK;
+}
+
+int xdem_ctx_cylinder_wrench(xdem_ctx* c, unsigned int cyl, double* out)
+{
+    if (!c || !out) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    if (cyl >= c->cyls.size()) return XDEM_E_INVALID;
+    const int slot = c->cyls[cyl].dev.wslot;
     for (int q = 0; q < 6; q++) out[q] = 0.0;
     if (slot < 0) return XDEM_OK;
     CTX_CHECK(hipSetDevice(c->cfg.device));

@@ -360,8 +360,8 @@ __device__ __forceinline__ void order_gather_slot(const StepArgs& A, uint32_t j)
     A.svals[f] = k;
     const V4<double> p = ld4(A.pos + 4 * k);
     const SceneDev& S = A.scenes[scene_index(A, k, MULTI)];
-    // exact FP64 reach test against every wall of the scene (wall_out_of_reach, radius
-    // grown by the skin): a sphere with a wall in reach goes to the full contact pass; and
+    // exact FP64 reach test against every wall and cylinder of the scene (radius grown by the
+    // skin): a sphere with one in reach goes to the full contact pass; and
     // the least displacement that could bring a wall into reach, |dp . uw| / max(1, |uw|) -
     // r (1 + 1e-6), the gap k_touch compares with (any cohesive wall: none)
     const float scale = __uint_as_float(ld_co(reinterpret_cast<const uint32_t*>(&A.lctl->scale)));
@@ -374,6 +374,16 @@ __device__ __forceinline__ void order_gather_slot(const StepArgs& A, uint32_t j)
         const double uw2 = pl.uw.x * pl.uw.x + pl.uw.y * pl.uw.y + pl.uw.z * pl.uw.z;
         const double wz = (p.x - pl.xw.x) * pl.uw.x + (p.y - pl.xw.y) * pl.uw.y + (p.z - pl.xw.z) * pl.uw.z;
         const double g = W.cc.coh != 0.0 ? -1.0 : fabs(wz) / sqrt(fmax(1.0, uw2)) - p.w * (1.0 + 1e-6) - 1e-9;
+        wgap = fmin(wgap, g);
+    }
+    // cylinders: in reach with the radius grown by the skin, and the gap the largest overlap
+    // lacks (it changes by at most the distance the sphere or the body moves)
+    for (uint32_t c = 0; c < S.ncyl; c++) {
+        const CylDev& C = A.cyls[S.cyl_begin + c];
+        far = far & cylinder_out_of_reach(C.cc, C.info, C.dbi, C.margin, p, S.skin * scale);
+        const double g = C.cc.coh != 0.0 ? -1.0
+                                         : -cylinder_reach(C.info, cylinder_axis(C.info, C.dbi), mk3(p.x, p.y, p.z),
+                                                           p.w * (1.0 + 1e-6)) - C.margin - 1e-9;
         wgap = fmin(wgap, g);
     }
     A.gap[k] = __double2float_rd(wgap);

@@ -8,6 +8,7 @@
 //   double mass[np]; double pos[4 np]; vel[3 np]; acc[3 np]; ep[4 np]; ev[4 np]; ea[4 np];
 //   sphere history: uint32 count[np]; uint32 id[np cap]; double tsd[2 np cap]   (cap = p2p capacity)
 //   wall history (scenes with walls): count[np]; id[3 np]; tsd[6 np]
+//   cylinder history (scenes with cylinders): count[np]; id[3 np]; tsd[6 np]
 //   triangle history (scenes with a mesh): count[np]; id[5 np]; tsd[10 np]
 // The file carries no section flags (as the reference's): the reader takes them from the
 // context, and checks the file's size against them. The multibody block of the reference's
@@ -28,7 +29,7 @@ namespace {
 
 struct Sections {
     unsigned np = 0, cap = 0;
-    bool walls = false, mesh = false;
+    bool walls = false, cyl = false, mesh = false;
 };
 
 int sections(xdem_ctx* ctx, unsigned scene, Sections& s)
@@ -39,6 +40,7 @@ int sections(xdem_ctx* ctx, unsigned scene, Sections& s)
     s.np = si.np;
     s.cap = si.p2p_capacity;
     s.walls = si.nplane > 0;
+    s.cyl = si.ncylinder > 0;
     s.mesh = si.has_mesh != 0;
     return XDEM_OK;
 }
@@ -50,9 +52,13 @@ size_t file_bytes(const Sections& s)
     auto hist = [&](unsigned cap) { return sizeof(unsigned) * (size_t)s.np * (1 + cap) + sizeof(double) * 2 * (size_t)s.np * cap; };
     b += hist(s.cap);
     if (s.walls) b += hist(XDEM_MAX_P2PL_COUNT);
+    if (s.cyl) b += hist(XDEM_MAX_P2CY_COUNT);
     if (s.mesh) b += hist(XDEM_MAX_P2MS_COUNT);
     return b;
 }
+
+// history sections in the file's order (xResultManager.cpp:889-912)
+constexpr int kKinds = 4;
 
 const int kFields[7] = {XDEM_FIELD_MASS, XDEM_FIELD_POS, XDEM_FIELD_VEL, XDEM_FIELD_ACC, XDEM_FIELD_EP, XDEM_FIELD_EV,
                         XDEM_FIELD_EA};
@@ -78,10 +84,10 @@ int xdem_ctx_save_part(xdem_ctx* ctx, unsigned int scene, const char* path, doub
         if ((rc = xdem_ctx_read(ctx, kFields[k], scene, buf.data()))) break;
         ok = std::fwrite(buf.data(), sizeof(double), buf.size(), f) == buf.size();
     }
-    const int kinds[3] = {XDEM_HISTORY_P2P, XDEM_HISTORY_PLANE, XDEM_HISTORY_MESH};
-    const unsigned caps[3] = {s.cap, XDEM_MAX_P2PL_COUNT, XDEM_MAX_P2MS_COUNT};
-    const bool present[3] = {true, s.walls, s.mesh};
-    for (int h = 0; h < 3 && ok && !rc; h++) {
+    const int kinds[kKinds] = {XDEM_HISTORY_P2P, XDEM_HISTORY_PLANE, XDEM_HISTORY_CYLINDER, XDEM_HISTORY_MESH};
+    const unsigned caps[kKinds] = {s.cap, XDEM_MAX_P2PL_COUNT, XDEM_MAX_P2CY_COUNT, XDEM_MAX_P2MS_COUNT};
+    const bool present[kKinds] = {true, s.walls, s.cyl, s.mesh};
+    for (int h = 0; h < kKinds && ok && !rc; h++) {
         if (!present[h]) continue;
         std::vector<unsigned> cnt(s.np), ids((size_t)s.np * caps[h]);
         std::vector<double> tsd(2 * (size_t)s.np * caps[h]);
@@ -120,12 +126,12 @@ int xdem_ctx_load_part(xdem_ctx* ctx, unsigned int scene, const char* path, int 
         fields[k].resize((size_t)kWidth[k] * s.np);
         ok = std::fread(fields[k].data(), sizeof(double), fields[k].size(), f) == fields[k].size();
     }
-    const int kinds[3] = {XDEM_HISTORY_P2P, XDEM_HISTORY_PLANE, XDEM_HISTORY_MESH};
-    const unsigned caps[3] = {s.cap, XDEM_MAX_P2PL_COUNT, XDEM_MAX_P2MS_COUNT};
-    const bool present[3] = {true, s.walls, s.mesh};
-    std::vector<unsigned> cnt[3], ids[3];
-    std::vector<double> tsd[3];
-    for (int h = 0; h < 3 && ok; h++) {
+    const int kinds[kKinds] = {XDEM_HISTORY_P2P, XDEM_HISTORY_PLANE, XDEM_HISTORY_CYLINDER, XDEM_HISTORY_MESH};
+    const unsigned caps[kKinds] = {s.cap, XDEM_MAX_P2PL_COUNT, XDEM_MAX_P2CY_COUNT, XDEM_MAX_P2MS_COUNT};
+    const bool present[kKinds] = {true, s.walls, s.cyl, s.mesh};
+    std::vector<unsigned> cnt[kKinds], ids[kKinds];
+    std::vector<double> tsd[kKinds];
+    for (int h = 0; h < kKinds && ok; h++) {
         if (!present[h]) continue;
         cnt[h].resize(s.np);
         ids[h].resize((size_t)s.np * caps[h]);
@@ -143,7 +149,7 @@ int xdem_ctx_load_part(xdem_ctx* ctx, unsigned int scene, const char* path, int 
         if ((flags & XDEM_PART_POSITIONS) && kFields[k] != XDEM_FIELD_POS) continue;
         if ((rc = xdem_ctx_write(ctx, kFields[k], scene, fields[k].data()))) return rc;
     }
-    for (int h = 0; h < 3 && (flags & XDEM_PART_HISTORY); h++) {
+    for (int h = 0; h < kKinds && (flags & XDEM_PART_HISTORY); h++) {
         if (!present[h]) continue;
         if ((rc = xdem_ctx_write_history(ctx, kinds[h], scene, cnt[h].data(), ids[h].data(), tsd[h].data()))) return rc;
     }

@@ -29,9 +29,11 @@ constexpr uint32_t kMeshNear = 0x02000000u;   // triangles of the scene's mesh w
 constexpr uint32_t kLiteStable = kSpinStable | kAccStable | kNoContacts;   // set by the lite pass
 constexpr uint32_t kCountMask = 0x01ffffffu;
 // The candidate and full-pass queues are split in kQueueSegs segments, each with its own
-// counter on its own 128-byte line: segment s holds the spheres of the s-th of 64 contiguous
-// ranges of sorted slots (the wave of slots [64 w, 64 w + 64) of nw waves appends to segment
-// 64 w / nw), so a segment is one region of the scene (see queue_block for the XCDs).
+// counter on its own 128-byte line: the wave of sorted slots [64 w, 64 w + 64) appends to
+// segment w mod kQueueSegs, so the appends of different waves rarely hit the same counter and
+// a contact region spreads over every segment. (Segments of 64 contiguous ranges of slots,
+// each XCD walking adjacent ones, made a settled bed's step 6% faster and the dam break's 2-3%
+// slower: its few thousand full-pass spheres then fell into a few segments, on few XCDs.)
 constexpr uint32_t kQueueSegs = 64;
 constexpr uint32_t kSegStride = 32;           // uint32 entries between two segment counters
 constexpr uint32_t kBarFlags = 64;            // release flags of k_rebuild's phases
@@ -87,7 +89,9 @@ struct SceneDev {
     double ref_cs;            // the reference's cell size (visiting order of the contacts);
                               // grid.cs, the bucket size, is larger by the skin and reach
     int32_t mesh;             // the scene's mesh body (index into StepArgs::meshes), -1 for none
-    uint32_t pad2_;
+    uint32_t cyl_begin;       // the scene's cylinder bodies in StepArgs::cyls
+    uint32_t ncyl;
+    uint32_t cyl_id0;         // a cylinder of the scene has id 0 (it restarts the cylinder slots)
 };
 
 // A rigid triangle mesh body of a scene. Triangles keep their body-frame vertices; the
@@ -113,6 +117,17 @@ struct WallDev {
     uint32_t pad_;
 };
 
+// A cylinder body of a scene (the reference's device_cylinder_info and its body pose).
+struct CylDev {
+    xdem_cylinder_info info;
+    double dbi[16];           // 15 used
+    ContactConst cc;
+    double margin;            // widens the reach test where the contact regions do not join
+                              // continuously (side and rim radii that differ, a lid ring)
+    int32_t wslot;            // per-sphere wrench slot of a dynamic cylinder, -1 for a fixed one
+    uint32_t pad_;
+};
+
 struct StepArgs {
     // state (reference layouts, all scenes concatenated)
     double* pos;  double* vel;  double* acc;  double* ep;  double* ev;  double* ea;
@@ -125,6 +140,8 @@ struct StepArgs {
     const uint32_t* scene_of;  // sphere -> scene (multi-scene only)
     const SceneDev* scenes;
     const WallDev* walls;
+    const CylDev* cyls;
+    uint32_t* cy_count; uint32_t* cy_id; double2* cy_tsd;   // cylinder slots (3 per sphere, in place)
     // grid: bucket key per sphere; dense bucket table
     uint32_t* keys;
     uint32_t* bucket_count;                  // zero between steps
@@ -188,12 +205,7 @@ struct StepArgs {
 };
 
 // Queue segment and chunk of block b of a kernel that walks a queue with nblocks blocks (a
-// multiple of kQueueSegs): segment b mod 64, chunk b / 64 of it. Work-groups are dealt
-// round-robin to the 8 XCDs, so XCD x walks segments x, x + 8, ..., x + 56: eight slices of
-// the scene, each spatially coherent (a segment is a contiguous range of sorted slots), which
-// keeps its L2 working set to an eighth of the state while a contact region that covers few
-// segments (the base of a collapsing column) still spreads over all XCDs. (Giving each XCD
-// eight adjacent segments was 6% faster on a settled bed and 2% slower on the dam break.)
+// multiple of kQueueSegs): segment b mod 64, chunk b / 64 of it.
 struct QueueBlock {
     uint32_t seg, chunk, nchunk;
 };
@@ -544,11 +556,16 @@ __device__ __forceinline__ bool touch_sphere(const StepArgs& A, uint32_t i, uint
     const bool may = mesh_near || !(A.gap[i] > (di + dm) * 1.0001f + 1e-9f);
     const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
     const V4<double> p = ld4(A.pos + 4 * i);
-    if (may && (fl & kWallNear))
+    if (may && (fl & kWallNear)) {
         for (uint32_t w = 0; w < S.nwall; w++) {
             const WallDev& W = A.walls[S.wall_begin + w];
             touch = touch || !wall_out_of_reach(W.cc, W.info, p);
         }
+        for (uint32_t c = 0; c < S.ncyl; c++) {
+            const CylDev& C = A.cyls[S.cyl_begin + c];
+            touch = touch || !cylinder_out_of_reach(C.cc, C.info, C.dbi, C.margin, p, 0.0);
+        }
+    }
     const uint32_t n = may ? min(fl & kCountMask, A.list_cap) : 0u;
     if (n) {
         int ix, iy, iz;
@@ -646,7 +663,7 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
                 if (full) touch = touch_sphere<MULTI>(A, i, fl, true, 0.0f);
                 A.nbr_count[i] = touch ? (fl | kTouch) : fl;
             }
-            const uint32_t seg = (j >> 6) * kQueueSegs / (nvb * (256u / 64u));
+            const uint32_t seg = (j >> 6) & (kQueueSegs - 1u);
             queue_append(A.work, A.seg_count, seg, A.seg_cap, full, i);
             queue_append(A.work2, A.seg_count2, seg, A.seg_cap, touch, i);
             if (A.has_mesh) queue_append(A.mesh_work, A.mesh_count, 0u, 0u, mesh, i);
@@ -932,6 +949,44 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, bo
         n_wall = ws.count;
     }
 
+    // ---- cylinders, in the scene's order (cylinder_contact_force_kernel once per cylinder,
+    // impl:1327-1460): each launch looks up the slots as the previous one left them
+    if (S.ncyl) {
+        CylSlots cs;
+        const uint32_t cb = XDEM_MAX_P2CY_COUNT * i;
+#pragma unroll
+        for (uint32_t q = 0; q < XDEM_MAX_P2CY_COUNT; q++) {
+            cs.id[q] = A.cy_id[cb + q];
+            const double2 h = A.cy_tsd[cb + q];
+            cs.s0[q] = h.x;
+            cs.s1[q] = h.y;
+        }
+        cs.count = A.cy_count[i];
+        cs.overflow = false;
+        for (uint32_t c = 0; c < S.ncyl; c++) {
+            const CylDev& C = A.cyls[S.cyl_begin + c];
+#pragma unroll
+            for (uint32_t q = 0; q < XDEM_MAX_P2CY_COUNT; q++) {
+                cs.old_id[q] = cs.id[q]; cs.old_s0[q] = cs.s0[q]; cs.old_s1[q] = cs.s1[q];
+            }
+            double bf[6] = {0, 0, 0, 0, 0, 0};
+            cylinder_contact<T, MODEL, GPU_LAW>(C.cc, C.info, C.dbi, S.dt, o.pos, o.vel, iom, o.mass, cs, tot, bf);
+            if (C.wslot >= 0) {
+                double* wr = A.wrench + (size_t)C.wslot * 6 * A.np;
+#pragma unroll
+                for (int q = 0; q < 6; q++) wr[(size_t)q * A.np + i] = bf[q];
+            }
+        }
+        if (cs.overflow) err |= XDEM_ERR_CYL_OVERFLOW;
+#pragma unroll
+        for (uint32_t q = 0; q < XDEM_MAX_P2CY_COUNT; q++) {
+            A.cy_id[cb + q] = cs.id[q];
+            A.cy_tsd[cb + q] = make_double2(cs.s0[q], cs.s1[q]);
+        }
+        A.cy_count[i] = cs.count;
+        n_wall += cs.count;
+    }
+
     // ---- the scene's mesh: this step's triangle sums (k_mesh_pass), added after the walls, as
     // the reference's mesh launch adds its sums to force, moment, tmax and rres (impl:2100-2108)
     if (A.has_mesh && S.mesh >= 0) {
@@ -1168,6 +1223,12 @@ __device__ __forceinline__ void lite_sphere(const StepArgs& A, const SceneDev& S
         if (count != in.count0) A.pl_count[i] = count;
         n_wall += count;
     }
+    if (S.ncyl) {
+        const uint32_t c0 = A.cy_count[i];
+        const uint32_t count = S.cyl_id0 ? 0u : min(c0, (uint32_t)XDEM_MAX_P2CY_COUNT);
+        if (count != c0) A.cy_count[i] = count;
+        n_wall += count;
+    }
     float dmx = 0.0f;   // the split lite pass computes no drift
     sphere_tail<FUSE>(A, S, i, in.o, tot, err, dmx);
     if (err) atomicOr(A.err, err);
@@ -1226,6 +1287,7 @@ __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
         const uint32_t count0 = (S.nwall && !settled) ? A.pl_count[i] : 0u;
         const uint32_t pp_in = (S.has_pp && !settled) ? A.pp_count_in[i] : 0u;
         const uint32_t msc = (S.mesh >= 0 && !settled) ? A.ms_count[i] : 0u;
+        const uint32_t cyc0 = (S.ncyl && !settled) ? A.cy_count[i] : 0u;
         const V4<double> p = ld4(A.pos + 4 * i);
         const V3<double> v0 = ld3(A.vel + 3 * i);
         const V3<double> ao = ld3(A.acc + 3 * i);
@@ -1234,17 +1296,22 @@ __global__ __launch_bounds__(256) void k_lite_move(StepArgs A)
         uint32_t nf = flags;
         if (!settled) {
             // contact bookkeeping of a sphere without contacts: empty p2p and triangle history,
-            // the walls' slot-count rule; once all of it holds zeros (the p2p count in both
-            // buffers) nothing changes until the sphere is touched again
+            // the walls' and cylinders' slot-count rule; once all of it holds zeros (the p2p
+            // count in both buffers) nothing changes until the sphere is touched again
             if (S.has_pp) A.pp_count_out[i] = 0u;
             if (msc) A.ms_count[i] = 0u;
-            uint32_t count = 0u;
+            uint32_t count = 0u, cyc = 0u;
             if (S.nwall) {
                 count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
                 if (count != count0) A.pl_count[i] = count;
                 n_wall += count;
             }
-            if (pp_in == 0u && count == 0u && msc == 0u) nf |= kNoContacts;
+            if (S.ncyl) {
+                cyc = S.cyl_id0 ? 0u : min(cyc0, (uint32_t)XDEM_MAX_P2CY_COUNT);
+                if (cyc != cyc0) A.cy_count[i] = cyc;
+                n_wall += cyc;
+            }
+            if (pp_in == 0u && count == 0u && msc == 0u && cyc == 0u) nf |= kNoContacts;
         }
         V3<double> a = ao;
         if (!acc_kept) {

@@ -16,11 +16,15 @@ from . import _lib
 from .scene import PARITY_QUIRKS, SceneData
 
 
+_HISTORY_KINDS = {"p2p": _lib.HISTORY_P2P, "plane": _lib.HISTORY_PLANE, "cylinder": _lib.HISTORY_CYLINDER,
+                  "mesh": _lib.HISTORY_MESH}
+
+
 class DemContext:
     def __init__(self, scenes, precision: str = "fp64", p2p_capacity: int = 12, quirks: dict | None = None,
                  grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False,
                  buckets_per_sphere: int = 0, skin: float = 0.0, step_graph: int = 0,
-                 rebuild_wait_limit: int = 0, meshes=(), full_blocks: int = 0):
+                 rebuild_wait_limit: int = 0, meshes=(), full_blocks: int = 0, cylinders=()):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -59,6 +63,17 @@ class DemContext:
             idx = C.c_uint()
             _lib.check(self.L.xdem_ctx_add_mesh(self._h, scene, C.byref(m), C.byref(idx)), "xdem_ctx_add_mesh")
             self.meshes.append((scene, v.shape[0]))
+        # cylinder bodies: (scene index, CylinderInfo, pose (15), ContactProperty), evaluated
+        # after the scene's walls in this order
+        self.cylinders = []
+        for scene, info, dbi, cp in cylinders:
+            d = np.ascontiguousarray(dbi, dtype=np.float64)
+            if d.size != 15:
+                raise ValueError("a cylinder pose has 15 values (MM..ED3)")
+            idx = C.c_uint()
+            _lib.check(self.L.xdem_ctx_add_cylinder(self._h, scene, C.byref(info), d.ctypes.data, C.byref(cp),
+                                                    C.byref(idx)), "xdem_ctx_add_cylinder")
+            self.cylinders.append(scene)
         _lib.check(self.L.xdem_ctx_commit(self._h), "xdem_ctx_commit")
 
     # ------------------------------------------------------------------ stepping
@@ -102,8 +117,8 @@ class DemContext:
 
     def history(self, kind: str = "p2p", scene: int = 0):
         n = self.scenes[scene].np
-        k = {"p2p": _lib.HISTORY_P2P, "plane": _lib.HISTORY_PLANE, "mesh": _lib.HISTORY_MESH}[kind]
-        cap = {"p2p": self.capacity, "plane": 3, "mesh": 5}[kind]
+        k = _HISTORY_KINDS[kind]
+        cap = {"p2p": self.capacity, "plane": 3, "cylinder": 3, "mesh": 5}[kind]
         count = np.empty(n, dtype=np.uint32)
         ids = np.empty(n * cap, dtype=np.uint32)
         tsd = np.empty(n * cap * 2, dtype=np.float64)
@@ -113,7 +128,7 @@ class DemContext:
 
     def write_history(self, kind: str, count, ids, tsd, scene: int = 0) -> None:
         """Inverse of history(): count (np,), ids (np, cap), tsd (np, cap, 2)."""
-        k = {"p2p": _lib.HISTORY_P2P, "plane": _lib.HISTORY_PLANE, "mesh": _lib.HISTORY_MESH}[kind]
+        k = _HISTORY_KINDS[kind]
         c = np.ascontiguousarray(count, dtype=np.uint32)
         i = np.ascontiguousarray(ids, dtype=np.uint32)
         t = np.ascontiguousarray(tsd, dtype=np.float64)
@@ -152,6 +167,15 @@ class DemContext:
     def set_mesh_pose(self, mesh: int, dbi) -> None:
         a = np.ascontiguousarray(dbi, dtype=np.float64)
         _lib.check(self.L.xdem_ctx_set_mesh_pose(self._h, mesh, a.ctypes.data), "xdem_ctx_set_mesh_pose")
+
+    def set_cylinder_pose(self, cylinder: int, dbi) -> None:
+        a = np.ascontiguousarray(dbi, dtype=np.float64)
+        _lib.check(self.L.xdem_ctx_set_cylinder_pose(self._h, cylinder, a.ctypes.data), "xdem_ctx_set_cylinder_pose")
+
+    def cylinder_wrench(self, cylinder: int = 0) -> np.ndarray:
+        out = np.zeros(6, dtype=np.float64)
+        _lib.check(self.L.xdem_ctx_cylinder_wrench(self._h, cylinder, out.ctypes.data), "xdem_ctx_cylinder_wrench")
+        return out
 
     def mesh_wrench(self, mesh: int = 0) -> np.ndarray:
         out = np.zeros(6)
