@@ -109,12 +109,13 @@ def dominant_roofline(kr, traffic=None):
 
 
 def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_sphere=0, skin=0.0, graph=0,
-            full_blocks=0):
+            full_blocks=0, merge_below=0):
     import torch
     from xdynamics_dll_amd import DemContext
 
     ctx = DemContext(scene, precision=precision, p2p_capacity=capacity, fuse_velocity=fuse,
-                     buckets_per_sphere=buckets_per_sphere, skin=skin, full_blocks=full_blocks)
+                     buckets_per_sphere=buckets_per_sphere, skin=skin, full_blocks=full_blocks,
+                     merge_pairs_below=merge_below)
     ctx.step(warmup)
     ctx.synchronize()
     torch.cuda.synchronize()
@@ -595,6 +596,8 @@ def main():
     ap.add_argument("--settle", type=int, default=20000, help="untimed settling steps of the wheel's bed")
     ap.add_argument("--full-blocks", type=int, default=0,
                     help="blocks of the full-pass kernels (0 = library default)")
+    ap.add_argument("--merge-below", type=int, default=0,
+                    help="full passes smaller than this walk their pairs in the contact kernel (0 = library default)")
     ap.add_argument("--nscene", type=int, default=256, help="scenes of the batch workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
@@ -626,7 +629,8 @@ def main():
     n = scene.np
 
     fuse = not args.split
-    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin, args.graph, args.full_blocks)
+    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin,
+                    args.graph, args.full_blocks, args.merge_below)
     value = n * args.steps / mixed["seconds"]
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
@@ -667,7 +671,7 @@ def main():
     }
     if not args.no_fp64:
         f64 = run_gpu(scene, "fp64", args.steps, args.warmup, args.capacity,
-                      fuse, args.buckets_per_sphere, args.skin, args.graph, args.full_blocks)
+                      fuse, args.buckets_per_sphere, args.skin, args.graph, args.full_blocks, args.merge_below)
         f64_value = n * args.steps / f64["seconds"]
         out["fp64_value"] = f64_value
         out["mixed_over_fp64"] = value / f64_value

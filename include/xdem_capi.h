@@ -167,6 +167,10 @@ typedef struct xdem_ctx_config {
                                     stride over the queued spheres, so results do not depend on
                                     it; a few blocks per CU keep several waves per SIMD to hide
                                     the neighbour gathers. */
+    unsigned int merge_pairs_below; /* below this many spheres in a step's full pass the contact
+                                    kernel walks their pairs itself instead of a pair pass of
+                                    its own (0 = 65536; 1 = never). Results do not depend on
+                                    it. */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */

@@ -146,7 +146,8 @@ class CtxConfig(C.Structure):
     _fields_ = [("precision", C.c_int), ("p2p_capacity", C.c_uint), ("quirks", Quirks),
                 ("block_size", C.c_uint), ("grid_dim_max", C.c_uint), ("fuse_velocity", C.c_int),
                 ("device", C.c_int), ("buckets_per_sphere", C.c_uint), ("skin", C.c_double),
-                ("step_graph", C.c_int), ("rebuild_wait_limit", C.c_uint), ("full_blocks", C.c_uint)]
+                ("step_graph", C.c_int), ("rebuild_wait_limit", C.c_uint), ("full_blocks", C.c_uint),
+                ("merge_pairs_below", C.c_uint)]
 
 
 class CtxInfo(C.Structure):

@@ -315,6 +315,7 @@ struct xdem_ctx {
         A.keep_velocity = cfg.quirks.velocity_not_updated; A.tmax_persist = tmax_persist;
         A.zero_force = 1;
         A.full_blocks = cfg.full_blocks ? std::max(cfg.full_blocks, kQueueSegs) : 1024u;
+        A.merge_below = cfg.merge_pairs_below ? cfg.merge_pairs_below : kMergePairsBelow;
         return A;
     }
 
@@ -337,17 +338,15 @@ struct xdem_ctx {
         if (ev && (e = hipEventRecord(ev[1], stream)) != hipSuccess) return e;
         if ((e = launch_touch(A, multi, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[2], stream)) != hipSuccess) return e;
-        // the pair pass of a dense full pass, before the fork: on a small full pass it returns
-        // at once, and launched on an idle device that costs a few microseconds instead of
-        // waiting behind the lite pass's blocks (k_collide then walks the pairs)
+        // the full pass (spheres k_touch queued) on the second stream, concurrently with the
+        // lite pass (all others) on the first; the first stream waits for both. The pair pass
+        // leads it (on a small full pass it returns at once: k_collide walks the pairs); on the
+        // second stream its launch overlaps the lite pass instead of lengthening the step.
         const bool gpu_law = cfg.quirks.gpu_force_law != 0;
-        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 3, stream)) != hipSuccess) return e;
-        if (ev && (e = hipEventRecord(ev[9], stream)) != hipSuccess) return e;
-        // the rest of the full pass (spheres k_touch queued) on the second stream,
-        // concurrently with the lite pass (all others) on the first; the first stream waits
-        // for both
         if ((e = hipEventRecord(fork_ev, stream)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(stream2, fork_ev, 0)) != hipSuccess) return e;
+        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 3, stream2)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[9], stream2)) != hipSuccess) return e;
         if (!meshes.empty() && (e = collide_fn(A, multi, fuse, model, gpu_law, 2, stream2)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[7], stream2)) != hipSuccess) return e;
         if ((e = collide_fn(A, multi, fuse, model, gpu_law, 1, stream2)) != hipSuccess) return e;

@@ -200,6 +200,7 @@ struct StepArgs {
     uint32_t nwslot, nbucket;
     uint32_t list_cap;                       // neighbour-list entries per sorted slot
     uint32_t full_blocks;                    // grid of the full-pass kernels (host side: launch shape)
+    uint32_t merge_below;                    // full passes smaller than this: k_collide walks the pairs
     int stale_ft, flat_inertia, keep_velocity, tmax_persist;
     int zero_force;  // k_finish leaves force/moment zero (the reference's state after a step)
 };
@@ -828,9 +829,9 @@ __device__ __forceinline__ uint32_t full_queue_total(const StepArgs& A)
     return c;
 }
 
-// Below this many full-pass spheres k_collide walks the pairs itself and k_pairs returns at
-// once: the spheres then fit in one wave per SIMD of k_collide, and the pair sums skip a
-// launch and two dependent memory round trips (ppsum, own state) on the step's critical path.
+// Below StepArgs::merge_below full-pass spheres (default kMergePairsBelow) k_collide walks
+// the pairs itself and k_pairs returns at once: the spheres then fit in one wave per SIMD of
+// k_collide, and the pair sums skip two dependent memory round trips (ppsum, own state).
 // Above it k_pairs (three waves per SIMD) hides the neighbour gathers of dense packs.
 constexpr uint32_t kMergePairsBelow = 65536;
 
@@ -1141,7 +1142,7 @@ __global__ __launch_bounds__(256) void k_mesh_pass(StepArgs A)
 template <class T, bool MULTI, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 {
-    if (full_queue_total(A) < kMergePairsBelow) return;   // k_collide walks the pairs
+    if (full_queue_total(A) < A.merge_below) return;   // k_collide walks the pairs
     const QueueBlock qb = queue_block(blockIdx.x, gridDim.x);
     const uint32_t stride = qb.nchunk * blockDim.x;
     const uint32_t* q = A.work2 + (size_t)qb.seg * A.seg_cap;
@@ -1399,7 +1400,7 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
     const uint32_t cnt = A.seg_count2[qb.seg * kSegStride];
     unsigned long long c0 = 0, c1 = 0;
     float dmx = 0.0f;
-    const bool merged = full_queue_total(A) < kMergePairsBelow;
+    const bool merged = full_queue_total(A) < A.merge_below;
     while (e < cnt) {
         uint32_t n_pp = 0, n_wall = 0;
         collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, i, merged, n_pp, n_wall, dmx);

@@ -24,7 +24,8 @@ class DemContext:
     def __init__(self, scenes, precision: str = "fp64", p2p_capacity: int = 12, quirks: dict | None = None,
                  grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False,
                  buckets_per_sphere: int = 0, skin: float = 0.0, step_graph: int = 0,
-                 rebuild_wait_limit: int = 0, meshes=(), full_blocks: int = 0, cylinders=()):
+                 rebuild_wait_limit: int = 0, meshes=(), full_blocks: int = 0, cylinders=(),
+                 merge_pairs_below: int = 0):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -43,6 +44,7 @@ class DemContext:
         cfg.step_graph = step_graph   # > 0: step() replays HIP graphs of that many steps; 0: eager launches
         cfg.rebuild_wait_limit = rebuild_wait_limit   # 0: ~1 s before a rebuild phase gives up
         cfg.full_blocks = full_blocks   # 0: 1024 blocks for the full-pass kernels
+        cfg.merge_pairs_below = merge_pairs_below   # 0: 65536; 1: the pair pass always runs
         self.precision = precision
         self.capacity = p2p_capacity
         self._h = C.c_void_p()
