@@ -572,6 +572,103 @@ def lattice_main(args):
     print(json.dumps(out), flush=True)
 
 
+def calib_main(args):
+    """The calibration workload (seed 1005): 20,000 spheres r ~ U[1, 1.5] mm released from a
+    30 mm cylinder region onto a rough floor, FP64 steps over 0.08 s (16,000 steps at dt = 5e-6 s:
+    at the nominal 2e-5 s the CPU force law's damping, beta 2 sqrt(m kn), exceeds 2 m / dt for
+    the lighter spheres and the run diverges; the kernel's law is stable there but has no
+    meaningful derivative, its tangential force keeping a finite size along the direction of a
+    vanishing slip velocity). Target: the
+    64-bin radial height profile of a hidden run (pile seed 2005) at friction 0.45, restitution
+    0.35, E = 2e7 Pa. Timed: one loss and gradient at the initial guess (friction 0.3,
+    restitution 0.5, E = 3e7 Pa): the forward run with checkpoints every 100 steps, then the
+    adjoint (recomputation per segment and the reverse sweep). Checked against central
+    differences with step 1e-4 (and 1e-7) on (friction, restitution, log E)."""
+    import torch
+    from xdynamics_dll_amd import SceneData
+    from xdynamics_dll_amd.calib import Calibration, pile_center, profile_loss, radial_profile
+
+    steps = args.calib_steps
+    hidden = SceneData.build(5, 2005, np_override=args.np)
+    scene = SceneData.build(5, 1005, np_override=args.np)
+    n = scene.np
+    every = 100
+    t0 = time.perf_counter()
+    hidden.dt = scene.dt = args.calib_dt
+    hcal = Calibration(hidden, max_steps=steps, checkpoint_every=every)
+    target = radial_profile(hcal.forward(hcal.theta_of(0.45, 0.35, 2e7), steps), pile_center(hidden))
+    hcal.close()
+    t_target = time.perf_counter() - t0
+    center = pile_center(scene)
+    cal = Calibration(scene, max_steps=steps, checkpoint_every=every)
+    theta = cal.theta_of(0.3, 0.5, 3e7)
+    cal.forward(theta, 10)   # warm-up (first launches)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pos = cal.forward(theta, steps)
+    t_fwd = time.perf_counter() - t0
+    loss, g = profile_loss(pos, target, center)
+    t0 = time.perf_counter()
+    grad = cal.backward(g)
+    t_bwd = time.perf_counter() - t0
+    print(f"[calib] loss {loss:.6e} gradient {grad} forward {t_fwd:.2f} s adjoint {t_bwd:.2f} s", file=sys.stderr,
+          flush=True)
+    fds = {}
+    for h in (1e-4, 1e-7):
+        fd = np.zeros(3)
+        for k in range(3):
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += h
+            tm[k] -= h
+            fd[k] = (profile_loss(cal.forward(tp, steps), target, center)[0] -
+                     profile_loss(cal.forward(tm, steps), target, center)[0]) / (2 * h)
+        fds[h] = fd
+        print(f"[calib] central differences h={h:g}: {fd}", file=sys.stderr, flush=True)
+    # the finite-horizon gradient: the run continued from step steps - horizon at theta, the
+    # adjoint over those steps, central differences of the continued runs
+    horizon = args.calib_horizon
+    start = steps - horizon
+    cal.forward(theta, steps)
+    fd_h = np.zeros(3)
+    hh = 1e-7
+    for k in range(3):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += hh
+        tm[k] -= hh
+        fd_h[k] = (profile_loss(cal.forward(tp, steps, start), target, center)[0] -
+                   profile_loss(cal.forward(tm, steps, start), target, center)[0]) / (2 * hh)
+    loss_h, g_h = profile_loss(cal.forward(theta, steps, start), target, center)
+    t0 = time.perf_counter()
+    grad_h = cal.backward(g_h)
+    t_bwd_h = time.perf_counter() - t0
+    print(f"[calib] last {horizon} steps: adjoint {grad_h} central differences {fd_h}", file=sys.stderr, flush=True)
+    err = cal.error_word()
+    cal.close()
+    value = n * steps / (t_fwd + t_bwd)
+    out = {
+        "metric": "particle-steps/s of one loss + gradient (forward with checkpoints + adjoint), pile calibration",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": 10,
+        "ms_per_step": (t_fwd + t_bwd) / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"pile calibration: {n:,} spheres r~U[1,1.5] mm from a 30 mm cylinder region onto a "
+                               f"floor, {steps} steps dt={args.calib_dt:g} s, checkpoints every {every} steps, loss = "
+                               "MSE of the 64-bin radial height profile vs a hidden run (seed 2005, mu=0.45, e=0.35, "
+                               "E=2e7), the CPU force law of xContact.cpp",
+                   "np": n, "seed": 1005, "parallelism": "single GPU"},
+        "forward_s": t_fwd, "adjoint_s": t_bwd, "target_run_s": t_target,
+        "theta": {"friction": theta[0], "restitution": theta[1], "log_E": theta[2]},
+        "loss": loss, "gradient": grad.tolist(),
+        "central_differences": {f"{h:g}": fd.tolist() for h, fd in fds.items()},
+        "relative_difference": {f"{h:g}": (np.abs(fd - grad) / np.maximum(np.abs(fd), 1e-300)).tolist()
+                                for h, fd in fds.items()},
+        "finite_horizon": {"steps": horizon, "start": start, "gradient": grad_h.tolist(),
+                           "central_differences_1e-07": fd_h.tolist(), "adjoint_s": t_bwd_h,
+                           "relative_difference": (np.abs(fd_h - grad_h) / np.abs(fd_h).max()).tolist()},
+        "error_word": err,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -589,11 +686,15 @@ def main():
                     help="neighbour-list skin in m (0: adaptive, <0: rebuild every step)")
     ap.add_argument("--buckets-per-sphere", type=int, default=0,
                     help="bucket budget of the grid per sphere (0 = library default)")
-    ap.add_argument("--workload", choices=["dam_break", "batch", "wheel", "lattice"], default="dam_break",
+    ap.add_argument("--workload", choices=["dam_break", "batch", "wheel", "lattice", "calib"], default="dam_break",
                     help="dam_break: the 1,000,000-sphere column (default); batch: the batched scene sweep; "
                          "wheel: the wheel on a 500,000-sphere bed with HHT coupling; lattice: the 10,000-sphere "
-                         "lattice box in FP64")
+                         "lattice box in FP64; calib: loss and adjoint gradient of the 20,000-sphere pile")
     ap.add_argument("--settle", type=int, default=20000, help="untimed settling steps of the wheel's bed")
+    ap.add_argument("--calib-steps", type=int, default=16000, help="steps of the calibration workload")
+    ap.add_argument("--calib-dt", type=float, default=5e-6, help="time step of the calibration workload")
+    ap.add_argument("--calib-horizon", type=int, default=100,
+                    help="steps of the calibration's finite-horizon gradient (a multiple of 100)")
     ap.add_argument("--full-blocks", type=int, default=0,
                     help="blocks of the full-pass kernels (0 = library default)")
     ap.add_argument("--merge-below", type=int, default=0,
@@ -619,6 +720,9 @@ def main():
         return
     if args.workload == "wheel":
         wheel_main(args)
+        return
+    if args.workload == "calib":
+        calib_main(args)
         return
     if args.workload == "lattice":
         lattice_main(args)

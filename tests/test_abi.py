@@ -9,7 +9,7 @@ import pytest
 from conftest import ROOT
 import xdynamics_dll_amd as xd
 
-HEADERS = ["xdem_capi.h", "xdem_host.h", "xdem_mbd.h"]
+HEADERS = ["xdem_capi.h", "xdem_host.h", "xdem_mbd.h", "xdem_calib.h"]
 
 
 def declared_functions(header):
@@ -47,6 +47,8 @@ def test_bindings_cover_the_header():
     mbd = set(declared_functions("xdem_mbd.h"))
     mbd_bound = {n for n in xd._lib.HOST_SIGNATURES if n.startswith("xdem_hht")} | set(xd._lib.COSIM_SIGNATURES)
     assert mbd == mbd_bound, (mbd ^ mbd_bound)
+    calib = set(declared_functions("xdem_calib.h"))
+    assert calib == set(xd._lib.CALIB_SIGNATURES), (calib ^ set(xd._lib.CALIB_SIGNATURES))
 
 
 def test_struct_layouts_match_reference_sizes():

@@ -248,6 +248,23 @@ COSIM_SIGNATURES = {
     "xdem_cosim_wheel_steps": [P, P, U, U, P],
 }
 
+class CalibConfig(C.Structure):
+    _fields_ = [("capacity", C.c_uint), ("checkpoint_every", C.c_uint), ("max_steps", C.c_uint), ("device", C.c_int),
+                ("quirks", Quirks)]
+
+
+# include/xdem_calib.h (libxdem.so)
+CALIB_SIGNATURES = {
+    "xdem_calib_create": [C.POINTER(Scene), C.POINTER(CalibConfig), P],
+    "xdem_calib_destroy": [P],
+    "xdem_calib_forward": [P, P, U, P],
+    "xdem_calib_forward_from": [P, P, U, U, P],
+    "xdem_calib_state": [P, P, P, P, P],
+    "xdem_calib_backward": [P, P, P],
+    "xdem_calib_error_word": [P, C.POINTER(U), I],
+    "xdem_calib_pair_eval": [C.POINTER(ContactProperty), I, C.POINTER(Quirks), D, D, D, D, D, P, P, P],
+}
+
 HOST_SIGNATURES = {
     "xdem_scene_build": ([I, C.c_ulonglong, U, U, C.POINTER(Scene)], C.c_int),
     "xdem_scene_free": ([C.POINTER(Scene)], None),
@@ -306,6 +323,7 @@ def lib() -> C.CDLL:
         _bind(L, LAUNCHER_SIGNATURES)
         _bind(L, CTX_SIGNATURES)
         _bind(L, COSIM_SIGNATURES)
+        _bind(L, CALIB_SIGNATURES)
         for name, (argtypes, restype) in HOST_SIGNATURES.items():
             fn = getattr(L, name)
             fn.argtypes = argtypes

@@ -11,6 +11,13 @@
 
 namespace xdem {
 
+// the double versions stay visible next to the dual overloads below
+using ::exp;
+using ::fabs;
+using ::fmin;
+using ::log;
+using ::sqrt;
+
 template <int N> struct Dual {
     double v;
     double d[N];
