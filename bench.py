@@ -119,9 +119,14 @@ def run_gpu(scene, precision, steps, warmup, capacity, fuse=False, buckets_per_s
     ctx.step(warmup)
     ctx.synchronize()
     torch.cuda.synchronize()
-    if graph:
-        ctx.graph_capture(graph)   # graph steps recorded once; replays do the timed steps
+    # a graph of at most the timed steps (an even count): short timed runs replay one graph
+    # too instead of launching their steps one by one
+    graph = min(graph, steps - steps % 2) if graph else 0
+    if graph >= 2:
+        ctx.graph_capture(graph)   # graph steps recorded (not run) once; replays do the timed steps
         ctx.synchronize()
+    else:
+        graph = 0
     info0 = ctx.info()
     t0 = time.perf_counter()
     if graph:

@@ -316,6 +316,7 @@ struct xdem_ctx {
         A.zero_force = 1;
         A.full_blocks = cfg.full_blocks ? std::max(cfg.full_blocks, kQueueSegs) : 1024u;
         A.merge_below = cfg.merge_pairs_below ? cfg.merge_pairs_below : kMergePairsBelow;
+        A.has_cyl = cyls.empty() ? 0 : 1;
         return A;
     }
 
@@ -339,15 +340,16 @@ struct xdem_ctx {
         if ((e = launch_touch(A, multi, stream)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[2], stream)) != hipSuccess) return e;
         // the full pass (spheres k_touch queued) on the second stream, concurrently with the
-        // lite pass (all others) on the first; the first stream waits for both. The pair pass
-        // leads it (on a small full pass it returns at once: k_collide walks the pairs); on the
-        // second stream its launch overlaps the lite pass instead of lengthening the step.
+        // lite pass (all others) on the first; the first stream waits for both: the triangle
+        // pass, the pair pass (on a small full pass it returns at once: k_collide walks the
+        // pairs; on the second stream its launch overlaps the lite pass instead of lengthening
+        // the step), the contact kernel
         const bool gpu_law = cfg.quirks.gpu_force_law != 0;
         if ((e = hipEventRecord(fork_ev, stream)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(stream2, fork_ev, 0)) != hipSuccess) return e;
-        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 3, stream2)) != hipSuccess) return e;
-        if (ev && (e = hipEventRecord(ev[9], stream2)) != hipSuccess) return e;
         if (!meshes.empty() && (e = collide_fn(A, multi, fuse, model, gpu_law, 2, stream2)) != hipSuccess) return e;
+        if (ev && (e = hipEventRecord(ev[9], stream2)) != hipSuccess) return e;
+        if ((e = collide_fn(A, multi, fuse, model, gpu_law, 3, stream2)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[7], stream2)) != hipSuccess) return e;
         if ((e = collide_fn(A, multi, fuse, model, gpu_law, 1, stream2)) != hipSuccess) return e;
         if (ev && (e = hipEventRecord(ev[8], stream2)) != hipSuccess) return e;
@@ -905,6 +907,9 @@ static int capture_steps(xdem_ctx* c, unsigned nsteps, hipGraph_t* graph, hipGra
     }
     *graph = g;
     CTX_CHECK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    // the executable's one-time transfer to the device now, not inside the first replay
+    CTX_CHECK(hipGraphUpload(*exec, c->stream));
+    CTX_CHECK(hipStreamSynchronize(c->stream));
     return XDEM_OK;
 }
 
@@ -968,8 +973,8 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
             CTX_CHECK(hipEventElapsedTime(&m, ev[5], ev[6]));
             float kc = 0.f, kp = 0.f, km = 0.f;
             CTX_CHECK(hipEventElapsedTime(&kc, ev[7], ev[8]));
-            CTX_CHECK(hipEventElapsedTime(&kp, ev[2], ev[9]));
-            CTX_CHECK(hipEventElapsedTime(&km, ev[9], ev[7]));
+            CTX_CHECK(hipEventElapsedTime(&km, ev[2], ev[9]));
+            CTX_CHECK(hipEventElapsedTime(&kp, ev[9], ev[7]));
             c->t_collide += kc;
             c->t_meshpass += km;
             c->t_pairpass += kp;

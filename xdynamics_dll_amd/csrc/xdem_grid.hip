@@ -345,7 +345,7 @@ __device__ __forceinline__ void scatter_sphere(const StepArgs& A, uint32_t i)
     else atomicOr(A.err, XDEM_ERR_GRID_SYNC);
 }
 
-template <bool MULTI>
+template <bool MULTI, bool CYL>
 __device__ __forceinline__ void order_gather_slot(const StepArgs& A, uint32_t j)
 {
     const uint32_t k = ld_co(A.slot_tmp + j);
@@ -378,7 +378,7 @@ __device__ __forceinline__ void order_gather_slot(const StepArgs& A, uint32_t j)
     }
     // cylinders: in reach with the radius grown by the skin, and the gap the largest overlap
     // lacks (it changes by at most the distance the sphere or the body moves)
-    for (uint32_t c = 0; c < S.ncyl; c++) {
+    for (uint32_t c = 0; CYL && c < S.ncyl; c++) {
         const CylDev& C = A.cyls[S.cyl_begin + c];
         far = far & cylinder_out_of_reach(C.cc, C.info, C.dbi, C.margin, p, S.skin * scale);
         const double g = C.cc.coh != 0.0 ? -1.0
@@ -397,7 +397,7 @@ __device__ __forceinline__ void order_gather_slot(const StepArgs& A, uint32_t j)
 // First kernel of every step: the step's counters (block 0), then, on a rebuild only, the
 // bucket grid in phases (work units claimed from per-phase cursors; the other steps return
 // at once, so a step that does not rebuild costs one launch, not six).
-template <bool MULTI>
+template <bool MULTI, bool CYL>
 __global__ __launch_bounds__(kBlock) void k_rebuild(StepArgs A, uint32_t ntile)
 {
     step_counters(A);
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(StepArgs A, uint32_t ntile)
     run_phase(A, 5, sl_units, [&](uint32_t u) {
         for (uint32_t r = 0; r < 4; r++) {
             const uint32_t j = u * kUnit + r * kBlock + threadIdx.x;
-            if (j < A.nsorted) order_gather_slot<MULTI>(A, j);
+            if (j < A.nsorted) order_gather_slot<MULTI, CYL>(A, j);
         }
     });
 }
@@ -459,8 +459,13 @@ hipError_t launch_touch(const StepArgs& A, bool multi, hipStream_t s)
 {
     // kQueueChunks blocks per queue segment (a build step loops over the sorted slots)
     const dim3 g(kQueueSegs * kQueueChunks);
-    if (multi) hipLaunchKernelGGL(k_touch<true>, g, dim3(kBlock), 0, s, A);
-    else hipLaunchKernelGGL(k_touch<false>, g, dim3(kBlock), 0, s, A);
+    if (multi) {
+        if (A.has_cyl) hipLaunchKernelGGL((k_touch<true, true>), g, dim3(kBlock), 0, s, A);
+        else hipLaunchKernelGGL((k_touch<true, false>), g, dim3(kBlock), 0, s, A);
+    } else {
+        if (A.has_cyl) hipLaunchKernelGGL((k_touch<false, true>), g, dim3(kBlock), 0, s, A);
+        else hipLaunchKernelGGL((k_touch<false, false>), g, dim3(kBlock), 0, s, A);
+    }
     return hipGetLastError();
 }
 
@@ -469,8 +474,13 @@ hipError_t launch_grid(const StepArgs& A, bool multi, hipStream_t s)
     const uint32_t ntile = bucket_table_size(A.nbucket) / kScanTile;
     if (A.rebuild_blocks == 0) return hipErrorInvalidValue;
     const dim3 g(A.rebuild_blocks);
-    if (multi) hipLaunchKernelGGL(k_rebuild<true>, g, dim3(kBlock), 0, s, A, ntile);
-    else hipLaunchKernelGGL(k_rebuild<false>, g, dim3(kBlock), 0, s, A, ntile);
+    if (multi) {
+        if (A.has_cyl) hipLaunchKernelGGL((k_rebuild<true, true>), g, dim3(kBlock), 0, s, A, ntile);
+        else hipLaunchKernelGGL((k_rebuild<true, false>), g, dim3(kBlock), 0, s, A, ntile);
+    } else {
+        if (A.has_cyl) hipLaunchKernelGGL((k_rebuild<false, true>), g, dim3(kBlock), 0, s, A, ntile);
+        else hipLaunchKernelGGL((k_rebuild<false, false>), g, dim3(kBlock), 0, s, A, ntile);
+    }
     return hipGetLastError();
 }
 
@@ -480,8 +490,14 @@ uint32_t rebuild_grid_size(bool multi)
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 0;
-    const hipError_t e = multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<true>, kBlock, 0)
-                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<false>, kBlock, 0);
+    // the smaller residency of the two instantiations (with and without cylinders)
+    int per2 = 0;
+    hipError_t e = multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<true, false>, kBlock, 0)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_rebuild<false, false>, kBlock, 0);
+    if (e == hipSuccess)
+        e = multi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_rebuild<true, true>, kBlock, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_rebuild<false, true>, kBlock, 0);
+    per = std::min(per, per2);
     if (e != hipSuccess || ncu <= 0 || per <= 0) return 0;
     // two blocks per CU (the phases do not need them all resident: work goes to the blocks
     // that run; more blocks only add claims on the phase cursors: measured slower)

@@ -201,6 +201,7 @@ struct StepArgs {
     uint32_t list_cap;                       // neighbour-list entries per sorted slot
     uint32_t full_blocks;                    // grid of the full-pass kernels (host side: launch shape)
     uint32_t merge_below;                    // full passes smaller than this: k_collide walks the pairs
+    int has_cyl;                             // a scene has cylinders (host side: the CYL kernels)
     int stale_ft, flat_inertia, keep_velocity, tmax_persist;
     int zero_force;  // k_finish leaves force/moment zero (the reference's state after a step)
 };
@@ -544,7 +545,7 @@ __device__ __forceinline__ bool pair_passes(const ContactConst& cc, double ir, d
 // (in the reference's visiting order, the list re-sorted when broken), their mask. built: the
 // lists were made this step (no displacement since); dm: the largest displacement since the
 // build.
-template <bool MULTI>
+template <bool MULTI, bool CYL>
 __device__ __forceinline__ bool touch_sphere(const StepArgs& A, uint32_t i, uint32_t fl, bool built, float dm)
 {
     // a sphere with listed triangles goes to the full pass every step (the triangle pass
@@ -562,10 +563,11 @@ __device__ __forceinline__ bool touch_sphere(const StepArgs& A, uint32_t i, uint
             const WallDev& W = A.walls[S.wall_begin + w];
             touch = touch || !wall_out_of_reach(W.cc, W.info, p);
         }
-        for (uint32_t c = 0; c < S.ncyl; c++) {
-            const CylDev& C = A.cyls[S.cyl_begin + c];
-            touch = touch || !cylinder_out_of_reach(C.cc, C.info, C.dbi, C.margin, p, 0.0);
-        }
+        if (CYL)
+            for (uint32_t c = 0; c < S.ncyl; c++) {
+                const CylDev& C = A.cyls[S.cyl_begin + c];
+                touch = touch || !cylinder_out_of_reach(C.cc, C.info, C.dbi, C.margin, p, 0.0);
+            }
     }
     const uint32_t n = may ? min(fl & kCountMask, A.list_cap) : 0u;
     if (n) {
@@ -629,7 +631,7 @@ __device__ __forceinline__ void queue_append(uint32_t* q, uint32_t* count, uint3
 // segment never holds more than seg_cap); the others are completed by the lite pass, as the
 // full pass would add only zero sums for them. The grid has a multiple of kQueueSegs
 // blocks.
-template <bool MULTI>
+template <bool MULTI, bool CYL>
 __global__ __launch_bounds__(256) void k_touch(StepArgs A)
 {
     const bool built = *A.rebuild_cur != 0u;
@@ -661,7 +663,7 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
                     mesh = (fl & kMeshNear) != 0u;
                 }
                 full = (fl & kCandidate) != 0u;
-                if (full) touch = touch_sphere<MULTI>(A, i, fl, true, 0.0f);
+                if (full) touch = touch_sphere<MULTI, CYL>(A, i, fl, true, 0.0f);
                 A.nbr_count[i] = touch ? (fl | kTouch) : fl;
             }
             const uint32_t seg = (j >> 6) & (kQueueSegs - 1u);
@@ -699,7 +701,7 @@ __global__ __launch_bounds__(256) void k_touch(StepArgs A)
         if (e + nchunk * blockDim.x < cnt) i_next = wq[e + nchunk * blockDim.x];
         if (e < cnt) {
             const uint32_t fl = A.nbr_count[i];
-            touch = touch_sphere<MULTI>(A, i, fl, built, dm);
+            touch = touch_sphere<MULTI, CYL>(A, i, fl, built, dm);
             // a touched sphere's rotation, acceleration and contact counts are written by the
             // full pass: no longer known stable
             A.nbr_count[i] = touch ? ((fl & ~kLiteStable) | kTouch) : (fl & ~kTouch);
@@ -883,10 +885,55 @@ __device__ __forceinline__ uint32_t pairs_sphere(const StepArgs& A, const SceneD
     return n_new;
 }
 
+// Sphere i against the cylinders of its scene, in order: each looks up the slots as the previous
+// one left them (one kernel launch per cylinder in the reference); the slots are updated in
+// place and the stored count returned. The full-pass and grid kernels take it only in their
+// CYL instantiation (contexts with cylinders): its registers stay out of the others.
+template <class T, int MODEL, bool GPU_LAW>
+__device__ __forceinline__ uint32_t collide_cylinders(const CylDev* cyls, uint32_t ncyl, uint32_t* cy_count,
+                                                   uint32_t* cy_id, double2* cy_tsd, double* wrench, uint32_t np,
+                                                   uint32_t i, double dt, V4<double> pos, V3<double> vel,
+                                                   V3<double> iom, double mass, Accum& tot, uint32_t& err)
+{
+    CylSlots cs;
+    const uint32_t cb = XDEM_MAX_P2CY_COUNT * i;
+#pragma unroll
+    for (uint32_t q = 0; q < XDEM_MAX_P2CY_COUNT; q++) {
+        cs.id[q] = cy_id[cb + q];
+        const double2 h = cy_tsd[cb + q];
+        cs.s0[q] = h.x;
+        cs.s1[q] = h.y;
+    }
+    cs.count = cy_count[i];
+    cs.overflow = false;
+    for (uint32_t c = 0; c < ncyl; c++) {
+        const CylDev& C = cyls[c];
+#pragma unroll
+        for (uint32_t q = 0; q < XDEM_MAX_P2CY_COUNT; q++) {
+            cs.old_id[q] = cs.id[q]; cs.old_s0[q] = cs.s0[q]; cs.old_s1[q] = cs.s1[q];
+        }
+        double bf[6] = {0, 0, 0, 0, 0, 0};
+        cylinder_contact<T, MODEL, GPU_LAW>(C.cc, C.info, C.dbi, dt, pos, vel, iom, mass, cs, tot, bf);
+        if (C.wslot >= 0) {
+            double* wr = wrench + (size_t)C.wslot * 6 * np;
+#pragma unroll
+            for (int q = 0; q < 6; q++) wr[(size_t)q * np + i] = bf[q];
+        }
+    }
+    if (cs.overflow) err |= XDEM_ERR_CYL_OVERFLOW;
+#pragma unroll
+    for (uint32_t q = 0; q < XDEM_MAX_P2CY_COUNT; q++) {
+        cy_id[cb + q] = cs.id[q];
+        cy_tsd[cb + q] = make_double2(cs.s0[q], cs.s1[q]);
+    }
+    cy_count[i] = cs.count;
+    return cs.count;
+}
+
 // Full contact pass of sphere i: the near neighbours the pair search listed, in their order, under
 // the exact FP64 predicate; every wall of the scene in id order; then sphere_tail. The
 // sphere's own state and outputs are read and written by id.
-template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
+template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW, bool CYL>
 __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, bool merged, uint32_t& n_pp,
                                                uint32_t& n_wall, float& dmx)
 {
@@ -951,42 +998,10 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, bo
     }
 
     // ---- cylinders, in the scene's order (cylinder_contact_force_kernel once per cylinder,
-    // impl:1327-1460): each launch looks up the slots as the previous one left them
-    if (S.ncyl) {
-        CylSlots cs;
-        const uint32_t cb = XDEM_MAX_P2CY_COUNT * i;
-#pragma unroll
-        for (uint32_t q = 0; q < XDEM_MAX_P2CY_COUNT; q++) {
-            cs.id[q] = A.cy_id[cb + q];
-            const double2 h = A.cy_tsd[cb + q];
-            cs.s0[q] = h.x;
-            cs.s1[q] = h.y;
-        }
-        cs.count = A.cy_count[i];
-        cs.overflow = false;
-        for (uint32_t c = 0; c < S.ncyl; c++) {
-            const CylDev& C = A.cyls[S.cyl_begin + c];
-#pragma unroll
-            for (uint32_t q = 0; q < XDEM_MAX_P2CY_COUNT; q++) {
-                cs.old_id[q] = cs.id[q]; cs.old_s0[q] = cs.s0[q]; cs.old_s1[q] = cs.s1[q];
-            }
-            double bf[6] = {0, 0, 0, 0, 0, 0};
-            cylinder_contact<T, MODEL, GPU_LAW>(C.cc, C.info, C.dbi, S.dt, o.pos, o.vel, iom, o.mass, cs, tot, bf);
-            if (C.wslot >= 0) {
-                double* wr = A.wrench + (size_t)C.wslot * 6 * A.np;
-#pragma unroll
-                for (int q = 0; q < 6; q++) wr[(size_t)q * A.np + i] = bf[q];
-            }
-        }
-        if (cs.overflow) err |= XDEM_ERR_CYL_OVERFLOW;
-#pragma unroll
-        for (uint32_t q = 0; q < XDEM_MAX_P2CY_COUNT; q++) {
-            A.cy_id[cb + q] = cs.id[q];
-            A.cy_tsd[cb + q] = make_double2(cs.s0[q], cs.s1[q]);
-        }
-        A.cy_count[i] = cs.count;
-        n_wall += cs.count;
-    }
+    // impl:1327-1460)
+    if (CYL && S.ncyl)
+        n_wall += collide_cylinders<T, MODEL, GPU_LAW>(A.cyls + S.cyl_begin, S.ncyl, A.cy_count, A.cy_id, A.cy_tsd,
+                                                       A.wrench, A.np, i, S.dt, o.pos, o.vel, iom, o.mass, tot, err);
 
     // ---- the scene's mesh: this step's triangle sums (k_mesh_pass), added after the walls, as
     // the reference's mesh launch adds its sums to force, moment, tmax and rres (impl:2100-2108)
@@ -1389,7 +1404,7 @@ __global__ __launch_bounds__(256) void k_lite_turn(StepArgs A)
 // Full pass over the spheres k_touch queued: block b takes queue segment b mod kQueueSegs,
 // its entries chunk b / kQueueSegs, ... strided by the chunks per segment. The first entry's
 // sphere id is read together with the segment's count.
-template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW>
+template <class T, bool MULTI, bool FUSE, int MODEL, bool GPU_LAW, bool CYL>
 __global__ __launch_bounds__(256) void k_collide(StepArgs A)
 {
     const QueueBlock qb = queue_block(blockIdx.x, gridDim.x);
@@ -1403,7 +1418,7 @@ __global__ __launch_bounds__(256) void k_collide(StepArgs A)
     const bool merged = full_queue_total(A) < A.merge_below;
     while (e < cnt) {
         uint32_t n_pp = 0, n_wall = 0;
-        collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW>(A, i, merged, n_pp, n_wall, dmx);
+        collide_sphere<T, MULTI, FUSE, MODEL, GPU_LAW, CYL>(A, i, merged, n_pp, n_wall, dmx);
         c0 += n_pp;
         c1 += n_wall;
         e += stride;

@@ -28,7 +28,8 @@ void collide_passes(const StepArgs& A, int pass, hipStream_t s)
 {
     const dim3 g = grid_for(A.np), b(kBlock);
     if (pass == 1) {
-        hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW>), full_grid(A), b, 0, s, A);
+        if (A.has_cyl) hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW, true>), full_grid(A), b, 0, s, A);
+        else hipLaunchKernelGGL((k_collide<double, MULTI, FUSE, MODEL, GPU_LAW, false>), full_grid(A), b, 0, s, A);
     } else if (pass == 3) {
         // the pair pass: one thread per queued sphere
         hipLaunchKernelGGL((k_pairs<double, MULTI, MODEL, GPU_LAW>), full_grid(A), b, 0, s, A);
