@@ -647,6 +647,26 @@ def calib_main(args):
     grad_h = cal.backward(g_h)
     t_bwd_h = time.perf_counter() - t0
     print(f"[calib] last {horizon} steps: adjoint {grad_h} central differences {fd_h}", file=sys.stderr, flush=True)
+    # where the whole-run gradient stays meaningful: runs of 500, 1,000 and 2,000 steps from the
+    # start (targets from the hidden pile at the same times), adjoint against central differences
+    study = []
+    for hs in (500, 1000, 2000):
+        hc = Calibration(hidden, max_steps=hs, checkpoint_every=every)
+        tgt = radial_profile(hc.forward(hc.theta_of(0.45, 0.35, 2e7), hs), pile_center(hidden))
+        hc.close()
+        loss_s, g_s = profile_loss(cal.forward(theta, hs), tgt, center)
+        grad_s = cal.backward(g_s)
+        fd_s = np.zeros(3)
+        for k in range(3):
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += hh
+            tm[k] -= hh
+            fd_s[k] = (profile_loss(cal.forward(tp, hs), tgt, center)[0] -
+                       profile_loss(cal.forward(tm, hs), tgt, center)[0]) / (2 * hh)
+        study.append({"steps": hs, "loss": loss_s, "adjoint": grad_s.tolist(), "central_differences_1e-07": fd_s.tolist(),
+                      "max_difference_over_max_fd": float(np.abs(grad_s - fd_s).max() / np.abs(fd_s).max())})
+        print(f"[calib] {hs} steps from the start: adjoint {grad_s} central differences {fd_s}", file=sys.stderr,
+              flush=True)
     err = cal.error_word()
     cal.close()
     value = n * steps / (t_fwd + t_bwd)
@@ -666,6 +686,7 @@ def calib_main(args):
         "central_differences": {f"{h:g}": fd.tolist() for h, fd in fds.items()},
         "relative_difference": {f"{h:g}": (np.abs(fd - grad) / np.maximum(np.abs(fd), 1e-300)).tolist()
                                 for h, fd in fds.items()},
+        "horizon_study": study,
         "finite_horizon": {"steps": horizon, "start": start, "gradient": grad_h.tolist(),
                            "central_differences_1e-07": fd_h.tolist(), "adjoint_s": t_bwd_h,
                            "relative_difference": (np.abs(fd_h - grad_h) / np.abs(fd_h).max()).tolist()},
@@ -741,6 +762,7 @@ def main():
     mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin,
                     args.graph, args.full_blocks, args.merge_below)
     value = n * args.steps / mixed["seconds"]
+    print(f"[dam] mixed leg: {mixed['seconds'] / args.steps * 1e3:.4f} ms per step", file=sys.stderr, flush=True)
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n
     tm = mixed["timing"]
     step_gbps = step_bytes(cbar) * n * args.steps / mixed["seconds"] / 1e9
@@ -782,9 +804,11 @@ def main():
         f64 = run_gpu(scene, "fp64", args.steps, args.warmup, args.capacity,
                       fuse, args.buckets_per_sphere, args.skin, args.graph, args.full_blocks, args.merge_below)
         f64_value = n * args.steps / f64["seconds"]
+        print(f"[dam] FP64 leg: {f64['seconds'] / args.steps * 1e3:.4f} ms per step", file=sys.stderr, flush=True)
         out["fp64_value"] = f64_value
         out["mixed_over_fp64"] = value / f64_value
         out["force_check"] = force_error(scene, args.capacity)
+        print("[dam] force check done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb = cpu_baseline(scene, args.capacity)
         out["cpu_baseline"] = cb
