@@ -1158,13 +1158,15 @@ template <class T, bool MULTI, int MODEL, bool GPU_LAW>
 __global__ __launch_bounds__(256) void k_pairs(StepArgs A)
 {
     if (full_queue_total(A) < A.merge_below) return;   // k_collide walks the pairs
-    const QueueBlock qb = queue_block(blockIdx.x, gridDim.x);
-    const uint32_t stride = qb.nchunk * blockDim.x;
-    const uint32_t* q = A.work2 + (size_t)qb.seg * A.seg_cap;
-    const uint32_t cnt = A.seg_count2[qb.seg * kSegStride];
+    // the touched spheres in sorted-slot order rather than from the queue: each XCD sweeps one
+    // contiguous eighth of the slots, so the neighbours a block gathers were mostly fetched
+    // into its XCD's L2 by the blocks of the adjacent slots
     unsigned long long c0 = 0;
-    for (uint32_t e = qb.chunk * blockDim.x + threadIdx.x; e < cnt; e += stride) {
-        const uint32_t i = q[e];
+    const uint32_t nvb = (A.nsorted + blockDim.x - 1) / blockDim.x;
+    for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        const uint32_t j = xcd_block(vb, nvb) * blockDim.x + threadIdx.x;
+        const uint32_t i = j < A.nsorted ? A.svals[j] : kInvalid;
+        if (i >= A.np || !(A.nbr_count[i] & kTouch)) continue;
         const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
         if (!S.has_pp) continue;
         OwnState o;
