@@ -37,21 +37,25 @@ def test_box_lattice_10k_fp64_matches_oracle():
     step 2,200 (~700 sphere and 300 wall contacts) the FP64 step is bit-identical to the
     oracle: positions, velocities, accelerations, Euler parameters and their rates."""
     scene = SceneData.build(1, 1001)
-    gpu = DemContext(scene, precision="fp64", fuse_velocity=True)
+    # the same run with the pairs walked by the pair pass on every step (merge_pairs_below 1)
+    gpus = [DemContext(scene, precision="fp64", fuse_velocity=True, merge_pairs_below=m) for m in (0, 1)]
     ref = xdem_ref.RefSim(scene)
     done = 0
     for target in (1, 10, 100, 1500, 1800, 2200):
         print(f"[cfg1 lattice] np={scene.np} step {target}", flush=True)
-        gpu.step(target - done)
+        for gpu in gpus:
+            gpu.step(target - done)
         ref.step(target - done)
         done = target
-        assert_state_equal(gpu, ref, f"step {target}")
-    c, _, _ = gpu.history("p2p")
-    assert np.array_equal(c, ref.pp_count) and ref.pp_count.sum() > 100
-    cw, _, _ = gpu.history("plane")
-    assert np.array_equal(cw, ref.pl_count) and ref.pl_count.sum() > 100
-    assert gpu.error_word() == 0
-    gpu.close()
+        for m, gpu in enumerate(gpus):
+            assert_state_equal(gpu, ref, f"merge {m} step {target}")
+    for gpu in gpus:
+        c, _, _ = gpu.history("p2p")
+        assert np.array_equal(c, ref.pp_count) and ref.pp_count.sum() > 100
+        cw, _, _ = gpu.history("plane")
+        assert np.array_equal(cw, ref.pl_count) and ref.pl_count.sum() > 100
+        assert gpu.error_word() == 0
+        gpu.close()
 
 
 def test_batch_members_in_one_context_match_oracle():
@@ -60,20 +64,24 @@ def test_batch_members_in_one_context_match_oracle():
     equals its own oracle run after 1, 10 and 400 steps."""
     scenes = [SceneData.build(4, 1004, variant=k) for k in range(8)]
     print(f"[cfg4 members] np={[s.np for s in scenes]}", flush=True)
-    gpu = DemContext(scenes, precision="fp64", fuse_velocity=True)
+    # the default context and one whose pairs go through the pair pass on every step
+    gpus = [DemContext(scenes, precision="fp64", fuse_velocity=True, merge_pairs_below=m) for m in (0, 1)]
     refs = [xdem_ref.RefSim(s) for s in scenes]
     done = 0
     for target in (1, 10, 400):
-        gpu.step(target - done)
+        for gpu in gpus:
+            gpu.step(target - done)
         for k, r in enumerate(refs):
             r.step(target - done)
-            assert_state_equal(gpu, r, f"scene {k} step {target}", scene=k)
+            for m, gpu in enumerate(gpus):
+                assert_state_equal(gpu, r, f"merge {m} scene {k} step {target}", scene=k)
         done = target
     contacts = sum(int(r.pp_count.sum() + r.pl_count.sum()) for r in refs)
     print(f"[cfg4 members] contacts at step 400: {contacts}", flush=True)
     assert contacts > 0
-    assert gpu.error_word() == 0
-    gpu.close()
+    for gpu in gpus:
+        assert gpu.error_word() == 0
+        gpu.close()
 
 
 def test_batch_256_scenes_full_size():

@@ -134,21 +134,26 @@ def test_terrain_mixed_precision_forces():
 def test_wheel_over_20k_bed_matches_oracle():
     scene, tris, dbi, cp = wheel_over_bed(20000)
     print(f"[wheel 20k] np={scene.np} ntri={len(tris)}", flush=True)
-    gpu = DemContext(scene, precision="fp64", fuse_velocity=True, meshes=[(0, tris, dbi, cp)])
+    # the default context and one whose pairs go through the pair pass on every step
+    gpus = [DemContext(scene, precision="fp64", fuse_velocity=True, meshes=[(0, tris, dbi, cp)], merge_pairs_below=m)
+            for m in (0, 1)]
     ref = xdem_ref.RefSim(scene)
     ref.add_mesh(tris, dbi, cp)
     done = 0
     for target in (1, 10, 40):
-        gpu.step(target - done)
+        for gpu in gpus:
+            gpu.step(target - done)
         ref.step(target - done)
         done = target
-        assert_state_equal(gpu, ref, f"step {target}")
-        assert_mesh_history_equal(gpu, ref, f"step {target}")
+        for m, gpu in enumerate(gpus):
+            assert_state_equal(gpu, ref, f"merge {m} step {target}")
+            assert_mesh_history_equal(gpu, ref, f"merge {m} step {target}")
     assert ref.ms_count.sum() > 0
-    info = gpu.info()
-    assert info["ntriangle"] == 4096 and info["mesh_spheres"] > 0
-    assert gpu.error_word() == 0
-    gpu.close()
+    for gpu in gpus:
+        info = gpu.info()
+        assert info["ntriangle"] == 4096 and info["mesh_spheres"] > 0
+        assert gpu.error_word() == 0
+        gpu.close()
 
 
 def test_wheel_over_full_bed_properties():

@@ -59,9 +59,10 @@ def test_skin_lists_are_bit_identical_to_rebuilding_every_step(precision, fuse, 
             assert np.array_equal(x, y), h
 
 
-def test_skin_history_slot_order_matches_oracle():
+@pytest.mark.parametrize("merge", [0, 1])
+def test_skin_history_slot_order_matches_oracle(merge):
     scene = shaken_pack(n=1500, seed=5)
-    gpu = DemContext(scene, precision="fp64", fuse_velocity=True, skin=1.0e-3)
+    gpu = DemContext(scene, precision="fp64", fuse_velocity=True, skin=1.0e-3, merge_pairs_below=merge)
     ref = xdem_ref.RefSim(scene)
     for n in (7, 33):
         gpu.step(n)

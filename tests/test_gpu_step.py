@@ -5,6 +5,11 @@ FP64 path: forces, moments, rolling sums, histories and states equal the oracle'
 absorbs libm differences in log/pow); pair sets are compared bit-exactly.
 Mixed path: per-sphere net-force error normalised by the sum of that sphere's pair-force
 magnitudes (FP64) <= 1e-4, pair sets bit-exact at the same input state.
+
+Sphere pairs are walked by the contact kernel itself when the full pass is small and by the
+pair pass (k_pairs, lanes packed over the pairs of 64 spheres) when it holds at least
+merge_pairs_below spheres; these scenes are below the default threshold, so the tests run
+with merge_pairs_below = 1 (every step through the pair pass) as well as with the default.
 """
 import numpy as np
 import pytest
@@ -47,12 +52,18 @@ def compare_histories(gpu, ref, tol=1e-12):
     assert np.allclose(tsd, ref.pl_tsd, rtol=tol, atol=1e-300)
 
 
+# merge_pairs_below: 0 = the library's threshold (these scenes: the contact kernel walks the
+# pairs), 1 = the pair pass on every step
+REGIMES = [0, 1]
+
+
+@pytest.mark.parametrize("merge", REGIMES)
 @pytest.mark.parametrize("name", list(SCENES))
-def test_fp64_collide_matches_oracle(name):
+def test_fp64_collide_matches_oracle(name, merge):
     scene = SCENES[name]()
-    gpu = DemContext(scene, precision="fp64")
+    gpu = DemContext(scene, precision="fp64", merge_pairs_below=merge)
     ref = xdem_ref.RefSim(scene)
-    print(f"[fp64 collide] scene={name} np={scene.np}", flush=True)
+    print(f"[fp64 collide] scene={name} np={scene.np} merge={merge}", flush=True)
     for step in range(3):
         gpu.collide()
         ref.collide()
@@ -70,11 +81,12 @@ def test_fp64_collide_matches_oracle(name):
     gpu.close()
 
 
+@pytest.mark.parametrize("merge", REGIMES)
 @pytest.mark.parametrize("fuse", [False, True])
 @pytest.mark.parametrize("name", ["lattice", "block"])
-def test_fp64_fused_steps_match_oracle(name, fuse):
+def test_fp64_fused_steps_match_oracle(name, fuse, merge):
     scene = SCENES[name]()
-    gpu = DemContext(scene, precision="fp64", fuse_velocity=fuse)
+    gpu = DemContext(scene, precision="fp64", fuse_velocity=fuse, merge_pairs_below=merge)
     ref = xdem_ref.RefSim(scene)
     for n in (1, 9, 40):
         gpu.step(n)
@@ -85,10 +97,11 @@ def test_fp64_fused_steps_match_oracle(name, fuse):
     assert gpu.error_word() == 0
 
 
+@pytest.mark.parametrize("merge", REGIMES)
 @pytest.mark.parametrize("name", ["lattice", "block", "cohesive_pack"])
-def test_mixed_force_error_and_pairs(name):
+def test_mixed_force_error_and_pairs(name, merge):
     scene = SCENES[name]()
-    gpu = DemContext(scene, precision="mixed")
+    gpu = DemContext(scene, precision="mixed", merge_pairs_below=merge)
     ref = xdem_ref.RefSim(scene)
     mag = np.zeros(scene.np)
     gpu.collide()
@@ -105,10 +118,11 @@ def test_mixed_force_error_and_pairs(name):
     assert err.max() <= 1e-4
 
 
+@pytest.mark.parametrize("merge", REGIMES)
 @pytest.mark.parametrize("precision,fuse", [("fp64", False), ("fp64", True), ("mixed", True)])
-def test_multi_scene_equals_single_scenes(precision, fuse):
+def test_multi_scene_equals_single_scenes(precision, fuse, merge):
     a, b = compressed_lattice(), block_scene()
-    both = DemContext([a, b], precision=precision, fuse_velocity=fuse)
+    both = DemContext([a, b], precision=precision, fuse_velocity=fuse, merge_pairs_below=merge)
     sa = DemContext(a, precision=precision, fuse_velocity=fuse)
     sb = DemContext(b, precision=precision, fuse_velocity=fuse)
     both.step(5)
@@ -133,9 +147,10 @@ def test_graph_replay_equals_eager(fuse):
             assert np.array_equal(graph.read(f), eager.read(f)), (k, f)
 
 
-def test_history_overflow_is_flagged_not_written_past_capacity():
+@pytest.mark.parametrize("merge", REGIMES)
+def test_history_overflow_is_flagged_not_written_past_capacity(merge):
     scene = compressed_lattice()
-    gpu = DemContext(scene, precision="fp64", p2p_capacity=3)
+    gpu = DemContext(scene, precision="fp64", p2p_capacity=3, merge_pairs_below=merge)
     ref = xdem_ref.RefSim(scene, capacity=3)
     gpu.collide()
     ref.collide()
@@ -149,9 +164,10 @@ def test_history_overflow_is_flagged_not_written_past_capacity():
 
 @pytest.mark.parametrize("q", [dict(gpu_force_law=1), dict(velocity_not_updated=1), dict(stale_ft=0),
                                dict(flat_inertia=0)])
-def test_quirk_profiles_match_oracle(q):
+@pytest.mark.parametrize("merge", REGIMES)
+def test_quirk_profiles_match_oracle(q, merge):
     scene = compressed_lattice()
-    gpu = DemContext(scene, precision="fp64", quirks=q)
+    gpu = DemContext(scene, precision="fp64", quirks=q, merge_pairs_below=merge)
     ref = xdem_ref.RefSim(scene, quirks=q)
     gpu.step(4)
     ref.step(4)
@@ -193,3 +209,24 @@ def test_state_written_between_steps_matches_oracle(fuse):
     for f in ("pos", "vel", "ev"):
         assert rel(gpu.read(f), getattr(ref, f)) <= 1e-10, f
     assert gpu.error_word() == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pair_pass_and_contact_kernel_walks_are_bit_identical(precision):
+    """The same scene stepped with the pairs walked by the contact kernel and by the pair pass:
+    states and histories equal bit for bit (both add the pairs in rank order with the same
+    operations)."""
+    scene = random_pack(n=3000)
+    a = DemContext(scene, precision=precision, fuse_velocity=True)
+    b = DemContext(scene, precision=precision, fuse_velocity=True, merge_pairs_below=1)
+    for n in (1, 7, 30):
+        a.step(n)
+        b.step(n)
+        for f in ("pos", "vel", "ep", "ev", "acc", "ea"):
+            assert np.array_equal(a.read(f), b.read(f)), (precision, n, f)
+        for kind in ("p2p", "plane"):
+            for x, y in zip(a.history(kind), b.history(kind)):
+                assert np.array_equal(x, y), (precision, n, kind)
+    assert a.error_word() == 0 and b.error_word() == 0
+    a.close()
+    b.close()

@@ -1154,33 +1154,167 @@ __global__ __launch_bounds__(256) void k_mesh_pass(StepArgs A)
     }
 }
 
-template <class T, bool MULTI, int MODEL, bool GPU_LAW>
-__global__ __launch_bounds__(256) void k_pairs(StepArgs A)
+// Pair results handed from the lane that evaluated a pair to the lane of its sphere, per wave:
+// ten values (sphere_pair_eval's out[] or the cohesion attraction) and the kind, column-major.
+struct PairStage {
+    double v[10][64];
+    uint32_t kind[64];
+    uint32_t ids[64][kPairIdRegs];   // each lane's sphere: its first slot ids of the previous step
+};
+
+// LDS written by some lanes of a wave and then read by others: the wave's own accesses complete
+// in program order; the fences keep the compiler from moving them across this point.
+__device__ __forceinline__ void wave_lds_sync()
 {
-    if (full_queue_total(A) < A.merge_below) return;   // k_collide walks the pairs
-    // the touched spheres in sorted-slot order rather than from the queue: each XCD sweeps one
-    // contiguous eighth of the slots, so the neighbours a block gathers were mostly fetched
-    // into its XCD's L2 by the blocks of the adjacent slots
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The pair pass of a large full pass. A wave takes 64 touched spheres, one per lane: when at
+// least half the sorted slots are touched (a dense pack), found in sorted-slot order, each XCD
+// sweeping one contiguous eighth of the slots, so the neighbours a block gathers were mostly
+// fetched into its XCD's L2 by the blocks of adjacent slots; otherwise from the full-pass
+// queue (block b takes segment b mod 64). It numbers their passing pairs consecutively, sphere after sphere, in rank order (an exclusive
+// scan of the pass-mask counts). The lanes then evaluate the pairs 64 at a time: entry x
+// belongs to the lane L whose range [s_L, s_L + c_L) holds it (a binary search over the
+// scanned ends), its rank is x - s_L and its list entry the rank-th set bit of L's mask; the
+// pair's history lookup, force law and new slot are pair_of's, as in pairs_sphere. Each
+// sphere's lane then adds its pairs' results in rank order from LDS with sphere_pair_sum's
+// operations, so the sums, slots and counts are those of the one-thread walk, while no lane
+// idles through the longest walk of its wave.
+template <class T, bool MULTI, int MODEL, bool GPU_LAW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_pairs(StepArgs A)
+{
+    const uint32_t nfull = full_queue_total(A);
+    if (nfull < A.merge_below) return;   // k_collide walks the pairs
+    __shared__ PairStage stage_all[4];
+    PairStage& st = stage_all[threadIdx.x >> 6];
+    const uint32_t lane = threadIdx.x & 63u;
     unsigned long long c0 = 0;
+    // the wave's spheres: sorted slots (it = virtual block) or queue entries (it = the wave's
+    // first entry); either way the loop bounds are the same for the whole wave
+    const bool sweep = 2ull * nfull >= (unsigned long long)A.nsorted;
     const uint32_t nvb = (A.nsorted + blockDim.x - 1) / blockDim.x;
-    for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
-        const uint32_t j = xcd_block(vb, nvb) * blockDim.x + threadIdx.x;
-        const uint32_t i = j < A.nsorted ? A.svals[j] : kInvalid;
-        if (i >= A.np || !(A.nbr_count[i] & kTouch)) continue;
-        const SceneDev& S = A.scenes[scene_index(A, i, MULTI)];
-        if (!S.has_pp) continue;
-        OwnState o;
-        o.pos = ld4(A.pos + 4 * (size_t)i);
-        o.vel = ld3(A.vel + 3 * (size_t)i);
-        o.mass = A.mass[i];
-        const V3<double> iom = omega_of(ld4(A.ep + 4 * (size_t)i), ld4(A.ev + 4 * (size_t)i));
+    const QueueBlock qb = queue_block(blockIdx.x, gridDim.x);
+    const uint32_t* qseg = A.work2 + (size_t)qb.seg * A.seg_cap;
+    const uint32_t qcnt = sweep ? 0u : A.seg_count2[qb.seg * kSegStride];
+    const uint32_t it0 = sweep ? blockIdx.x : qb.chunk * blockDim.x + (threadIdx.x & ~63u);
+    const uint32_t it_end = sweep ? nvb : qcnt;
+    const uint32_t it_step = sweep ? gridDim.x : qb.nchunk * blockDim.x;
+    for (uint32_t it = it0; it < it_end; it += it_step) {
+        uint32_t i = kInvalid;
+        if (sweep) {
+            const uint32_t j = xcd_block(it, nvb) * blockDim.x + threadIdx.x;
+            if (j < A.nsorted) i = A.svals[j];
+        } else if (it + lane < qcnt) {
+            i = qseg[it + lane];
+        }
+        uint32_t flags = 0;
+        if (i < A.np) {
+            flags = A.nbr_count[i];
+            if (!(flags & kTouch) || !A.scenes[scene_index(A, i, MULTI)].has_pp) i = kInvalid;
+        } else {
+            i = kInvalid;
+        }
+        const uint32_t nlist_all = flags & kCountMask;
+        const unsigned long long m = (i != kInvalid && min(nlist_all, A.list_cap)) ? A.pass_mask[i] : 0ull;
+        const uint32_t c = (uint32_t)__popcll(m);
+        const uint32_t old_count = c ? min(A.pp_count_in[i], A.cap) : 0u;
+        if (c) {
+            const uint32_t hb = i * A.cap;
+            const uint32_t nid = min(old_count, kPairIdRegs);
+            for (uint32_t q = 0; q < nid; q++) st.ids[lane][q] = A.pp_id_in[hb + q];
+        }
+        // inclusive scan of the counts over the wave
+        uint32_t incl = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off, 64);
+            if ((int)lane >= off) incl += t;
+        }
+        const uint32_t total = __shfl(incl, 63, 64);
+        const uint32_t first = incl - c;
         Accum pp = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), 0.0};
-        c0 += pairs_sphere<T, MODEL, GPU_LAW>(A, S, i, o, iom, A.nbr_count[i], pp);
-        double* ps = A.ppsum + (size_t)i * 10;
-        st3(ps, pp.F);
-        st3(ps + 3, pp.M);
-        st3(ps + 6, pp.T);
-        ps[9] = pp.R;
+        V3<T> Ft = mk3(T(0), T(0), T(0));
+        wave_lds_sync();
+        // entry x's lane L, rank r and neighbour k; the next chunk's are found (and its list
+        // entry loaded) while this chunk's pairs are evaluated
+        uint32_t L, r, k;
+        auto locate = [&](uint32_t x, uint32_t& L_, uint32_t& r_, uint32_t& k_) {
+            // the lane of entry x: the number of lanes whose range ends at or before x
+            L_ = 0;
+#pragma unroll
+            for (uint32_t w = 32; w; w >>= 1) {
+                const uint32_t e = __shfl(incl, (int)(L_ + w - 1u), 64);
+                if (e <= x) L_ += w;
+            }
+            const uint32_t iq = __shfl(i, (int)L_, 64);
+            r_ = x - __shfl(first, (int)L_, 64);
+            const uint32_t mlo = __shfl((uint32_t)m, (int)L_, 64), mhi = __shfl((uint32_t)(m >> 32), (int)L_, 64);
+            k_ = 0;
+            if (x < total) {
+                unsigned long long mm = ((unsigned long long)mhi << 32) | mlo;
+                for (uint32_t u = 0; u < r_; u++) mm &= mm - 1ull;
+                k_ = A.nbr[(size_t)iq * A.list_cap + ((uint32_t)__ffsll((long long)mm) - 1u)];
+            }
+        };
+        locate(lane, L, r, k);
+        for (uint32_t b = 0; b < total; b += 64u) {
+            const uint32_t x = b + lane;
+            uint32_t Ln = 0, rn = 0, kn = 0;
+            if (b + 64u < total) locate(x + 64u, Ln, rn, kn);
+            const uint32_t iL = __shfl(i, (int)L, 64);
+            const uint32_t ocL = __shfl(old_count, (int)L, 64);
+            if (x < total) {
+                const SceneDev& S = A.scenes[scene_index(A, iL, MULTI)];
+                OwnState o;
+                o.pos = ld4(A.pos + 4 * (size_t)iL);
+                o.vel = ld3(A.vel + 3 * (size_t)iL);
+                o.mass = A.mass[iL];
+                const V3<double> iom = omega_of(ld4(A.ep + 4 * (size_t)iL), ld4(A.ev + 4 * (size_t)iL));
+                const uint32_t hb = iL * A.cap;
+                uint32_t slot = kInvalid;
+#pragma unroll
+                for (int q = (int)kPairIdRegs - 1; q >= 0; q--)   // downwards: the lowest match stays
+                    if ((uint32_t)q < ocL && st.ids[L][q] == k) slot = (uint32_t)q;
+                for (uint32_t q = kPairIdRegs; q < ocL && slot == kInvalid; q++)
+                    if (A.pp_id_in[hb + q] == k) slot = q;
+                double v[10];
+                const uint32_t kind = pair_of<T, MODEL, GPU_LAW>(A, S, iL, o, iom, k, r, slot, v);
+#pragma unroll
+                for (int q = 0; q < 10; q++) st.v[q][lane] = v[q];
+                st.kind[lane] = kind;
+            }
+            wave_lds_sync();
+            // this lane's sphere adds its pairs of this chunk, in rank order
+            const uint32_t lo = max(first, b), hi = min(incl, b + 64u);
+            for (uint32_t y = lo; y < hi; y++) {
+                const uint32_t q = y - b;
+                double v[10];
+#pragma unroll
+                for (int z = 0; z < 10; z++) v[z] = st.v[z][q];
+                const uint32_t kind = st.kind[q];
+                if (kind == kPairCoh) pp.F = pp.F - mk3(v[0], v[1], v[2]);
+                else sphere_pair_sum<T>(v, kind == kPairSet, A.stale_ft != 0, Ft, pp);
+            }
+            wave_lds_sync();
+            L = Ln; r = rn; k = kn;
+        }
+        if (i != kInvalid) {
+            const uint32_t n_new = min(c, A.cap);
+            A.pp_count_out[i] = n_new;
+            c0 += n_new;
+            uint32_t err = 0;
+            if (c > A.cap) err |= XDEM_ERR_P2P_OVERFLOW;
+            if (nlist_all > A.list_cap) err |= XDEM_ERR_NBR_OVERFLOW;
+            if (err) atomicOr(A.err, err);
+            double* ps = A.ppsum + (size_t)i * 10;
+            st3(ps, pp.F);
+            st3(ps + 3, pp.M);
+            st3(ps + 6, pp.T);
+            ps[9] = pp.R;
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) c0 += __shfl_xor(c0, off, 64);
