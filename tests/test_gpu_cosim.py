@@ -5,6 +5,9 @@ integrator under the contact wrench the step layer reduces every step
 
 - the native loop (xdem_cosim_wheel_steps) equals the same calls made one by one from Python,
   bit for bit;
+- with step graphs enabled, the loop's single steps replay one-step HIP graphs (one per buffer
+  parity) and the pose is handed to the device as a kernel argument without a host wait: the
+  run equals the eager one bit for bit;
 - against the oracle driving the same integrator: the wrench agrees to the reduction's rounding
   every step, so the wheel's pose and the spheres' state agree to 1e-12 m after 30 steps.
 """
@@ -54,6 +57,27 @@ def test_native_loop_equals_python_loop():
     assert a.error_word() == 0
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_one_step_graphs_equal_eager_steps(precision):
+    scene, tris, dbi, cp = setup()
+    runs = []
+    for sg in (0, 50):
+        ctx = DemContext(scene, precision=precision, fuse_velocity=True, step_graph=sg, meshes=[(0, tris, dbi, cp)])
+        w = wheel(scene, dbi)
+        track = cosim_wheel_steps(ctx, w, 0, 31)
+        runs.append((track, {f: ctx.read(f) for f in ("pos", "vel", "ep", "ev", "acc", "ea")}, w.state()[0],
+                     ctx.history("p2p"), ctx.history("mesh")))
+        assert ctx.error_word() == 0
+        ctx.close()
+    (ta, sa, pa, ha, ma), (tb, sb, pb, hb, mb) = runs
+    assert np.array_equal(ta, tb)
+    for f in sa:
+        assert np.array_equal(sa[f], sb[f]), f
+    assert np.array_equal(pa, pb)
+    for x, y in zip(ha + ma, hb + mb):
+        assert np.array_equal(x, y)
 
 
 def test_coupled_loop_matches_oracle():

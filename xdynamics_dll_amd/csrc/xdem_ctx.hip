@@ -52,6 +52,18 @@ struct WrenchSlot {
     uint32_t begin, end, mask, pad;
 };
 
+// A body pose handed to the device by value (kernel argument): the host's array may change
+// as soon as the launch returns, with no copy in flight and no wait.
+struct PoseArg {
+    double v[15];
+};
+
+__global__ void k_set_mesh_pose(MeshDev* meshes, uint32_t m, PoseArg p)
+{
+    const uint32_t q = threadIdx.x;
+    if (q < 15) meshes[m].dbi[q] = p.v[q];
+}
+
 // Triangles and proxy centres of mesh m at its pose (impl:2544-2601; the proxies of the first
 // mesh are the reference's proxy spheres 0..T-1, tid = index).
 __global__ __launch_bounds__(256) void k_mesh_pose(const MeshDev* meshes, uint32_t m, const double* vlist,
@@ -231,6 +243,7 @@ struct xdem_ctx {
     double2* pl_tsd = nullptr;
     double* wrench_pp = nullptr;
     double* wrench_out = nullptr;
+    double* h_wrench = nullptr;   // pinned host copy of one body's wrench (the coupled loop's readback)
     double* wrench_partial = nullptr;
     WrenchSlot* d_wslots = nullptr;
     // meshes (all scenes' triangles concatenated)
@@ -254,6 +267,11 @@ struct xdem_ctx {
     // xdem_ctx_step's own graph of auto_steps steps (replayed when the parities match)
     hipGraph_t agraph = nullptr;
     hipGraphExec_t agexec = nullptr;
+    // one-step graphs for runs shorter than auto_steps (a coupled loop steps one at a time): a
+    // step flips every buffer parity, so one graph per position parity covers every step
+    hipGraph_t sgraph[2] = {nullptr, nullptr};
+    hipGraphExec_t sgexec[2] = {nullptr, nullptr};
+    int spar[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     unsigned auto_steps = 0;
     int apar[4] = {0, 0, 0, 0};
     int graph_parity = 0, graph_sparity = 0, graph_pparity = 0;
@@ -499,8 +517,13 @@ int xdem_ctx_destroy(xdem_ctx* c)
     if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->agexec) (void)hipGraphExecDestroy(c->agexec);
     if (c->agraph) (void)hipGraphDestroy(c->agraph);
+    for (int q = 0; q < 2; q++) {
+        if (c->sgexec[q]) (void)hipGraphExecDestroy(c->sgexec[q]);
+        if (c->sgraph[q]) (void)hipGraphDestroy(c->sgraph[q]);
+    }
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
+    if (c->h_wrench) (void)hipHostFree(c->h_wrench);
     if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
@@ -738,6 +761,7 @@ int xdem_ctx_commit(xdem_ctx* c)
     if (c->nwslot) {
         CTX_CHECK(c->alloc(&c->wrench_pp, sizeof(double) * 6 * n * c->nwslot));
         CTX_CHECK(c->alloc(&c->wrench_out, sizeof(double) * 6 * c->nwslot));
+        CTX_CHECK(hipHostMalloc(reinterpret_cast<void**>(&c->h_wrench), sizeof(double) * 6, hipHostMallocDefault));
         CTX_CHECK(c->alloc(&c->wrench_partial, sizeof(double) * 6 * kWrenchBlocks * c->nwslot));
         CTX_CHECK(c->alloc(&c->d_wslots, sizeof(WrenchSlot) * c->nwslot));
     }
@@ -883,9 +907,10 @@ int xdem_ctx_commit(xdem_ctx* c)
     return XDEM_OK;
 }
 
-// Records nsteps steps (even) into a HIP graph without running them: the drift must be valid
-// (it is computed first otherwise); the parities (history, state, position, rebuild flag)
-// the graph's buffers assume are returned in par and restored afterwards.
+// Records nsteps steps into a HIP graph without running them: the drift must be valid (it is
+// computed first otherwise); the parities (history, state, position, rebuild flag) the graph's
+// buffers assume are returned in par and restored afterwards. An even count leaves them as it
+// found them (the graph can be replayed back to back); one step flips them all.
 static int capture_steps(xdem_ctx* c, unsigned nsteps, hipGraph_t* graph, hipGraphExec_t* exec, int par[4])
 {
     if (!c->drift_valid) CTX_CHECK(c->run_drift());   // the graph's steps start from a precomputed drift
@@ -944,6 +969,25 @@ static int run_steps(xdem_ctx* c, unsigned nsteps)
                 CTX_CHECK(hipGraphLaunch(c->agexec, c->stream));
                 c->steps_done += K;
                 k += K;
+                continue;
+            }
+        }
+        if (!time_it && K && c->drift_valid) {
+            const int q = c->spp;
+            if (!c->sgexec[q]) {
+                const int rc = capture_steps(c, 1, &c->sgraph[q], &c->sgexec[q], c->spar[q]);
+                if (rc != XDEM_OK) return rc;
+            }
+            if (parities_match(c, c->spar[q])) {
+                CTX_CHECK(hipGraphLaunch(c->sgexec[q], c->stream));
+                // the parities the step leaves (collide_half's flips)
+                c->rbp ^= 1;
+                c->set_pos_parity(c->spp ^ 1);
+                c->cur ^= 1;
+                if (c->cfg.fuse_velocity) c->set_state_parity(c->sv ^ 1);
+                c->drift_valid = true;
+                c->steps_done++;
+                k++;
                 continue;
             }
         }
@@ -1404,12 +1448,13 @@ int xdem_ctx_set_mesh_pose(xdem_ctx* c, unsigned int mesh, const double dbi[15])
     std::memcpy(m.dev.dbi, dbi, sizeof(double) * 15);
     if (c->committed) {
         CTX_CHECK(hipSetDevice(c->cfg.device));
-        CTX_CHECK(hipMemcpyAsync(c->d_meshes + mesh, &m.dev, sizeof(MeshDev), hipMemcpyHostToDevice, c->stream));
+        PoseArg pa;
+        std::memcpy(pa.v, dbi, sizeof(pa.v));
+        hipLaunchKernelGGL(k_set_mesh_pose, dim3(1), dim3(64), 0, c->stream, c->d_meshes, mesh, pa);
         hipLaunchKernelGGL(k_mesh_pose, dim3((m.dev.ntri + 255) / 256), dim3(256), 0, c->stream, c->d_meshes, mesh,
                            c->d_vlist, c->d_plocal, c->d_prad, c->d_tri, c->d_prox);
         CTX_CHECK(hipGetLastError());
         CTX_CHECK(c->body_moved(dmax));
-        CTX_CHECK(hipStreamSynchronize(c->stream));   // the host's pose copy may change after return
     }
     return XDEM_OK;
 }
@@ -1423,8 +1468,10 @@ int xdem_ctx_mesh_wrench(xdem_ctx* c, unsigned int mesh, double* out)
     for (int q = 0; q < 6; q++) out[q] = 0.0;
     if (slot < 0) return XDEM_OK;
     CTX_CHECK(hipSetDevice(c->cfg.device));
-    CTX_CHECK(hipMemcpyAsync(out, c->wrench_out + 6 * (size_t)slot, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipMemcpyAsync(c->h_wrench, c->wrench_out + 6 * (size_t)slot, sizeof(double) * 6, hipMemcpyDeviceToHost,
+                             c->stream));
     CTX_CHECK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 6; q++) out[q] = c->h_wrench[q];
     return XDEM_OK;
 }
 
@@ -1539,8 +1586,10 @@ int xdem_ctx_cylinder_wrench(xdem_ctx* c, unsigned int cyl, double* out)
     for (int q = 0; q < 6; q++) out[q] = 0.0;
     if (slot < 0) return XDEM_OK;
     CTX_CHECK(hipSetDevice(c->cfg.device));
-    CTX_CHECK(hipMemcpyAsync(out, c->wrench_out + 6 * (size_t)slot, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipMemcpyAsync(c->h_wrench, c->wrench_out + 6 * (size_t)slot, sizeof(double) * 6, hipMemcpyDeviceToHost,
+                             c->stream));
     CTX_CHECK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 6; q++) out[q] = c->h_wrench[q];
     return XDEM_OK;
 }
 
