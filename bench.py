@@ -460,8 +460,9 @@ def wheel_main(args):
     park[0], park[7] = 40.0, 1.0
     park[1:4] = (xc, yc, 1.0)          # above the bin while the bed settles
     cp_wheel = contact_property(2e11, 5e7, 0.3, 0.25, 0.4, 0.6, 0.05)
+    quirks = dict(gpu_force_law=1) if args.force_law == "kernel" else None
     ctx = DemContext(scene, precision="mixed", p2p_capacity=args.capacity, fuse_velocity=True, step_graph=50,
-                     meshes=[(0, tris, park, cp_wheel)], full_blocks=args.full_blocks)
+                     meshes=[(0, tris, park, cp_wheel)], full_blocks=args.full_blocks, quirks=quirks)
     t_build = time.perf_counter() - t_build
     t0 = time.perf_counter()
     done = 0
@@ -476,8 +477,9 @@ def wheel_main(args):
     vel = ctx.read("vel")
     under = (np.abs(pos[:, 0] - xc) < 0.08) & (np.abs(pos[:, 1] - yc) < 0.06)
     top = float((pos[under, 2] + pos[under, 3]).max())
-    bed = {"top_m": top, "max_speed_after_settle": float(np.linalg.norm(vel, axis=1).max()),
-           "mean_speed_after_settle": float(np.linalg.norm(vel, axis=1).mean())}
+    speed = np.linalg.norm(vel, axis=1)
+    bed = {"top_m": top, "max_speed_after_settle": float(speed.max()), "mean_speed_after_settle": float(speed.mean()),
+           "kinetic_energy_after_settle_J": float(0.5 * (scene.mass * speed ** 2).sum())}
     omega = 2.0
     start = park.copy()
     start[3] = top + R - 1e-3
@@ -519,7 +521,8 @@ def wheel_main(args):
         "config": {"workload": f"wheel on a granular bed: {n:,} spheres r~U[2,3] mm settled {args.settle} steps in a "
                                f"{L:.2f}x{W:.2f} m bin, 4,096-triangle grousered wheel R=0.15 m, 40 kg, "
                                "2 rad/s about y, 200 N load, HHT alpha=-0.1, mu=0.6, dt=1e-5 s, mixed",
-                   "np": n, "ntriangle": len(tris), "parallelism": "single GPU + host HHT"},
+                   "np": n, "ntriangle": len(tris), "parallelism": "single GPU + host HHT",
+                   "force_law": args.force_law},
         "roofline": dominant_roofline(kr),
         "kernels": kr,
         "mean_contacts_per_sphere": (n_pp + n_wall + n_mesh) / n,
@@ -724,6 +727,9 @@ def main():
                          "wheel: the wheel on a 500,000-sphere bed with HHT coupling; lattice: the 10,000-sphere "
                          "lattice box in FP64; calib: loss and adjoint gradient of the 20,000-sphere pile")
     ap.add_argument("--settle", type=int, default=20000, help="untimed settling steps of the wheel's bed")
+    ap.add_argument("--force-law", choices=["cpu", "kernel"], default="cpu",
+                    help="wheel: the damping law of the sphere contacts, the CPU path's (xContact.cpp:238-264, "
+                         "the parity profile) or the GPU kernel's (impl:432-474)")
     ap.add_argument("--calib-steps", type=int, default=16000, help="steps of the calibration workload")
     ap.add_argument("--calib-dt", type=float, default=5e-6, help="time step of the calibration workload")
     ap.add_argument("--calib-horizon", type=int, default=100,
