@@ -271,6 +271,24 @@ def measured_traffic(np_, fused):
     return (k["fetch_kib"] * t["copy_scale_fetch"] + k["write_kib"] * t["copy_scale_write"]) * 1024.0
 
 
+WHEEL_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_wheel_500k.json")
+
+
+def wheel_traffic(np_, kernel="k_pairs"):
+    """HBM bytes per dispatch of a full-pass kernel of the wheel workload from the committed
+    rocprofv3 --pmc summary (scaled as in measured_traffic), or None when the summary is of
+    another bed size."""
+    try:
+        with open(WHEEL_TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k or t.get("np") != np_:
+        return None
+    return (k["fetch_kib"] * t["copy_scale_fetch"] + k["write_kib"] * t["copy_scale_write"]) * 1024.0
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of mib MiB, repeated 5 times: the calibration kernel
     for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -523,7 +541,9 @@ def wheel_main(args):
                                "2 rad/s about y, 200 N load, HHT alpha=-0.1, mu=0.6, dt=1e-5 s, mixed",
                    "np": n, "ntriangle": len(tris), "parallelism": "single GPU + host HHT",
                    "force_law": args.force_law},
-        "roofline": dominant_roofline(kr),
+        "roofline": dominant_roofline(kr, {"k_pairs": wheel_traffic(n, "k_pairs"),
+                                           "k_collide": wheel_traffic(n, "k_collide")}
+                                      if args.force_law == "cpu" else None),
         "kernels": kr,
         "mean_contacts_per_sphere": (n_pp + n_wall + n_mesh) / n,
         "sphere_contacts": n_pp, "wall_contacts": n_wall, "mesh_contacts": n_mesh,
