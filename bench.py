@@ -491,6 +491,9 @@ def wheel_main(args):
         done += k
         print(f"[wheel] settled {done} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     t_settle = time.perf_counter() - t0
+    # the settled bed's spheres re-sorted by their current buckets (the stored order was that
+    # of the loose column)
+    ctx.reorder()
     pos = ctx.read("pos")
     vel = ctx.read("vel")
     under = (np.abs(pos[:, 0] - xc) < 0.08) & (np.abs(pos[:, 1] - yc) < 0.06)

@@ -171,6 +171,17 @@ typedef struct xdem_ctx_config {
                                     kernel walks their pairs itself instead of a pair pass of
                                     its own (0 = 65536; 1 = never). Results do not depend on
                                     it. */
+    int keep_order;              /* 0 (default): the context stores each scene's spheres in the
+                                    bucket order of their initial positions (a stable sort on
+                                    the bucket key at commit), so that the neighbours a kernel
+                                    gathers lie close in memory whatever order the caller gave
+                                    them in; 1: in the caller's order. Every call that takes or
+                                    returns per-sphere data (read, write, histories, part
+                                    files) uses the caller's order either way, the reference's
+                                    visiting order breaks its ties by the caller's ids, and
+                                    results do not depend on it. Device pointers
+                                    (xdem_ctx_device_ptr) address the stored order:
+                                    xdem_ctx_sphere_order maps it. */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */
@@ -242,14 +253,26 @@ int xdem_ctx_synchronize(xdem_ctx* ctx);
 /* Copies a field of one scene (its particles in input order) to/from host memory. */
 int xdem_ctx_read(xdem_ctx* ctx, int field, unsigned int scene, double* host_out);
 int xdem_ctx_write(xdem_ctx* ctx, int field, unsigned int scene, const double* host_in);
-/* Device pointer of a field over all scenes (scenes concatenated in order). With
- * fuse_velocity = 1, XDEM_FIELD_VEL and XDEM_FIELD_EV alternate between two buffers from
- * step to step: fetch the pointer again after every step. */
+/* Device pointer of a field over all scenes (scenes concatenated in order), in the stored
+ * sphere order (see keep_order and xdem_ctx_sphere_order). With fuse_velocity = 1,
+ * XDEM_FIELD_VEL and XDEM_FIELD_EV alternate between two buffers from step to step: fetch the
+ * pointer again after every step. */
 int xdem_ctx_device_ptr(xdem_ctx* ctx, int field, void** out);
+/* The stored order of a committed scene's spheres: order[j] (np values) is the caller's index
+ * of the sphere stored j-th (row j of the scene's part of a device pointer). No counterpart in
+ * the reference, which keeps its callers' order; it is for callers that use device pointers
+ * with keep_order = 0. */
+int xdem_ctx_sphere_order(xdem_ctx* ctx, unsigned int scene, unsigned int* order);
+/* Re-sorts the stored order by the spheres' current positions (a stable sort on the bucket key,
+ * each scene in its own range), for a pack that has mixed or settled since commit; the next
+ * step rebuilds the lists. Call it between steps. Results do not depend on it. No counterpart
+ * in the reference; it allocates and frees a temporary buffer (the widest per-sphere row
+ * times the sphere count). */
+int xdem_ctx_reorder(xdem_ctx* ctx);
 /* History of one scene in the reference layout: count[np], ids[np*cap], tsd[2*np*cap]
  * (cap = p2p_capacity for XDEM_HISTORY_P2P, 3 for XDEM_HISTORY_PLANE, 5 for XDEM_HISTORY_MESH,
  * 3 for XDEM_HISTORY_CYLINDER).
- * Sphere ids are scene-local particle indices. */
+ * Sphere ids are scene-local particle indices; sphere-pair slots past a count read as zeros. */
 int xdem_ctx_read_history(xdem_ctx* ctx, int kind, unsigned int scene, unsigned int* count, unsigned int* ids,
                           double* tsd);
 /* Writes one scene's history in the same layout (the inverse of xdem_ctx_read_history; the

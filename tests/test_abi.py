@@ -109,3 +109,36 @@ def test_reference_named_shim_compiles_and_links(tmp_path):
            f"-L{libdir}", "-lxdem", f"-Wl,-rpath,{libdir}", "-Wl,--allow-shlib-undefined"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+CONFIG_LAYOUT = r"""
+#include "xdem_capi.h"
+#include <stddef.h>
+#include <stdio.h>
+int main(void)
+{
+    printf("%d %d %d\n", (int)sizeof(xdem_ctx_config), (int)offsetof(xdem_ctx_config, merge_pairs_below),
+           (int)offsetof(xdem_ctx_config, keep_order));
+    return 0;
+}
+"""
+
+
+def test_context_config_layout_matches_binding(tmp_path):
+    """xdem_ctx_config as C lays it out and as the ctypes binding does (size and the offsets of
+    its last fields)."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "layout.c"
+    src.write_text(CONFIG_LAYOUT)
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c11", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    size, off_merge, off_keep = map(int, subprocess.run([str(exe)], capture_output=True, text=True).stdout.split())
+    assert size == ctypes.sizeof(xd._lib.CtxConfig)
+    assert off_merge == xd._lib.CtxConfig.merge_pairs_below.offset
+    assert off_keep == xd._lib.CtxConfig.keep_order.offset

@@ -147,7 +147,7 @@ class CtxConfig(C.Structure):
                 ("block_size", C.c_uint), ("grid_dim_max", C.c_uint), ("fuse_velocity", C.c_int),
                 ("device", C.c_int), ("buckets_per_sphere", C.c_uint), ("skin", C.c_double),
                 ("step_graph", C.c_int), ("rebuild_wait_limit", C.c_uint), ("full_blocks", C.c_uint),
-                ("merge_pairs_below", C.c_uint)]
+                ("merge_pairs_below", C.c_uint), ("keep_order", C.c_int)]
 
 
 class CtxInfo(C.Structure):
@@ -219,6 +219,8 @@ CTX_SIGNATURES = {
     "xdem_ctx_read": [P, I, U, P],
     "xdem_ctx_write": [P, I, U, P],
     "xdem_ctx_device_ptr": [P, I, C.POINTER(P)],
+    "xdem_ctx_sphere_order": [P, U, C.POINTER(C.c_uint)],
+    "xdem_ctx_reorder": [P],
     "xdem_ctx_read_history": [P, I, U, P, P, P],
     "xdem_ctx_write_history": [P, I, U, P, P, P],
     "xdem_ctx_scene_info_get": [P, U, C.POINTER(CtxSceneInfo)],

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <numeric>
 #include <vector>
 
 #include "xdem_capi.h"
@@ -145,6 +146,36 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
     if (i < n) p[i] = v;
 }
 
+// Rows of w 32-bit words in a new order: dst row j = src row perm[j].
+__global__ void k_gather_rows(const uint32_t* src, uint32_t* dst, const uint32_t* perm, size_t n, uint32_t w)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * w) return;
+    const size_t j = t / w, c = t - j * w;
+    dst[t] = src[(size_t)perm[j] * w + c];
+}
+
+// Columns of n doubles (structure-of-arrays) in a new order: dst[col][j] = src[col][perm[j]].
+__global__ void k_gather_cols(const double* src, double* dst, const uint32_t* perm, size_t n, uint32_t ncol)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * ncol) return;
+    const size_t col = t / n, j = t - col * n;
+    dst[t] = src[col * n + perm[j]];
+}
+
+// Sphere-pair slot ids in a new order: row j from row perm[j], each stored id below the (new)
+// count renamed by inv (old stored index -> new).
+__global__ void k_gather_pair_ids(const uint32_t* src, uint32_t* dst, const uint32_t* count, const uint32_t* perm,
+                                  const uint32_t* inv, size_t n, uint32_t cap)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * cap) return;
+    const size_t j = t / cap, q = t - j * cap;
+    const uint32_t v = src[(size_t)perm[j] * cap + q];
+    dst[t] = q < count[j] ? inv[v] : v;
+}
+
 // Rotation matrix of Euler parameters e (xPointMass::setupTransformationMatrix,
 // source/xdynamics_object/xPointMass.cpp:296-301): o = A v, or A^T v when transpose is set.
 void euler_rotate(const double* e, const double v[3], bool transpose, double o[3])
@@ -210,6 +241,11 @@ struct xdem_ctx {
     uint32_t* grid_bar = nullptr;                        // k_rebuild's phase cursors and flags
     uint32_t rebuild_blocks = 0;                         // k_rebuild's grid
     unsigned long long* pass_mask = nullptr;
+    // stored order: stored sphere -> the caller's global index (scene begin + caller's index)
+    // and back; the same ranges per scene as the caller's
+    std::vector<uint32_t> ext_of, int_of;
+    uint32_t *d_ext = nullptr, *d_int = nullptr;
+    double* d_jflat = nullptr;   // quirk Q5's rolling inertia per stored sphere
     uint32_t seg_cap = 0;
     uint32_t list_cap = 48;
     uint32_t *bucket_count = nullptr, *bucket_start = nullptr, *scan_part = nullptr;
@@ -308,6 +344,7 @@ struct xdem_ctx {
         A.seg_count = seg_count; A.seg_cap = seg_cap; A.work2 = work2; A.seg_count2 = seg_count2;
         A.grid_bar = grid_bar; A.rebuild_blocks = rebuild_blocks; A.rebuild_wait_limit = cfg.rebuild_wait_limit;
         A.pass_mask = pass_mask;
+        A.ext_id = d_ext; A.int_id = d_int; A.jflat = d_jflat;
         A.sposf = sposf;
         A.rebuild_cur = rebuild + rbp; A.rebuild_next = rebuild + (rbp ^ 1); A.n_rebuild = n_rebuild;
         A.lctl = lctl;
@@ -710,6 +747,9 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->seg_count, sizeof(uint32_t) * kQueueSegs * kSegStride));
     CTX_CHECK(c->alloc(&c->work2, sizeof(uint32_t) * kQueueSegs * c->seg_cap));
     CTX_CHECK(c->alloc(&c->pass_mask, sizeof(unsigned long long) * n));
+    CTX_CHECK(c->alloc(&c->d_ext, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->d_int, sizeof(uint32_t) * n));
+    CTX_CHECK(c->alloc(&c->d_jflat, sizeof(double) * n));
     CTX_CHECK(c->alloc(&c->seg_count2, sizeof(uint32_t) * kQueueSegs * kSegStride));
     CTX_CHECK(c->alloc(&c->grid_bar, sizeof(uint32_t) * kSegStride * kBarSlots));
     c->rebuild_blocks = rebuild_grid_size(c->multi);
@@ -806,18 +846,42 @@ int xdem_ctx_commit(xdem_ctx* c)
     // host staging of the initial state (xDiscreteElementMethodSimulation.cpp:60-83)
     std::vector<double> pos(4 * n), vel(3 * n), acc(3 * n, 0.0), ep(4 * n, 0.0), mass(n), inertia(3 * n);
     std::vector<uint32_t> scene_of(c->multi ? n : 0);
+    std::vector<double> jflat(n);
     std::vector<SceneDev> sd;
+    c->ext_of.assign(n, 0u);
+    c->int_of.assign(n, 0u);
     for (size_t si = 0; si < c->scenes.size(); si++) {
         const SceneHost& h = c->scenes[si];
         const size_t b = h.p_begin, m = h.s.np;
-        std::copy(h.pos.begin(), h.pos.end(), pos.begin() + 4 * b);
-        std::copy(h.vel.begin(), h.vel.end(), vel.begin() + 3 * b);
-        std::copy(h.mass.begin(), h.mass.end(), mass.begin() + b);
-        std::copy(h.inertia.begin(), h.inertia.end(), inertia.begin() + 3 * b);
-        for (size_t i = 0; i < m; i++) {
-            acc[3 * (b + i) + 1] = h.s.gravity[1];  // acc = (0, g.y, 0) as the reference initialises it
-            ep[4 * (b + i)] = 1.0;
-            if (c->multi) scene_of[b + i] = (uint32_t)si;
+        // stored order: the caller's, or a stable sort on the bucket key of the initial
+        // position (the order of the grid's sorted slots at the first build)
+        std::vector<uint32_t> ord(m);
+        std::iota(ord.begin(), ord.end(), 0u);
+        if (!c->cfg.keep_order) {
+            const Grid& g = h.dev.grid;
+            std::vector<uint32_t> key(m);
+            for (size_t e = 0; e < m; e++) {
+                const int gx = (int)std::floor((h.pos[4 * e] - g.wo_x) / g.cs);
+                const int gy = (int)std::floor((h.pos[4 * e + 1] - g.wo_y) / g.cs);
+                const int gz = (int)std::floor((h.pos[4 * e + 2] - g.wo_z) / g.cs);
+                key[e] = ((uint32_t)gz & g.mask_z) * g.dim_xy + ((uint32_t)gy & g.mask_y) * g.dim_x +
+                         ((uint32_t)gx & g.mask_x);
+            }
+            std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
+        }
+        for (size_t j = 0; j < m; j++) {
+            const size_t e = ord[j], i = b + j;
+            c->ext_of[i] = (uint32_t)(b + e);
+            c->int_of[b + e] = (uint32_t)i;
+            std::copy(h.pos.begin() + 4 * e, h.pos.begin() + 4 * e + 4, pos.begin() + 4 * i);
+            std::copy(h.vel.begin() + 3 * e, h.vel.begin() + 3 * e + 3, vel.begin() + 3 * i);
+            mass[i] = h.mass[e];
+            std::copy(h.inertia.begin() + 3 * e, h.inertia.begin() + 3 * e + 3, inertia.begin() + 3 * i);
+            // quirk Q5 reads the scene's inertia array as flat doubles at the caller's index
+            jflat[i] = h.inertia[e];
+            acc[3 * i + 1] = h.s.gravity[1];  // acc = (0, g.y, 0) as the reference initialises it
+            ep[4 * i] = 1.0;
+            if (c->multi) scene_of[i] = (uint32_t)si;
         }
         sd.push_back(h.dev);
     }
@@ -828,6 +892,9 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(hipMemcpyAsync(c->ep, ep.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, s));
     CTX_CHECK(hipMemcpyAsync(c->mass, mass.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     CTX_CHECK(hipMemcpyAsync(c->inertia, inertia.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemcpyAsync(c->d_ext, c->ext_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemcpyAsync(c->d_int, c->int_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    CTX_CHECK(hipMemcpyAsync(c->d_jflat, jflat.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     if (c->multi) CTX_CHECK(hipMemcpyAsync(c->scene_of, scene_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
     CTX_CHECK(hipMemcpyAsync(c->d_scenes, sd.data(), sizeof(SceneDev) * sd.size(), hipMemcpyHostToDevice, s));
     if (!c->walls.empty())
@@ -1094,9 +1161,13 @@ int xdem_ctx_read(xdem_ctx* c, int field, unsigned int scene, double* out)
     int w;
     if (field_info(c, field, &p, &w)) return XDEM_E_INVALID;
     const SceneHost& h = c->scenes[scene];
+    const size_t b = h.p_begin, m = h.s.np;
+    std::vector<double> tmp((size_t)w * m);
     CTX_CHECK(hipSetDevice(c->cfg.device));
-    CTX_CHECK(hipMemcpyAsync(out, p + (size_t)w * h.p_begin, sizeof(double) * w * h.s.np, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipMemcpyAsync(tmp.data(), p + (size_t)w * b, sizeof(double) * w * m, hipMemcpyDeviceToHost, c->stream));
     CTX_CHECK(hipStreamSynchronize(c->stream));
+    for (size_t j = 0; j < m; j++)   // stored row j is the caller's row ext_of - b
+        std::copy(tmp.begin() + w * j, tmp.begin() + w * (j + 1), out + (size_t)w * (c->ext_of[b + j] - b));
     return XDEM_OK;
 }
 
@@ -1109,8 +1180,12 @@ int xdem_ctx_write(xdem_ctx* c, int field, unsigned int scene, const double* in)
     int w;
     if (field_info(c, field, &p, &w)) return XDEM_E_INVALID;
     const SceneHost& h = c->scenes[scene];
+    const size_t b = h.p_begin, m = h.s.np;
+    std::vector<double> tmp((size_t)w * m);
+    for (size_t j = 0; j < m; j++)
+        std::copy(in + (size_t)w * (c->ext_of[b + j] - b), in + (size_t)w * (c->ext_of[b + j] - b + 1), tmp.begin() + w * j);
     CTX_CHECK(hipSetDevice(c->cfg.device));
-    CTX_CHECK(hipMemcpyAsync(p + (size_t)w * h.p_begin, in, sizeof(double) * w * h.s.np, hipMemcpyHostToDevice, c->stream));
+    CTX_CHECK(hipMemcpyAsync(p + (size_t)w * b, tmp.data(), sizeof(double) * w * m, hipMemcpyHostToDevice, c->stream));
     CTX_CHECK(hipStreamSynchronize(c->stream));
     c->drift_valid = false;   // the precomputed drift used the old state
     return XDEM_OK;
@@ -1125,6 +1200,130 @@ int xdem_ctx_device_ptr(xdem_ctx* c, int field, void** out)
     if (field_info(c, field, &p, &w)) return XDEM_E_INVALID;
     *out = p;
     c->drift_valid = false;   // the caller may write the state through the pointer
+    return XDEM_OK;
+}
+
+// Re-sorts the stored order by the spheres' current positions: every per-sphere array that a
+// step carries to the next is gathered into the new order (the sphere-pair slots' ids renamed),
+// and the next step rebuilds the lists.
+int xdem_ctx_reorder(xdem_ctx* c)
+{
+    if (!c) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    const size_t n = c->np;
+    CTX_CHECK(hipSetDevice(c->cfg.device));
+    hipStream_t s = c->stream;
+    std::vector<double> pos(4 * n);
+    CTX_CHECK(hipMemcpyAsync(pos.data(), c->pos, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, s));
+    CTX_CHECK(hipStreamSynchronize(s));
+    std::vector<uint32_t> perm(n), inv(n);
+    for (const SceneHost& h : c->scenes) {
+        const size_t b = h.p_begin, m = h.s.np;
+        const Grid& g = h.dev.grid;
+        std::vector<uint32_t> key(m), ord(m);
+        std::iota(ord.begin(), ord.end(), 0u);
+        for (size_t j = 0; j < m; j++) {
+            const double* p = &pos[4 * (b + j)];
+            const int gx = (int)std::floor((p[0] - g.wo_x) / g.cs);
+            const int gy = (int)std::floor((p[1] - g.wo_y) / g.cs);
+            const int gz = (int)std::floor((p[2] - g.wo_z) / g.cs);
+            key[j] = ((uint32_t)gz & g.mask_z) * g.dim_xy + ((uint32_t)gy & g.mask_y) * g.dim_x + ((uint32_t)gx & g.mask_x);
+        }
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
+        for (size_t j = 0; j < m; j++) {
+            perm[b + j] = (uint32_t)(b + ord[j]);
+            inv[b + ord[j]] = (uint32_t)(b + j);
+        }
+    }
+    // the caller's index of each stored sphere, and back
+    std::vector<uint32_t> ext(n);
+    for (size_t j = 0; j < n; j++) ext[j] = c->ext_of[perm[j]];
+    c->ext_of.swap(ext);
+    for (size_t j = 0; j < n; j++) c->int_of[c->ext_of[j]] = (uint32_t)j;
+
+    // the widest row in 32-bit words: a state row (8), a kind's history slots (4 each) or the
+    // wrench slots (12 per body)
+    const size_t words = std::max({(size_t)8, 4u * (size_t)c->cap, 4u * (size_t)XDEM_MAX_P2PL_COUNT,
+                                   4u * (size_t)XDEM_MAX_P2CY_COUNT, 4u * (size_t)XDEM_MAX_P2MS_COUNT,
+                                   12u * (size_t)c->nwslot});
+    uint32_t *d_perm = nullptr, *d_inv = nullptr, *tmp = nullptr;
+    hipError_t e = hipMalloc(&d_perm, sizeof(uint32_t) * n);
+    if (e == hipSuccess) e = hipMalloc(&d_inv, sizeof(uint32_t) * n);
+    if (e == hipSuccess) e = hipMalloc(&tmp, sizeof(uint32_t) * words * n);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_perm, perm.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_inv, inv.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s);
+    auto rows = [&](void* p, uint32_t w) {   // w 32-bit words per sphere
+        if (e != hipSuccess || !p) return;
+        const size_t t = n * w;
+        hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const uint32_t*>(p), tmp, d_perm, n, w);
+        e = hipMemcpyAsync(p, tmp, sizeof(uint32_t) * t, hipMemcpyDeviceToDevice, s);
+    };
+    rows(c->pos, 8);
+    rows(c->ep, 8);
+    if (c->drift_valid) {   // the next step's drifted positions
+        rows(c->pos_buf[c->spp ^ 1], 8);
+        rows(c->ep_buf[c->spp ^ 1], 8);
+    }
+    rows(c->vel, 6);
+    rows(c->ev, 8);
+    rows(c->acc, 6);
+    rows(c->ea, 8);
+    rows(c->force, 6);
+    rows(c->moment, 6);
+    rows(c->mass, 2);
+    rows(c->inertia, 6);
+    rows(c->tmax, 6);
+    rows(c->rres, 2);
+    rows(c->d_jflat, 2);
+    rows(c->nbr_count, 1);
+    rows(c->pp_count[c->cur], 1);
+    rows(c->pp_tsd[c->cur], 4 * c->cap);
+    if (e == hipSuccess) {   // the pair slots' ids, renamed below the (already gathered) count
+        const size_t t = n * c->cap;
+        hipLaunchKernelGGL(k_gather_pair_ids, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, c->pp_id[c->cur], tmp,
+                           c->pp_count[c->cur], d_perm, d_inv, n, c->cap);
+        e = hipMemcpyAsync(c->pp_id[c->cur], tmp, sizeof(uint32_t) * t, hipMemcpyDeviceToDevice, s);
+    }
+    rows(c->pl_count, 1);
+    rows(c->pl_id, XDEM_MAX_P2PL_COUNT);
+    rows(c->pl_tsd, 4 * XDEM_MAX_P2PL_COUNT);
+    if (!c->cyls.empty()) {
+        rows(c->cy_count, 1);
+        rows(c->cy_id, XDEM_MAX_P2CY_COUNT);
+        rows(c->cy_tsd, 4 * XDEM_MAX_P2CY_COUNT);
+    }
+    if (!c->meshes.empty()) {
+        rows(c->ms_count, 1);
+        rows(c->ms_id, XDEM_MAX_P2MS_COUNT);
+        rows(c->ms_tsd, 4 * XDEM_MAX_P2MS_COUNT);
+    }
+    if (e == hipSuccess && c->nwslot) {   // the wrench slots (columns of n)
+        const size_t t = n * 6 * c->nwslot;
+        hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, s, c->wrench_pp,
+                           reinterpret_cast<double*>(tmp), d_perm, n, 6u * c->nwslot);
+        e = hipMemcpyAsync(c->wrench_pp, tmp, sizeof(double) * t, hipMemcpyDeviceToDevice, s);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_ext, c->ext_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_int, c->int_of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {   // the next step builds the lists (it reads the flag of its parity)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, c->rebuild + (c->rbp ^ 1), 1u, (size_t)1);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+    (void)hipFree(d_inv);
+    (void)hipFree(d_perm);
+    return hip_status(e);
+}
+
+int xdem_ctx_sphere_order(xdem_ctx* c, unsigned int scene, unsigned int* order)
+{
+    if (!c || !order) return XDEM_E_INVALID;
+    if (!c->committed) return XDEM_E_NOT_READY;
+    if (scene >= c->scenes.size()) return XDEM_E_INVALID;
+    const SceneHost& h = c->scenes[scene];
+    for (size_t j = 0; j < h.s.np; j++) order[j] = c->ext_of[h.p_begin + j] - h.p_begin;
     return XDEM_OK;
 }
 
@@ -1152,13 +1351,27 @@ int xdem_ctx_read_history(xdem_ctx* c, int kind, unsigned int scene, unsigned in
     } else {
         return XDEM_E_INVALID;
     }
-    CTX_CHECK(hipMemcpyAsync(count, dc + b, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c->stream));
-    CTX_CHECK(hipMemcpyAsync(ids, di + b * cap, sizeof(uint32_t) * m * cap, hipMemcpyDeviceToHost, c->stream));
-    CTX_CHECK(hipMemcpyAsync(tsd, dt + b * cap, sizeof(double2) * m * cap, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> tc(m), ti(m * cap);
+    std::vector<double> tt(2 * m * cap);
+    CTX_CHECK(hipMemcpyAsync(tc.data(), dc + b, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipMemcpyAsync(ti.data(), di + b * cap, sizeof(uint32_t) * m * cap, hipMemcpyDeviceToHost, c->stream));
+    CTX_CHECK(hipMemcpyAsync(tt.data(), dt + b * cap, sizeof(double2) * m * cap, hipMemcpyDeviceToHost, c->stream));
     CTX_CHECK(hipStreamSynchronize(c->stream));
-    if (kind == XDEM_HISTORY_P2P)
-        for (size_t i = 0; i < m; i++)
-            for (uint32_t s = 0; s < count[i] && s < cap; s++) ids[i * cap + s] -= (uint32_t)b;
+    // stored row j is the caller's row ext_of - b; a sphere-pair slot holds a stored sphere,
+    // returned as the caller's scene-local index. The sphere-pair slots past the count (left
+    // over from earlier steps in the step's double buffer) read as zeros.
+    for (size_t j = 0; j < m; j++) {
+        const size_t e = c->ext_of[b + j] - b;
+        count[e] = tc[j];
+        for (uint32_t q = 0; q < cap; q++) {
+            const bool unused = kind == XDEM_HISTORY_P2P && q >= tc[j];
+            uint32_t id = ti[j * cap + q];
+            if (kind == XDEM_HISTORY_P2P) id = unused ? 0u : c->ext_of[id] - (uint32_t)b;
+            ids[e * cap + q] = id;
+            tsd[2 * (e * cap + q)] = unused ? 0.0 : tt[2 * (j * cap + q)];
+            tsd[2 * (e * cap + q) + 1] = unused ? 0.0 : tt[2 * (j * cap + q) + 1];
+        }
+    }
     return XDEM_OK;
 }
 
@@ -1192,19 +1405,30 @@ int xdem_ctx_write_history(xdem_ctx* c, int kind, unsigned int scene, const unsi
     }
     // a count past the slots or an id outside the scene would index past its arrays in the
     // contact kernels: refuse the whole history
-    std::vector<uint32_t> idv(ids, ids + m * cap);
-    for (size_t i = 0; i < m; i++) {
-        if (count[i] > cap) return XDEM_E_INVALID;
-        for (uint32_t s = 0; s < count[i]; s++) {
-            uint32_t& v = idv[i * cap + s];
-            if (id_limit != UINT32_MAX && v >= id_limit) return XDEM_E_INVALID;
-            if (kind == XDEM_HISTORY_P2P) v += (uint32_t)b;
+    for (size_t e = 0; e < m; e++) {
+        if (count[e] > cap) return XDEM_E_INVALID;
+        for (uint32_t q = 0; q < count[e]; q++)
+            if (id_limit != UINT32_MAX && ids[e * cap + q] >= id_limit) return XDEM_E_INVALID;
+    }
+    // the caller's row e goes to stored row int_of - b; a sphere-pair slot's caller's index to
+    // its stored sphere
+    std::vector<uint32_t> tc(m), ti(m * cap);
+    std::vector<double> tt(2 * m * cap);
+    for (size_t e = 0; e < m; e++) {
+        const size_t j = c->int_of[b + e] - b;
+        tc[j] = count[e];
+        for (uint32_t q = 0; q < cap; q++) {
+            uint32_t id = ids[e * cap + q];
+            if (kind == XDEM_HISTORY_P2P && q < count[e]) id = c->int_of[b + id];
+            ti[j * cap + q] = id;
+            tt[2 * (j * cap + q)] = tsd[2 * (e * cap + q)];
+            tt[2 * (j * cap + q) + 1] = tsd[2 * (e * cap + q) + 1];
         }
     }
     CTX_CHECK(hipSetDevice(c->cfg.device));
-    CTX_CHECK(hipMemcpyAsync(dc + b, count, sizeof(uint32_t) * m, hipMemcpyHostToDevice, c->stream));
-    CTX_CHECK(hipMemcpyAsync(di + b * cap, idv.data(), sizeof(uint32_t) * m * cap, hipMemcpyHostToDevice, c->stream));
-    CTX_CHECK(hipMemcpyAsync(dt + b * cap, tsd, sizeof(double2) * m * cap, hipMemcpyHostToDevice, c->stream));
+    CTX_CHECK(hipMemcpyAsync(dc + b, tc.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, c->stream));
+    CTX_CHECK(hipMemcpyAsync(di + b * cap, ti.data(), sizeof(uint32_t) * m * cap, hipMemcpyHostToDevice, c->stream));
+    CTX_CHECK(hipMemcpyAsync(dt + b * cap, tt.data(), sizeof(double2) * m * cap, hipMemcpyHostToDevice, c->stream));
     CTX_CHECK(hipStreamSynchronize(c->stream));
     return XDEM_OK;
 }

@@ -20,7 +20,19 @@ int xDiscreteElementMethodSimulation::Initialize(const xdem_scene* scenes, unsig
         ctx = nullptr;
     }
     isInitilize = false;
-    int rc = xdem_ctx_create(cfg, &ctx);
+    // Position() and Velocity() hand out device pointers, which the reference's callers index
+    // in their own particle order: the context keeps it (keep_order). Without a configuration,
+    // xdem_ctx_create's defaults: FP64, the parity quirks.
+    xdem_ctx_config local;
+    if (cfg) {
+        local = *cfg;
+    } else {
+        std::memset(&local, 0, sizeof(local));
+        local.quirks.stale_ft = 1;
+        local.quirks.flat_inertia = 1;
+    }
+    local.keep_order = 1;
+    int rc = xdem_ctx_create(&local, &ctx);
     if (rc) return rc;
     np = 0;
     scene_np.clear();

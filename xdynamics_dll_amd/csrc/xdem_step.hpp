@@ -152,6 +152,9 @@ struct StepArgs {
     uint32_t* work;                          // candidate spheres (pair search), by queue segment
     uint32_t* work2;                         // spheres of this step's full contact pass (k_touch)
     unsigned long long* pass_mask;           // per sphere of work2: list entries that pass (k_touch)
+    const uint32_t* ext_id;                  // per stored sphere: the caller's (global) index
+    const uint32_t* int_id;                  // per caller's (global) index: the stored sphere
+    const double* jflat;                     // per stored sphere: quirk Q5's rolling inertia
     uint32_t* nbr_count;                     // listed neighbours per sphere | kCandidate, kWallNear, kTouch,
                                              // kSpinStable, kAccStable, kNoContacts
     uint32_t* nbr;                           // their sphere ids, list_cap per sphere
@@ -459,8 +462,9 @@ __device__ __forceinline__ void load_own_fused(const StepArgs& A, const SceneDev
     o.ea = ld4(A.ea + 4 * i);
     o.acc = ld3(A.acc + 3 * i);
     o.inertia = ld3(A.inertia + 3 * i);
-    // quirk Q5: the scene's inertia array read as flat doubles at the scene-local id
-    o.J = A.flat_inertia ? A.inertia[i + 2 * S.p_begin] : o.inertia.x;
+    // quirk Q5: the scene's inertia array read as flat doubles at the scene-local id (in the
+    // caller's order), taken at commit
+    o.J = A.flat_inertia ? A.jflat[i] : o.inertia.x;
 }
 
 // FUSED_TOO: also load the velocity-update fields now (the lite pass, which keeps few
@@ -508,8 +512,9 @@ __device__ __forceinline__ uint32_t visit_code(const SceneDev& S, int ix, int iy
 }
 
 // Puts a neighbour list in the reference's visiting order at the current positions: codes
-// are packed above the ids, the list is insertion-sorted in place (it is short and this
-// thread's own), then the codes are removed. k_touch does it when the pairs of a list that
+// are packed above the caller's ids (the reference's ties within a cell), the list is
+// insertion-sorted in place (it is short and this thread's own), then each entry is turned
+// back into the stored sphere. k_touch does it when the pairs of a list that
 // pass are not in that order (after a build, the lists are in bucket order; later, a sphere
 // may have changed cells).
 __device__ __forceinline__ void order_list(const StepArgs& A, const SceneDev& S, uint32_t* lst, uint32_t n, int ix,
@@ -517,7 +522,7 @@ __device__ __forceinline__ void order_list(const StepArgs& A, const SceneDev& S,
 {
     for (uint32_t t = 0; t < n; t++) {
         const uint32_t k = lst[t];
-        lst[t] = (visit_code(S, ix, iy, iz, ld4(A.pos + 4 * (size_t)k)) << kIdBits) | k;
+        lst[t] = (visit_code(S, ix, iy, iz, ld4(A.pos + 4 * (size_t)k)) << kIdBits) | A.ext_id[k];
     }
     for (uint32_t t = 1; t < n; t++) {
         const uint32_t x = lst[t];
@@ -528,7 +533,7 @@ __device__ __forceinline__ void order_list(const StepArgs& A, const SceneDev& S,
         }
         lst[u] = x;
     }
-    for (uint32_t t = 0; t < n; t++) lst[t] &= kIdMask;
+    for (uint32_t t = 0; t < n; t++) lst[t] = A.int_id[lst[t] & kIdMask];
 }
 
 // The reference's pair predicate: overlap, or inside the JKR separation depth of a cohesive
@@ -584,7 +589,7 @@ __device__ __forceinline__ bool touch_sphere(const StepArgs& A, uint32_t i, uint
             const uint32_t code = visit_code(S, ix, iy, iz, pj);
             if (code == kNotVisited) continue;
             m |= 1ull << t;
-            const uint32_t key = (code << kIdBits) | k;
+            const uint32_t key = (code << kIdBits) | A.ext_id[k];
             if (have && key <= prev) ordered = false;
             prev = key;
             have = true;
@@ -1576,7 +1581,7 @@ __global__ __launch_bounds__(256) void k_finish(StepArgs A)
         const V4<double> e = ld4(A.ep + 4 * i);
         V4<double> av = ld4(A.ev + 4 * i);
         V3<double> M = ld3(A.moment + 3 * i);
-        const double J = A.flat_inertia ? A.inertia[i + 2 * S.p_begin] : A.inertia[3 * i];
+        const double J = A.flat_inertia ? A.jflat[i] : A.inertia[3 * i];
         if (rolling_moment(S.dt, J, omega_of(e, av), ld3(A.tmax + 3 * i), A.rres[i], M)) {
             st3(A.tmax + 3 * i, mk3(0.0, 0.0, 0.0));
             A.rres[i] = 0.0;

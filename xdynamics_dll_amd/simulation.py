@@ -25,7 +25,7 @@ class DemContext:
                  grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False,
                  buckets_per_sphere: int = 0, skin: float = 0.0, step_graph: int = 0,
                  rebuild_wait_limit: int = 0, meshes=(), full_blocks: int = 0, cylinders=(),
-                 merge_pairs_below: int = 0):
+                 merge_pairs_below: int = 0, keep_order: bool = False):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -45,6 +45,7 @@ class DemContext:
         cfg.rebuild_wait_limit = rebuild_wait_limit   # 0: ~1 s before a rebuild phase gives up
         cfg.full_blocks = full_blocks   # 0: 1024 blocks for the full-pass kernels
         cfg.merge_pairs_below = merge_pairs_below   # 0: 65536; 1: the pair pass always runs
+        cfg.keep_order = 1 if keep_order else 0   # 0: spheres stored in bucket order (API calls keep the caller's)
         self.precision = precision
         self.capacity = p2p_capacity
         self._h = C.c_void_p()
@@ -116,6 +117,17 @@ class DemContext:
         p = C.c_void_p()
         _lib.check(self.L.xdem_ctx_device_ptr(self._h, _lib.FIELD[name][0], C.byref(p)), "xdem_ctx_device_ptr")
         return p.value
+
+    def reorder(self) -> None:
+        """Re-sorts the stored sphere order by the current positions (xdem_ctx_reorder)."""
+        _lib.check(self.L.xdem_ctx_reorder(self._h), "xdem_ctx_reorder")
+
+    def sphere_order(self, scene: int = 0) -> np.ndarray:
+        """order[j]: the caller's index of the sphere stored j-th (the rows of device_ptr)."""
+        out = np.empty(self.scenes[scene].np, dtype=np.uint32)
+        _lib.check(self.L.xdem_ctx_sphere_order(self._h, scene, out.ctypes.data_as(C.POINTER(C.c_uint))),
+                   "xdem_ctx_sphere_order")
+        return out
 
     def history(self, kind: str = "p2p", scene: int = 0):
         n = self.scenes[scene].np
@@ -251,6 +263,7 @@ class xIntegratorVV:
         self._scenes: list[SceneData] = []
 
     def Initialize(self, scenes, precision: str = "fp64", **kw) -> int:
+        kw.setdefault("keep_order", True)   # Position()/Velocity() are device pointers in the caller's order
         self.ctx = DemContext(scenes, precision=precision, **kw)
         self._scenes = self.ctx.scenes
         return 0
