@@ -582,17 +582,28 @@ __device__ __forceinline__ bool touch_sphere(const StepArgs& A, uint32_t i, uint
         uint32_t prev = 0;
         bool have = false, ordered = true;
         unsigned long long m = 0;
-        for (uint32_t t = 0; t < n; t++) {
-            const uint32_t k = lst[t];
-            const V4<double> pj = ld4(A.pos + 4 * (size_t)k);
-            if (!pair_passes(S.cc, p.w, pj.w, pair_geom(p, pj))) continue;
-            const uint32_t code = visit_code(S, ix, iy, iz, pj);
-            if (code == kNotVisited) continue;
-            m |= 1ull << t;
-            const uint32_t key = (code << kIdBits) | A.ext_id[k];
-            if (have && key <= prev) ordered = false;
-            prev = key;
-            have = true;
+        // four list entries at a time: their ids, then their positions, are loaded together
+        for (uint32_t t0 = 0; t0 < n; t0 += 4u) {
+            uint32_t kk[4];
+            V4<double> pp[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; u++) kk[u] = t0 + u < n ? lst[t0 + u] : i;
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; u++) pp[u] = ld4(A.pos + 4 * (size_t)kk[u]);
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; u++) {
+                const uint32_t t = t0 + u;
+                if (t >= n) break;
+                const V4<double> pj = pp[u];
+                if (!pair_passes(S.cc, p.w, pj.w, pair_geom(p, pj))) continue;
+                const uint32_t code = visit_code(S, ix, iy, iz, pj);
+                if (code == kNotVisited) continue;
+                m |= 1ull << t;
+                const uint32_t key = (code << kIdBits) | A.ext_id[kk[u]];
+                if (have && key <= prev) ordered = false;
+                prev = key;
+                have = true;
+            }
         }
         if (!ordered) {
             // sort, then find the passing entries at their new places
