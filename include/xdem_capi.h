@@ -182,6 +182,12 @@ typedef struct xdem_ctx_config {
                                     results do not depend on it. Device pointers
                                     (xdem_ctx_device_ptr) address the stored order:
                                     xdem_ctx_sphere_order maps it. */
+    double skin_cap;             /* the adaptive skin's largest value as a fraction of each
+                                    scene's smallest radius (0 = 0.2); the skin adapts between
+                                    a sixteenth of it and it, and buckets are sized for it. A
+                                    larger cap means fewer list builds and more listed pairs
+                                    (xdem_autotune tries 0.2, 0.4 and 0.8). Results do not
+                                    depend on it. */
 } xdem_ctx_config;
 
 /* Fields for xdem_ctx_read / xdem_ctx_write / xdem_ctx_device_ptr (reference layouts). */
@@ -359,8 +365,8 @@ int xdem_ctx_graph_launch(xdem_ctx* ctx, unsigned int nlaunch);
  * into the contact passes), out[4] = per-step pair test (k_touch) ms, out[5] = k_lite_move
  * ms (fused mode; 0 otherwise), out[6] = k_collide ms and out[7] = the triangle pass before it
  * (k_mesh_pass, with the wait for the fork; about 0 without meshes) ms, both on the second
- * stream, out[8] = the sphere-pair pass (k_pairs, on the context's stream before the fork;
- * it returns at once on a small full pass, whose pairs k_collide walks) ms. */
+ * stream, out[8] = the sphere-pair pass (k_pairs, on the second stream between the two; it
+ * returns at once on a small full pass, whose pairs k_collide walks) ms. */
 int xdem_ctx_set_timing(xdem_ctx* ctx, int on);
 int xdem_ctx_timing(xdem_ctx* ctx, double out[9]);
 
@@ -369,13 +375,14 @@ int xdem_ctx_timing(xdem_ctx* ctx, double out[9]);
  * xDiscreteElementMethodSimulation.cpp:100). Times candidate launch shapes of the step layer
  * on the caller's scenes: for each candidate a context is built from `base` with that shape,
  * `warmup` steps run untimed and `steps` are timed; coordinate descent over the full-pass
- * blocks (256, 512, 1024, 2048) at 4 buckets per sphere, then the bucket budget (2, 4, 8).
+ * blocks (256, 512, 1024, 2048) at 4 buckets per sphere, then the bucket budget (2, 4, 8),
+ * then the adaptive skin's cap (0.2, 0.4, 0.8 of the smallest radius).
  * Results do not depend on the shapes, so only speed is compared; a candidate whose run sets
  * an error bit other than XDEM_ERR_GRID_SYNC is discarded. The winner is appended to
  * `cache_path` (a text file, one line per scene signature; NULL: no cache) and read back from
  * it on later calls with the same signature (no trials then, from_cache = 1). Use the result
- * as xdem_ctx_config.full_blocks / buckets_per_sphere. Allocates and frees contexts: call it
- * at set-up. */
+ * as xdem_ctx_config.full_blocks / buckets_per_sphere / skin_cap (the cap is tried only when
+ * base->skin is 0, the adaptive skin). Allocates and frees contexts: call it at set-up. */
 #define XDEM_AUTOTUNE_MAX_TRIALS 16
 typedef struct xdem_autotune_result {
     unsigned int full_blocks, buckets_per_sphere;
@@ -386,6 +393,8 @@ typedef struct xdem_autotune_result {
     unsigned int trial_buckets_per_sphere[XDEM_AUTOTUNE_MAX_TRIALS];
     double trial_ms_per_step[XDEM_AUTOTUNE_MAX_TRIALS];
     char key[256];                  /* the scene signature */
+    double skin_cap;                /* of the winner (xdem_ctx_config.skin_cap) */
+    double trial_skin_cap[XDEM_AUTOTUNE_MAX_TRIALS];
 } xdem_autotune_result;
 int xdem_autotune(const xdem_scene* scenes, unsigned int nscene, const xdem_ctx_config* base, unsigned int warmup,
                   unsigned int steps, const char* cache_path, xdem_autotune_result* out);

@@ -147,7 +147,7 @@ class CtxConfig(C.Structure):
                 ("block_size", C.c_uint), ("grid_dim_max", C.c_uint), ("fuse_velocity", C.c_int),
                 ("device", C.c_int), ("buckets_per_sphere", C.c_uint), ("skin", C.c_double),
                 ("step_graph", C.c_int), ("rebuild_wait_limit", C.c_uint), ("full_blocks", C.c_uint),
-                ("merge_pairs_below", C.c_uint), ("keep_order", C.c_int)]
+                ("merge_pairs_below", C.c_uint), ("keep_order", C.c_int), ("skin_cap", C.c_double)]
 
 
 class CtxInfo(C.Structure):
@@ -167,7 +167,7 @@ class AutotuneResult(C.Structure):
     _fields_ = [("full_blocks", C.c_uint), ("buckets_per_sphere", C.c_uint), ("ms_per_step", C.c_double),
                 ("from_cache", C.c_int), ("ntrial", C.c_uint), ("trial_full_blocks", C.c_uint * 16),
                 ("trial_buckets_per_sphere", C.c_uint * 16), ("trial_ms_per_step", C.c_double * 16),
-                ("key", C.c_char * 256)]
+                ("key", C.c_char * 256), ("skin_cap", C.c_double), ("trial_skin_cap", C.c_double * 16)]
 
 
 assert C.sizeof(DemParameters) == 144

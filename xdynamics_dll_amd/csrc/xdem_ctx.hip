@@ -616,8 +616,9 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
     // skin in neighbouring buckets (the reference's cell size stays the one of the visiting
     // order)
     const ContactConst cc_pp = make_contact_const(s->cp_pp, 0.5 * rmax);
-    // default: adaptive, up to 20% of the smallest radius, starting at 2.5%
-    const double skin = c->cfg.skin > 0.0 ? c->cfg.skin : (c->cfg.skin == 0.0 ? 0.2 * rmin : 0.0);
+    // default: adaptive, up to skin_cap (0.2) of the smallest radius, starting at an eighth of it
+    const double cap = c->cfg.skin_cap > 0.0 ? c->cfg.skin_cap : 0.2;
+    const double skin = c->cfg.skin > 0.0 ? c->cfg.skin : (c->cfg.skin == 0.0 ? cap * rmin : 0.0);
     const double bucket_cs = std::max(s->cell_size, 2.0 * rmax + cc_pp.coh_reach + skin) * (1.0 + 1e-9);
     // bucket grid: the reference's origin and cell size, power-of-two dims covering the
     // scene (wrapping beyond them, as the reference's 128^3 grid does)

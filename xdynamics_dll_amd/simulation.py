@@ -25,7 +25,7 @@ class DemContext:
                  grid_dim_max: int = 512, device: int = 0, fuse_velocity: bool = False,
                  buckets_per_sphere: int = 0, skin: float = 0.0, step_graph: int = 0,
                  rebuild_wait_limit: int = 0, meshes=(), full_blocks: int = 0, cylinders=(),
-                 merge_pairs_below: int = 0, keep_order: bool = False):
+                 merge_pairs_below: int = 0, keep_order: bool = False, skin_cap: float = 0.0):
         if isinstance(scenes, SceneData):
             scenes = [scenes]
         self.scenes = list(scenes)
@@ -46,6 +46,7 @@ class DemContext:
         cfg.full_blocks = full_blocks   # 0: 1024 blocks for the full-pass kernels
         cfg.merge_pairs_below = merge_pairs_below   # 0: 65536; 1: the pair pass always runs
         cfg.keep_order = 1 if keep_order else 0   # 0: spheres stored in bucket order (API calls keep the caller's)
+        cfg.skin_cap = skin_cap   # 0: the adaptive skin goes up to 0.2 of the smallest radius
         self.precision = precision
         self.capacity = p2p_capacity
         self._h = C.c_void_p()
@@ -230,9 +231,10 @@ class DemContext:
 
 def autotune(scenes, precision: str = "mixed", fuse_velocity: bool = True, p2p_capacity: int = 12,
              warmup: int = 20, steps: int = 40, cache_path: str | None = None, step_graph: int = 0) -> dict:
-    """Launch shapes of the step layer for these scenes (xdem_autotune): full-pass blocks and
-    buckets per sphere, chosen by timing short runs, cached per scene signature in cache_path.
-    Returns the choice, whether it came from the cache, and every trial's time."""
+    """Launch shapes of the step layer for these scenes (xdem_autotune): full-pass blocks,
+    buckets per sphere and the adaptive skin's cap, chosen by timing short runs, cached per
+    scene signature in cache_path. Returns the choice, whether it came from the cache, and
+    every trial's time."""
     if isinstance(scenes, SceneData):
         scenes = [scenes]
     L = _lib.lib()
@@ -249,10 +251,11 @@ def autotune(scenes, precision: str = "mixed", fuse_velocity: bool = True, p2p_c
     path = None if cache_path is None else os.fsencode(cache_path)
     _lib.check(L.xdem_autotune(arr, len(scenes), C.byref(cfg), warmup, steps, path, C.byref(res)), "xdem_autotune")
     del keeps
-    return {"full_blocks": res.full_blocks, "buckets_per_sphere": res.buckets_per_sphere,
+    return {"full_blocks": res.full_blocks, "buckets_per_sphere": res.buckets_per_sphere, "skin_cap": res.skin_cap,
             "ms_per_step": res.ms_per_step, "from_cache": bool(res.from_cache), "key": res.key.decode(),
             "trials": [{"full_blocks": res.trial_full_blocks[k], "buckets_per_sphere": res.trial_buckets_per_sphere[k],
-                        "ms_per_step": res.trial_ms_per_step[k]} for k in range(res.ntrial)]}
+                        "skin_cap": res.trial_skin_cap[k], "ms_per_step": res.trial_ms_per_step[k]}
+                       for k in range(res.ntrial)]}
 
 
 class xIntegratorVV:
