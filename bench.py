@@ -385,8 +385,8 @@ def batch_main(args):
     n = sum(s.np for s in scenes)
     fuse = not args.split
     tuned = None
-    skin_cap = 0.0
-    if not args.full_blocks and not args.buckets_per_sphere:
+    skin_cap = args.skin_cap
+    if not args.full_blocks and not args.buckets_per_sphere and not args.skin_cap:
         t0 = time.perf_counter()
         cache = os.path.join(tempfile.gettempdir(), "xdem_autotune_bench.txt")
         # trials over the timed run's own steps: the skin's cap (list builds against listed
@@ -486,7 +486,8 @@ def wheel_main(args):
     cp_wheel = contact_property(2e11, 5e7, 0.3, 0.25, 0.4, 0.6, 0.05)
     quirks = dict(gpu_force_law=1) if args.force_law == "kernel" else None
     ctx = DemContext(scene, precision="mixed", p2p_capacity=args.capacity, fuse_velocity=True, step_graph=50,
-                     meshes=[(0, tris, park, cp_wheel)], full_blocks=args.full_blocks, quirks=quirks)
+                     meshes=[(0, tris, park, cp_wheel)], full_blocks=args.full_blocks, quirks=quirks,
+                     skin_cap=args.skin_cap)
     t_build = time.perf_counter() - t_build
     t0 = time.perf_counter()
     done = 0
@@ -587,7 +588,7 @@ def lattice_main(args):
     n = scene.np
     fuse = not args.split
     r = run_gpu(scene, "fp64", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin,
-                args.graph, args.full_blocks)
+                args.graph, args.full_blocks, skin_cap=args.skin_cap)
     value = n * args.steps / r["seconds"]
     tm = r["timing"]
     kr = kernel_rooflines(tm, n, r["info"], r["n_pp"], r["n_wall"], fuse)
@@ -747,6 +748,9 @@ def main():
                     help="rolling + velocity update in a kernel of their own (default: inside the contact pass)")
     ap.add_argument("--graph", type=int, default=50,
                     help="replay the timed steps from a HIP graph of this many steps (even; 0: eager launches)")
+    ap.add_argument("--skin-cap", type=float, default=0.0,
+                    help="the adaptive skin's cap as a fraction of the smallest radius (0: the library's 0.4; "
+                         "the batch: autotuned)")
     ap.add_argument("--skin", type=float, default=0.0,
                     help="neighbour-list skin in m (0: adaptive, <0: rebuild every step)")
     ap.add_argument("--buckets-per-sphere", type=int, default=0,
@@ -804,7 +808,7 @@ def main():
 
     fuse = not args.split
     mixed = run_gpu(scene, "mixed", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin,
-                    args.graph, args.full_blocks, args.merge_below)
+                    args.graph, args.full_blocks, args.merge_below, skin_cap=args.skin_cap)
     value = n * args.steps / mixed["seconds"]
     print(f"[dam] mixed leg: {mixed['seconds'] / args.steps * 1e3:.4f} ms per step", file=sys.stderr, flush=True)
     cbar = (mixed["n_pp"] + mixed["n_wall"]) / n

@@ -145,8 +145,9 @@ typedef struct xdem_ctx_config {
     double skin;                 /* neighbour-list skin [m]: the grid and the lists are rebuilt
                                     only on steps after a sphere has moved half of it since
                                     the last build (or after a state write or a wall move).
-                                    0 = adaptive: between 1.25% and 20% of the scene's
-                                    smallest radius, starting at 2.5%, doubled when the lists
+                                    0 = adaptive: between a sixteenth of skin_cap and
+                                    skin_cap times the scene's smallest radius (2.5% and 40%
+                                    by default), starting at a sixteenth, doubled when the lists
                                     lasted fewer than 16 steps and halved when they lasted
                                     more than 64; < 0 = no skin (rebuilt every step).
                                     Results do not depend on it. */
@@ -183,7 +184,7 @@ typedef struct xdem_ctx_config {
                                     (xdem_ctx_device_ptr) address the stored order:
                                     xdem_ctx_sphere_order maps it. */
     double skin_cap;             /* the adaptive skin's largest value as a fraction of each
-                                    scene's smallest radius (0 = 0.2); the skin adapts between
+                                    scene's smallest radius (0 = 0.4); the skin adapts between
                                     a sixteenth of it and it, and buckets are sized for it. A
                                     larger cap means fewer list builds and more listed pairs
                                     (xdem_autotune tries 0.2, 0.4 and 0.8). Results do not

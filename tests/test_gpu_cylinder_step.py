@@ -107,7 +107,9 @@ def test_moving_drum_matches_oracle():
         if k % 10 == 9:
             assert_state(gpu, ref, f"step {k}")
             assert_cyl_history(gpu, ref, f"step {k}")
-    assert gpu.info()["list_builds"] >= 6   # the shifts wore the lists out
+    # the shifts wore the lists out: builds beyond the first (12 shifts of 50 um against an
+    # adaptive skin of up to 0.4 of the smallest radius)
+    assert gpu.info()["list_builds"] >= 3
     assert gpu.error_word() == 0
     gpu.close()
 

@@ -142,7 +142,7 @@ extern "C" int xdem_autotune(const xdem_scene* scenes, unsigned int nscene, cons
     static const unsigned kBlocks[] = {256, 512, 1024, 2048};
     static const unsigned kBuckets[] = {2, 4, 8};
     static const double kCaps[] = {0.2, 0.4, 0.8};
-    const double cap0 = cfg.skin_cap > 0.0 ? cfg.skin_cap : 0.2;
+    const double cap0 = cfg.skin_cap > 0.0 ? cfg.skin_cap : 0.4;
     std::vector<Trial> trials;
     auto trial = [&](unsigned f, unsigned b, double c) -> int {
         for (const Trial& t : trials)

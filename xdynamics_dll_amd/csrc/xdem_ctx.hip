@@ -616,8 +616,8 @@ int xdem_ctx_add_scene(xdem_ctx* c, const xdem_scene* s, unsigned int* idx)
     // skin in neighbouring buckets (the reference's cell size stays the one of the visiting
     // order)
     const ContactConst cc_pp = make_contact_const(s->cp_pp, 0.5 * rmax);
-    // default: adaptive, up to skin_cap (0.2) of the smallest radius, starting at an eighth of it
-    const double cap = c->cfg.skin_cap > 0.0 ? c->cfg.skin_cap : 0.2;
+    // default: adaptive, up to skin_cap (0.4) of the smallest radius, starting at a sixteenth of it
+    const double cap = c->cfg.skin_cap > 0.0 ? c->cfg.skin_cap : 0.4;
     const double skin = c->cfg.skin > 0.0 ? c->cfg.skin : (c->cfg.skin == 0.0 ? cap * rmin : 0.0);
     const double bucket_cs = std::max(s->cell_size, 2.0 * rmax + cc_pp.coh_reach + skin) * (1.0 + 1e-9);
     // bucket grid: the reference's origin and cell size, power-of-two dims covering the
@@ -952,7 +952,9 @@ int xdem_ctx_commit(xdem_ctx* c)
         ListCtl L;
         std::memset(&L, 0, sizeof(L));
         L.adaptive = c->cfg.skin == 0.0 ? 1u : 0u;
-        L.scale = L.adaptive ? 0.125f : 1.0f;
+        // the adaptive skin starts at its smallest value, a sixteenth of the cap: short runs
+        // keep short lists, long ones grow the skin as builds come too often
+        L.scale = L.adaptive ? 0.0625f : 1.0f;
         L.min_scale = L.adaptive ? 0.0625f : 1.0f;
         L.half_skin_min = 1e300;
         for (const SceneHost& h : c->scenes) L.half_skin_min = std::min(L.half_skin_min, 0.5 * h.dev.skin);

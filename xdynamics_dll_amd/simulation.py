@@ -40,13 +40,13 @@ class DemContext:
         cfg.fuse_velocity = 1 if fuse_velocity else 0
         cfg.device = device
         cfg.buckets_per_sphere = buckets_per_sphere
-        cfg.skin = skin   # 0: adaptive (1.25-20% of the smallest radius); < 0: lists rebuilt every step
+        cfg.skin = skin   # 0: adaptive (up to skin_cap of the smallest radius); < 0: lists rebuilt every step
         cfg.step_graph = step_graph   # > 0: step() replays HIP graphs of that many steps; 0: eager launches
         cfg.rebuild_wait_limit = rebuild_wait_limit   # 0: ~1 s before a rebuild phase gives up
         cfg.full_blocks = full_blocks   # 0: 1024 blocks for the full-pass kernels
         cfg.merge_pairs_below = merge_pairs_below   # 0: 65536; 1: the pair pass always runs
         cfg.keep_order = 1 if keep_order else 0   # 0: spheres stored in bucket order (API calls keep the caller's)
-        cfg.skin_cap = skin_cap   # 0: the adaptive skin goes up to 0.2 of the smallest radius
+        cfg.skin_cap = skin_cap   # 0: the adaptive skin goes up to 0.4 of the smallest radius
         self.precision = precision
         self.capacity = p2p_capacity
         self._h = C.c_void_p()
