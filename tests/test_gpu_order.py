@@ -166,3 +166,32 @@ def test_reorder_in_the_coupled_wheel_loop():
         assert np.array_equal(sa[f], sb[f]), f
     for x, y in zip(ma + pa, mb + pb):
         assert np.array_equal(x, y)
+
+
+def test_reorder_with_cylinders_and_two_scenes():
+    """A drum (cylinder slots) next to a shuffled pack in one context, re-sorted mid-run: both
+    scenes go on bit for bit."""
+    from cylinder_scenes import drum
+
+    dscene, info, dbi, cp = drum()
+    pack = shuffled(random_pack(n=2000, seed=8), 15)
+    runs = []
+    for cut in (True, False):
+        ctx = DemContext([dscene, pack], precision="fp64", fuse_velocity=True, cylinders=[(0, info, dbi, cp)])
+        if cut:
+            ctx.step(9)
+            ctx.reorder()
+            ctx.step(21)
+        else:
+            ctx.step(30)
+        runs.append(({(f, s): ctx.read(f, s) for f in STATE for s in (0, 1)},
+                     ctx.history("cylinder", 0), ctx.history("p2p", 0), ctx.history("p2p", 1)))
+        assert ctx.error_word() == 0
+        ctx.close()
+    (sa, *ha), (sb, *hb) = runs
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    for x, y in zip(ha, hb):
+        for u, v in zip(x, y):
+            assert np.array_equal(u, v)
+    assert runs[0][1][0].sum() > 0   # the drum's slots are in use
