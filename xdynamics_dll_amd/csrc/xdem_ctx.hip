@@ -240,6 +240,7 @@ struct xdem_ctx {
     uint32_t *work2 = nullptr, *seg_count2 = nullptr;
     uint32_t* grid_bar = nullptr;                        // k_rebuild's phase cursors and flags
     uint32_t rebuild_blocks = 0;                         // k_rebuild's grid
+    uint32_t resident_blocks = kQueueSegs;               // k_touch's and k_pairs' grid
     unsigned long long* pass_mask = nullptr;
     // stored order: stored sphere -> the caller's global index (scene begin + caller's index)
     // and back; the same ranges per scene as the caller's
@@ -370,6 +371,7 @@ struct xdem_ctx {
         A.keep_velocity = cfg.quirks.velocity_not_updated; A.tmax_persist = tmax_persist;
         A.zero_force = 1;
         A.full_blocks = cfg.full_blocks ? std::max(cfg.full_blocks, kQueueSegs) : 1024u;
+        A.resident_blocks = resident_blocks;
         A.merge_below = cfg.merge_pairs_below ? cfg.merge_pairs_below : kMergePairsBelow;
         A.has_cyl = cyls.empty() ? 0 : 1;
         return A;
@@ -754,6 +756,11 @@ int xdem_ctx_commit(xdem_ctx* c)
     CTX_CHECK(c->alloc(&c->seg_count2, sizeof(uint32_t) * kQueueSegs * kSegStride));
     CTX_CHECK(c->alloc(&c->grid_bar, sizeof(uint32_t) * kSegStride * kBarSlots));
     c->rebuild_blocks = rebuild_grid_size(c->multi);
+    {
+        int cus = 0;
+        CTX_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
+        c->resident_blocks = resident_grid(cus);
+    }
     if (c->rebuild_blocks == 0) return XDEM_E_HIP;
     CTX_CHECK(c->alloc(&c->nbr, sizeof(uint32_t) * n * c->list_cap));
     const size_t nbt = bucket_table_size(c->nbucket);

@@ -457,8 +457,9 @@ hipError_t launch_drift(const StepArgs& A, bool multi, hipStream_t s)
 
 hipError_t launch_touch(const StepArgs& A, bool multi, hipStream_t s)
 {
-    // kQueueChunks blocks per queue segment (a build step loops over the sorted slots)
-    const dim3 g(kQueueSegs * kQueueChunks);
+    // the resident grid: a build step loops over the sorted slots, the other steps stride
+    // over the candidate queue's segments
+    const dim3 g(A.resident_blocks);
     if (multi) {
         if (A.has_cyl) hipLaunchKernelGGL((k_touch<true, true>), g, dim3(kBlock), 0, s, A);
         else hipLaunchKernelGGL((k_touch<true, false>), g, dim3(kBlock), 0, s, A);

@@ -203,6 +203,8 @@ struct StepArgs {
     uint32_t nwslot, nbucket;
     uint32_t list_cap;                       // neighbour-list entries per sorted slot
     uint32_t full_blocks;                    // grid of the full-pass kernels (host side: launch shape)
+    uint32_t resident_blocks;                // grid of k_touch and k_pairs: the blocks the device holds
+                                             // at once at three waves per SIMD (host side)
     uint32_t merge_below;                    // full passes smaller than this: k_collide walks the pairs
     int has_cyl;                             // a scene has cylinders (host side: the CYL kernels)
     int stale_ft, flat_inertia, keep_velocity, tmax_persist;
@@ -1636,7 +1638,14 @@ using CollideLaunch = hipError_t (*)(const StepArgs&, bool multi, bool fuse, int
                                      hipStream_t);
 // which: 0 = k_lite_move then k_lite_turn, 1 = k_lite_move only, 2 = k_lite_turn only
 void launch_lite_fused(const StepArgs&, bool multi, hipStream_t, int which = 0);
-constexpr unsigned kQueueChunks = 16;   // blocks per queue segment of k_touch
+// Blocks of k_touch and k_pairs: as many as the device holds at once at three waves per SIMD
+// (both kernels fit three: k_pairs by its launch bounds, k_touch by its registers), one
+// round with no tail, a multiple of the queue segments.
+inline uint32_t resident_grid(int cus)
+{
+    const uint32_t b = 3u * (uint32_t)(cus > 0 ? cus : 1) / kQueueSegs * kQueueSegs;
+    return b > kQueueSegs ? b : kQueueSegs;
+}
 CollideLaunch collide_launch_f64();
 CollideLaunch collide_launch_mixed();
 // xdem_grid.hip

@@ -31,8 +31,8 @@ void collide_passes(const StepArgs& A, int pass, hipStream_t s)
         if (A.has_cyl) hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, MODEL, GPU_LAW, true>), full_grid(A), b, 0, s, A);
         else hipLaunchKernelGGL((k_collide<float, MULTI, FUSE, MODEL, GPU_LAW, false>), full_grid(A), b, 0, s, A);
     } else if (pass == 3) {
-        // the pair pass: one thread per queued sphere
-        hipLaunchKernelGGL((k_pairs<float, MULTI, MODEL, GPU_LAW>), full_grid(A), b, 0, s, A);
+        // the pair pass: the resident grid (every block in one round)
+        hipLaunchKernelGGL((k_pairs<float, MULTI, MODEL, GPU_LAW>), dim3(A.resident_blocks), b, 0, s, A);
     } else if (pass == 2) {
         // the triangle pass (grid-strided over the mesh queue, whose length is on the device)
         hipLaunchKernelGGL((k_mesh_pass<float, MULTI, MODEL, GPU_LAW>), full_grid(A), b, 0, s, A);
