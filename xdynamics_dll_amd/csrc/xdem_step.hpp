@@ -990,7 +990,22 @@ __device__ __forceinline__ void collide_sphere(const StepArgs& A, uint32_t i, bo
     tot.M = mk3(0.0, 0.0, 0.0) + pp.M;
     tot.T = o.t0 + pp.T;
     tot.R = 0.0 + pp.R;
-    if (S.nwall) {
+    if (S.nwall && !(flags & kWallNear)) {
+        // no wall in reach + skin since the list build (the build's exact test): every wall's
+        // contact is empty, so the slot count follows wall_no_contact's rule as in the lite
+        // pass, and a dynamic wall's wrench slot gets zeros
+        const uint32_t count = S.wall_id0 ? 0u : min(count0, (uint32_t)XDEM_MAX_P2PL_COUNT);
+        if (count != count0) A.pl_count[i] = count;
+        n_wall = count;
+        if (S.wall_dynamic)
+            for (uint32_t w = 0; w < S.nwall; w++) {
+                const int32_t slot = A.walls[S.wall_begin + w].wslot;
+                if (slot < 0) continue;
+                double* wr = A.wrench + (size_t)slot * 6 * A.np;
+#pragma unroll
+                for (int c = 0; c < 6; c++) wr[(size_t)c * A.np + i] = 0.0;
+            }
+    } else if (S.nwall) {
         // wall slots are read only when a wall needs them and written back only
         // when one changed (the values in memory are the same either way)
         for (uint32_t w = 0; w < S.nwall; w++) {
