@@ -588,7 +588,7 @@ def lattice_main(args):
     n = scene.np
     fuse = not args.split
     r = run_gpu(scene, "fp64", args.steps, args.warmup, args.capacity, fuse, args.buckets_per_sphere, args.skin,
-                args.graph, args.full_blocks, skin_cap=args.skin_cap)
+                args.graph, args.full_blocks, args.merge_below, skin_cap=args.skin_cap)
     value = n * args.steps / r["seconds"]
     tm = r["timing"]
     kr = kernel_rooflines(tm, n, r["info"], r["n_pp"], r["n_wall"], fuse)
