@@ -778,8 +778,8 @@ def main():
     if args.steps is None:
         args.steps = 1000 if args.workload == "batch" else 2000
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described "
-              f"in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout (one spatial slab "
+              f"of spheres per GPU with halo exchange) is not built", file=sys.stderr)
         sys.exit(2)
 
     import torch

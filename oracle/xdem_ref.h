@@ -62,7 +62,8 @@ typedef struct ref_triangle_info {
     double P[3], Q[3], R[3];
 } ref_triangle_info;
 
-/* Quirk switches (the survey's quirk register; defaults give the parity profile). */
+/* Quirk switches: behaviours of the reference that are kept or changed (defaults give the
+ * parity profile). */
 /* device_cylinder_info (decl:220-228): len_rr = (length, side-wall radius, inner/rim radius). */
 typedef struct ref_cylinder_info {
     unsigned int id, empty_part;
